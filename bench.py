@@ -1,0 +1,136 @@
+"""Benchmark: timesteps/s of the FP64 advance! step on one MI355X.
+
+Workload: impulsively started cylinder (diameter 1) at Re = 200 in uniform
+flow, finest grid 1024 x 1024 cells with dx = 1/128 on (-2, 6) x (-4, 4), 5
+nested levels, dt = 1e-3, 402 body points, initial vorticity a Gaussian vortex
+of circulation 0.01 and width 0.1 at (1.5, 0.3) (the 5-level 1024^2 precision
+configuration run in FP64).  A step is one advance!(state, prob, t).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line.  Timed region: K consecutive steps enqueued by
+ibpm_advance_n with the state resident in HBM, bracketed by device
+synchronisation (the call returns after its stream has drained).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in ("ibpm.jl_amd", "oracle", "tests"):
+    sys.path.insert(0, os.path.join(ROOT, p))
+
+import numpy as np  # noqa: E402
+
+METRIC = "timesteps/s, 5-level 1024² grid: FP64 vs mixed @CD err≤1e-6; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_step(nx, ny, mg, nb):
+    """8 [L (4Q + 24N) + 2.5 (L-1) N + 8N + (2 nb)^2] bytes: every operator of
+    advance! reads its inputs and writes its output once (static body)."""
+    N = (nx - 1) * (ny - 1)
+    Q = ny * (nx + 1) + nx * (ny + 1)
+    L = mg
+    return 8.0 * (L * (4 * Q + 24 * N) + 2.5 * (L - 1) * N + 8 * N + (2 * nb) ** 2)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import ibpm
+    from ibpm import _capi as C
+    from cases import case, gaussian_vortex_gamma
+
+    c = case("config4")
+    g = c["grid"]
+    nb = len(c["bodies"][0].ds)
+    t0 = time.time()
+    prob = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
+    setup_s = time.time() - t0
+    state = ibpm.IBState(prob)
+    state.Γ = gaussian_vortex_gamma(g)
+    ibpm.vort2flux_(state, prob)
+    dt = c["dt"]
+    print(f"# setup {setup_s:.2f}s (B assembly on GPU {prob.info.setup_seconds:.2f}s, "
+          f"|B-B'|/|B| = {prob.info.b_asymmetry:.2e}); warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    ts_w = np.arange(args.warmup) * dt
+    ibpm.advance_n_(state, prob, ts_w)
+    ts = (args.warmup + np.arange(args.steps)) * dt
+    t1 = time.perf_counter()
+    cd, cl, done = ibpm.advance_n_(state, prob, ts)
+    el = time.perf_counter() - t1
+    if done != args.steps:
+        raise RuntimeError(f"non-finite forces after {done} steps")
+    sps = args.steps / el
+    print(f"# timed {args.steps} steps in {el:.3f}s: {sps:.1f} steps/s; CD={cd[-1, 0]:.6f} CL={cl[-1, 0]:.3e}",
+          file=sys.stderr, flush=True)
+
+    # dominant-kernel roofline, HIP events on the problem's stream
+    L = g.mg
+    N = g.nΓ
+    kb = {}
+    for which, name in ((0, "k_xpass<X_PLAIN> (all levels)"), (1, "k_ypass<8> op=A (all levels)"),
+                        (2, "k_nonlinear (all levels)")):
+        v = __import__("ctypes").c_double(0.0)
+        prob.ctx.check(C.lib().ibpm_time_kernel(prob.h, which, args.kernel_reps, __import__("ctypes").byref(v)))
+        kb[name] = v.value
+    # algorithmic bytes per launch: read N + write N doubles per level for a DST pass
+    dst_bytes = 2.0 * 8.0 * N * L
+    ypass_ms = kb["k_ypass<8> op=A (all levels)"]
+    roof = {"bound": "hbm", "kernel": "k_ypass<8> (fused DST-I y-pass, 5 levels)",
+            "achieved": round(dst_bytes / (ypass_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(dst_bytes / (ypass_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": dst_bytes, "avg_launch_ms": round(ypass_ms, 5)}
+    step_bytes = algorithmic_bytes_per_step(g.nx, g.ny, g.mg, nb)
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        from cases import boundary_of, oracle_bodies, sc7
+
+        O.set_threads(1)
+        op = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], dt, oracle_bodies(c), binv=prob.Binv)
+        S = op.new_state(sc7(c, 0.0))
+        S.G[:] = gaussian_vortex_gamma(g)
+        op.run(S, sc7(c, 0.0)[None, :])  # first step outside the timing
+        tc = time.perf_counter()
+        op.run(S, np.tile(sc7(c, 0.0), (args.cpu_steps, 1)))
+        ec = time.perf_counter() - tc
+        cpu = {"value": round(args.cpu_steps / ec, 4), "unit": "timesteps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} advance! steps of the same 1024x1024 5-level case (after 1 untimed), "
+                         f"C oracle, 1 thread, {ec:.1f}s"}
+
+    out = {
+        "metric": METRIC, "value": round(sps, 2), "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "FP64 advance!, cylinder Re=200, 1024x1024 cells (dx=1/128 on [-2,6]x[-4,4]), "
+                               "5 levels, dt=1e-3, 402 body points, Gaussian-vortex start",
+                   "grid": [g.nx, g.ny], "levels": g.mg, "body_points": nb, "precision": "fp64"},
+        "roofline": roof,
+        "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
+                     "achieved_GBs": round(step_bytes * sps / 1e9, 1),
+                     "frac_of_peak": round(step_bytes * sps / 1e9 / HBM_PEAK_GBS, 4)},
+        "kernel_ms": {k: round(v, 5) for k, v in kb.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1048 @@
+// capi.hip -- host side of the library: problem setup, the advance! step
+// schedule and the C ABI declared in include/ibpm_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ibpm_hip.h"
+#include "kernels.h"
+
+using namespace ibpm;
+
+struct ibpm_ctx {
+    int device = 0;
+    std::string err;
+};
+
+namespace {
+
+struct DevTables {
+    DstPlan plan{};
+    double2* tw = nullptr;
+    double* sn = nullptr;
+    double2* wm = nullptr;
+};
+
+struct CoarsifyRange {
+    int i0, ic0, ci, j0, jc0, cj;
+};
+
+}  // namespace
+
+struct ibpm_prob {
+    ibpm_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    GridDev g{};
+    int mg = 1;
+    double Re = 0, dt = 0, beta[2] = {1.5, -0.5};
+    double offx = 0, offy = 0;
+    int nbodies = 0, nbt = 0, nf = 0, motion = 0;
+    std::vector<int> nb, boff;
+    std::vector<double> xb, ds;
+    std::vector<int> body_idx;  // nbt x 2, Julia 1-based values
+    std::vector<double> weight; // [(2k + c) * 169 + l + 13 m]
+    std::vector<double> binv;   // nf x nf column-major
+    double b_asym = 0.0, setup_s = 0.0;
+    int ytile = 8;
+
+    // device state (IBState) and work arrays
+    double *q = nullptr, *q0 = nullptr, *G = nullptr, *Gs = nullptr, *psi = nullptr;
+    double* nl[2] = {nullptr, nullptr};
+    int nl1 = 0, nl2 = 0;  // buffers holding state.nonlin[1], state.nonlin[2]
+    double *qs1 = nullptr, *T1 = nullptr, *T2 = nullptr, *Gtmp = nullptr, *qreg = nullptr;
+    double *Fb = nullptr, *Ftmp = nullptr, *fbv = nullptr, *CD = nullptr, *CL = nullptr, *scal = nullptr;
+    double *BinvT = nullptr, *w = nullptr, *ds_d = nullptr;
+    int *bidx = nullptr, *boff_d = nullptr;
+    int *sc_cell = nullptr, *sc_off = nullptr, *sc_f = nullptr;
+    double* sc_w = nullptr;
+    int sc_ncell = 0;
+    int rb_i0 = 2, rb_i1 = 1, rb_j0 = 2, rb_j1 = 1;  // rot! bounding box of E' support
+    double *cx = nullptr, *cy = nullptr;
+    DevTables tx, ty;
+    std::vector<CoarsifyRange> crange;
+    double* hist = nullptr;
+    size_t hist_cap = 0;
+    std::vector<void*> allocs;
+};
+
+// ---------------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------------
+static int fail(ibpm_ctx* c, int code, const std::string& m)
+{
+    if (c) c->err = m;
+    return code;
+}
+
+#define HIPTRY(ctx, expr)                                                                           \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail((ctx), IBPM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+template <typename T>
+static hipError_t dalloc(ibpm_prob* P, T** p, size_t n)
+{
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, sizeof(T) * (n > 0 ? n : 1));
+    if (e == hipSuccess) {
+        P->allocs.push_back(v);
+        *p = (T*)v;
+    }
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// host-side setup pieces
+// ---------------------------------------------------------------------------
+static const long double PI_L = 3.14159265358979323846264338327950288L;
+
+// DST-I plan for line length M-1.  Returns false when M is unsupported.
+static bool make_plan(int M, DstPlan& pl, std::vector<double2>& tw, std::vector<double>& sn, std::vector<double2>& wm)
+{
+    pl = DstPlan{};
+    pl.M = M;
+    if (M < 2) return false;
+    bool fft = (M % 2 == 0) && (M <= MAXM);
+    int P = M / 2;
+    std::vector<int> rad;
+    if (fft) {
+        int m = P, e2 = 0;
+        while (m % 2 == 0) { m /= 2; e2++; }
+        int e3 = 0, e5 = 0;
+        while (m % 3 == 0) { m /= 3; e3++; }
+        while (m % 5 == 0) { m /= 5; e5++; }
+        if (m != 1) fft = false;
+        else {
+            while (e2 >= 3) { rad.push_back(8); e2 -= 3; }
+            if (e2 == 2) rad.push_back(4);
+            if (e2 == 1) rad.push_back(2);
+            for (int k = 0; k < e5; k++) rad.push_back(5);
+            for (int k = 0; k < e3; k++) rad.push_back(3);
+        }
+    }
+    if (fft) {
+        pl.kind = 0;
+        pl.P = P;
+        pl.nst = (int)rad.size();
+        for (int k = 0; k < pl.nst; k++) pl.radix[k] = rad[k];
+        tw.resize(P);
+        for (int e = 0; e < P; e++) {
+            long double a = -2.0L * PI_L * (long double)e / (long double)P;
+            tw[e] = make_double2((double)cosl(a), (double)sinl(a));
+        }
+        sn.resize(P + 1);
+        for (int j = 0; j <= P; j++) sn[j] = (double)sinl(PI_L * (long double)j / (long double)M);
+        wm.resize(P);
+        for (int k = 0; k < P; k++) {
+            long double a = -2.0L * PI_L * (long double)k / (long double)M;
+            wm[k] = make_double2((double)cosl(a), (double)sinl(a));
+        }
+        return true;
+    }
+    if (M - 1 > MAXM) return false;
+    pl.kind = 1;
+    pl.P = 0;
+    pl.nst = 0;
+    tw.assign(1, make_double2(0, 0));
+    wm.assign(1, make_double2(0, 0));
+    sn.resize(2 * M);
+    for (int t = 0; t < 2 * M; t++) sn[t] = (double)sinl(PI_L * (long double)t / (long double)M);
+    return true;
+}
+
+// δh (interface-coupling.jl:6-62): Yang-3 smoothed delta times h.
+static double delta_h(double rf, double rb, double dr)
+{
+    double r = std::fabs(rf - rb);
+    double r1 = r / dr, r2 = r1 * r1, r3 = r2 * r1, r4 = r3 * r1;
+    double del_h;
+    if (r1 <= 1.0) {
+        double a5 = std::asin((1.0 / 2.0) * std::sqrt(3.0) * (2.0 * r1 - 1.0));
+        double a8 = std::sqrt(1.0 - 12.0 * r2 + 12.0 * r1);
+        del_h = 4.166666667e-2 * r4 + (-0.1388888889 + 3.472222222e-2 * a8) * r3 +
+                (-7.121664902e-2 - 5.208333333e-2 * a8 + 0.2405626122 * a5) * r2 +
+                (-.2405626122 * a5 - 0.3792313933 + 0.1012731481 * a8) * r1 + 8.0187537413e-2 * a5 -
+                4.195601852e-2 * a8 + 0.6485698427;
+    } else if (r1 <= 2.0) {
+        double a6 = std::asin((1.0 / 2.0) * std::sqrt(3.0) * (-3.0 + 2.0 * r1));
+        double a9 = std::sqrt(-23.0 + 36.0 * r1 - 12.0 * r2);
+        del_h = -6.250000000e-2 * r4 + (0.4861111111 - 1.736111111e-2 * a9) * r3 +
+                (-1.143175026 + 7.812500000e-2 * a9 - 0.1202813061 * a6) * r2 +
+                (0.8751991178 + 0.3608439183 * a6 - 0.1548032407 * a9) * r1 - 0.2806563809 * a6 +
+                8.22848104e-3 + 0.1150173611 * a9;
+    } else if (r1 <= 3.0) {
+        double a1 = std::asin((1.0 / 2.0 * (2.0 * r1 - 5.0)) * std::sqrt(3.0));
+        double a7 = std::sqrt(-71.0 - 12.0 * r2 + 60.0 * r1);
+        del_h = 2.083333333e-2 * r4 + (3.472222222e-3 * a7 - 0.2638888889) * r3 +
+                (1.214391675 - 2.604166667e-2 * a7 + 2.405626122e-2 * a1) * r2 +
+                (-0.1202813061 * a1 - 2.449273192 + 7.262731481e-2 * a7) * r1 + 0.1523563211 * a1 +
+                1.843201677 - 7.306134259e-2 * a7;
+    } else {
+        del_h = 0.0;
+    }
+    return del_h;
+}
+
+// In-place inverse of a column-major n x n matrix: LU with partial pivoting
+// (getrf), then solves against the identity (getri), as inv() does.
+static bool invert_lu(std::vector<double>& A, int n)
+{
+    std::vector<double> LU(A);
+    std::vector<int> piv(n);
+    auto at = [&](int i, int j) -> double& { return LU[(size_t)i + (size_t)n * j]; };
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double best = std::fabs(at(k, k));
+        for (int i = k + 1; i < n; i++)
+            if (std::fabs(at(i, k)) > best) { best = std::fabs(at(i, k)); p = i; }
+        piv[k] = p;
+        if (best == 0.0) return false;
+        if (p != k)
+            for (int j = 0; j < n; j++) std::swap(at(k, j), at(p, j));
+        double inv = 1.0 / at(k, k);
+        for (int i = k + 1; i < n; i++) at(i, k) *= inv;
+        for (int j = k + 1; j < n; j++) {
+            double akj = at(k, j);
+            if (akj != 0.0) {
+                double* cj = &LU[(size_t)n * j];
+                const double* ck = &LU[(size_t)n * k];
+                for (int i = k + 1; i < n; i++) cj[i] -= ck[i] * akj;
+            }
+        }
+    }
+    for (int c = 0; c < n; c++) {
+        double* x = &A[(size_t)n * c];
+        for (int i = 0; i < n; i++) x[i] = 0.0;
+        x[c] = 1.0;
+        for (int k = 0; k < n; k++)
+            if (piv[k] != k) std::swap(x[k], x[piv[k]]);
+        for (int j = 0; j < n; j++) {
+            double xj = x[j];
+            if (xj != 0.0) {
+                const double* cj = &LU[(size_t)n * j];
+                for (int i = j + 1; i < n; i++) x[i] -= cj[i] * xj;
+            }
+        }
+        for (int j = n - 1; j >= 0; j--) {
+            const double* cj = &LU[(size_t)n * j];
+            x[j] /= cj[j];
+            double xj = x[j];
+            for (int i = 0; i < j; i++) x[i] -= cj[i] * xj;
+        }
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// operator schedules on the device
+// ---------------------------------------------------------------------------
+static XArgs xargs(ibpm_prob* P)
+{
+    XArgs a{};
+    a.lstride = P->g.nG;
+    a.mg = P->mg;
+    a.pl = P->tx.plan;
+    a.g = P->g;
+    a.fac = 1.0;
+    return a;
+}
+
+static YArgs yargs(ibpm_prob* P, int op, int lev0, int nlev, double* buf)
+{
+    YArgs a{};
+    a.buf = buf;
+    a.lstride = P->g.nG;
+    a.nlev = nlev;
+    a.lev0 = lev0;
+    a.op = op;
+    a.cx = P->cx;
+    a.cy = P->cy;
+    a.dt = P->dt;
+    for (int l = 0; l < P->mg; l++) {
+        double hc = P->g.h * (double)(1L << l);
+        a.cdiv[l] = (2.0 * P->Re) * (hc * hc);
+    }
+    a.scale = 1.0 / (double)(4L * P->g.nx * P->g.ny);
+    a.pl = P->ty.plan;
+    a.g = P->g;
+    return a;
+}
+
+// x = L b for one level (lev 0-based), L one of Δinv / A / Ainv (laplacian.jl:24-76),
+// with optional apply_bc! prologue (bcr != null) and subtract epilogue.
+static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x, const double* bcr, bool sub)
+{
+    hipStream_t s = P->stream;
+    XArgs a = xargs(P);
+    a.src = b;
+    a.dst = P->T1 + (long)lev * P->g.nG;
+    a.nlev = 1;
+    a.lev0 = lev;
+    a.bcr = bcr;
+    launch_xpass(bcr ? X_BC : X_PLAIN, a, s);
+    launch_ypass(yargs(P, op, lev, 1, P->T1 + (long)lev * P->g.nG), P->ytile, s);
+    XArgs c = xargs(P);
+    c.src = P->T1 + (long)lev * P->g.nG;
+    c.dst = x;
+    c.nlev = 1;
+    c.lev0 = lev;
+    launch_xpass(sub ? X_SUB : X_PLAIN, c, s);
+}
+
+// vort2flux! (curl.jl:133-165).  Gb is mutated on coarse levels.  When
+// `level1_only`, only q on level 1 is produced (into qout), which is all the
+// trial stage and B_times! read of it.
+static void vort2flux_dev(ibpm_prob* P, double* Gb, double* psib, double* qout, bool level1_only)
+{
+    hipStream_t s = P->stream;
+    const long nG = P->g.nG;
+    const int L = P->mg;
+    for (int lev = 2; lev <= L; lev++) {
+        CoarsifyArgs c{};
+        const CoarsifyRange& r = P->crange[0];
+        c.Gc = Gb + (long)(lev - 1) * nG;
+        c.G = Gb + (long)(lev - 2) * nG;
+        c.i0 = r.i0; c.ic0 = r.ic0; c.ci = r.ci; c.j0 = r.j0; c.jc0 = r.jc0; c.cj = r.cj;
+        c.g = P->g;
+        launch_coarsify(c, s);
+    }
+    lap_level(P, Y_DINV, L - 1, Gb + (long)(L - 1) * nG, psib + (long)(L - 1) * nG, nullptr, false);
+    for (int lev = L - 1; lev >= 1; lev--) {
+        int l = lev - 1;
+        lap_level(P, Y_DINV, l, Gb + (long)l * nG, psib + (long)l * nG, psib + (long)(l + 1) * nG, false);
+    }
+    CurlArgs ca{};
+    ca.psi = psib;
+    ca.q = qout;
+    ca.qstride = P->g.nq;
+    ca.nlev = level1_only ? 1 : L;
+    ca.lev0 = 0;
+    ca.mg = L;
+    ca.g = P->g;
+    launch_curl(ca, s);
+}
+
+// Γout[:,1] = Ainv[1] * rot(reg(f)) -- the body-force projection of
+// project_circ! (timestepping.jl:217-218) and of B_times! (laplacian.jl:162).
+// sub: Γout[:,1] -= result instead of storing it.
+static void project_forces(ibpm_prob* P, const double* f, double* Gout, bool sub)
+{
+    hipStream_t s = P->stream;
+    ScatterArgs sa{};
+    sa.cell = P->sc_cell; sa.off = P->sc_off; sa.ent_f = P->sc_f; sa.ent_w = P->sc_w;
+    sa.fb = f; sa.q = P->qreg; sa.ncell = P->sc_ncell;
+    launch_scatter(sa, s);
+    launch_rot(P->qreg, P->Gtmp, P->rb_i0, P->rb_i1, P->rb_j0, P->rb_j1, P->g, s);
+    lap_level(P, Y_AINV, 0, P->Gtmp, Gout, nullptr, sub);
+}
+
+// advance!(state, prob, t) -- timestepping.jl:35-67.  hist: optional CD/CL slot.
+static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, double* clh)
+{
+    hipStream_t s = P->stream;
+    const GridDev& g = P->g;
+    const long nG = g.nG;
+    const int L = P->mg;
+    if (P->motion == IBPM_MOTION_MOVING_GRID) {
+        // base_flux! MovingGrid (timestepping.jl:284-298)
+        double Om = -sc.theta_dot;
+        double al = -sc.theta;
+        double Ux0 = sc.U * std::cos(al) - sc.V * std::sin(al);
+        double Uy0 = sc.U * std::sin(al) + sc.V * std::cos(al);
+        Ux0 += sc.Ux * std::cos(sc.inclination) - sc.Uy * std::sin(sc.inclination);
+        Uy0 += sc.Ux * std::sin(sc.inclination) + sc.Uy * std::cos(sc.inclination);
+        BaseFluxArgs b{};
+        b.q0 = P->q0; b.mg = L; b.Ux0 = Ux0; b.Uy0 = Uy0; b.Om = Om; b.offx = P->offx; b.offy = P->offy; b.g = g;
+        launch_base_flux_moving(b, s);
+    }
+    // ---- get_trial_state! (timestepping.jl:86-129), all levels at once
+    const int out = (P->nl1 != P->nl2) ? P->nl1 : 1 - P->nl2;
+    NLArgs na{};
+    na.q = P->q; na.q0 = P->q0; na.G = P->G; na.out = P->nl[out];
+    na.nlev = L; na.lev0 = 0; na.mg = L; na.g = g;
+    for (int l = 0; l < L; l++) {
+        double hc = g.h * (double)(1L << l);
+        na.s_lev[l] = 1.0 / (4.0 * (hc * hc));
+    }
+    launch_nonlinear(na, s);
+    XArgs a = xargs(P);
+    a.src = P->G; a.dst = P->T1; a.nlev = L; a.lev0 = 0;
+    launch_xpass(X_PLAIN, a, s);
+    launch_ypass(yargs(P, Y_A, 0, L, P->T1), P->ytile, s);
+    XArgs t = xargs(P);
+    t.src = P->T1; t.dst = P->T2; t.nlev = L; t.lev0 = 0;
+    t.n1 = P->nl[out]; t.n0 = P->nl[P->nl2]; t.gam = P->G;
+    t.c1 = P->beta[0] * P->dt;
+    t.c2 = P->beta[1] * P->dt;
+    for (int l = 0; l < L; l++) {
+        double hc = g.h * (double)(1L << l);
+        t.fac_lev[l] = 0.25 * P->dt / (P->Re * (hc * hc));
+    }
+    launch_xpass(X_TRIAL, t, s);
+    launch_ypass(yargs(P, Y_AINV, 0, L, P->T2), P->ytile, s);
+    XArgs c = xargs(P);
+    c.src = P->T2; c.dst = P->Gs; c.nlev = L; c.lev0 = 0;
+    launch_xpass(X_PLAIN, c, s);
+    P->nl1 = P->nl2 = out;  // state.nonlin[2] .= state.nonlin[1]
+    vort2flux_dev(P, P->Gs, P->psi, P->qs1, true);
+    // ---- boundary_forces! (timestepping.jl:154-163) and update_stress!
+    if (P->nbt > 0) {
+        GatherArgs ga{};
+        ga.q = P->qs1; ga.q0 = P->q0; ga.w = P->w; ga.bidx = P->bidx; ga.out = P->Ftmp; ga.nbt = P->nbt; ga.g = g;
+        launch_gather(ga, s);
+        launch_gemv_rows(P->BinvT, P->Ftmp, P->Fb, P->nf, s);
+        ForceArgs fa{};
+        fa.Fb = P->Fb; fa.ds = P->ds_d; fa.boff = P->boff_d; fa.fb = P->fbv; fa.CD = P->CD; fa.CL = P->CL;
+        fa.cd_hist = cdh; fa.cl_hist = clh; fa.nbodies = P->nbodies; fa.nbt = P->nbt; fa.hdt = g.h / P->dt;
+        launch_forces(fa, s);
+        // ---- project_circ! (timestepping.jl:204-223): Γ = Γs; Γ[:,1] -= Ainv rot(reg(F̃b))
+        project_forces(P, P->Fb, P->Gs, true);
+    }
+    std::swap(P->G, P->Gs);
+    // ---- vort2flux!(state.ψ, state.q, state.Γ) (timestepping.jl:64)
+    vort2flux_dev(P, P->G, P->psi, P->q, false);
+    (void)nG;
+}
+
+// get_Binv (laplacian.jl:98-119): B[:, j] = B_times!(e_j) on the device,
+// then inv(B) on the host.
+static int build_binv(ibpm_prob* P)
+{
+    ibpm_ctx* C = P->ctx;
+    hipStream_t s = P->stream;
+    const int nf = P->nf;
+    const long nG = P->g.nG;
+    double* Bd = nullptr;
+    HIPTRY(C, hipMalloc(&Bd, sizeof(double) * (size_t)nf * nf));
+    // scratch: Gs (Γ), psi (ψ), qs1 (q[:,1]); the coarse-level ring of Γ stays 0
+    launch_fill(P->Gs, nG * P->mg, 0.0, s);
+    for (int j = 0; j < nf; j++) {
+        launch_set_unit(P->Ftmp, nf, j, s);
+        project_forces(P, P->Ftmp, P->Gs, false);
+        vort2flux_dev(P, P->Gs, P->psi, P->qs1, true);
+        GatherArgs ga{};
+        ga.q = P->qs1; ga.q0 = nullptr; ga.w = P->w; ga.bidx = P->bidx; ga.out = Bd + (long)nf * j;
+        ga.nbt = P->nbt; ga.g = P->g;
+        launch_gather(ga, s);
+    }
+    std::vector<double> B((size_t)nf * nf);
+    hipError_t e = hipMemcpyAsync(B.data(), Bd, sizeof(double) * B.size(), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(Bd);
+    if (e != hipSuccess) return fail(C, IBPM_E_HIP, std::string("B assembly: ") + hipGetErrorString(e));
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < nf; i++)
+        for (int k = 0; k < nf; k++) {
+            double d = B[(size_t)i + (size_t)nf * k] - B[(size_t)k + (size_t)nf * i];
+            num += d * d;
+            den += B[(size_t)i + (size_t)nf * k] * B[(size_t)i + (size_t)nf * k];
+        }
+    P->b_asym = den > 0 ? std::sqrt(num / den) : 0.0;
+    P->binv = B;
+    if (!invert_lu(P->binv, nf)) return fail(C, IBPM_E_SINGULAR, "B is singular");
+    return IBPM_OK;
+}
+
+static int upload_binv(ibpm_prob* P)
+{
+    const int nf = P->nf;
+    std::vector<double> T((size_t)nf * nf);
+    for (int i = 0; i < nf; i++)
+        for (int k = 0; k < nf; k++) T[(size_t)i * nf + k] = P->binv[(size_t)i + (size_t)nf * k];
+    HIPTRY(P->ctx, hipMemcpy(P->BinvT, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice));
+    return IBPM_OK;
+}
+
+static int zero_state(ibpm_prob* P)
+{
+    hipStream_t s = P->stream;
+    const long nG = P->g.nG, nq = P->g.nq;
+    launch_fill(P->q, nq * P->mg, 0.0, s);
+    launch_fill(P->q0, nq * P->mg, 0.0, s);
+    launch_fill(P->G, nG * P->mg, 0.0, s);
+    launch_fill(P->Gs, nG * P->mg, 0.0, s);
+    launch_fill(P->psi, nG * P->mg, 0.0, s);
+    launch_fill(P->nl[0], nG * P->mg, 0.0, s);
+    launch_fill(P->nl[1], nG * P->mg, 0.0, s);
+    launch_fill(P->Fb, P->nf, 0.0, s);
+    launch_fill(P->fbv, P->nf, 0.0, s);
+    launch_fill(P->CD, P->nbodies, 0.0, s);
+    launch_fill(P->CL, P->nbodies, 0.0, s);
+    P->nl1 = P->nl2 = 0;
+    return IBPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int ibpm_ctx_create(int device, ibpm_ctx** out)
+{
+    if (!out) return IBPM_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return IBPM_E_HIP;
+    if (hipSetDevice(device) != hipSuccess) return IBPM_E_HIP;
+    kernels_init();
+    ibpm_ctx* c = new ibpm_ctx();
+    c->device = device;
+    *out = c;
+    return IBPM_OK;
+}
+
+int ibpm_ctx_destroy(ibpm_ctx* ctx)
+{
+    delete ctx;
+    return IBPM_OK;
+}
+
+const char* ibpm_last_error(const ibpm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body_desc* bd, const double* binv,
+                        ibpm_prob** out)
+{
+    if (!ctx || !gd || !bd || !out) return IBPM_E_ARG;
+    *out = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    const int nx = gd->nx, ny = gd->ny, mg = gd->mg;
+    if (nx < 4 || ny < 4 || mg < 1 || mg > MAXLEV)
+        return fail(ctx, IBPM_E_ARG, "grid needs nx, ny >= 4 and 1 <= mg <= 16");
+    if (mg > 1 && (nx % 4 != 0 || ny % 4 != 0))
+        return fail(ctx, IBPM_E_UNSUPPORTED, "multi-level grids need nx and ny divisible by 4 (get_bc!/coarsify!)");
+    if (bd->motion == IBPM_MOTION_FUNCTION)
+        return fail(ctx, IBPM_E_UNSUPPORTED, "MotionFunction (lab-frame moving body) is not run by this library");
+    if (bd->motion == IBPM_MOTION_MOVING_GRID && bd->nbodies != 1)
+        return fail(ctx, IBPM_E_ARG, "MovingGrid needs exactly one body (timestepping.jl:277)");
+    ibpm_prob* P = new ibpm_prob();
+    P->ctx = ctx;
+    GridDev& g = P->g;
+    g.nx = nx; g.ny = ny; g.n1 = nx - 1; g.n2 = ny - 1;
+    g.nG = (long)(nx - 1) * (ny - 1);
+    g.nu = (long)ny * (nx + 1); g.nv = (long)nx * (ny + 1); g.nq = g.nu + g.nv;
+    g.LEFT = 0; g.RIGHT = ny + 1; g.BOT = 2 * (ny + 1); g.TOP = 2 * (ny + 1) + nx + 1;
+    g.nbc = 2 * (nx + 1) + 2 * (ny + 1);
+    g.h = gd->h;
+    P->mg = mg; P->Re = gd->Re; P->dt = gd->dt; P->beta[0] = gd->beta[0]; P->beta[1] = gd->beta[1];
+    P->offx = gd->offx; P->offy = gd->offy;
+    P->motion = bd->motion;
+    P->nbodies = bd->nbodies;
+    P->boff.push_back(0);
+    for (int b = 0; b < bd->nbodies; b++) {
+        P->nb.push_back(bd->nb[b]);
+        P->boff.push_back(P->boff.back() + bd->nb[b]);
+    }
+    P->nbt = P->boff.back();
+    P->nf = 2 * P->nbt;
+    const int nbt = P->nbt;
+    P->xb.assign(bd->xb, bd->xb + 2 * (size_t)nbt);
+    P->ds.assign(bd->ds, bd->ds + (size_t)nbt);
+    auto bail = [&](int code, const std::string& m) {
+        ibpm_problem_destroy(P);
+        return fail(ctx, code, m);
+    };
+    // ---- setup_reg (interface-coupling.jl:72-96)
+    const double h = g.h;
+    P->body_idx.resize(2 * (size_t)nbt);
+    for (int k = 0; k < nbt; k++) {
+        P->body_idx[k] = (int)std::floor((P->xb[k] + P->offx) / h);
+        P->body_idx[nbt + k] = (int)std::floor((P->xb[nbt + k] + P->offy) / h);
+        int bx = P->body_idx[k], by = P->body_idx[nbt + k];
+        if (bx - 6 < 1 || bx + 6 > nx || by - 6 < 1 || by + 6 > ny)
+            return bail(IBPM_E_ARG, "body point closer than 6 cells to the domain edge");
+    }
+    P->weight.assign((size_t)2 * nbt * 169, 0.0);
+    for (int k = 0; k < nbt; k++) {
+        double xk = P->xb[k], yk = P->xb[nbt + k];
+        int bx = P->body_idx[k], by = P->body_idx[nbt + k];
+        for (int m = 0; m < 13; m++) {
+            double y = h * (double)(by - 1 + (m - 6)) - P->offy;
+            for (int l = 0; l < 13; l++) {
+                double x = h * (double)(bx - 1 + (l - 6)) - P->offx;
+                P->weight[(size_t)(2 * k) * 169 + l + 13 * m] = delta_h(x, xk, h) * delta_h(y + h / 2, yk, h);
+                P->weight[(size_t)(2 * k + 1) * 169 + l + 13 * m] = delta_h(x + h / 2, xk, h) * delta_h(y, yk, h);
+            }
+        }
+    }
+    // E' as a segmented gather: (flux cell, F̃b index, weight), ordered by cell then point
+    struct Ent { long p; int f; double w; };
+    std::vector<Ent> ents;
+    ents.reserve((size_t)2 * nbt * 169);
+    int ib0 = nx + 1, ib1 = 0, jb0 = ny + 2, jb1 = 0;
+    for (int c = 0; c < 2; c++)
+        for (int k = 0; k < nbt; k++) {
+            int bx = P->body_idx[k], by = P->body_idx[nbt + k];
+            for (int m = 0; m < 13; m++)
+                for (int l = 0; l < 13; l++) {
+                    int i = bx + l - 6, j = by + m - 6;
+                    long p = (c == 0) ? (i - 1) + (long)(nx + 1) * (j - 1) : g.nu + (i - 1) + (long)nx * (j - 1);
+                    ents.push_back({p, k + c * nbt, P->weight[(size_t)(2 * k + c) * 169 + l + 13 * m]});
+                    // rot! outputs touched: x-flux (i,j) -> Γ(i,j),(i,j+1); y-flux -> Γ(i,j),(i+1,j)
+                    ib0 = std::min(ib0, i); jb0 = std::min(jb0, j);
+                    ib1 = std::max(ib1, i + 1); jb1 = std::max(jb1, j + 1);
+                }
+        }
+    std::stable_sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) { return a.p < b.p; });
+    std::vector<int> cell, off, ef;
+    std::vector<double> ew;
+    for (size_t e = 0; e < ents.size(); e++) {
+        if (e == 0 || ents[e].p != ents[e - 1].p) {
+            cell.push_back((int)ents[e].p);
+            off.push_back((int)e);
+        }
+        ef.push_back(ents[e].f);
+        ew.push_back(ents[e].w);
+    }
+    off.push_back((int)ents.size());
+    P->sc_ncell = (int)cell.size();
+    P->rb_i0 = std::max(ib0, 2); P->rb_i1 = std::min(ib1, nx);
+    P->rb_j0 = std::max(jb0, 2); P->rb_j1 = std::min(jb1, ny);
+    // ---- coarsify! ranges (mg-utils.jl:296-299, zip truncates to the shorter range)
+    {
+        CoarsifyRange r{};
+        int j0 = ny / 2 + ((-ny) / 2 + 2), j1 = ny / 2 + (ny / 2 - 2);
+        int jc0 = ny / 2 + ((-ny) / 4 + 1), jc1 = ny / 2 + (ny / 4 - 1);
+        int i0 = nx / 2 + ((-nx) / 2 + 2), i1 = nx / 2 + (nx / 2 - 2);
+        int ic0 = nx / 2 + ((-nx) / 4 + 1), ic1 = nx / 2 + (nx / 4 - 1);
+        int nj = j1 >= j0 ? (j1 - j0) / 2 + 1 : 0, njc = jc1 >= jc0 ? jc1 - jc0 + 1 : 0;
+        int ni = i1 >= i0 ? (i1 - i0) / 2 + 1 : 0, nic = ic1 >= ic0 ? ic1 - ic0 + 1 : 0;
+        r.i0 = i0; r.ic0 = ic0; r.ci = std::min(ni, nic);
+        r.j0 = j0; r.jc0 = jc0; r.cj = std::min(nj, njc);
+        P->crange.push_back(r);
+    }
+    // ---- DST plans and eigenvalue tables
+    std::vector<double2> twx, wmx, twy, wmy;
+    std::vector<double> snx, sny;
+    if (!make_plan(nx, P->tx.plan, twx, snx, wmx) || !make_plan(ny, P->ty.plan, twy, sny, wmy))
+        return bail(IBPM_E_UNSUPPORTED, "DST length unsupported (need nx, ny <= 1024, even with nx/2, ny/2 of the form 2^a 3^b 5^c for the FFT path)");
+    std::vector<double> cxh(nx - 1), cyh(ny - 1);
+    const long double fpi = (long double)(double)PI_L;
+    for (int i = 1; i <= nx - 1; i++) cxh[i - 1] = std::cos((double)(fpi * (long double)i / (long double)nx));
+    for (int j = 1; j <= ny - 1; j++) cyh[j - 1] = std::cos((double)(fpi * (long double)j / (long double)ny));
+    // ---- device allocations
+    hipError_t e = hipStreamCreate(&P->stream);
+    if (e != hipSuccess) return bail(IBPM_E_HIP, "hipStreamCreate failed");
+    const long nG = g.nG, nq = g.nq;
+#define DA(ptr, n)                                                              \
+    if ((e = dalloc(P, &(ptr), (size_t)(n))) != hipSuccess)                     \
+        return bail(IBPM_E_HIP, std::string("hipMalloc failed for " #ptr));
+    DA(P->q, nq * mg); DA(P->q0, nq * mg); DA(P->G, nG * mg); DA(P->Gs, nG * mg); DA(P->psi, nG * mg);
+    DA(P->nl[0], nG * mg); DA(P->nl[1], nG * mg);
+    DA(P->qs1, nq); DA(P->T1, nG * mg); DA(P->T2, nG * mg); DA(P->Gtmp, nG); DA(P->qreg, nq);
+    DA(P->Fb, P->nf); DA(P->Ftmp, P->nf); DA(P->fbv, P->nf); DA(P->CD, P->nbodies); DA(P->CL, P->nbodies);
+    DA(P->scal, 16);
+    DA(P->BinvT, (size_t)P->nf * P->nf); DA(P->w, P->weight.size());
+    DA(P->ds_d, nbt); DA(P->bidx, 2 * nbt); DA(P->boff_d, P->boff.size());
+    DA(P->sc_cell, cell.size()); DA(P->sc_off, off.size()); DA(P->sc_f, ef.size()); DA(P->sc_w, ew.size());
+    DA(P->cx, nx - 1); DA(P->cy, ny - 1);
+    DA(P->tx.tw, twx.size()); DA(P->tx.sn, snx.size()); DA(P->tx.wm, wmx.size());
+    DA(P->ty.tw, twy.size()); DA(P->ty.sn, sny.size()); DA(P->ty.wm, wmy.size());
+#undef DA
+    auto up = [&](void* d, const void* h_, size_t bytes) -> bool {
+        return bytes == 0 || hipMemcpy(d, h_, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    bool ok = up(P->w, P->weight.data(), sizeof(double) * P->weight.size()) &&
+              up(P->ds_d, P->ds.data(), sizeof(double) * P->ds.size()) &&
+              up(P->bidx, P->body_idx.data(), sizeof(int) * P->body_idx.size()) &&
+              up(P->boff_d, P->boff.data(), sizeof(int) * P->boff.size()) &&
+              up(P->sc_cell, cell.data(), sizeof(int) * cell.size()) && up(P->sc_off, off.data(), sizeof(int) * off.size()) &&
+              up(P->sc_f, ef.data(), sizeof(int) * ef.size()) && up(P->sc_w, ew.data(), sizeof(double) * ew.size()) &&
+              up(P->cx, cxh.data(), sizeof(double) * cxh.size()) && up(P->cy, cyh.data(), sizeof(double) * cyh.size()) &&
+              up(P->tx.tw, twx.data(), sizeof(double2) * twx.size()) && up(P->tx.sn, snx.data(), sizeof(double) * snx.size()) &&
+              up(P->tx.wm, wmx.data(), sizeof(double2) * wmx.size()) && up(P->ty.tw, twy.data(), sizeof(double2) * twy.size()) &&
+              up(P->ty.sn, sny.data(), sizeof(double) * sny.size()) && up(P->ty.wm, wmy.data(), sizeof(double2) * wmy.size());
+    if (!ok) return bail(IBPM_E_HIP, "upload of setup tables failed");
+    P->tx.plan.tw = P->tx.tw; P->tx.plan.sn = P->tx.sn; P->tx.plan.wm = P->tx.wm;
+    P->ty.plan.tw = P->ty.tw; P->ty.plan.sn = P->ty.sn; P->ty.plan.wm = P->ty.wm;
+    // scratch fields whose untouched entries must stay zero
+    launch_fill(P->qreg, nq, 0.0, P->stream);
+    launch_fill(P->Gtmp, nG, 0.0, P->stream);
+    zero_state(P);
+    // ---- Binv
+    if (P->nf > 0) {
+        if (binv) {
+            P->binv.assign(binv, binv + (size_t)P->nf * P->nf);
+        } else {
+            int rc = build_binv(P);
+            if (rc != IBPM_OK) { std::string m = ctx->err; ibpm_problem_destroy(P); return fail(ctx, rc, m); }
+        }
+        int rc = upload_binv(P);
+        if (rc != IBPM_OK) { std::string m = ctx->err; ibpm_problem_destroy(P); return fail(ctx, rc, m); }
+    }
+    zero_state(P);
+    if (hipStreamSynchronize(P->stream) != hipSuccess) return bail(IBPM_E_HIP, "setup failed on the device");
+    P->setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    *out = P;
+    return IBPM_OK;
+}
+
+int ibpm_problem_destroy(ibpm_prob* P)
+{
+    if (!P) return IBPM_OK;
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    for (void* p : P->allocs) (void)hipFree(p);
+    if (P->hist) (void)hipFree(P->hist);
+    if (P->stream) (void)hipStreamDestroy(P->stream);
+    delete P;
+    return IBPM_OK;
+}
+
+int ibpm_problem_get_info(const ibpm_prob* P, ibpm_problem_info* info)
+{
+    if (!P || !info) return IBPM_E_ARG;
+    info->nx = P->g.nx; info->ny = P->g.ny; info->mg = P->mg; info->nbodies = P->nbodies; info->nb_total = P->nbt;
+    info->nbc = P->g.nbc; info->nG = P->g.nG; info->nq = P->g.nq; info->h = P->g.h; info->offx = P->offx;
+    info->offy = P->offy; info->b_asymmetry = P->b_asym; info->setup_seconds = P->setup_s;
+    return IBPM_OK;
+}
+
+int ibpm_problem_get_binv(ibpm_prob* P, double* out, size_t n)
+{
+    if (!P || !out || n != (size_t)P->nf * P->nf) return fail(P ? P->ctx : nullptr, IBPM_E_ARG, "binv size");
+    std::copy(P->binv.begin(), P->binv.end(), out);
+    return IBPM_OK;
+}
+
+int ibpm_problem_get_body_idx(ibpm_prob* P, int32_t* out, size_t n)
+{
+    if (!P || !out || n != 2 * (size_t)P->nbt) return fail(P ? P->ctx : nullptr, IBPM_E_ARG, "body_idx size");
+    std::copy(P->body_idx.begin(), P->body_idx.end(), out);
+    return IBPM_OK;
+}
+
+static double* field_ptr(ibpm_prob* P, int field, size_t* n, bool for_upload)
+{
+    const size_t nq = P->g.nq * P->mg, nG = P->g.nG * P->mg;
+    switch (field) {
+    case IBPM_FIELD_Q: *n = nq; return P->q;
+    case IBPM_FIELD_Q0: *n = nq; return P->q0;
+    case IBPM_FIELD_GAMMA: *n = nG; return P->G;
+    case IBPM_FIELD_PSI: *n = nG; return P->psi;
+    case IBPM_FIELD_NONLIN1:
+        *n = nG;
+        if (for_upload && P->nl1 == P->nl2) P->nl1 = 1 - P->nl2;
+        return P->nl[P->nl1];
+    case IBPM_FIELD_NONLIN2:
+        *n = nG;
+        if (for_upload && P->nl1 == P->nl2) P->nl2 = 1 - P->nl1;
+        return P->nl[P->nl2];
+    case IBPM_FIELD_FB_TILDE: *n = P->nf; return P->Fb;
+    case IBPM_FIELD_FB: *n = P->nf; return P->fbv;
+    case IBPM_FIELD_CD: *n = P->nbodies; return P->CD;
+    case IBPM_FIELD_CL: *n = P->nbodies; return P->CL;
+    default: return nullptr;
+    }
+}
+
+int ibpm_state_upload(ibpm_prob* P, int field, const double* host, size_t n)
+{
+    if (!P || !host) return IBPM_E_ARG;
+    size_t want = 0;
+    double* d = field_ptr(P, field, &want, true);
+    if (!d) return fail(P->ctx, IBPM_E_ARG, "unknown field");
+    if (n != want) return fail(P->ctx, IBPM_E_ARG, "upload size mismatch");
+    HIPTRY(P->ctx, hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, P->stream));
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    return IBPM_OK;
+}
+
+int ibpm_state_download(ibpm_prob* P, int field, double* host, size_t n)
+{
+    if (!P || !host) return IBPM_E_ARG;
+    size_t want = 0;
+    double* d = field_ptr(P, field, &want, false);
+    if (!d) return fail(P->ctx, IBPM_E_ARG, "unknown field");
+    if (n != want) return fail(P->ctx, IBPM_E_ARG, "download size mismatch");
+    HIPTRY(P->ctx, hipMemcpyAsync(host, d, sizeof(double) * n, hipMemcpyDeviceToHost, P->stream));
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    return IBPM_OK;
+}
+
+int ibpm_base_flux(ibpm_prob* P, const ibpm_step_scalars* s)
+{
+    if (!P || !s) return IBPM_E_ARG;
+    if (P->motion == IBPM_MOTION_MOVING_GRID) {
+        double Om = -s->theta_dot, al = -s->theta;
+        double Ux0 = s->U * std::cos(al) - s->V * std::sin(al);
+        double Uy0 = s->U * std::sin(al) + s->V * std::cos(al);
+        Ux0 += s->Ux * std::cos(s->inclination) - s->Uy * std::sin(s->inclination);
+        Uy0 += s->Ux * std::sin(s->inclination) + s->Uy * std::cos(s->inclination);
+        BaseFluxArgs b{};
+        b.q0 = P->q0; b.mg = P->mg; b.Ux0 = Ux0; b.Uy0 = Uy0; b.Om = Om; b.offx = P->offx; b.offy = P->offy; b.g = P->g;
+        launch_base_flux_moving(b, P->stream);
+    } else {
+        double a = s->inclination;
+        double vx = s->Ux * std::cos(a) - s->Uy * std::sin(a);
+        double vy = s->Ux * std::sin(a) + s->Uy * std::cos(a);
+        launch_base_flux_inertial(P->q0, P->mg, vx, vy, P->g, P->stream);
+    }
+    HIPTRY(P->ctx, hipGetLastError());
+    return IBPM_OK;
+}
+
+int ibpm_state_init(ibpm_prob* P, const ibpm_step_scalars* s)
+{
+    if (!P || !s) return IBPM_E_ARG;
+    zero_state(P);
+    return ibpm_base_flux(P, s);
+}
+
+int ibpm_vort2flux(ibpm_prob* P)
+{
+    if (!P) return IBPM_E_ARG;
+    vort2flux_dev(P, P->G, P->psi, P->q, false);
+    HIPTRY(P->ctx, hipGetLastError());
+    return IBPM_OK;
+}
+
+int ibpm_advance(ibpm_prob* P, double t, const ibpm_step_scalars* s)
+{
+    (void)t;
+    if (!P || !s) return IBPM_E_ARG;
+    step_dev(P, *s, nullptr, nullptr);
+    HIPTRY(P->ctx, hipGetLastError());
+    return IBPM_OK;
+}
+
+int ibpm_advance_n(ibpm_prob* P, int nsteps, const double* t, const ibpm_step_scalars* s, double* cd_hist,
+                   double* cl_hist, int* steps_done)
+{
+    (void)t;
+    if (!P || !s || nsteps < 0) return IBPM_E_ARG;
+    if (steps_done) *steps_done = 0;
+    const size_t need = 2 * (size_t)nsteps * (size_t)std::max(P->nbodies, 1);
+    if (need > P->hist_cap) {
+        if (P->hist) (void)hipFree(P->hist);
+        P->hist = nullptr;
+        P->hist_cap = 0;
+        HIPTRY(P->ctx, hipMalloc(&P->hist, sizeof(double) * need));
+        P->hist_cap = need;
+    }
+    const int nb = std::max(P->nbodies, 1);
+    for (int k = 0; k < nsteps; k++)
+        step_dev(P, s[k], P->hist + (size_t)k * nb, P->hist + (size_t)nsteps * nb + (size_t)k * nb);
+    HIPTRY(P->ctx, hipGetLastError());
+    std::vector<double> hh(2 * (size_t)nsteps * nb);
+    HIPTRY(P->ctx, hipMemcpyAsync(hh.data(), P->hist, sizeof(double) * hh.size(), hipMemcpyDeviceToHost, P->stream));
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    int done = nsteps;
+    for (int k = 0; k < nsteps && done == nsteps; k++)
+        for (int b = 0; b < P->nbodies; b++)
+            if (!std::isfinite(hh[(size_t)nsteps * nb + (size_t)k * nb + b])) { done = k + 1; break; }
+    if (cd_hist && P->nbodies > 0) std::copy(hh.begin(), hh.begin() + (size_t)nsteps * nb, cd_hist);
+    if (cl_hist && P->nbodies > 0) std::copy(hh.begin() + (size_t)nsteps * nb, hh.end(), cl_hist);
+    if (steps_done) *steps_done = done;
+    return done < nsteps ? fail(P->ctx, IBPM_E_NONFINITE, "non-finite CL") : IBPM_OK;
+}
+
+int ibpm_get_forces(ibpm_prob* P, double* CD, double* CL, double* Fb)
+{
+    if (!P) return IBPM_E_ARG;
+    if (CD && P->nbodies) HIPTRY(P->ctx, hipMemcpyAsync(CD, P->CD, sizeof(double) * P->nbodies, hipMemcpyDeviceToHost, P->stream));
+    if (CL && P->nbodies) HIPTRY(P->ctx, hipMemcpyAsync(CL, P->CL, sizeof(double) * P->nbodies, hipMemcpyDeviceToHost, P->stream));
+    if (Fb && P->nf) HIPTRY(P->ctx, hipMemcpyAsync(Fb, P->Fb, sizeof(double) * P->nf, hipMemcpyDeviceToHost, P->stream));
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    return IBPM_OK;
+}
+
+int ibpm_compute_cfl(ibpm_prob* P, double* cfl)
+{
+    if (!P || !cfl) return IBPM_E_ARG;
+    launch_absmax(P->q, P->g.nq, P->scal, P->stream);
+    double m = 0.0;
+    HIPTRY(P->ctx, hipMemcpyAsync(&m, P->scal, sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    *cfl = m * P->dt / P->g.h;
+    return IBPM_OK;
+}
+
+// ---- operator-level entry points --------------------------------------------
+struct HostIO {
+    ibpm_prob* P;
+    std::vector<double*> tmp;
+    ~HostIO() { for (double* p : tmp) (void)hipFree(p); }
+    double* put(const double* h, size_t n)
+    {
+        double* d = nullptr;
+        if (hipMalloc(&d, sizeof(double) * (n ? n : 1)) != hipSuccess) return nullptr;
+        tmp.push_back(d);
+        if (h && n) (void)hipMemcpy(d, h, sizeof(double) * n, hipMemcpyHostToDevice);
+        else (void)hipMemset(d, 0, sizeof(double) * (n ? n : 1));
+        return d;
+    }
+};
+
+static int get_back(ibpm_prob* P, double* h, const double* d, size_t n)
+{
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    HIPTRY(P->ctx, hipMemcpy(h, d, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return IBPM_OK;
+}
+
+int ibpm_op_lapinv(ibpm_prob* P, int kind, int lev, const double* b, double* x)
+{
+    if (!P || !b || !x || kind < 0 || kind > 2 || lev < 1 || lev > P->mg) return IBPM_E_ARG;
+    HostIO io{P};
+    const size_t n = P->g.nG;
+    double* db = io.put(b, n);
+    double* dx = io.put(nullptr, n);
+    if (!db || !dx) return fail(P->ctx, IBPM_E_HIP, "alloc");
+    lap_level(P, kind == 0 ? Y_DINV : (kind == 1 ? Y_A : Y_AINV), lev - 1, db, dx, nullptr, false);
+    return get_back(P, x, dx, n);
+}
+
+int ibpm_op_dst2(ibpm_prob* P, const double* x, double* y)
+{
+    if (!P || !x || !y) return IBPM_E_ARG;
+    // 2D RODFT00 from two x-direction passes: transform along x, transpose on
+    // the host, transform the former y lines with the y plan, transpose back.
+    HostIO io{P};
+    const int n1 = P->g.n1, n2 = P->g.n2;
+    const size_t n = P->g.nG;
+    double* dx = io.put(x, n);
+    double* dy = io.put(nullptr, n);
+    if (!dx || !dy) return fail(P->ctx, IBPM_E_HIP, "alloc");
+    XArgs a = xargs(P);
+    a.src = dx; a.dst = dy; a.nlev = 1; a.lev0 = 0;
+    launch_xpass(X_PLAIN, a, P->stream);
+    std::vector<double> hx(n), ht(n);
+    int rc = get_back(P, hx.data(), dy, n);
+    if (rc) return rc;
+    for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++) ht[(size_t)j + (size_t)n2 * i] = hx[(size_t)i + (size_t)n1 * j];
+    // transform along the former y (now contiguous) with the y plan
+    GridDev gt = P->g;
+    std::swap(gt.n1, gt.n2);
+    XArgs b = xargs(P);
+    b.g = gt;
+    b.pl = P->ty.plan;
+    double* dt2 = io.put(ht.data(), n);
+    double* dt3 = io.put(nullptr, n);
+    b.src = dt2; b.dst = dt3; b.nlev = 1; b.lev0 = 0;
+    launch_xpass(X_PLAIN, b, P->stream);
+    rc = get_back(P, ht.data(), dt3, n);
+    if (rc) return rc;
+    for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++) y[(size_t)i + (size_t)n1 * j] = ht[(size_t)j + (size_t)n2 * i];
+    return IBPM_OK;
+}
+
+int ibpm_op_nonlinear(ibpm_prob* P, int lev, const double* q, const double* q0, const double* G, double* out)
+{
+    if (!P || !q || !q0 || !G || !out || lev < 1 || lev > P->mg) return IBPM_E_ARG;
+    HostIO io{P};
+    const size_t nq = P->g.nq * P->mg, nG = P->g.nG * P->mg;
+    double *dq = io.put(q, nq), *dq0 = io.put(q0, nq), *dG = io.put(G, nG), *dout = io.put(nullptr, P->g.nG);
+    if (!dq || !dq0 || !dG || !dout) return fail(P->ctx, IBPM_E_HIP, "alloc");
+    NLArgs na{};
+    na.q = dq; na.q0 = dq0; na.G = dG; na.out = dout; na.nlev = 1; na.lev0 = lev - 1; na.mg = P->mg; na.g = P->g;
+    for (int l = 0; l < P->mg; l++) {
+        double hc = P->g.h * (double)(1L << l);
+        na.s_lev[l] = 1.0 / (4.0 * (hc * hc));
+    }
+    launch_nonlinear(na, P->stream);
+    return get_back(P, out, dout, P->g.nG);
+}
+
+int ibpm_op_vort2flux(ibpm_prob* P, double* G, double* psi, double* q)
+{
+    if (!P || !G || !psi || !q) return IBPM_E_ARG;
+    HostIO io{P};
+    const size_t nq = P->g.nq * P->mg, nG = P->g.nG * P->mg;
+    double *dG = io.put(G, nG), *dpsi = io.put(nullptr, nG), *dq = io.put(nullptr, nq);
+    if (!dG || !dpsi || !dq) return fail(P->ctx, IBPM_E_HIP, "alloc");
+    vort2flux_dev(P, dG, dpsi, dq, false);
+    int rc = get_back(P, G, dG, nG);
+    if (!rc) rc = get_back(P, psi, dpsi, nG);
+    if (!rc) rc = get_back(P, q, dq, nq);
+    return rc;
+}
+
+int ibpm_op_regT(ibpm_prob* P, const double* q, double* fb)
+{
+    if (!P || !q || !fb) return IBPM_E_ARG;
+    HostIO io{P};
+    double *dq = io.put(q, P->g.nq), *dfb = io.put(nullptr, P->nf);
+    if (!dq || !dfb) return fail(P->ctx, IBPM_E_HIP, "alloc");
+    GatherArgs ga{};
+    ga.q = dq; ga.q0 = nullptr; ga.w = P->w; ga.bidx = P->bidx; ga.out = dfb; ga.nbt = P->nbt; ga.g = P->g;
+    launch_gather(ga, P->stream);
+    return get_back(P, fb, dfb, P->nf);
+}
+
+int ibpm_op_reg(ibpm_prob* P, const double* fb, double* q)
+{
+    if (!P || !fb || !q) return IBPM_E_ARG;
+    HostIO io{P};
+    double *dfb = io.put(fb, P->nf), *dq = io.put(nullptr, P->g.nq);
+    if (!dq || !dfb) return fail(P->ctx, IBPM_E_HIP, "alloc");
+    ScatterArgs sa{};
+    sa.cell = P->sc_cell; sa.off = P->sc_off; sa.ent_f = P->sc_f; sa.ent_w = P->sc_w;
+    sa.fb = dfb; sa.q = dq; sa.ncell = P->sc_ncell;
+    launch_scatter(sa, P->stream);
+    return get_back(P, q, dq, P->g.nq);
+}
+
+int ibpm_op_rot(ibpm_prob* P, const double* q, double* G)
+{
+    if (!P || !q || !G) return IBPM_E_ARG;
+    HostIO io{P};
+    double *dq = io.put(q, P->g.nq), *dG = io.put(nullptr, P->g.nG);
+    if (!dq || !dG) return fail(P->ctx, IBPM_E_HIP, "alloc");
+    launch_rot(dq, dG, 2, P->g.nx, 2, P->g.ny, P->g, P->stream);
+    return get_back(P, G, dG, P->g.nG);
+}
+
+int ibpm_time_kernel(ibpm_prob* P, int which, int reps, double* ms_avg)
+{
+    if (!P || !ms_avg || reps < 1) return IBPM_E_ARG;
+    hipEvent_t e0, e1;
+    HIPTRY(P->ctx, hipEventCreate(&e0));
+    HIPTRY(P->ctx, hipEventCreate(&e1));
+    const int L = P->mg;
+    auto one = [&]() {
+        if (which == 0) {
+            XArgs a = xargs(P);
+            a.src = P->G; a.dst = P->T1; a.nlev = L; a.lev0 = 0;
+            launch_xpass(X_PLAIN, a, P->stream);
+        } else if (which == 1) {
+            launch_ypass(yargs(P, Y_A, 0, L, P->T1), P->ytile, P->stream);
+        } else if (which == 2) {
+            NLArgs na{};
+            na.q = P->q; na.q0 = P->q0; na.G = P->G; na.out = P->T2; na.nlev = L; na.lev0 = 0; na.mg = L; na.g = P->g;
+            for (int l = 0; l < L; l++) {
+                double hc = P->g.h * (double)(1L << l);
+                na.s_lev[l] = 1.0 / (4.0 * (hc * hc));
+            }
+            launch_nonlinear(na, P->stream);
+        }
+    };
+    if (which == 3) {
+        // full steps are timed by the caller through ibpm_advance_n
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return fail(P->ctx, IBPM_E_ARG, "use ibpm_advance_n to time whole steps");
+    }
+    one();
+    HIPTRY(P->ctx, hipEventRecord(e0, P->stream));
+    for (int r = 0; r < reps; r++) one();
+    HIPTRY(P->ctx, hipEventRecord(e1, P->stream));
+    HIPTRY(P->ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPTRY(P->ctx, hipEventElapsedTime(&ms, e0, e1));
+    *ms_avg = (double)ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return IBPM_OK;
+}
+
+}  // extern "C"

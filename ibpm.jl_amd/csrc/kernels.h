@@ -1,0 +1,137 @@
+// kernels.h -- argument blocks and launchers of the HIP kernels (kernels.hip).
+#pragma once
+#include "ibpm_dev.h"
+
+namespace ibpm {
+
+constexpr int MAXLEV = 16;
+
+// ---- DST passes -----------------------------------------------------------
+enum XMode { X_PLAIN = 0, X_BC = 1, X_SUB = 2, X_TRIAL = 3 };
+
+struct XArgs {
+    const double* src;
+    double* dst;
+    long lstride;         // element stride between batch entries (levels)
+    int nlev;             // batch size
+    int lev0;             // 0-based grid level of batch entry 0
+    int mg;
+    // X_BC: apply_bc!(line, get_bc!(bcr), fac) prologue (one level)
+    const double* bcr;
+    double fac;
+    // X_TRIAL: rhs = ((A Γ + c1 N1) + c2 N0) + rhsbc, then a second x-DST
+    const double* n1;
+    const double* n0;
+    const double* gam;    // state Γ (all levels) for rhsbc of level lev from lev+1
+    double c1, c2;
+    double fac_lev[MAXLEV];  // 0.25 dt / (Re hc^2)
+    DstPlan pl;
+    GridDev g;
+};
+void launch_xpass(int mode, const XArgs& a, hipStream_t s);
+
+enum YOp { Y_DINV = 0, Y_A = 1, Y_AINV = 2 };
+struct YArgs {
+    double* buf;
+    long lstride;
+    int nlev, lev0;
+    int op;
+    const double* cx;   // cos(pi i / nx), i = 1..nx-1
+    const double* cy;   // cos(pi j / ny), j = 1..ny-1
+    double dt;
+    double cdiv[MAXLEV];  // (2 Re) hc^2 per level
+    double scale;         // 1 / (4 nx ny)
+    DstPlan pl;
+    GridDev g;
+};
+void launch_ypass(const YArgs& a, int tile, hipStream_t s);
+
+// ---- field kernels ---------------------------------------------------------
+struct NLArgs {
+    const double* q;
+    const double* q0;
+    const double* G;
+    double* out;
+    int nlev, lev0, mg;
+    double s_lev[MAXLEV];  // 1 / (4 hc^2)
+    GridDev g;
+};
+void launch_nonlinear(const NLArgs& a, hipStream_t s);
+
+struct CurlArgs {
+    const double* psi;  // all levels (for the bc of level lev from lev+1)
+    double* q;          // output: entry b is level lev0 + b
+    long qstride;
+    int nlev, lev0, mg;
+    GridDev g;
+};
+void launch_curl(const CurlArgs& a, hipStream_t s);
+
+struct CoarsifyArgs {
+    double* Gc;
+    const double* G;
+    int i0, ic0, ci, j0, jc0, cj;
+    GridDev g;
+};
+void launch_coarsify(const CoarsifyArgs& a, hipStream_t s);
+
+// rot! over the rectangle [ib0, ib1] x [jb0, jb1] of 1-based Γ indices (i, j
+// here are the flux-grid indices 2..nx, 2..ny of rot!, curl.jl:26)
+void launch_rot(const double* q, double* G, int ib0, int ib1, int jb0, int jb1, const GridDev& g, hipStream_t s);
+
+struct GatherArgs {
+    const double* q;
+    const double* q0;  // may be null
+    const double* w;   // [(2k + c) * 169 + l + 13 m]
+    const int* bidx;   // [k] = bx, [nbt + k] = by (1-based)
+    double* out;       // 2 nbt
+    int nbt;
+    GridDev g;
+};
+void launch_gather(const GatherArgs& a, hipStream_t s);
+
+struct ScatterArgs {
+    const int* cell;   // flux index of each touched cell
+    const int* off;    // CSR offsets (ncell + 1)
+    const int* ent_f;  // F̃b index of each contribution
+    const double* ent_w;
+    const double* fb;
+    double* q;
+    int ncell;
+};
+void launch_scatter(const ScatterArgs& a, hipStream_t s);
+
+void launch_gemv_rows(const double* AT, const double* x, double* y, int n, hipStream_t s);
+
+struct ForceArgs {
+    const double* Fb;
+    const double* ds;
+    const int* boff;  // body offsets (nbodies + 1)
+    double* fb;
+    double* CD;
+    double* CL;
+    double* cd_hist;  // may be null
+    double* cl_hist;
+    int nbodies, nbt;
+    double hdt;       // h / dt
+};
+void launch_forces(const ForceArgs& a, hipStream_t s);
+
+struct BaseFluxArgs {
+    double* q0;
+    int mg;
+    double Ux0, Uy0, Om;   // translation in the body frame and Ω = -θ̇
+    double offx, offy;
+    GridDev g;
+};
+void launch_base_flux_moving(const BaseFluxArgs& a, hipStream_t s);
+void launch_base_flux_inertial(double* q0, int mg, double vx, double vy, const GridDev& g, hipStream_t s);
+
+void launch_fill(double* p, long n, double v, hipStream_t s);
+void launch_copy(double* dst, const double* src, long n, hipStream_t s);
+void launch_set_unit(double* p, int n, int j, hipStream_t s);
+void launch_absmax(const double* p, long n, double* out, hipStream_t s);  // out: single double (>= 0)
+
+void kernels_init();  // sets large-LDS attributes
+
+}  // namespace ibpm

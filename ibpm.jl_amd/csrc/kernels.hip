@@ -1,0 +1,545 @@
+// kernels.hip -- HIP kernels of the advance! step for gfx950.
+//
+// All arithmetic is FP64.  Element-wise expressions that the reference fixes
+// (stencils, eigenvalue scaling, right-hand-side assembly) are written in the
+// reference's order of operations; the library is compiled with
+// -ffp-contract=off so each a*b+c stays two rounded operations as in Julia.
+// Transform internals (FFT butterflies) use explicit fma().
+#include "kernels.h"
+
+namespace ibpm {
+
+// ===========================================================================
+// DST x-pass: one wavefront per x-line (fixed j) of one level.
+// ===========================================================================
+constexpr int XWAVES = 4;  // wavefronts per workgroup
+
+// apply_bc! term of element (i, j) (1-based), mg-utils.jl:43-60: rows first
+// (j = 1 then j = ny-1), then columns (i = 1 then i = nx-1).
+__device__ __forceinline__ double add_bc(double v, const double* __restrict__ r, const GridDev& g, double fac, int i, int j)
+{
+    if (j == 1) v += fac * get_bc_val(r, g, (g.BOT + 1) + i);
+    if (j == g.ny - 1) v += fac * get_bc_val(r, g, (g.TOP + 1) + i);
+    if (i == 1) v += fac * get_bc_val(r, g, (g.LEFT + 1) + j);
+    if (i == g.nx - 1) v += fac * get_bc_val(r, g, (g.RIGHT + 1) + j);
+    return v;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
+{
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int w = threadIdx.x / WAVE;
+    const int n1 = a.g.n1, n2 = a.g.n2;
+    const int line = blockIdx.x * XWAVES + w;  // 0-based j - 1
+    const int b = blockIdx.y;
+    if (line >= n2) return;
+    const int ld = a.pl.M + 2;
+    double* s = lds + w * ld;
+    const int j = line + 1;
+    const double* src = a.src + b * a.lstride + (long)line * n1;
+    for (int n = lane; n < n1; n += WAVE) {
+        double v = src[n];
+        if constexpr (MODE == X_BC) v = add_bc(v, a.bcr, a.g, a.fac, n + 1, j);
+        s[n] = v;
+    }
+    wave_sync();
+    dst1_line(s, a.pl, lane);
+    if constexpr (MODE == X_TRIAL) {
+        // get_trial_state! (timestepping.jl:111-118): rhs = A Γ; rhs += (β1 dt) N1;
+        // rhs += (β2 dt) N0; rhs += rhsbc, with rhsbc = apply_bc!(0, get_bc!(Γ[:, lev+1]), fac)
+        const int lev = a.lev0 + b;
+        const double* n1p = a.n1 + b * a.lstride + (long)line * n1;
+        const double* n0p = a.n0 + b * a.lstride + (long)line * n1;
+        const bool hasbc = lev < a.mg - 1;
+        const double* gc = a.gam + (long)(lev + 1) * a.lstride;
+        const double fac = a.fac_lev[lev];
+        for (int n = lane; n < n1; n += WAVE) {
+            double rhs = s[n];
+            rhs += a.c1 * n1p[n];
+            rhs += a.c2 * n0p[n];
+            if (hasbc) {
+                const int i = n + 1;
+                if (j == 1 || j == a.g.ny - 1 || i == 1 || i == a.g.nx - 1) {
+                    double rb = 0.0;
+                    rb = add_bc(rb, gc, a.g, fac, i, j);
+                    rhs += rb;
+                }
+            }
+            s[n] = rhs;
+        }
+        wave_sync();
+        dst1_line(s, a.pl, lane);
+    }
+    double* dst = a.dst + b * a.lstride + (long)line * n1;
+    for (int n = lane; n < n1; n += WAVE) {
+        if constexpr (MODE == X_SUB) dst[n] = dst[n] - s[n];
+        else dst[n] = s[n];
+    }
+}
+
+void launch_xpass(int mode, const XArgs& a, hipStream_t s)
+{
+    dim3 grid((a.g.n2 + XWAVES - 1) / XWAVES, a.nlev);
+    size_t lds = sizeof(double) * (size_t)XWAVES * (a.pl.M + 2);
+    switch (mode) {
+    case X_PLAIN: hipLaunchKernelGGL(k_xpass<X_PLAIN>, grid, dim3(XWAVES * WAVE), lds, s, a); break;
+    case X_BC: hipLaunchKernelGGL(k_xpass<X_BC>, grid, dim3(XWAVES * WAVE), lds, s, a); break;
+    case X_SUB: hipLaunchKernelGGL(k_xpass<X_SUB>, grid, dim3(XWAVES * WAVE), lds, s, a); break;
+    default: hipLaunchKernelGGL(k_xpass<X_TRIAL>, grid, dim3(XWAVES * WAVE), lds, s, a); break;
+    }
+}
+
+// ===========================================================================
+// DST y-pass, fused: forward DST along y, eigenvalue scaling, DST along y.
+// A workgroup stages TI adjacent columns (all rows) of one level in LDS.
+// ===========================================================================
+constexpr int YWAVES = 4;
+
+template <int TI>
+__global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
+{
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int w = tid / WAVE;
+    const int n1 = a.g.n1, n2 = a.g.n2;
+    const int b = blockIdx.y;
+    const int i0 = blockIdx.x * TI;
+    const int ld = a.pl.M + 2;
+    double* base = a.buf + b * a.lstride;
+    for (int idx = tid; idx < TI * n2; idx += YWAVES * WAVE) {
+        int r = idx / TI, c = idx % TI;
+        int i = i0 + c;
+        lds[c * ld + r] = (i < n1) ? base[i + (long)n1 * r] : 0.0;
+    }
+    __syncthreads();
+    const int lev = a.lev0 + b;
+    const double cdiv = a.cdiv[lev];
+    for (int c = w; c < TI; c += YWAVES) {
+        if (i0 + c >= n1) continue;
+        double* s = lds + c * ld;
+        dst1_line(s, a.pl, lane);
+        // dst_inv!: w ./= Λ̃; w .*= scale (dst-inversion.jl:46-48) with
+        // Λ = -2 (cos(pi i/nx) + cos(pi j/ny) - 2) (laplacian.jl:17) and
+        // Λ̃ per operator (laplacian.jl:69, 74-75)
+        const double cxk = a.cx[i0 + c];
+        for (int kj = lane; kj < n2; kj += WAVE) {
+            double lam = -2.0 * ((cxk + a.cy[kj]) - 2.0);
+            double d;
+            if (a.op == Y_DINV) d = lam;
+            else if (a.op == Y_A) d = 1.0 / (1.0 - (lam * a.dt) / cdiv);
+            else d = 1.0 + (lam * a.dt) / cdiv;
+            s[kj] = (s[kj] / d) * a.scale;
+        }
+        wave_sync();
+        dst1_line(s, a.pl, lane);
+    }
+    __syncthreads();
+    for (int idx = tid; idx < TI * n2; idx += YWAVES * WAVE) {
+        int r = idx / TI, c = idx % TI;
+        int i = i0 + c;
+        if (i < n1) base[i + (long)n1 * r] = lds[c * ld + r];
+    }
+}
+
+void launch_ypass(const YArgs& a, int tile, hipStream_t s)
+{
+    size_t ldsb = sizeof(double) * (size_t)tile * (a.pl.M + 2);
+    dim3 grid((a.g.n1 + tile - 1) / tile, a.nlev);
+    switch (tile) {
+    case 4: hipLaunchKernelGGL(k_ypass<4>, grid, dim3(YWAVES * WAVE), ldsb, s, a); break;
+    case 16: hipLaunchKernelGGL(k_ypass<16>, grid, dim3(YWAVES * WAVE), ldsb, s, a); break;
+    default: hipLaunchKernelGGL(k_ypass<8>, grid, dim3(YWAVES * WAVE), ldsb, s, a); break;
+    }
+}
+
+// ===========================================================================
+// nonlinear! (nonlinear.jl:168-201), fused: avg_flux (:17-38),
+// direct_product_loops! (:73-114), rot! (curl.jl:22-34) and the 1/(4 hc^2)
+// scaling.  Boundary circulation is get_bc!(Γ[:, lev+1]) .* 0.25.
+// One thread per output Γ point.
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
+{
+    const GridDev& g = a.g;
+    const int nx = g.nx, ny = g.ny, n1 = g.n1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;  // 2..nx
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;  // 2..ny
+    const int b = blockIdx.z;
+    if (i > nx || j > ny) return;
+    const int lev = a.lev0 + b;
+    const long nq = g.nq, nu = g.nu;
+    const double* q = a.q + (long)lev * nq;
+    const double* q0 = a.q0 + (long)lev * nq;
+    const double* G = a.G + (long)lev * g.nG;
+    const bool hasbc = lev < a.mg - 1;
+    const double* Gc = a.G + (long)(lev + 1) * g.nG;
+
+    auto GA = [&](int ii, int jj) -> double { return G[(ii - 1) + (long)n1 * (jj - 1)]; };
+    auto BCV = [&](int k) -> double { return hasbc ? get_bc_val(Gc, g, k) * 0.25 : 0.0; };
+    // Qx[ii, jj] (jj >= 2) and Qy[ii, jj] (ii >= 2) of avg_flux
+    auto QX = [&](int ii, int jj) -> double {
+        long p = (ii - 1) + (long)(nx + 1) * (jj - 1);
+        long pm = p - (nx + 1);
+        return ((q[p] + q0[p]) + q[pm]) + q0[pm];
+    };
+    auto QY = [&](int ii, int jj) -> double {
+        long p = nu + (ii - 1) + (long)nx * (jj - 1);
+        long pm = p - 1;
+        return ((q[p] + q0[p]) + q[pm]) + q0[pm];
+    };
+    // fqx[ii, jj] for ii = 2..nx, jj = 1..ny
+    auto FQX = [&](int ii, int jj) -> double {
+        if (jj == 1) return QY(ii, 2) * GA(ii - 1, 1) + QY(ii, 1) * BCV(g.BOT + ii);
+        if (jj == ny) return QY(ii, ny) * GA(ii - 1, ny - 1) + QY(ii, ny + 1) * BCV(g.TOP + ii);
+        return QY(ii, jj + 1) * GA(ii - 1, jj) + QY(ii, jj) * GA(ii - 1, jj - 1);
+    };
+    // fqy[ii, jj] for ii = 1..nx, jj = 2..ny
+    auto FQY = [&](int ii, int jj) -> double {
+        double v;
+        if (ii == 1) v = QX(2, jj) * GA(1, jj - 1) + QX(1, jj) * BCV(g.LEFT + jj);
+        else if (ii == nx) v = QX(nx, jj) * GA(nx - 1, jj - 1) + QX(nx + 1, jj) * BCV(g.RIGHT + jj);
+        else v = QX(ii + 1, jj) * GA(ii, jj - 1) + QX(ii, jj) * GA(ii - 1, jj - 1);
+        return v * -1.0;
+    };
+    double r = (FQX(i, j - 1) - FQX(i, j)) + (FQY(i, j) - FQY(i - 1, j));
+    a.out[(long)b * g.nG + (i - 2) + (long)n1 * (j - 2)] = r * a.s_lev[lev];
+}
+
+void launch_nonlinear(const NLArgs& a, hipStream_t s)
+{
+    dim3 blk(64, 4);
+    dim3 grid((a.g.nx - 1 + 63) / 64, (a.g.ny - 1 + 3) / 4, a.nlev);
+    hipLaunchKernelGGL(k_nonlinear, grid, blk, 0, s, a);
+}
+
+// ===========================================================================
+// curl! on the multigrid (curl.jl:81-116): q = C ψ with ψbc = get_bc!(ψ[:, lev+1])
+// (zero on the coarsest level).  One thread per flux point.
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_curl(CurlArgs a)
+{
+    const GridDev& g = a.g;
+    const int nx = g.nx, ny = g.ny, n1 = g.n1;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (p >= g.nq) return;
+    const int lev = a.lev0 + b;
+    const double* psi = a.psi + (long)lev * g.nG;
+    const bool hasbc = lev < a.mg - 1;
+    const double* pc = a.psi + (long)(lev + 1) * g.nG;
+    auto PS = [&](int ii, int jj) -> double { return psi[(ii - 1) + (long)n1 * (jj - 1)]; };
+    auto BC = [&](int k) -> double { return hasbc ? get_bc_val(pc, g, k) : 0.0; };
+    const int T = g.TOP, B = g.BOT, L = g.LEFT, R = g.RIGHT;
+    double v;
+    if (p < g.nu) {
+        const int i = (int)(p % (nx + 1)) + 1, j = (int)(p / (nx + 1)) + 1;
+        if (i == 1) v = BC((L + 1) + j) - BC(L + j);
+        else if (i == nx + 1) v = BC((R + 1) + j) - BC(R + j);
+        else if (j == 1) v = PS(i - 1, 1) - BC((B - 1) + i);
+        else if (j == ny) v = BC(i + T) - PS(i - 1, ny - 1);
+        else v = PS(i - 1, j) - PS(i - 1, j - 1);
+    } else {
+        const long pp = p - g.nu;
+        const int i = (int)(pp % nx) + 1, j = (int)(pp / nx) + 1;
+        if (j == 1) v = BC(B + i) - BC((B + 1) + i);
+        else if (j == ny + 1) v = BC(T + i) - BC((T + 1) + i);
+        else if (i == 1) v = BC(L + j) - PS(1, j - 1);
+        else if (i == nx) v = PS(nx - 1, j - 1) - BC(R + j);
+        else v = PS(i - 1, j - 1) - PS(i, j - 1);
+    }
+    a.q[(long)b * a.qstride + p] = v;
+}
+
+void launch_curl(const CurlArgs& a, hipStream_t s)
+{
+    dim3 grid((unsigned)((a.g.nq + 255) / 256), a.nlev);
+    hipLaunchKernelGGL(k_curl, grid, dim3(256), 0, s, a);
+}
+
+// ===========================================================================
+// coarsify! (mg-utils.jl:71-106): full-weighting restriction into the central
+// block of the next coarser level.  One thread per coarse point.
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_coarsify(CoarsifyArgs a)
+{
+    const int n1 = a.g.n1;
+    const int ai = blockIdx.x * blockDim.x + threadIdx.x;
+    const int bj = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ai >= a.ci || bj >= a.cj) return;
+    const int i = a.i0 + 2 * ai, ic = a.ic0 + ai;
+    const int j = a.j0 + 2 * bj, jc = a.jc0 + bj;
+    auto F = [&](int ii, int jj) -> double { return a.G[(ii - 1) + (long)n1 * (jj - 1)]; };
+    double s1 = ((F(i + 1, j) + F(i, j + 1)) + F(i - 1, j)) + F(i, j - 1);
+    double s2 = ((F(i + 1, j + 1) + F(i + 1, j - 1)) + F(i - 1, j - 1)) + F(i - 1, j + 1);
+    a.Gc[(ic - 1) + (long)n1 * (jc - 1)] = (F(i, j) + 0.5 * s1) + 0.25 * s2;
+}
+
+void launch_coarsify(const CoarsifyArgs& a, hipStream_t s)
+{
+    if (a.ci <= 0 || a.cj <= 0) return;
+    dim3 blk(64, 4);
+    dim3 grid((a.ci + 63) / 64, (a.cj + 3) / 4);
+    hipLaunchKernelGGL(k_coarsify, grid, blk, 0, s, a);
+}
+
+// ===========================================================================
+// rot! (curl.jl:22-34) over a rectangle of output points
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_rot(const double* __restrict__ q, double* __restrict__ G, int ib0, int ib1,
+                                             int jb0, int jb1, GridDev g)
+{
+    const int i = ib0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = jb0 + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > ib1 || j > jb1) return;
+    const int nx = g.nx;
+    auto QXv = [&](int ii, int jj) -> double { return q[(ii - 1) + (long)(nx + 1) * (jj - 1)]; };
+    auto QYv = [&](int ii, int jj) -> double { return q[g.nu + (ii - 1) + (long)nx * (jj - 1)]; };
+    double a = QXv(i, j - 1) - QXv(i, j);
+    double c = QYv(i, j) - QYv(i - 1, j);
+    G[(i - 2) + (long)g.n1 * (j - 2)] = a + c;
+}
+
+void launch_rot(const double* q, double* G, int ib0, int ib1, int jb0, int jb1, const GridDev& g, hipStream_t s)
+{
+    if (ib1 < ib0 || jb1 < jb0) return;
+    dim3 blk(64, 4);
+    dim3 grid((ib1 - ib0 + 1 + 63) / 64, (jb1 - jb0 + 1 + 3) / 4);
+    hipLaunchKernelGGL(k_rot, grid, blk, 0, s, q, G, ib0, ib1, jb0, jb1, g);
+}
+
+// ===========================================================================
+// E = regT! (interface-coupling.jl:122-133): one wavefront per (point, component)
+// over its 13 x 13 window; fb = sum(weight .* (q + q0)) (boundary_forces!
+// forms q + q0 first, timestepping.jl:158-159).
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_gather(GatherArgs a)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    if (wid >= 2 * a.nbt) return;
+    const int c = wid / a.nbt, k = wid % a.nbt;
+    const int bx = a.bidx[k], by = a.bidx[a.nbt + k];
+    const double* w = a.w + (long)(2 * k + c) * 169;
+    const int nx = a.g.nx;
+    double acc = 0.0;
+    for (int e = lane; e < 169; e += WAVE) {
+        const int l = e % 13, m = e / 13;
+        const int i = bx + l - 6, j = by + m - 6;
+        long p = (c == 0) ? (i - 1) + (long)(nx + 1) * (j - 1) : a.g.nu + (i - 1) + (long)nx * (j - 1);
+        double v = a.q0 ? (a.q[p] + a.q0[p]) : a.q[p];
+        acc += w[e] * v;
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) a.out[c * a.nbt + k] = acc;
+}
+
+void launch_gather(const GatherArgs& a, hipStream_t s)
+{
+    int waves = 2 * a.nbt;
+    if (waves == 0) return;
+    hipLaunchKernelGGL(k_gather, dim3((waves + 3) / 4), dim3(256), 0, s, a);
+}
+
+// ===========================================================================
+// E' = reg! (interface-coupling.jl:99-119) as an ordered segmented gather:
+// each touched flux cell sums its contributions in increasing body-point
+// order, which is the order of the reference's serial accumulation.
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_scatter(ScatterArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ncell) return;
+    double acc = 0.0;
+    for (int e = a.off[t]; e < a.off[t + 1]; e++) acc = acc + a.ent_w[e] * a.fb[a.ent_f[e]];
+    a.q[a.cell[t]] = acc;
+}
+
+void launch_scatter(const ScatterArgs& a, hipStream_t s)
+{
+    if (a.ncell == 0) return;
+    hipLaunchKernelGGL(k_scatter, dim3((a.ncell + 255) / 256), dim3(256), 0, s, a);
+}
+
+// ===========================================================================
+// y = A x with A given row-major (AT = transpose of the column-major matrix);
+// one wavefront per row.  (prob.Binv * F̃b, timestepping.jl:160)
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_gemv_rows(const double* __restrict__ AT, const double* __restrict__ x,
+                                                   double* __restrict__ y, int n)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int r = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    if (r >= n) return;
+    const double* row = AT + (long)r * n;
+    double acc = 0.0;
+    for (int c = lane; c < n; c += WAVE) acc = fma(row[c], x[c], acc);
+    acc = wave_sum(acc);
+    if (lane == 0) y[r] = acc;
+}
+
+void launch_gemv_rows(const double* AT, const double* x, double* y, int n, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_gemv_rows, dim3((n + 3) / 4), dim3(256), 0, s, AT, x, y, n);
+}
+
+// ===========================================================================
+// update_stress! (timestepping.jl:330-354): fb = F̃b (h/dt) ./ ds and
+// CD = 2 sum(ds .* fb[:,1]), CL = 2 sum(ds .* fb[:,2]) per body.  One block.
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_forces(ForceArgs a)
+{
+    __shared__ double red[2][4];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
+    for (int bd = 0; bd < a.nbodies; bd++) {
+        const int o = a.boff[bd], n = a.boff[bd + 1] - o;
+        double sx = 0.0, sy = 0.0;
+        for (int k = tid; k < n; k += blockDim.x) {
+            double ds = a.ds[o + k];
+            double fx = (a.Fb[o + k] * a.hdt) / ds;
+            double fy = (a.Fb[a.nbt + o + k] * a.hdt) / ds;
+            a.fb[2L * o + k] = fx;
+            a.fb[2L * o + n + k] = fy;
+            sx += ds * fx;
+            sy += ds * fy;
+        }
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        if (lane == 0) { red[0][w] = sx; red[1][w] = sy; }
+        __syncthreads();
+        if (tid == 0) {
+            double X = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+            double Y = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+            a.CD[bd] = 2.0 * X;
+            a.CL[bd] = 2.0 * Y;
+            if (a.cd_hist) { a.cd_hist[bd] = 2.0 * X; a.cl_hist[bd] = 2.0 * Y; }
+        }
+        __syncthreads();
+    }
+}
+
+void launch_forces(const ForceArgs& a, hipStream_t s)
+{
+    if (a.nbodies == 0) return;
+    hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, s, a);
+}
+
+// ===========================================================================
+// base_flux! for MovingGrid (timestepping.jl:273-318) with XX, YY of
+// models.jl:140-153 (xc = yc = 0) generated from indices.
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_base_flux_moving(BaseFluxArgs a)
+{
+    const GridDev& g = a.g;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lev = blockIdx.y;
+    if (p >= g.nq) return;
+    const double hc = g.h * (double)(1L << lev);
+    double v;
+    if (p < g.nu) {
+        const int j = (int)(p / (g.nx + 1)) + 1;
+        double y = ((((double)j - 0.5) - (double)g.ny / 2.0) * hc + ((double)g.ny / 2.0) * g.h) - a.offy;
+        double yy = y - 0.0;
+        v = yy * (-hc * a.Om);
+        v += hc * a.Ux0;
+    } else {
+        const long pp = p - g.nu;
+        const int i = (int)(pp % g.nx) + 1;
+        double x = ((((double)i - 0.5) - (double)g.nx / 2.0) * hc + ((double)g.nx / 2.0) * g.h) - a.offx;
+        double xx = x - 0.0;
+        v = xx * (hc * a.Om);
+        v += hc * a.Uy0;
+    }
+    a.q0[(long)lev * g.nq + p] = v;
+}
+
+void launch_base_flux_moving(const BaseFluxArgs& a, hipStream_t s)
+{
+    dim3 grid((unsigned)((a.g.nq + 255) / 256), a.mg);
+    hipLaunchKernelGGL(k_base_flux_moving, grid, dim3(256), 0, s, a);
+}
+
+// base_flux! for InertialMotion (timestepping.jl:252-270): uniform per level.
+__global__ void __launch_bounds__(256) k_base_flux_inertial(double* q0, double vx, double vy, GridDev g)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lev = blockIdx.y;
+    if (p >= g.nq) return;
+    const double hc = g.h * (double)(1L << lev);
+    q0[(long)lev * g.nq + p] = (p < g.nu) ? vx * hc : vy * hc;
+}
+
+void launch_base_flux_inertial(double* q0, int mg, double vx, double vy, const GridDev& g, hipStream_t s)
+{
+    dim3 grid((unsigned)((g.nq + 255) / 256), mg);
+    hipLaunchKernelGGL(k_base_flux_inertial, grid, dim3(256), 0, s, q0, vx, vy, g);
+}
+
+// ===========================================================================
+// small utilities
+// ===========================================================================
+__global__ void k_fill(double* p, long n, double v)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+void launch_fill(double* p, long n, double v, hipStream_t s)
+{
+    if (n <= 0) return;
+    long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
+__global__ void k_copy(double* d, const double* src, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) d[i] = src[i];
+}
+void launch_copy(double* dst, const double* src, long n, hipStream_t s)
+{
+    if (n <= 0) return;
+    long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
+}
+
+__global__ void k_set_unit(double* p, int n, int j)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = (i == j) ? 1.0 : 0.0;
+}
+void launch_set_unit(double* p, int n, int j, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_unit, dim3(1), dim3(256), 0, s, p, n, j);
+}
+
+__global__ void __launch_bounds__(256) k_absmax(const double* p, long n, double* out)
+{
+    __shared__ double red[4];
+    double m = 0.0;
+    for (long i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(p[i]));
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, WAVE));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+void launch_absmax(const double* p, long n, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_absmax, dim3(1), dim3(256), 0, s, p, n, out);
+}
+
+void kernels_init()
+{
+    const int big = 160 * 1024;
+    (void)hipFuncSetAttribute((const void*)k_xpass<X_PLAIN>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)k_xpass<X_BC>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)k_xpass<X_SUB>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)k_xpass<X_TRIAL>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)k_ypass<4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)k_ypass<8>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    (void)hipFuncSetAttribute((const void*)k_ypass<16>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+}
+
+}  // namespace ibpm

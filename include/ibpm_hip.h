@@ -1,0 +1,179 @@
+/*
+ * ibpm_hip.h -- C ABI of the MI355X (gfx950) implementation of IBPM.jl's
+ * `advance!(state, prob, t)` hot path.
+ *
+ * The reference has no FFI for this path: its boundary is the Julia method
+ *   advance!(state::IBState, prob::AbstractIBProblem, t::Float64)
+ * (src/timestepping/timestepping.jl:35-67), with state and problem built by
+ *   IBProblem(grid, bodies, dt, Re; freestream)  (src/pre-processing/problem-types.jl:188-200)
+ *   IBState(prob)                                (src/pre-processing/state-types.jl:270-293)
+ * The functions below are what a `ccall` layer binding that method would
+ * call.  Every array crossing this ABI uses the reference's exact memory
+ * layout (Julia column-major, one column per grid level):
+ *   q, q0           : nq x mg     nq = ny(nx+1) + nx(ny+1); x-fluxes first,
+ *                                  as (nx+1) x ny, then y-fluxes as nx x (ny+1)
+ *   Γ, ψ, nonlin[k] : nΓ x mg     nΓ = (nx-1)(ny-1)
+ *   F̃b              : 2 nb_total  [fx(1..nb_total); fy(1..nb_total)]
+ *   fb              : for each body j its (nb_j x 2) block, bodies in order
+ *   CD, CL          : one value per body
+ * (src/fluid-domain/fluid-domain.jl:181-210, src/pre-processing/state-types.jl:7-47).
+ *
+ * Conventions: every function returns 0 on success or a negative IBPM_E_*
+ * code; the message of the last failure on a context is returned by
+ * ibpm_last_error().  Host pointers are borrowed for the duration of the call
+ * only.  The library owns all device memory.  Calls on one problem must be
+ * serialised (the reference's `advance!` is likewise non-reentrant: it works
+ * in prob.model.work, src/pre-processing/models.jl:79-106); different problems
+ * may be driven from different host threads.
+ */
+#ifndef IBPM_HIP_H
+#define IBPM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ibpm_ctx ibpm_ctx;
+typedef struct ibpm_prob ibpm_prob;
+
+/* status codes */
+#define IBPM_OK 0
+#define IBPM_E_ARG -1        /* invalid argument / size mismatch */
+#define IBPM_E_HIP -2        /* HIP runtime failure */
+#define IBPM_E_UNSUPPORTED -3/* motion type or grid shape the library does not run */
+#define IBPM_E_SINGULAR -4   /* Schur matrix B singular during setup */
+#define IBPM_E_NONFINITE -5  /* non-finite body force after a step (ibpm.jl:95-97 test) */
+
+/* motion kinds: MotionType(bodies) (src/structure-domain/move-body-utils.jl:15-26) */
+#define IBPM_MOTION_STATIC 0
+#define IBPM_MOTION_MOVING_GRID 1
+#define IBPM_MOTION_FUNCTION 2   /* lab-frame moving body: rejected (IBPM_E_UNSUPPORTED) */
+
+/* state fields for upload/download (IBState, src/pre-processing/state-types.jl:7-20) */
+#define IBPM_FIELD_Q 0
+#define IBPM_FIELD_Q0 1
+#define IBPM_FIELD_GAMMA 2
+#define IBPM_FIELD_PSI 3
+#define IBPM_FIELD_NONLIN1 4
+#define IBPM_FIELD_NONLIN2 5
+#define IBPM_FIELD_FB_TILDE 6   /* F̃b */
+#define IBPM_FIELD_FB 7         /* fb, all bodies stacked */
+#define IBPM_FIELD_CD 8
+#define IBPM_FIELD_CL 9
+
+/* Grid + scheme description: make_grid(h, boundary; mg)
+ * (src/fluid-domain/fluid-domain.jl:181-210), Re and dt of IBProblem
+ * (problem-types.jl:188-192), AB2 coefficients (models.jl:70-72). */
+typedef struct {
+    int32_t nx, ny, mg;
+    double h;            /* grid.h = len/nx */
+    double offx, offy;   /* grid.offx, grid.offy */
+    double Re, dt;
+    double beta[2];      /* AdamsBashforth β; AB2 = {1.5, -0.5} */
+} ibpm_grid_desc;
+
+/* Bodies: vcat of RigidBody.xb / .ds (src/structure-domain/body-types.jl:10-16). */
+typedef struct {
+    int32_t nbodies;
+    const int32_t* nb;   /* points per body */
+    const double* xb;    /* nb_total x 2, column-major (all x, then all y) */
+    const double* ds;    /* nb_total */
+    int32_t motion;      /* IBPM_MOTION_* */
+} ibpm_body_desc;
+
+/* Closures of the problem evaluated by the caller at the step time t
+ * (freestream: src/pre-processing/problem-types.jl:192; MovingGrid U,V,θ,θ̇:
+ * src/structure-domain/motion-types.jl:42-50). */
+typedef struct {
+    double Ux, Uy, inclination;      /* freestream.Ux(t), .Uy(t), .inclination(t) */
+    double U, V, theta, theta_dot;   /* motion.U(t), .V(t), .θ(t), .θ̇(t) (MovingGrid only) */
+} ibpm_step_scalars;
+
+/* Sizes and setup products of a problem (for host mirrors and tests). */
+typedef struct {
+    int32_t nx, ny, mg, nbodies, nb_total, nbc;
+    int64_t nG, nq;
+    double h, offx, offy;
+    double b_asymmetry;      /* ||B - B'||_F / ||B||_F of the assembled Schur matrix (0 if Binv given) */
+    double setup_seconds;
+} ibpm_problem_info;
+
+/* Context on one device.  Replaces nothing in the reference (it is the HIP
+ * device binding the Julia module would hold). */
+int ibpm_ctx_create(int device, ibpm_ctx** out);
+int ibpm_ctx_destroy(ibpm_ctx* ctx);
+const char* ibpm_last_error(const ibpm_ctx* ctx);
+
+/* IBProblem(grid, bodies, dt, Re; freestream) -- problem-types.jl:188-200,
+ * including IBModel/IBMatrices (models.jl:33-52,124-158: Λ, the DST plans,
+ * E = setup_reg, interface-coupling.jl:72-96) and get_AB (laplacian.jl:83-89).
+ * binv: if non-NULL, the (2 nb_total)^2 column-major prob.Binv to use; if
+ * NULL, B is assembled on the GPU column by column exactly as get_Binv
+ * (laplacian.jl:98-119, B_times! :143-169) and inverted in FP64 on the host
+ * (LU with partial pivoting, as inv() does). */
+int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* grid, const ibpm_body_desc* bodies,
+                        const double* binv, ibpm_prob** out);
+int ibpm_problem_destroy(ibpm_prob* prob);
+int ibpm_problem_get_info(const ibpm_prob* prob, ibpm_problem_info* info);
+/* prob.Binv (column-major), body_idx (nb_total x 2, column-major, Julia
+ * 1-based values as computed by setup_reg, interface-coupling.jl:85-87). */
+int ibpm_problem_get_binv(ibpm_prob* prob, double* out, size_t n);
+int ibpm_problem_get_body_idx(ibpm_prob* prob, int32_t* out, size_t n);
+
+/* IBState fields <-> device (state-types.jl:7-20).  n = element count. */
+int ibpm_state_upload(ibpm_prob* prob, int field, const double* host, size_t n);
+int ibpm_state_download(ibpm_prob* prob, int field, double* host, size_t n);
+/* IBState(prob): zero every field, then base_flux!(state, prob, 0)
+ * (state-types.jl:270-293; timestepping.jl:252-270 / 273-318). */
+int ibpm_state_init(ibpm_prob* prob, const ibpm_step_scalars* at_t0);
+/* base_flux!(state, prob, t) (timestepping.jl:230-318). */
+int ibpm_base_flux(ibpm_prob* prob, const ibpm_step_scalars* at_t);
+/* vort2flux!(state.ψ, state.q, state.Γ, model, mg) on the device state
+ * (curl.jl:133-165): makes ψ and q consistent with an uploaded Γ. */
+int ibpm_vort2flux(ibpm_prob* prob);
+
+/* advance!(state, prob, t) -- timestepping.jl:35-67.  `s` holds the problem's
+ * closures evaluated at t (the time at the start of the step).  Returns
+ * IBPM_E_NONFINITE if a CD/CL came out non-finite (state is still updated). */
+int ibpm_advance(ibpm_prob* prob, double t, const ibpm_step_scalars* s);
+/* nsteps calls of advance! with per-step scalars s[k] (times t[k]) without a
+ * host round trip in between: the time loop of run_sim! (ibpm.jl:93-105).
+ * cd_hist / cl_hist (nsteps x nbodies, row k = step k) may be NULL.  Stops at
+ * the first step whose CL is non-finite (ibpm.jl:95-97) and returns
+ * IBPM_E_NONFINITE with *steps_done set. */
+int ibpm_advance_n(ibpm_prob* prob, int nsteps, const double* t, const ibpm_step_scalars* s,
+                   double* cd_hist, double* cl_hist, int* steps_done);
+/* state.CD, state.CL (nbodies each) and state.F̃b (2 nb_total); any may be NULL. */
+int ibpm_get_forces(ibpm_prob* prob, double* CD, double* CL, double* Fb);
+/* compute_cfl(state, prob) = max|q[:,1]| dt / h (ibpm.jl:83-88). */
+int ibpm_compute_cfl(ibpm_prob* prob, double* cfl);
+
+/* --- operator-level entry points (unit parity tests; host arrays in/out) --- */
+/* kind: 0 = Δinv (laplacian.jl:24-41), 1 = A[lev] (:73-76), 2 = Ainv[lev] (:61-71) */
+int ibpm_op_lapinv(ibpm_prob* prob, int kind, int lev, const double* b, double* x);
+/* 2D RODFT00 of one (nx-1) x (ny-1) array (dst-inversion.jl:10-20) */
+int ibpm_op_dst2(ibpm_prob* prob, const double* x, double* y);
+/* nonlinear!(out, state, bc(lev+1), lev) on host copies of q, q0, Γ (nonlinear.jl:168-201) */
+int ibpm_op_nonlinear(ibpm_prob* prob, int lev, const double* q, const double* q0, const double* G, double* out);
+/* vort2flux! on host copies (curl.jl:133-165); G is updated (coarse levels) */
+int ibpm_op_vort2flux(ibpm_prob* prob, double* G, double* psi, double* q);
+/* E (regT!, interface-coupling.jl:122-133) and E' (reg!, :99-119) */
+int ibpm_op_regT(ibpm_prob* prob, const double* q, double* fb);
+int ibpm_op_reg(ibpm_prob* prob, const double* fb, double* q);
+/* rot! (curl.jl:22-34) */
+int ibpm_op_rot(ibpm_prob* prob, const double* q, double* G);
+
+/* --- timing (bench) --- */
+/* Average device time in ms of `reps` launches of kernel `which` on the
+ * problem's stream, bracketed by HIP events on that stream.  which:
+ * 0 = x-pass DST over all levels, 1 = fused y-pass (A operator) over all
+ * levels, 2 = nonlinear over all levels, 3 = one full advance! step. */
+int ibpm_time_kernel(ibpm_prob* prob, int which, int reps, double* ms_avg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IBPM_HIP_H */

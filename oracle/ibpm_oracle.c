@@ -1,0 +1,1001 @@
+/*
+ * ibpm_oracle.c -- CPU restatement of IBPM.jl's `advance!(state, prob, t)` path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: the parity tests, the
+ * smoke check and the CPU-baseline leg of the benchmark call it to produce the
+ * numbers the HIP path is compared against.  Nothing in the product library
+ * links, loads or calls it.
+ *
+ * Every function restates one reference function, operation by operation, in
+ * the reference's own Julia column-major layouts and (where it fixes rounding)
+ * in the reference's own order of floating-point operations.  Index arithmetic
+ * is written with 1-based accessor macros so that it reads like the Julia code
+ * it follows.  Citations are file:line into the reference tree.
+ *
+ * The two third-party numerics the reference calls are restated here from
+ * their published definitions:
+ *   - FFTW's RODFT00 (DST-I, unnormalised) used by `dst_inv!`
+ *     (src/fluid-operators/dst-inversion.jl:10-20,39-50): computed through a
+ *     complex mixed-radix FFT of the odd extension of length 2(n+1), two real
+ *     lines packed per complex transform.
+ *   - LAPACK getrf/getri behind `inv(B)` (src/fluid-operators/laplacian.jl:118):
+ *     LU with partial pivoting, then column-by-column solves.
+ *
+ * Build: oracle/Makefile (gcc -O2 -fPIC -shared -fopenmp -ffp-contract=off).
+ * -ffp-contract=off keeps every a*b+c as two rounded operations, as Julia does.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* complex mixed-radix FFT (forward, e^{-2 pi i jk/n})                        */
+/* ------------------------------------------------------------------------- */
+typedef struct { double re, im; } ocx;
+
+typedef struct {
+    int n;
+    int nfac;
+    int fac[64];
+    ocx* tw; /* tw[k] = exp(-2 pi i k / n) */
+} offt_plan;
+
+static void offt_plan_init(offt_plan* p, int n)
+{
+    p->n = n;
+    p->nfac = 0;
+    int m = n;
+    while (m % 4 == 0) { p->fac[p->nfac++] = 4; m /= 4; }
+    while (m % 2 == 0) { p->fac[p->nfac++] = 2; m /= 2; }
+    for (int f = 3; m > 1; f += 2) {
+        while (m % f == 0) { p->fac[p->nfac++] = f; m /= f; }
+    }
+    p->tw = (ocx*)malloc(sizeof(ocx) * (size_t)(n > 0 ? n : 1));
+    for (int k = 0; k < n; k++) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+        p->tw[k].re = (double)cosl(a);
+        p->tw[k].im = (double)sinl(a);
+    }
+}
+
+static void offt_plan_free(offt_plan* p) { free(p->tw); p->tw = NULL; }
+
+/* out[k] = sum_j in[j*is] W_n^{jk};  W_n^{e} = tw[e*tws mod N] */
+static void offt_rec(ocx* out, const ocx* in, int n, int is, const offt_plan* p, int d, int tws)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    /* q*k*tws < r*m*(N/n) = N, so first-stage twiddle indices need no wrap */
+    const int r = p->fac[d], m = n / r, N = p->n;
+    for (int q = 0; q < r; q++) offt_rec(out + q * m, in + q * is, m, is * r, p, d + 1, tws * r);
+    if (r == 2) {
+        for (int k = 0; k < m; k++) {
+            ocx a = out[k], b = out[m + k], w = p->tw[(long)k * tws];
+            ocx t = { b.re * w.re - b.im * w.im, b.re * w.im + b.im * w.re };
+            out[k].re = a.re + t.re; out[k].im = a.im + t.im;
+            out[m + k].re = a.re - t.re; out[m + k].im = a.im - t.im;
+        }
+    } else if (r == 4) {
+        for (int k = 0; k < m; k++) {
+            ocx v[4];
+            for (int q = 0; q < 4; q++) {
+                ocx a = out[q * m + k], w = p->tw[(long)q * k * tws];
+                v[q].re = a.re * w.re - a.im * w.im;
+                v[q].im = a.re * w.im + a.im * w.re;
+            }
+            /* DFT-4 with W_4 = -i */
+            ocx s02 = { v[0].re + v[2].re, v[0].im + v[2].im };
+            ocx d02 = { v[0].re - v[2].re, v[0].im - v[2].im };
+            ocx s13 = { v[1].re + v[3].re, v[1].im + v[3].im };
+            ocx d13 = { v[1].re - v[3].re, v[1].im - v[3].im };
+            out[k].re = s02.re + s13.re;         out[k].im = s02.im + s13.im;
+            out[2 * m + k].re = s02.re - s13.re; out[2 * m + k].im = s02.im - s13.im;
+            /* -i * d13 = (d13.im, -d13.re) */
+            out[m + k].re = d02.re + d13.im;     out[m + k].im = d02.im - d13.re;
+            out[3 * m + k].re = d02.re - d13.im; out[3 * m + k].im = d02.im + d13.re;
+        }
+    } else {
+        ocx v[64], o[64];
+        for (int k = 0; k < m; k++) {
+            for (int q = 0; q < r; q++) {
+                ocx a = out[q * m + k], w = p->tw[(long)q * k * tws];
+                v[q].re = a.re * w.re - a.im * w.im;
+                v[q].im = a.re * w.im + a.im * w.re;
+            }
+            for (int s = 0; s < r; s++) {
+                double re = 0.0, im = 0.0;
+                for (int q = 0; q < r; q++) {
+                    ocx w = p->tw[(long)q * s * m * tws % N];
+                    re += v[q].re * w.re - v[q].im * w.im;
+                    im += v[q].re * w.im + v[q].im * w.re;
+                }
+                o[s].re = re; o[s].im = im;
+            }
+            for (int s = 0; s < r; s++) out[k + s * m] = o[s];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* DST-I (FFTW RODFT00): Y_k = 2 sum_{j=0}^{n-1} X_j sin(pi (j+1)(k+1)/(n+1))  */
+/* Two real lines a, b per complex FFT of the odd extension z of length      */
+/* N = 2(n+1): Z_k = -i Ya_{k-1} + Yb_{k-1}.                                 */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int n;           /* line length */
+    offt_plan plan;  /* N = 2(n+1) */
+} odst_plan;
+
+static void odst_plan_init(odst_plan* p, int n) { p->n = n; offt_plan_init(&p->plan, 2 * (n + 1)); }
+static void odst_plan_free(odst_plan* p) { offt_plan_free(&p->plan); }
+
+/* a, b: input lines (stride sa/sb); b may be NULL.  ya, yb: outputs. */
+static void odst_pair(const odst_plan* p, const double* a, long sa, const double* b, long sb,
+                      double* ya, long sya, double* yb, long syb, ocx* zin, ocx* zout)
+{
+    const int n = p->n, N = 2 * (n + 1);
+    zin[0].re = 0.0; zin[0].im = 0.0;
+    zin[n + 1].re = 0.0; zin[n + 1].im = 0.0;
+    for (int j = 0; j < n; j++) {
+        double ar = a[j * sa], bi = b ? b[j * sb] : 0.0;
+        zin[j + 1].re = ar; zin[j + 1].im = bi;
+        zin[N - 1 - j].re = -ar; zin[N - 1 - j].im = -bi;
+    }
+    offt_rec(zout, zin, N, 1, &p->plan, 0, 1);
+    for (int k = 0; k < n; k++) {
+        ya[k * sya] = -zout[k + 1].im;
+        if (yb) yb[k * syb] = zout[k + 1].re;
+    }
+}
+
+static int o_nthreads = 1;
+
+void oracle_set_threads(int n) { o_nthreads = n > 0 ? n : 1; }
+
+/* 2D RODFT00 over dims [1,2] of the n1 x n2 column-major array x -> y (may alias).
+ * Same transform as FFTW.plan_r2r(b, RODFT00, [1,2]) (dst-inversion.jl:10-20). */
+static void o_dst2(double* y, const double* x, int n1, int n2, const odst_plan* p1, const odst_plan* p2)
+{
+    const int N1 = 2 * (n1 + 1), N2 = 2 * (n2 + 1);
+    const int Nmax = N1 > N2 ? N1 : N2;
+    /* dimension 1: contiguous columns, paired */
+#pragma omp parallel num_threads(o_nthreads)
+    {
+        ocx* zin = (ocx*)malloc(sizeof(ocx) * (size_t)Nmax);
+        ocx* zout = (ocx*)malloc(sizeof(ocx) * (size_t)Nmax);
+        double* ta = (double*)malloc(sizeof(double) * (size_t)(n2 > n1 ? n2 : n1));
+        double* tb = (double*)malloc(sizeof(double) * (size_t)(n2 > n1 ? n2 : n1));
+#pragma omp for schedule(static)
+        for (int jp = 0; jp < (n2 + 1) / 2; jp++) {
+            int j = 2 * jp;
+            const double* a = x + (long)j * n1;
+            const double* b = (j + 1 < n2) ? x + (long)(j + 1) * n1 : NULL;
+            /* copy inputs first so that y may alias x */
+            memcpy(ta, a, sizeof(double) * (size_t)n1);
+            if (b) memcpy(tb, b, sizeof(double) * (size_t)n1);
+            odst_pair(p1, ta, 1, b ? tb : NULL, 1, y + (long)j * n1, 1,
+                      b ? y + (long)(j + 1) * n1 : NULL, 1, zin, zout);
+        }
+        free(zin); free(zout); free(ta); free(tb);
+    }
+    /* dimension 2: strided rows, paired */
+#pragma omp parallel num_threads(o_nthreads)
+    {
+        ocx* zin = (ocx*)malloc(sizeof(ocx) * (size_t)Nmax);
+        ocx* zout = (ocx*)malloc(sizeof(ocx) * (size_t)Nmax);
+        double* ta = (double*)malloc(sizeof(double) * (size_t)n2);
+        double* tb = (double*)malloc(sizeof(double) * (size_t)n2);
+        double* ua = (double*)malloc(sizeof(double) * (size_t)n2);
+        double* ub = (double*)malloc(sizeof(double) * (size_t)n2);
+#pragma omp for schedule(static)
+        for (int ip = 0; ip < (n1 + 1) / 2; ip++) {
+            int i = 2 * ip;
+            int hb = (i + 1 < n1);
+            for (int j = 0; j < n2; j++) {
+                ta[j] = y[i + (long)j * n1];
+                if (hb) tb[j] = y[i + 1 + (long)j * n1];
+            }
+            odst_pair(p2, ta, 1, hb ? tb : NULL, 1, ua, 1, hb ? ub : NULL, 1, zin, zout);
+            for (int j = 0; j < n2; j++) {
+                y[i + (long)j * n1] = ua[j];
+                if (hb) y[i + 1 + (long)j * n1] = ub[j];
+            }
+        }
+        free(zin); free(zout); free(ta); free(tb); free(ua); free(ub);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* grid (src/fluid-domain/fluid-domain.jl:10-25,181-210)                      */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int nx, ny, nG, nu, nv, nq, mg, nbc;
+    double offx, offy, len, h;
+    int LEFT, RIGHT, BOT, TOP;
+} og_grid;
+
+/* make_grid(h, boundary; mg) -- fluid-domain.jl:181-210.  Julia's round() is
+ * round-half-to-even, which is nearbyint() under the default rounding mode. */
+static void og_make_grid(og_grid* g, double h, double x0, double x1, double y0, double y1, int mg)
+{
+    g->offx = -x0;
+    g->offy = -y0;
+    g->len = x1 - x0;
+    double ylen = y1 - y0;
+    g->nx = (int)nearbyint(g->len / h);
+    g->ny = (int)nearbyint(ylen * (double)g->nx / g->len);
+    g->nG = (g->nx - 1) * (g->ny - 1);
+    g->nu = g->ny * (g->nx + 1);
+    g->nv = g->nx * (g->ny + 1);
+    g->nq = g->nu + g->nv;
+    g->h = g->len / (double)g->nx;
+    g->mg = mg;
+    g->LEFT = 0;
+    g->RIGHT = g->ny + 1;
+    g->BOT = 2 * (g->ny + 1);
+    g->TOP = 2 * (g->ny + 1) + g->nx + 1;
+    g->nbc = 2 * (g->nx + 1) + 2 * (g->ny + 1);
+}
+
+/* 1-based accessors in the reference's reshaped views */
+#define GM(a, i, j) (a)[((i) - 1) + (long)(nx - 1) * ((j) - 1)]   /* (nx-1) x (ny-1) */
+#define QX(a, i, j) (a)[((i) - 1) + (long)(nx + 1) * ((j) - 1)]   /* (nx+1) x ny      */
+#define QY(a, i, j) (a)[nu + ((i) - 1) + (long)nx * ((j) - 1)]    /* nx x (ny+1)      */
+#define BC(a, k) (a)[(k) - 1]
+
+/* ------------------------------------------------------------------------- */
+/* fluid operators                                                            */
+/* ------------------------------------------------------------------------- */
+
+/* lap_eigs -- laplacian.jl:14-19.  Lam = -2*(cos(pi*i/nx) + cos(pi*j/ny) - 2) */
+static void o_lap_eigs(const og_grid* g, double* Lam)
+{
+    const int nx = g->nx, ny = g->ny;
+    double* cxv = (double*)malloc(sizeof(double) * (size_t)nx);
+    double* cyv = (double*)malloc(sizeof(double) * (size_t)ny);
+    const long double PI = 3.14159265358979323846264338327950288L;
+    /* Julia's pi*(1:n)/n is a twice-precision range: each argument is the
+     * double nearest to fl(pi)*i/n; long double reproduces that rounding. */
+    const long double fpi = (long double)(double)PI;
+    for (int i = 1; i <= nx - 1; i++) cxv[i] = cos((double)(fpi * (long double)i / (long double)nx));
+    for (int j = 1; j <= ny - 1; j++) cyv[j] = cos((double)(fpi * (long double)j / (long double)ny));
+    for (int j = 1; j <= ny - 1; j++)
+        for (int i = 1; i <= nx - 1; i++)
+            GM(Lam, i, j) = -2.0 * ((cxv[i] + cyv[j]) - 2.0);
+    free(cxv); free(cyv);
+}
+
+/* rot! (C') -- curl.jl:22-34 */
+static void o_rot(const og_grid* g, double* G, const double* q)
+{
+    const int nx = g->nx, ny = g->ny, nu = g->nu;
+    for (int j = 2; j <= ny; j++)
+        for (int i = 2; i <= nx; i++) {
+            double a = QX(q, i, j - 1) - QX(q, i, j);
+            double b = QY(q, i, j) - QY(q, i - 1, j);
+            GM(G, i - 1, j - 1) = a + b;
+        }
+}
+
+/* curl! single grid (C) -- curl.jl:51-71.  Used only by adjointness checks. */
+static void o_curl_sg(const og_grid* g, double* q, const double* psi)
+{
+    const int nx = g->nx, ny = g->ny, nu = g->nu;
+    for (int j = 2; j <= ny - 1; j++)
+        for (int i = 2; i <= nx; i++) QX(q, i, j) = GM(psi, i - 1, j) - GM(psi, i - 1, j - 1);
+    for (int i = 2; i <= nx; i++) QX(q, i, 1) = GM(psi, i - 1, 1);
+    for (int i = 2; i <= nx; i++) QX(q, i, ny) = -GM(psi, i - 1, ny - 1);
+    for (int j = 2; j <= ny; j++)
+        for (int i = 2; i <= nx - 1; i++) QY(q, i, j) = GM(psi, i - 1, j - 1) - GM(psi, i, j - 1);
+    for (int j = 2; j <= ny; j++) QY(q, 1, j) = -GM(psi, 1, j - 1);
+    for (int j = 2; j <= ny; j++) QY(q, nx, j) = GM(psi, nx - 1, j - 1);
+}
+
+/* curl! multigrid -- curl.jl:81-116 (note the bottom edge reads psibc[(B-1)+i]) */
+static void o_curl_mg(const og_grid* g, double* q, const double* psi, const double* psibc)
+{
+    const int nx = g->nx, ny = g->ny, nu = g->nu;
+    const int T = g->TOP, B = g->BOT, L = g->LEFT, R = g->RIGHT;
+    for (int j = 2; j <= ny - 1; j++)
+        for (int i = 2; i <= nx; i++) QX(q, i, j) = GM(psi, i - 1, j) - GM(psi, i - 1, j - 1);
+    for (int i = 2; i <= nx; i++) QX(q, i, 1) = GM(psi, i - 1, 1) - BC(psibc, (B - 1) + i);
+    for (int i = 2; i <= nx; i++) QX(q, i, ny) = BC(psibc, i + T) - GM(psi, i - 1, ny - 1);
+    for (int j = 1; j <= ny; j++) QX(q, 1, j) = BC(psibc, (L + 1) + j) - BC(psibc, L + j);
+    for (int j = 1; j <= ny; j++) QX(q, nx + 1, j) = BC(psibc, (R + 1) + j) - BC(psibc, R + j);
+    for (int j = 2; j <= ny; j++)
+        for (int i = 2; i <= nx - 1; i++) QY(q, i, j) = GM(psi, i - 1, j - 1) - GM(psi, i, j - 1);
+    for (int j = 2; j <= ny; j++) QY(q, 1, j) = BC(psibc, L + j) - GM(psi, 1, j - 1);
+    for (int j = 2; j <= ny; j++) QY(q, nx, j) = GM(psi, nx - 1, j - 1) - BC(psibc, R + j);
+    for (int i = 1; i <= nx; i++) QY(q, i, 1) = BC(psibc, B + i) - BC(psibc, (B + 1) + i);
+    for (int i = 1; i <= nx; i++) QY(q, i, ny + 1) = BC(psibc, T + i) - BC(psibc, (T + 1) + i);
+}
+
+/* get_bc! -- mg-utils.jl:7-32 */
+static void o_get_bc(const og_grid* g, double* rbc, const double* r)
+{
+    const int nx = g->nx, ny = g->ny;
+    const int T = g->TOP, B = g->BOT, L = g->LEFT, R = g->RIGHT;
+    const int nx4 = nx / 4, ny4 = ny / 4, nx34 = 3 * nx / 4, ny34 = 3 * ny / 4;
+    for (int m = 0; m <= nx / 2; m++) {
+        int i = nx4 + m, ibc = 1 + 2 * m;
+        BC(rbc, B + ibc) = GM(r, i, ny4);
+        BC(rbc, T + ibc) = GM(r, i, ny34);
+    }
+    for (int m = 1; m <= nx / 2; m++) {
+        int i = nx4 + m, ibc = 2 * m;
+        BC(rbc, B + ibc) = 0.5 * (GM(r, i, ny4) + GM(r, i - 1, ny4));
+        BC(rbc, T + ibc) = 0.5 * (GM(r, i, ny34) + GM(r, i - 1, ny34));
+    }
+    for (int m = 0; m <= ny / 2; m++) {
+        int j = ny4 + m, jbc = 1 + 2 * m;
+        BC(rbc, L + jbc) = GM(r, nx4, j);
+        BC(rbc, R + jbc) = GM(r, nx34, j);
+    }
+    for (int m = 1; m <= ny / 2; m++) {
+        int j = ny4 + m, jbc = 2 * m;
+        BC(rbc, L + jbc) = 0.5 * (GM(r, nx4, j) + GM(r, nx4, j - 1));
+        BC(rbc, R + jbc) = 0.5 * (GM(r, nx34, j) + GM(r, nx34, j - 1));
+    }
+}
+
+/* apply_bc! -- mg-utils.jl:43-60 */
+static void o_apply_bc(const og_grid* g, double* r, const double* rbc, double fac)
+{
+    const int nx = g->nx, ny = g->ny;
+    const int T = g->TOP, B = g->BOT, L = g->LEFT, R = g->RIGHT;
+    for (int i = 1; i <= nx - 1; i++) GM(r, i, 1) += fac * BC(rbc, (B + 1) + i);
+    for (int i = 1; i <= nx - 1; i++) GM(r, i, ny - 1) += fac * BC(rbc, (T + 1) + i);
+    for (int j = 1; j <= ny - 1; j++) GM(r, 1, j) += fac * BC(rbc, (L + 1) + j);
+    for (int j = 1; j <= ny - 1; j++) GM(r, nx - 1, j) += fac * BC(rbc, (R + 1) + j);
+}
+
+/* Julia integer division ÷ truncates toward zero, like C's '/'. */
+/* coarsify! -- mg-utils.jl:71-106 (zip stops at the shorter range) */
+static void o_coarsify(const og_grid* g, double* Gc, const double* G)
+{
+    const int nx = g->nx, ny = g->ny;
+    int j0 = ny / 2 + ((-ny) / 2 + 2), j1 = ny / 2 + (ny / 2 - 2);
+    int jc0 = ny / 2 + ((-ny) / 4 + 1), jc1 = ny / 2 + (ny / 4 - 1);
+    int i0 = nx / 2 + ((-nx) / 2 + 2), i1 = nx / 2 + (nx / 2 - 2);
+    int ic0 = nx / 2 + ((-nx) / 4 + 1), ic1 = nx / 2 + (nx / 4 - 1);
+    int nj = j1 >= j0 ? (j1 - j0) / 2 + 1 : 0, njc = jc1 >= jc0 ? jc1 - jc0 + 1 : 0;
+    int ni = i1 >= i0 ? (i1 - i0) / 2 + 1 : 0, nic = ic1 >= ic0 ? ic1 - ic0 + 1 : 0;
+    int cj = nj < njc ? nj : njc, ci = ni < nic ? ni : nic;
+    for (int b = 0; b < cj; b++) {
+        int j = j0 + 2 * b, jc = jc0 + b;
+        for (int a = 0; a < ci; a++) {
+            int i = i0 + 2 * a, ic = ic0 + a;
+            double s1 = ((GM(G, i + 1, j) + GM(G, i, j + 1)) + GM(G, i - 1, j)) + GM(G, i, j - 1);
+            double s2 = ((GM(G, i + 1, j + 1) + GM(G, i + 1, j - 1)) + GM(G, i - 1, j - 1)) + GM(G, i - 1, j + 1);
+            GM(Gc, ic, jc) = (GM(G, i, j) + 0.5 * s1) + 0.25 * s2;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* problem                                                                    */
+/* ------------------------------------------------------------------------- */
+#define SUPP 6
+#define WIN 13 /* 2*SUPP+1 */
+
+typedef struct {
+    og_grid g;
+    double Re, dt;
+    int nbeta;
+    double beta[4];
+    int nbodies, nbt;
+    int* nb;
+    double* xb; /* nbt x 2 col-major */
+    double* ds;
+    int motion; /* 0 = Static, 1 = MovingGrid */
+    /* precomputed operators */
+    double* Lam;     /* mats.Λ */
+    double* divA;    /* mg x nG: 1 ./ (1 .- Λ dt/(2 Re hc^2)) */
+    double* divAinv; /* mg x nG: 1 .+ Λ dt/(2 Re hc^2) */
+    double lap_scale;
+    int* body_idx;   /* nbt x 2 */
+    double* weight;  /* [k][c][l + 13 m] */
+    double* Binv;    /* nf x nf col-major */
+    double* XX;      /* nx(ny+1) x mg */
+    double* YY;      /* ny(nx+1) x mg */
+    odst_plan p1, p2;
+    /* WorkingMemory (models.jl:79-106) */
+    double *q1, *G1, *q2, *G2, *q3, *q4, *q5, *q6, *G3, *Gbc, *rhsbc, *Gwork, *wdst;
+    double *bt_G, *bt_psi, *bt_q; /* get_Binv scratch */
+} oprob;
+
+/* dst_inv!(x, b, Λ, plan; scale) -- dst-inversion.jl:39-50, wrapped by
+ * get_lap_inv -- laplacian.jl:24-41 (copy through b_temp/x_temp). */
+static void o_dst_inv(const oprob* P, double* x, const double* b, const double* Lam, double scale)
+{
+    const og_grid* g = &P->g;
+    double* w = P->wdst;
+    o_dst2(w, b, g->nx - 1, g->ny - 1, &P->p1, &P->p2);
+    for (long k = 0; k < g->nG; k++) w[k] /= Lam[k];
+    for (long k = 0; k < g->nG; k++) w[k] *= scale;
+    o_dst2(x, w, g->nx - 1, g->ny - 1, &P->p1, &P->p2);
+}
+
+static void o_Ainv(const oprob* P, double* x, const double* b, int lev)
+{ o_dst_inv(P, x, b, P->divAinv + (long)(lev - 1) * P->g.nG, P->lap_scale); }
+static void o_A(const oprob* P, double* x, const double* b, int lev)
+{ o_dst_inv(P, x, b, P->divA + (long)(lev - 1) * P->g.nG, P->lap_scale); }
+static void o_Dinv(const oprob* P, double* x, const double* b)
+{ o_dst_inv(P, x, b, P->Lam, P->lap_scale); }
+
+/* vort2flux! -- curl.jl:133-165.  Mutates the coarse levels of G. */
+static void o_vort2flux(oprob* P, double* psi, double* q, double* G, int ngrids)
+{
+    const og_grid* g = &P->g;
+    const long nG = g->nG, nq = g->nq;
+    double* psibc = P->Gbc;
+    double* Gwork = P->G3;
+    for (int lev = 2; lev <= ngrids; lev++) o_coarsify(g, G + (lev - 1) * nG, G + (lev - 2) * nG);
+    for (long k = 0; k < nG * g->mg; k++) psi[k] *= 0.0;
+    for (int k = 0; k < g->nbc; k++) psibc[k] *= 0.0;
+    o_Dinv(P, psi + (ngrids - 1) * nG, G + (ngrids - 1) * nG);
+    o_curl_mg(g, q + (ngrids - 1) * nq, psi + (ngrids - 1) * nG, psibc);
+    for (int lev = ngrids - 1; lev >= 1; lev--) {
+        memcpy(Gwork, G + (lev - 1) * nG, sizeof(double) * (size_t)nG);
+        o_get_bc(g, psibc, psi + lev * nG);
+        o_apply_bc(g, Gwork, psibc, 1.0);
+        o_Dinv(P, psi + (lev - 1) * nG, Gwork);
+        o_curl_mg(g, q + (lev - 1) * nq, psi + (lev - 1) * nG, psibc);
+    }
+}
+
+/* avg_flux -- nonlinear.jl:17-38 (IBProblem method) */
+static void o_avg_flux(const og_grid* g, double* Q, const double* q, const double* q0)
+{
+    const int nx = g->nx, ny = g->ny, nu = g->nu;
+    for (long k = 0; k < g->nq; k++) Q[k] *= 0.0;
+    for (int j = 2; j <= ny; j++)
+        for (int i = 1; i <= nx + 1; i++)
+            QX(Q, i, j) = ((QX(q, i, j) + QX(q0, i, j)) + QX(q, i, j - 1)) + QX(q0, i, j - 1);
+    for (int j = 1; j <= ny + 1; j++)
+        for (int i = 2; i <= nx; i++)
+            QY(Q, i, j) = ((QY(q, i, j) + QY(q0, i, j)) + QY(q, i - 1, j)) + QY(q0, i - 1, j);
+}
+
+/* direct_product_loops! -- nonlinear.jl:73-114 (nl_avg! :2-10) */
+static void o_direct_product_loops(const og_grid* g, double* fq, const double* Q, const double* G, const double* Gbc)
+{
+    const int nx = g->nx, ny = g->ny, nu = g->nu;
+    const int T = g->TOP, B = g->BOT, L = g->LEFT, R = g->RIGHT;
+    for (int j = 2; j <= ny - 1; j++)
+        for (int i = 2; i <= nx; i++)
+            QX(fq, i, j) = QY(Q, i, j + 1) * GM(G, i - 1, j) + QY(Q, i, j) * GM(G, i - 1, j - 1);
+    for (int i = 2; i <= nx; i++)
+        QX(fq, i, 1) = QY(Q, i, 2) * GM(G, i - 1, 1) + QY(Q, i, 1) * BC(Gbc, B + i);
+    for (int i = 2; i <= nx; i++)
+        QX(fq, i, ny) = QY(Q, i, ny) * GM(G, i - 1, ny - 1) + QY(Q, i, ny + 1) * BC(Gbc, T + i);
+    for (int j = 2; j <= ny; j++)
+        for (int i = 2; i <= nx - 1; i++)
+            QY(fq, i, j) = (QX(Q, i + 1, j) * GM(G, i, j - 1) + QX(Q, i, j) * GM(G, i - 1, j - 1)) * -1.0;
+    for (int j = 2; j <= ny; j++)
+        QY(fq, 1, j) = (QX(Q, 2, j) * GM(G, 1, j - 1) + QX(Q, 1, j) * BC(Gbc, L + j)) * -1.0;
+    for (int j = 2; j <= ny; j++)
+        QY(fq, nx, j) = (QX(Q, nx, j) * GM(G, nx - 1, j - 1) + QX(Q, nx + 1, j) * BC(Gbc, R + j)) * -1.0;
+}
+
+/* nonlinear! -- nonlinear.jl:168-201.  Scales Gbc by 0.25 in place. */
+static void o_nonlinear(oprob* P, double* nonlin, const double* q, const double* q0, const double* G,
+                        double* Gbc, int lev)
+{
+    const og_grid* g = &P->g;
+    double* fq = P->q3;
+    for (long k = 0; k < g->nq; k++) fq[k] *= 0.0;
+    for (int k = 0; k < g->nbc; k++) Gbc[k] *= 0.25;
+    double* Q = P->q2;
+    o_avg_flux(g, Q, q, q0);
+    for (long k = 0; k < g->nq; k++) fq[k] *= 0.0;
+    o_direct_product_loops(g, fq, Q, G, Gbc);
+    o_rot(g, nonlin, fq);
+    double hc = g->h * (double)(1L << (lev - 1));
+    double s = 1.0 / (4.0 * (hc * hc));
+    for (long k = 0; k < g->nG; k++) nonlin[k] *= s;
+}
+
+/* ------------------------------------------------------------------------- */
+/* interface coupling (src/interface-coupling/interface-coupling.jl)          */
+/* ------------------------------------------------------------------------- */
+
+/* δh -- interface-coupling.jl:6-62 (Yang-3 smoothed delta times h) */
+double oracle_delta_h(double rf, double rb, double dr)
+{
+    double r = fabs(rf - rb);
+    double r1 = r / dr, r2 = r1 * r1, r3 = r2 * r1, r4 = r3 * r1;
+    double del_h;
+    if (r1 <= 1.0) {
+        double a5 = asin((1.0 / 2.0) * sqrt(3.0) * (2.0 * r1 - 1.0));
+        double a8 = sqrt(1.0 - 12.0 * r2 + 12.0 * r1);
+        del_h = 4.166666667e-2 * r4 + (-0.1388888889 + 3.472222222e-2 * a8) * r3 +
+                (-7.121664902e-2 - 5.208333333e-2 * a8 + 0.2405626122 * a5) * r2 +
+                (-.2405626122 * a5 - 0.3792313933 + 0.1012731481 * a8) * r1 + 8.0187537413e-2 * a5 -
+                4.195601852e-2 * a8 + 0.6485698427;
+    } else if (r1 <= 2.0) {
+        double a6 = asin((1.0 / 2.0) * sqrt(3.0) * (-3.0 + 2.0 * r1));
+        double a9 = sqrt(-23.0 + 36.0 * r1 - 12.0 * r2);
+        del_h = -6.250000000e-2 * r4 + (0.4861111111 - 1.736111111e-2 * a9) * r3 +
+                (-1.143175026 + 7.812500000e-2 * a9 - 0.1202813061 * a6) * r2 +
+                (0.8751991178 + 0.3608439183 * a6 - 0.1548032407 * a9) * r1 - 0.2806563809 * a6 +
+                8.22848104e-3 + 0.1150173611 * a9;
+    } else if (r1 <= 3.0) {
+        double a1 = asin((1.0 / 2.0 * (2.0 * r1 - 5.0)) * sqrt(3.0));
+        double a7 = sqrt(-71.0 - 12.0 * r2 + 60.0 * r1);
+        del_h = 2.083333333e-2 * r4 + (3.472222222e-3 * a7 - 0.2638888889) * r3 +
+                (1.214391675 - 2.604166667e-2 * a7 + 2.405626122e-2 * a1) * r2 +
+                (-0.1202813061 * a1 - 2.449273192 + 7.262731481e-2 * a7) * r1 + 0.1523563211 * a1 +
+                1.843201677 - 7.306134259e-2 * a7;
+    } else {
+        del_h = 0.0;
+    }
+    return del_h;
+}
+
+/* setup_reg -- interface-coupling.jl:72-96 */
+static void o_setup_reg(oprob* P)
+{
+    const og_grid* g = &P->g;
+    const int nbt = P->nbt;
+    const double h = g->h;
+    for (int k = 0; k < nbt; k++) {
+        P->body_idx[k] = (int)floor((P->xb[k] + g->offx) / h);
+        P->body_idx[k + nbt] = (int)floor((P->xb[k + nbt] + g->offy) / h);
+    }
+    for (int k = 0; k < nbt; k++) {
+        double xk = P->xb[k], yk = P->xb[k + nbt];
+        int bx = P->body_idx[k], by = P->body_idx[k + nbt];
+        double* w1 = P->weight + (long)(2 * k) * WIN * WIN;
+        double* w2 = P->weight + (long)(2 * k + 1) * WIN * WIN;
+        for (int m = 0; m < WIN; m++) {
+            double y = h * (double)(by - 1 + (m - SUPP)) - g->offy;
+            for (int l = 0; l < WIN; l++) {
+                double x = h * (double)(bx - 1 + (l - SUPP)) - g->offx;
+                w1[l + WIN * m] = oracle_delta_h(x, xk, h) * oracle_delta_h(y + h / 2, yk, h);
+                w2[l + WIN * m] = oracle_delta_h(x + h / 2, xk, h) * oracle_delta_h(y, yk, h);
+            }
+        }
+    }
+}
+
+/* reg! (E') -- interface-coupling.jl:99-119 */
+static void o_reg(const oprob* P, double* q, const double* fb)
+{
+    const og_grid* g = &P->g;
+    const int nx = g->nx, nu = g->nu, nbt = P->nbt;
+    (void)nx;
+    for (long k = 0; k < g->nq; k++) q[k] = 0.0;
+    for (int k = 0; k < nbt; k++) {
+        int bx = P->body_idx[k], by = P->body_idx[k + nbt];
+        const double* w1 = P->weight + (long)(2 * k) * WIN * WIN;
+        const double* w2 = P->weight + (long)(2 * k + 1) * WIN * WIN;
+        double fx = fb[k], fy = fb[k + nbt];
+        for (int m = 0; m < WIN; m++)
+            for (int l = 0; l < WIN; l++) {
+                int i = bx + l - SUPP, j = by + m - SUPP;
+                QX(q, i, j) = QX(q, i, j) + w1[l + WIN * m] * fx;
+                QY(q, i, j) = QY(q, i, j) + w2[l + WIN * m] * fy;
+            }
+    }
+}
+
+/* regT! (E) -- interface-coupling.jl:122-133 */
+static void o_regT(const oprob* P, double* fb, const double* q)
+{
+    const og_grid* g = &P->g;
+    const int nx = g->nx, nu = g->nu, nbt = P->nbt;
+    (void)nx;
+    for (int k = 0; k < 2 * nbt; k++) fb[k] = 0.0;
+    for (int k = 0; k < nbt; k++) {
+        int bx = P->body_idx[k], by = P->body_idx[k + nbt];
+        const double* w1 = P->weight + (long)(2 * k) * WIN * WIN;
+        const double* w2 = P->weight + (long)(2 * k + 1) * WIN * WIN;
+        double s1 = 0.0, s2 = 0.0;
+        for (int m = 0; m < WIN; m++)
+            for (int l = 0; l < WIN; l++) {
+                int i = bx + l - SUPP, j = by + m - SUPP;
+                s1 += w1[l + WIN * m] * QX(q, i, j);
+                s2 += w2[l + WIN * m] * QY(q, i, j);
+            }
+        fb[k] += s1;
+        fb[k + nbt] += s2;
+    }
+}
+
+/* B_times! -- laplacian.jl:143-169 */
+static void o_B_times(oprob* P, double* x, const double* z)
+{
+    const og_grid* g = &P->g;
+    const long nG = g->nG;
+    double* G = P->bt_G;
+    for (long k = 0; k < nG * g->mg; k++) G[k] *= 0.0;
+    /* Γ[:,1] = Ainv * ((E*C)' * z) = Ainv(rot(reg(z))) */
+    o_reg(P, P->q4, z);
+    o_rot(g, P->Gwork, P->q4);
+    o_Ainv(P, G, P->Gwork, 1);
+    o_vort2flux(P, P->bt_psi, P->bt_q, G, g->mg);
+    o_regT(P, x, P->bt_q);
+}
+
+/* in-place inverse of a dense column-major n x n matrix by LU with partial
+ * pivoting (the algorithm of LAPACK getrf/getri, which inv() calls). */
+static int o_invert(double* A, int n)
+{
+    int* piv = (int*)malloc(sizeof(int) * (size_t)n);
+    double* LU = (double*)malloc(sizeof(double) * (size_t)n * n);
+    memcpy(LU, A, sizeof(double) * (size_t)n * n);
+#define LUA(i, j) LU[(i) + (long)n * (j)]
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double best = fabs(LUA(k, k));
+        for (int i = k + 1; i < n; i++) if (fabs(LUA(i, k)) > best) { best = fabs(LUA(i, k)); p = i; }
+        piv[k] = p;
+        if (best == 0.0) { free(piv); free(LU); return -1; }
+        if (p != k) for (int j = 0; j < n; j++) { double t = LUA(k, j); LUA(k, j) = LUA(p, j); LUA(p, j) = t; }
+        double inv = 1.0 / LUA(k, k);
+        for (int i = k + 1; i < n; i++) LUA(i, k) *= inv;
+#pragma omp parallel for num_threads(o_nthreads) schedule(static)
+        for (int j = k + 1; j < n; j++) {
+            double akj = LUA(k, j);
+            if (akj != 0.0) for (int i = k + 1; i < n; i++) LUA(i, j) -= LUA(i, k) * akj;
+        }
+    }
+    /* solve A X = I column by column: P A = L U */
+#pragma omp parallel for num_threads(o_nthreads) schedule(static)
+    for (int c = 0; c < n; c++) {
+        double* x = A + (long)n * c;
+        for (int i = 0; i < n; i++) x[i] = 0.0;
+        x[c] = 1.0;
+        for (int k = 0; k < n; k++) if (piv[k] != k) { double t = x[k]; x[k] = x[piv[k]]; x[piv[k]] = t; }
+        for (int j = 0; j < n; j++) { double xj = x[j]; if (xj != 0.0) for (int i = j + 1; i < n; i++) x[i] -= LUA(i, j) * xj; }
+        for (int j = n - 1; j >= 0; j--) { x[j] /= LUA(j, j); double xj = x[j]; for (int i = 0; i < j; i++) x[i] -= LUA(i, j) * xj; }
+    }
+#undef LUA
+    free(piv); free(LU);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* public API (ctypes)                                                        */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    double *q, *q0, *G, *psi, *nl1, *nl2, *Fb, *fb, *CD, *CL;
+} ostate;
+
+void oracle_problem_free(oprob* P);
+
+/* IBProblem(grid, bodies, dt, Re) -- problem-types.jl:188-200 with IBModel
+ * (models.jl:124-158), IBMatrices (models.jl:33-52) and get_AB
+ * (laplacian.jl:83-89).  When binv == NULL, Binv is formed by get_Binv
+ * (laplacian.jl:98-119); otherwise it is copied from binv. */
+oprob* oracle_problem_create(double h, double x0, double x1, double y0, double y1, int mg,
+                             double Re, double dt, int nbodies, const int* nb, const double* xb,
+                             const double* ds, int motion, const double* binv, int* status)
+{
+    oprob* P = (oprob*)calloc(1, sizeof(oprob));
+    og_grid* g = &P->g;
+    og_make_grid(g, h, x0, x1, y0, y1, mg);
+    const int nx = g->nx, ny = g->ny;
+    P->Re = Re; P->dt = dt;
+    P->nbeta = 2; P->beta[0] = 1.5; P->beta[1] = -0.5; /* AB2 -- models.jl:70-72 */
+    P->nbodies = nbodies;
+    P->nb = (int*)malloc(sizeof(int) * (size_t)(nbodies > 0 ? nbodies : 1));
+    int nbt = 0;
+    for (int b = 0; b < nbodies; b++) { P->nb[b] = nb[b]; nbt += nb[b]; }
+    P->nbt = nbt;
+    P->xb = (double*)malloc(sizeof(double) * 2 * (size_t)(nbt > 0 ? nbt : 1));
+    P->ds = (double*)malloc(sizeof(double) * (size_t)(nbt > 0 ? nbt : 1));
+    memcpy(P->xb, xb, sizeof(double) * 2 * (size_t)nbt);
+    memcpy(P->ds, ds, sizeof(double) * (size_t)nbt);
+    P->motion = motion;
+    const long nG = g->nG, nq = g->nq;
+    P->Lam = (double*)malloc(sizeof(double) * nG);
+    o_lap_eigs(g, P->Lam);
+    P->lap_scale = 1.0 / (double)(4 * nx * ny);
+    P->divA = (double*)malloc(sizeof(double) * nG * mg);
+    P->divAinv = (double*)malloc(sizeof(double) * nG * mg);
+    for (int lev = 1; lev <= mg; lev++) {
+        double hc = g->h * (double)(1L << (lev - 1));
+        double c = (2.0 * Re) * (hc * hc);
+        for (long k = 0; k < nG; k++) {
+            double lt = 1.0 - (P->Lam[k] * dt) / c;                    /* get_A    laplacian.jl:74 */
+            P->divA[(lev - 1) * nG + k] = 1.0 / lt;                      /* 1 ./ Λ̃   laplacian.jl:75 */
+            P->divAinv[(lev - 1) * nG + k] = 1.0 + (P->Lam[k] * dt) / c; /* get_Ainv laplacian.jl:69 */
+        }
+    }
+    odst_plan_init(&P->p1, nx - 1);
+    odst_plan_init(&P->p2, ny - 1);
+    /* WorkingMemory -- models.jl:91-105 */
+    P->q1 = (double*)calloc(nq * mg, sizeof(double));
+    P->G1 = (double*)calloc(nG * mg, sizeof(double));
+    P->q2 = (double*)calloc(nq, sizeof(double));
+    P->G2 = (double*)calloc(nG, sizeof(double));
+    P->q3 = (double*)calloc(nq, sizeof(double));
+    P->q4 = (double*)calloc(nq, sizeof(double));
+    P->q5 = (double*)calloc(nq, sizeof(double));
+    P->q6 = (double*)calloc(nq, sizeof(double));
+    P->G3 = (double*)calloc(nG, sizeof(double));
+    P->Gbc = (double*)calloc(g->nbc, sizeof(double));
+    P->rhsbc = (double*)calloc(nG, sizeof(double));
+    P->Gwork = (double*)calloc(nG, sizeof(double));
+    P->wdst = (double*)calloc(nG, sizeof(double));
+    P->bt_G = (double*)calloc(nG * mg, sizeof(double));
+    P->bt_psi = (double*)calloc(nG * mg, sizeof(double));
+    P->bt_q = (double*)calloc(nq * mg, sizeof(double));
+    /* E -- setup_reg */
+    P->body_idx = (int*)malloc(sizeof(int) * 2 * (size_t)(nbt > 0 ? nbt : 1));
+    P->weight = (double*)calloc((size_t)(2 * (nbt > 0 ? nbt : 1)) * WIN * WIN, sizeof(double));
+    o_setup_reg(P);
+    *status = 0;
+    for (int k = 0; k < nbt; k++) {
+        int bx = P->body_idx[k], by = P->body_idx[k + nbt];
+        if (bx - SUPP < 1 || bx + SUPP > nx || by - SUPP < 1 || by + SUPP > ny) *status = -2;
+    }
+    if (*status) { oracle_problem_free(P); return NULL; }
+    /* moving-grid coordinates -- models.jl:135-153 (xc = yc = 0) */
+    if (motion == 1) {
+        P->XX = (double*)malloc(sizeof(double) * (size_t)nx * (ny + 1) * mg);
+        P->YY = (double*)malloc(sizeof(double) * (size_t)ny * (nx + 1) * mg);
+        for (int lev = 1; lev <= mg; lev++) {
+            double hc = g->h * (double)(1L << (lev - 1));
+            for (int j = 1; j <= ny; j++) {
+                double y = (((double)j - 0.5) - (double)ny / 2.0) * hc + ((double)ny / 2.0) * g->h - g->offy;
+                for (int i = 1; i <= nx + 1; i++) P->YY[(long)(lev - 1) * g->nu + (i - 1) + (long)(nx + 1) * (j - 1)] = y - 0.0;
+            }
+            for (int j = 1; j <= ny + 1; j++)
+                for (int i = 1; i <= nx; i++) {
+                    double x = (((double)i - 0.5) - (double)nx / 2.0) * hc + ((double)nx / 2.0) * g->h - g->offx;
+                    P->XX[(long)(lev - 1) * g->nv + (i - 1) + (long)nx * (j - 1)] = x - 0.0;
+                }
+        }
+    }
+    const int nf = 2 * nbt;
+    P->Binv = (double*)malloc(sizeof(double) * (size_t)(nf > 0 ? nf : 1) * (nf > 0 ? nf : 1));
+    if (binv) {
+        memcpy(P->Binv, binv, sizeof(double) * (size_t)nf * nf);
+    } else {
+        double* e = (double*)calloc(nf, sizeof(double));
+        for (int j = 0; j < nf; j++) {
+            for (int k = 0; k < nf; k++) e[k] *= 0.0;
+            e[j] = 1.0;
+            o_B_times(P, P->Binv + (long)nf * j, e);
+        }
+        free(e);
+        if (nf > 0 && o_invert(P->Binv, nf) != 0) { *status = -3; oracle_problem_free(P); return NULL; }
+    }
+    return P;
+}
+
+void oracle_problem_free(oprob* P)
+{
+    if (!P) return;
+    odst_plan_free(&P->p1); odst_plan_free(&P->p2);
+    free(P->nb); free(P->xb); free(P->ds); free(P->Lam); free(P->divA); free(P->divAinv);
+    free(P->body_idx); free(P->weight); free(P->Binv); free(P->XX); free(P->YY);
+    free(P->q1); free(P->G1); free(P->q2); free(P->G2); free(P->q3); free(P->q4); free(P->q5); free(P->q6);
+    free(P->G3); free(P->Gbc); free(P->rhsbc); free(P->Gwork); free(P->wdst);
+    free(P->bt_G); free(P->bt_psi); free(P->bt_q);
+    free(P);
+}
+
+void oracle_problem_sizes(const oprob* P, int* out /* nx ny nG nq mg nbt nbc */, double* dout /* h offx offy len */)
+{
+    out[0] = P->g.nx; out[1] = P->g.ny; out[2] = P->g.nG; out[3] = P->g.nq; out[4] = P->g.mg;
+    out[5] = P->nbt; out[6] = P->g.nbc;
+    dout[0] = P->g.h; dout[1] = P->g.offx; dout[2] = P->g.offy; dout[3] = P->g.len;
+}
+
+void oracle_get_binv(const oprob* P, double* out) { memcpy(out, P->Binv, sizeof(double) * 4 * (size_t)P->nbt * P->nbt); }
+void oracle_get_body_idx(const oprob* P, int* out) { memcpy(out, P->body_idx, sizeof(int) * 2 * (size_t)P->nbt); }
+void oracle_get_weights(const oprob* P, double* out) { memcpy(out, P->weight, sizeof(double) * 2 * (size_t)P->nbt * WIN * WIN); }
+void oracle_get_lam(const oprob* P, double* out) { memcpy(out, P->Lam, sizeof(double) * (size_t)P->g.nG); }
+
+/* base_flux! for InertialMotion -- timestepping.jl:252-270 (used by IBState) */
+static void o_base_flux_inertial(const oprob* P, double* q0, double Ux, double Uy, double alpha)
+{
+    const og_grid* g = &P->g;
+    const int nu = g->nu;
+    for (int lev = 1; lev <= g->mg; lev++) {
+        double hc = g->h * (double)(1L << (lev - 1));
+        double vx = (Ux * cos(alpha) - Uy * sin(alpha)) * hc;
+        double vy = (Ux * sin(alpha) + Uy * cos(alpha)) * hc;
+        double* col = q0 + (long)(lev - 1) * g->nq;
+        for (int k = 0; k < nu; k++) col[k] = vx;
+        for (long k = nu; k < g->nq; k++) col[k] = vy;
+    }
+}
+
+/* base_flux! for MovingGrid -- timestepping.jl:273-318 */
+static void o_base_flux_moving(const oprob* P, double* q0, double U, double V, double th, double thdot,
+                               double Uxf, double Uyf, double alf)
+{
+    const og_grid* g = &P->g;
+    const int nu = g->nu;
+    const long nq = g->nq;
+    double Om = -thdot;
+    double al = -th;
+    double Ux0 = U * cos(al) - V * sin(al);
+    double Uy0 = U * sin(al) + V * cos(al);
+    Ux0 += Uxf * cos(alf) - Uyf * sin(alf);
+    Uy0 += Uxf * sin(alf) + Uyf * cos(alf);
+    for (long k = 0; k < nq * g->mg; k++) q0[k] *= 0.0;
+    for (int lev = 1; lev <= g->mg; lev++) {
+        double hc = g->h * pow(2.0, (double)lev - 1.0);
+        double* col = q0 + (long)(lev - 1) * nq;
+        const double* YY = P->YY + (long)(lev - 1) * nu;
+        const double* XX = P->XX + (long)(lev - 1) * g->nv;
+        double sx = -hc * Om, sy = hc * Om;
+        for (int k = 0; k < nu; k++) { col[k] = YY[k]; col[k] *= sx; }
+        for (long k = nu; k < nq; k++) { col[k] = XX[k - nu]; col[k] *= sy; }
+        for (int k = 0; k < nu; k++) col[k] += hc * Ux0;
+        for (long k = nu; k < nq; k++) col[k] += hc * Uy0;
+    }
+}
+
+/* scalars[] = {Ux, Uy, inclination, U, V, theta, theta_dot} evaluated at t */
+void oracle_base_flux(const oprob* P, ostate* S, const double* sc)
+{
+    if (P->motion == 1) o_base_flux_moving(P, S->q0, sc[3], sc[4], sc[5], sc[6], sc[0], sc[1], sc[2]);
+    else o_base_flux_inertial(P, S->q0, sc[0], sc[1], sc[2]);
+}
+
+/* get_trial_state! -- timestepping.jl:86-129 */
+static void o_get_trial_state(oprob* P, double* qs, double* Gs, ostate* S)
+{
+    const og_grid* g = &P->g;
+    const long nG = g->nG, nq = g->nq;
+    const double dt = P->dt;
+    double* rhsbc = P->rhsbc;
+    double* rhs = P->G2;
+    double* bc = P->Gbc;
+    for (int lev = g->mg; lev >= 1; lev--) {
+        for (int k = 0; k < g->nbc; k++) bc[k] *= 0.0;
+        for (long k = 0; k < nG; k++) rhsbc[k] *= 0.0;
+        double hc = g->h * (double)(1L << (lev - 1));
+        if (lev < g->mg) {
+            o_get_bc(g, bc, S->G + (long)lev * nG);
+            double fac = 0.25 * dt / (P->Re * (hc * hc));
+            o_apply_bc(g, rhsbc, bc, fac);
+        }
+        o_nonlinear(P, S->nl1 + (lev - 1) * nG, S->q + (lev - 1) * nq, S->q0 + (lev - 1) * nq,
+                    S->G + (lev - 1) * nG, bc, lev);
+        o_A(P, rhs, S->G + (lev - 1) * nG, lev);
+        double c1 = P->beta[0] * dt, c2 = P->beta[1] * dt;
+        const double* n1 = S->nl1 + (lev - 1) * nG;
+        const double* n2 = S->nl2 + (lev - 1) * nG;
+        for (long k = 0; k < nG; k++) rhs[k] += c1 * n1[k];
+        for (long k = 0; k < nG; k++) rhs[k] += c2 * n2[k];
+        for (long k = 0; k < nG; k++) rhs[k] += rhsbc[k];
+        o_Ainv(P, Gs + (lev - 1) * nG, rhs, lev);
+    }
+    memcpy(S->nl2, S->nl1, sizeof(double) * nG * g->mg);
+    o_vort2flux(P, S->psi, qs, Gs, g->mg);
+}
+
+/* boundary_forces! (Static / MovingGrid) -- timestepping.jl:154-163 */
+static void o_boundary_forces(oprob* P, double* Fb, const double* qs1, const double* q01)
+{
+    const og_grid* g = &P->g;
+    const int nf = 2 * P->nbt;
+    double* Q = P->q2;
+    for (long k = 0; k < g->nq; k++) Q[k] = qs1[k] + q01[k];
+    double* tmp = (double*)malloc(sizeof(double) * (size_t)(nf > 0 ? nf : 1));
+    o_regT(P, tmp, Q);
+    for (int i = 0; i < nf; i++) {
+        double s = 0.0;
+        for (int j = 0; j < nf; j++) s += P->Binv[i + (long)nf * j] * tmp[j];
+        Fb[i] = s;
+    }
+    free(tmp);
+}
+
+/* update_stress! -- timestepping.jl:330-354 */
+static void o_update_stress(const oprob* P, ostate* S)
+{
+    const double h = P->g.h, dt = P->dt;
+    const int nbt = P->nbt;
+    int tally = 0;
+    for (int b = 0; b < P->nbodies; b++) {
+        int n = P->nb[b];
+        double* fb = S->fb + 2L * tally; /* body b: n x 2 col-major */
+        double sx = 0.0, sy = 0.0;
+        for (int k = 0; k < n; k++) {
+            double ds = P->ds[tally + k];
+            fb[k] = (S->Fb[tally + k] * (h / dt)) / ds;
+            fb[n + k] = (S->Fb[nbt + tally + k] * (h / dt)) / ds;
+        }
+        for (int k = 0; k < n; k++) sx += P->ds[tally + k] * fb[k];
+        for (int k = 0; k < n; k++) sy += P->ds[tally + k] * fb[n + k];
+        S->CD[b] = 2.0 * sx;
+        S->CL[b] = 2.0 * sy;
+        tally += n;
+    }
+}
+
+/* project_circ! -- timestepping.jl:204-223 */
+static void o_project_circ(oprob* P, double* Gs, ostate* S)
+{
+    const og_grid* g = &P->g;
+    const long nG = g->nG;
+    double* Gwork = P->G2;
+    memcpy(S->G, Gs, sizeof(double) * nG * g->mg);
+    o_reg(P, P->q4, S->Fb);
+    o_rot(g, Gs, P->q4);
+    o_Ainv(P, Gwork, Gs, 1);
+    for (long k = 0; k < nG; k++) S->G[k] -= Gwork[k];
+}
+
+/* advance!(state, prob, t) -- timestepping.jl:35-67.  sc: closures at t. */
+int oracle_advance(oprob* P, ostate* S, const double* sc)
+{
+    if (P->motion == 1) oracle_base_flux(P, S, sc); /* timestepping.jl:41-43 */
+    double* qs = P->q1;
+    double* Gs = P->G1;
+    o_get_trial_state(P, qs, Gs, S);
+    o_boundary_forces(P, S->Fb, qs, S->q0);
+    o_update_stress(P, S);
+    o_project_circ(P, Gs, S);
+    o_vort2flux(P, S->psi, S->q, S->G, P->g.mg);
+    return 0;
+}
+
+/* run nsteps from times t[0..nsteps-1]; sc: 7 scalars per step; CD/CL
+ * history (nsteps x nbodies) is recorded when hist != NULL. */
+int oracle_run(oprob* P, ostate* S, int nsteps, const double* sc, double* cd_hist, double* cl_hist)
+{
+    for (int s = 0; s < nsteps; s++) {
+        oracle_advance(P, S, sc + 7L * s);
+        if (cd_hist) for (int b = 0; b < P->nbodies; b++) {
+            cd_hist[(long)s * P->nbodies + b] = S->CD[b];
+            cl_hist[(long)s * P->nbodies + b] = S->CL[b];
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* operator-level entry points for unit tests                                 */
+/* ------------------------------------------------------------------------- */
+void oracle_op_dst2(const oprob* P, double* y, const double* x) { o_dst2(y, x, P->g.nx - 1, P->g.ny - 1, &P->p1, &P->p2); }
+
+/* raw DST-I of n1 x n2 arrays without a problem */
+void oracle_dst2_raw(double* y, const double* x, int n1, int n2)
+{
+    odst_plan p1, p2;
+    odst_plan_init(&p1, n1); odst_plan_init(&p2, n2);
+    o_dst2(y, x, n1, n2, &p1, &p2);
+    odst_plan_free(&p1); odst_plan_free(&p2);
+}
+
+/* kind: 0 = Δinv, 1 = A[lev], 2 = Ainv[lev] */
+void oracle_op_lapinv(const oprob* P, double* x, const double* b, int kind, int lev)
+{
+    if (kind == 0) o_Dinv(P, x, b);
+    else if (kind == 1) o_A(P, x, b, lev);
+    else o_Ainv(P, x, b, lev);
+}
+void oracle_op_rot(const oprob* P, double* G, const double* q) { o_rot(&P->g, G, q); }
+void oracle_op_curl_sg(const oprob* P, double* q, const double* psi) { o_curl_sg(&P->g, q, psi); }
+void oracle_op_curl_mg(const oprob* P, double* q, const double* psi, const double* bc) { o_curl_mg(&P->g, q, psi, bc); }
+void oracle_op_get_bc(const oprob* P, double* bc, const double* r) { o_get_bc(&P->g, bc, r); }
+void oracle_op_apply_bc(const oprob* P, double* r, const double* bc, double fac) { o_apply_bc(&P->g, r, bc, fac); }
+void oracle_op_coarsify(const oprob* P, double* Gc, const double* G) { o_coarsify(&P->g, Gc, G); }
+void oracle_op_reg(const oprob* P, double* q, const double* fb) { o_reg(P, q, fb); }
+void oracle_op_regT(const oprob* P, double* fb, const double* q) { o_regT(P, fb, q); }
+void oracle_op_vort2flux(oprob* P, double* psi, double* q, double* G) { o_vort2flux(P, psi, q, G, P->g.mg); }
+/* nonlinear! for level lev with the coarse-level boundary condition taken from
+ * G[:, lev+1] (zero on the coarsest level), as get_trial_state! does. */
+void oracle_op_nonlinear(oprob* P, double* out, const double* q, const double* q0, const double* G, int lev)
+{
+    const og_grid* g = &P->g;
+    for (int k = 0; k < g->nbc; k++) P->Gbc[k] = 0.0;
+    if (lev < g->mg) o_get_bc(g, P->Gbc, G + (long)lev * g->nG);
+    o_nonlinear(P, out, q + (long)(lev - 1) * g->nq, q0 + (long)(lev - 1) * g->nq, G + (long)(lev - 1) * g->nG, P->Gbc, lev);
+}
+void oracle_op_B_times(oprob* P, double* x, const double* z) { o_B_times(P, x, z); }
+int oracle_invert(double* A, int n) { return o_invert(A, n); }

@@ -1,0 +1,98 @@
+"""Problem set-ups shared by the tests, the smoke check and the benchmark.
+
+Each builder returns the reference-style objects (grid, bodies, Re, dt,
+freestream) from the host mirror, plus the raw arrays the oracle takes.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+import ibpm
+
+
+def sc7(prob_like, t):
+    """Closures at t as the oracle's 7 scalars {Ux, Uy, incl, U, V, θ, θ̇}."""
+    fs = prob_like["freestream"]
+    s = [fs.get("Ux", lambda t: 0.0)(t), fs.get("Uy", lambda t: 0.0)(t), fs.get("inclination", lambda t: 0.0)(t),
+         0.0, 0.0, 0.0, 0.0]
+    m = prob_like.get("motion")
+    if m is not None:
+        s[3:] = [m.U(t), m.V(t), m.θ(t), m.θ̇(t)]
+    return np.array(s, dtype=np.float64)
+
+
+def case(name: str):
+    """Named configurations.  Sizes are small unless the name says otherwise."""
+    U1 = {"Ux": lambda t: 1.0}
+    if name == "tiny":  # 2 levels, 48 x 32, cylinder
+        grid = ibpm.make_grid(0.125, (-2.0, 4.0, -2.0, 2.0), mg=2)
+        bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.1, n=24)]
+        return dict(grid=grid, bodies=bodies, Re=40.0, dt=0.02, freestream=U1)
+    if name == "small3":  # 3 levels, 64 x 64 (M = 64: pure radix-8 path)
+        grid = ibpm.make_grid(1.0 / 16, (-1.5, 2.5, -2.0, 2.0), mg=3)
+        bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, n=40)]
+        return dict(grid=grid, bodies=bodies, Re=100.0, dt=0.01, freestream=U1)
+    if name == "mixed":  # radix 3/5 paths: nx = 120 (P = 60), ny = 40 (P = 20), 2 bodies, inclined stream
+        grid = ibpm.make_grid(0.05, (-2.0, 4.0, -1.0, 1.0), mg=2)
+        bodies = [ibpm.make_cylinder(0.25, grid.h, 0.0, 0.0, n=30), ibpm.make_plate(0.5, 0.3, grid.h, 1.0, 0.1, n=12)]
+        return dict(grid=grid, bodies=bodies, Re=200.0, dt=0.005,
+                    freestream={"Ux": lambda t: 1.0, "Uy": lambda t: 0.1, "inclination": lambda t: 0.05})
+    if name == "moving":  # MovingGrid pitching plate about its mid-chord at the origin
+        grid = ibpm.make_grid(1.0 / 32, (-1.0, 3.0, -1.0, 1.0), mg=3)
+        A, f = 15 * math.pi / 180, 0.2
+        mot = ibpm.MovingGrid(lambda t: 1.0, lambda t: 0.0, lambda t: A * math.sin(2 * math.pi * f * t),
+                              lambda t: 2 * math.pi * f * A * math.cos(2 * math.pi * f * t))
+        bodies = [ibpm.make_plate(1.0, 0.0, grid.h, -0.5, 0.0, motion=mot, n=33)]
+        return dict(grid=grid, bodies=bodies, Re=200.0, dt=0.004, freestream={}, motion=mot)
+    if name == "config1":  # Re 40 cylinder, 200 x 200, 3 levels
+        grid = ibpm.make_grid(0.02, (-1.0, 3.0, -2.0, 2.0), mg=3)
+        bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, n=157)]
+        return dict(grid=grid, bodies=bodies, Re=40.0, dt=0.004, freestream=U1)
+    if name == "steady_bl":  # benchmarks/eldredge/steady_bl.jl + config.jl
+        grid = ibpm.make_grid(0.02, (-1.5, 6.5, -2.0, 2.0), mg=5)
+        bodies = [ibpm.make_plate(1.0, 0.0, grid.h, 0.0, 0.0, n=51)]
+        return dict(grid=grid, bodies=bodies, Re=100.0, dt=1e-2,
+                    freestream={"Ux": lambda t: 1.0, "Uy": lambda t: 0.0, "inclination": lambda t: 0.0})
+    if name == "cyl40_single":  # benchmarks/cyl_40.jl
+        grid = ibpm.make_grid(0.02, (-1.0, 3.0, -2.0, 2.0), mg=1)
+        bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0)]
+        return dict(grid=grid, bodies=bodies, Re=40.0, dt=1e-2,
+                    freestream={"Ux": lambda t: 1.0, "Uy": lambda t: 0.0, "inclination": lambda t: 0.0})
+    if name == "config2":  # Re 100 cylinder, 800 x 400, 5 levels
+        grid = ibpm.make_grid(0.01, (-2.0, 6.0, -2.0, 2.0), mg=5)
+        bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, n=314)]
+        return dict(grid=grid, bodies=bodies, Re=100.0, dt=0.002, freestream=U1)
+    if name == "config4":  # Re 200 cylinder, 1024 x 1024, 5 levels, dx = 1/128
+        grid = ibpm.make_grid(1.0 / 128, (-2.0, 6.0, -4.0, 4.0), mg=5)
+        n = int(round(math.pi / grid.h))  # points spaced ~ dx on the unit-diameter circle
+        bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, n=n)]
+        return dict(grid=grid, bodies=bodies, Re=200.0, dt=0.001, freestream=U1)
+    raise KeyError(name)
+
+
+def oracle_bodies(c):
+    return [(b.xb, b.ds) for b in c["bodies"]]
+
+
+def boundary_of(grid):
+    return (-grid.offx, -grid.offx + grid.len, -grid.offy, -grid.offy + grid.ny * grid.h)
+
+
+def gaussian_vortex_gamma(grid, x0=1.5, y0=0.3, circ=0.01, width=0.1):
+    """Γ_l = hc^2 ω(x, y) on every level with ω a Gaussian vortex of the given
+    circulation (the seeded perturbation of the benchmark configurations)."""
+    nx, ny, mg, h = grid.nx, grid.ny, grid.mg, grid.h
+    G = np.zeros((grid.nΓ, mg), order="F")
+    for lev in range(1, mg + 1):
+        hc = h * 2 ** (lev - 1)
+        i = np.arange(1, nx)
+        j = np.arange(1, ny)
+        x = (i - nx / 2) * hc + (nx / 2) * h - grid.offx
+        y = (j - ny / 2) * hc + (ny / 2) * h - grid.offy
+        X, Y = np.meshgrid(x, y, indexing="ij")
+        r2 = (X - x0) ** 2 + (Y - y0) ** 2
+        om = circ / (math.pi * width ** 2) * np.exp(-r2 / width ** 2)
+        G[:, lev - 1] = (hc * hc * om).ravel(order="F")
+    return G

@@ -1,0 +1,260 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Tolerances (norm-relative, FP64): operators 1e-13 (same arithmetic up to the
+FFT and summation order); Binv 1e-9 (the inverse amplifies the 1e-16
+differences in B by cond(B) ~ 1e4-1e5); whole steps 1e-10, the bar of the
+drop-in.  Index work (body_idx, E' accumulation order, rot!) is bit-exact.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import ibpm
+import oracle as O
+from cases import boundary_of, case, gaussian_vortex_gamma, oracle_bodies, sc7
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+STEP_TOL = 1e-10
+OP_TOL = 1e-13
+
+
+def build(name, share_binv=False):
+    c = case(name)
+    g = c["grid"]
+    op = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], c["dt"], oracle_bodies(c),
+                         motion=1 if "motion" in c else 0)
+    gp = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"],
+                        Binv=op.binv() if share_binv else None)
+    return c, op, gp
+
+
+@pytest.fixture(scope="module", params=["tiny", "small3", "mixed", "moving"])
+def pair(request):
+    return build(request.param)
+
+
+def initial_states(c, op, gp, seed=7):
+    """Same initial condition on both sides: IBState(prob), Γ = Gaussian
+    vortex (+ seeded noise on level 1), then vort2flux! for q and ψ."""
+    g = c["grid"]
+    rng = np.random.default_rng(seed)
+    G = gaussian_vortex_gamma(g, x0=0.8, y0=0.2, circ=0.5, width=0.3)
+    G[:, 0] += 1e-3 * g.h * g.h * rng.standard_normal(g.nΓ)
+    S = op.new_state(sc7(c, 0.0))
+    psi, q, G2 = op.vort2flux(G)
+    S.G[:] = G2
+    S.psi[:] = psi
+    S.q[:] = q
+    st = ibpm.IBState(gp)
+    st.Γ = G
+    ibpm.vort2flux_(st, gp)
+    return S, st
+
+
+def compare_state(S, st, tol):
+    errs = {
+        "q": relerr(st.q, S.q), "q0": relerr(st.q0, S.q0), "Γ": relerr(st.Γ, S.G), "ψ": relerr(st.ψ, S.psi),
+        "nonlin1": relerr(st.nonlin[0], S.nl1), "nonlin2": relerr(st.nonlin[1], S.nl2),
+        "F̃b": relerr(st.F̃b, S.Fb), "CD": relerr(st.CD, S.CD),
+        # CL may vanish by symmetry: scale its error by the force magnitude
+        "CL": np.max(np.abs(st.CL - S.CL)) / max(np.max(np.abs(S.CD)), np.max(np.abs(S.CL)), 1e-300),
+    }
+    bad = {k: v for k, v in errs.items() if not v <= tol}
+    assert not bad, f"fields beyond {tol}: {bad}"
+    return errs
+
+
+# ---------------------------------------------------------------------------
+# setup products
+# ---------------------------------------------------------------------------
+def test_body_idx_bit_exact(pair):
+    c, op, gp = pair
+    assert np.array_equal(gp.body_idx, op.body_idx())
+
+
+def test_binv(pair):
+    c, op, gp = pair
+    assert relerr(gp.Binv, op.binv()) < 1e-9
+    assert gp.info.b_asymmetry < 1e-3
+
+
+# ---------------------------------------------------------------------------
+# operators
+# ---------------------------------------------------------------------------
+def _op(gp, name, *arrays, out_sizes):
+    import ctypes
+    from ibpm import _capi as C
+    outs = [np.zeros(n) for n in out_sizes]
+    args = [C.ptr(np.ascontiguousarray(a, dtype=np.float64)) for a in arrays] + [C.ptr(o) for o in outs]
+    gp.ctx.check(getattr(C.lib(), name)(gp.h, *args), name)
+    return outs
+
+
+def test_dst2(pair, rng):
+    c, op, gp = pair
+    g = c["grid"]
+    x = rng.standard_normal(g.nΓ)
+    (y,) = _op(gp, "ibpm_op_dst2", x, out_sizes=[g.nΓ])
+    assert relerr(y, op.dst2(x)) < OP_TOL
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_lapinv(pair, rng, kind):
+    import ctypes
+    from ibpm import _capi as C
+    c, op, gp = pair
+    g = c["grid"]
+    for lev in range(1, g.mg + 1):
+        b = rng.standard_normal(g.nΓ)
+        x = np.zeros(g.nΓ)
+        gp.ctx.check(C.lib().ibpm_op_lapinv(gp.h, kind, lev, C.ptr(b), C.ptr(x)))
+        assert relerr(x, op.lapinv(b, kind, lev)) < OP_TOL, (kind, lev)
+
+
+def test_nonlinear(pair, rng):
+    from ibpm import _capi as C
+    c, op, gp = pair
+    g = c["grid"]
+    q = np.asfortranarray(rng.standard_normal((g.nq, g.mg)))
+    q0 = np.asfortranarray(rng.standard_normal((g.nq, g.mg)))
+    G = np.asfortranarray(rng.standard_normal((g.nΓ, g.mg)))
+    for lev in range(1, g.mg + 1):
+        out = np.zeros(g.nΓ)
+        gp.ctx.check(C.lib().ibpm_op_nonlinear(gp.h, lev, C.ptr(q), C.ptr(q0), C.ptr(G), C.ptr(out)))
+        ref = op.nonlinear(q, q0, G, lev)
+        assert np.array_equal(out, ref), (lev, relerr(out, ref))
+
+
+def test_vort2flux(pair, rng):
+    from ibpm import _capi as C
+    c, op, gp = pair
+    g = c["grid"]
+    G = np.asfortranarray(rng.standard_normal((g.nΓ, g.mg)))
+    Gd = G.copy(order="F")
+    psi = np.zeros((g.nΓ, g.mg), order="F")
+    q = np.zeros((g.nq, g.mg), order="F")
+    gp.ctx.check(C.lib().ibpm_op_vort2flux(gp.h, C.ptr(Gd), C.ptr(psi), C.ptr(q)))
+    psi_o, q_o, G_o = op.vort2flux(G)
+    assert np.array_equal(Gd, G_o)  # coarsify! is exact arithmetic on the same inputs
+    assert relerr(psi, psi_o) < 1e-12
+    assert relerr(q, q_o) < 1e-12
+
+
+def test_E_and_Et(pair, rng):
+    c, op, gp = pair
+    g = c["grid"]
+    q = rng.standard_normal(g.nq)
+    (fb,) = _op(gp, "ibpm_op_regT", q, out_sizes=[gp.nf])
+    assert relerr(fb, op.regT(q)) < OP_TOL
+    f = rng.standard_normal(gp.nf)
+    (qq,) = _op(gp, "ibpm_op_reg", f, out_sizes=[g.nq])
+    assert np.array_equal(qq, op.reg(f))  # same accumulation order as reg!
+
+
+def test_rot_bit_exact(pair, rng):
+    c, op, gp = pair
+    g = c["grid"]
+    q = rng.standard_normal(g.nq)
+    (G,) = _op(gp, "ibpm_op_rot", q, out_sizes=[g.nΓ])
+    assert np.array_equal(G, op.rot(q))
+
+
+# ---------------------------------------------------------------------------
+# whole steps
+# ---------------------------------------------------------------------------
+def test_state_init_matches(pair):
+    c, op, gp = pair
+    S = op.new_state(sc7(c, 0.0))
+    st = ibpm.IBState(gp)
+    assert np.array_equal(st.q0, S.q0)
+    assert np.all(st.Γ == 0) and np.all(st.q == 0)
+
+
+@pytest.mark.parametrize("name", ["tiny", "small3", "mixed", "moving"])
+def test_one_step_shared_binv(name):
+    c, op, gp = build(name, share_binv=True)
+    S, st = initial_states(c, op, gp)
+    t = 0.0
+    op.advance(S, sc7(c, t))
+    ibpm.advance_(st, gp, t)
+    compare_state(S, st, 1e-12)
+
+
+@pytest.mark.parametrize("name,nsteps", [("tiny", 40), ("small3", 30), ("mixed", 30), ("moving", 40)])
+def test_many_steps(name, nsteps):
+    c, op, gp = build(name)
+    S, st = initial_states(c, op, gp)
+    ts = np.arange(nsteps) * c["dt"]
+    cd_o, cl_o = op.run(S, np.stack([sc7(c, t) for t in ts]))
+    cd, cl, done = ibpm.advance_n_(st, gp, ts)
+    assert done == nsteps
+    assert relerr(cd, cd_o) < STEP_TOL
+    # CL can sit near zero (symmetric flow): measure it against the force scale
+    assert np.max(np.abs(cl - cl_o)) < STEP_TOL * np.abs(cd_o).max()
+    compare_state(S, st, STEP_TOL)
+
+
+def test_advance_equals_advance_n():
+    c, op, gp = build("tiny")
+    _, st = initial_states(c, op, gp)
+    ts = np.arange(5) * c["dt"]
+    for t in ts:
+        ibpm.advance_(st, gp, t)
+    G1, q1, cd1 = st.Γ.copy(), st.q.copy(), st.CD.copy()
+    gp2 = ibpm.IBProblem(c["grid"], c["bodies"], c["dt"], c["Re"], freestream=c["freestream"], Binv=gp.Binv)
+    _, st2 = initial_states(c, op, gp2)
+    cd, cl, done = ibpm.advance_n_(st2, gp2, ts)
+    assert np.array_equal(st2.Γ, G1) and np.array_equal(st2.q, q1) and cd[-1, 0] == cd1[0]
+
+
+def test_config1_steps():
+    """BASELINE configs[0] (200 x 200, 3 levels, Re 40 cylinder), 20 steps."""
+    c, op, gp = build("config1")
+    assert relerr(gp.Binv, op.binv()) < 1e-9
+    S = op.new_state(sc7(c, 0.0))
+    st = ibpm.IBState(gp)
+    ts = np.arange(20) * c["dt"]
+    cd_o, cl_o = op.run(S, np.stack([sc7(c, t) for t in ts]))
+    cd, cl, _ = ibpm.advance_n_(st, gp, ts)
+    assert relerr(cd, cd_o) < STEP_TOL
+    assert np.max(np.abs(cl - cl_o)) < 1e-10 * np.abs(cd_o).max()
+    compare_state(S, st, STEP_TOL)
+
+
+# ---------------------------------------------------------------------------
+# full-size properties (1024 x 1024, 5 levels), no oracle run needed
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def big():
+    c = case("config4")
+    g = c["grid"]
+    nf = 2 * len(c["bodies"][0].ds)
+    gp = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"], Binv=np.eye(nf))
+    return c, gp
+
+
+def test_full_size_dst_involution(big, rng):
+    c, gp = big
+    g = c["grid"]
+    x = rng.standard_normal(g.nΓ)
+    (y,) = _op(gp, "ibpm_op_dst2", x, out_sizes=[g.nΓ])
+    (z,) = _op(gp, "ibpm_op_dst2", y, out_sizes=[g.nΓ])
+    assert relerr(z / (4.0 * g.nx * g.ny), x) < 1e-13
+
+
+def test_full_size_dinv_inverts_5pt(big, rng):
+    from ibpm import _capi as C
+    c, gp = big
+    g = c["grid"]
+    n1, n2 = g.nx - 1, g.ny - 1
+    psi = rng.standard_normal((n1, n2))
+    pad = np.zeros((n1 + 2, n2 + 2))
+    pad[1:-1, 1:-1] = psi
+    lap = 4 * psi - pad[2:, 1:-1] - pad[:-2, 1:-1] - pad[1:-1, 2:] - pad[1:-1, :-2]
+    x = np.zeros(g.nΓ)
+    b = np.ascontiguousarray(lap.ravel(order="F"))
+    gp.ctx.check(C.lib().ibpm_op_lapinv(gp.h, 0, 1, C.ptr(b), C.ptr(x)))
+    assert relerr(x, psi.ravel(order="F")) < 1e-11
