@@ -66,8 +66,12 @@ def main():
     state.Γ = gaussian_vortex_gamma(g)
     ibpm.vort2flux_(state, prob)
     dt = c["dt"]
+    from ibpm.autotune import autotune
+
+    tuned = {("ytile_batch", "ytile_single")[k]: v[0] for k, v in autotune(prob).items()}
     print(f"# setup {setup_s:.2f}s (B assembly on GPU {prob.info.setup_seconds:.2f}s, "
-          f"|B-B'|/|B| = {prob.info.b_asymmetry:.2e}); warmup {args.warmup} steps", file=sys.stderr, flush=True)
+          f"|B-B'|/|B| = {prob.info.b_asymmetry:.2e}); tiles {tuned}; warmup {args.warmup} steps",
+          file=sys.stderr, flush=True)
     ts_w = np.arange(args.warmup) * dt
     ibpm.advance_n_(state, prob, ts_w)
     ts = (args.warmup + np.arange(args.steps)) * dt
@@ -121,7 +125,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "FP64 advance!, cylinder Re=200, 1024x1024 cells (dx=1/128 on [-2,6]x[-4,4]), "
                                "5 levels, dt=1e-3, 402 body points, Gaussian-vortex start",
-                   "grid": [g.nx, g.ny], "levels": g.mg, "body_points": nb, "precision": "fp64"},
+                   "grid": [g.nx, g.ny], "levels": g.mg, "body_points": nb, "precision": "fp64",
+                   "tiles": tuned},
         "roofline": roof,
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
                      "achieved_GBs": round(step_bytes * sps / 1e9, 1),

@@ -49,7 +49,12 @@ struct ibpm_prob {
     std::vector<double> weight; // [(2k + c) * 169 + l + 13 m]
     std::vector<double> binv;   // nf x nf column-major
     double b_asym = 0.0, setup_s = 0.0;
-    int ytile = 8;
+    // q0 holds a per-level uniform value per component (inertial base flux,
+    // timestepping.jl:262-269): kernels then take the values, not the array
+    bool q0_uniform = true;
+    double q0vx = 0.0, q0vy = 0.0;  // q0 = (q0vx, q0vy) hc on every level
+    int ytile = 8;         // y-pass columns per workgroup, all-level launches
+    int ytile_single = 4;  // same, one-level launches (vort2flux chain)
 
     // device state (IBState) and work arrays
     double *q = nullptr, *q0 = nullptr, *G = nullptr, *Gs = nullptr, *psi = nullptr;
@@ -132,7 +137,9 @@ static bool make_plan(int M, DstPlan& pl, std::vector<double2>& tw, std::vector<
         pl.kind = 0;
         pl.P = P;
         pl.nst = (int)rad.size();
-        for (int k = 0; k < pl.nst; k++) pl.radix[k] = rad[k];
+        if (pl.nst > 8) return false;
+        pl.rad = 0;
+        for (int k = 0; k < pl.nst; k++) pl.rad |= (unsigned)rad[k] << (4 * k);
         tw.resize(P);
         for (int e = 0; e < P; e++) {
             long double a = -2.0L * PI_L * (long double)e / (long double)P;
@@ -255,6 +262,13 @@ static XArgs xargs(ibpm_prob* P)
     return a;
 }
 
+static void set_nl_q0(const ibpm_prob* P, NLArgs& na)
+{
+    na.q0_uniform = P->q0_uniform ? 1 : 0;
+    na.q0vx = P->q0vx;
+    na.q0vy = P->q0vy;
+}
+
 static YArgs yargs(ibpm_prob* P, int op, int lev0, int nlev, double* buf)
 {
     YArgs a{};
@@ -266,10 +280,7 @@ static YArgs yargs(ibpm_prob* P, int op, int lev0, int nlev, double* buf)
     a.cx = P->cx;
     a.cy = P->cy;
     a.dt = P->dt;
-    for (int l = 0; l < P->mg; l++) {
-        double hc = P->g.h * (double)(1L << l);
-        a.cdiv[l] = (2.0 * P->Re) * (hc * hc);
-    }
+    a.Re = P->Re;
     a.scale = 1.0 / (double)(4L * P->g.nx * P->g.ny);
     a.pl = P->ty.plan;
     a.g = P->g;
@@ -288,7 +299,7 @@ static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x,
     a.lev0 = lev;
     a.bcr = bcr;
     launch_xpass(bcr ? X_BC : X_PLAIN, a, s);
-    launch_ypass(yargs(P, op, lev, 1, P->T1 + (long)lev * P->g.nG), P->ytile, s);
+    launch_ypass(yargs(P, op, lev, 1, P->T1 + (long)lev * P->g.nG), P->ytile_single, s);
     XArgs c = xargs(P);
     c.src = P->T1 + (long)lev * P->g.nG;
     c.dst = x;
@@ -362,16 +373,14 @@ static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, dou
         BaseFluxArgs b{};
         b.q0 = P->q0; b.mg = L; b.Ux0 = Ux0; b.Uy0 = Uy0; b.Om = Om; b.offx = P->offx; b.offy = P->offy; b.g = g;
         launch_base_flux_moving(b, s);
+        P->q0_uniform = false;
     }
     // ---- get_trial_state! (timestepping.jl:86-129), all levels at once
     const int out = (P->nl1 != P->nl2) ? P->nl1 : 1 - P->nl2;
     NLArgs na{};
     na.q = P->q; na.q0 = P->q0; na.G = P->G; na.out = P->nl[out];
     na.nlev = L; na.lev0 = 0; na.mg = L; na.g = g;
-    for (int l = 0; l < L; l++) {
-        double hc = g.h * (double)(1L << l);
-        na.s_lev[l] = 1.0 / (4.0 * (hc * hc));
-    }
+    set_nl_q0(P, na);
     launch_nonlinear(na, s);
     XArgs a = xargs(P);
     a.src = P->G; a.dst = P->T1; a.nlev = L; a.lev0 = 0;
@@ -382,10 +391,8 @@ static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, dou
     t.n1 = P->nl[out]; t.n0 = P->nl[P->nl2]; t.gam = P->G;
     t.c1 = P->beta[0] * P->dt;
     t.c2 = P->beta[1] * P->dt;
-    for (int l = 0; l < L; l++) {
-        double hc = g.h * (double)(1L << l);
-        t.fac_lev[l] = 0.25 * P->dt / (P->Re * (hc * hc));
-    }
+    t.dt = P->dt;
+    t.Re = P->Re;
     launch_xpass(X_TRIAL, t, s);
     launch_ypass(yargs(P, Y_AINV, 0, L, P->T2), P->ytile, s);
     XArgs c = xargs(P);
@@ -477,6 +484,8 @@ static int zero_state(ibpm_prob* P)
     launch_fill(P->CD, P->nbodies, 0.0, s);
     launch_fill(P->CL, P->nbodies, 0.0, s);
     P->nl1 = P->nl2 = 0;
+    P->q0_uniform = true;
+    P->q0vx = P->q0vy = 0.0;
     return IBPM_OK;
 }
 
@@ -724,7 +733,10 @@ static double* field_ptr(ibpm_prob* P, int field, size_t* n, bool for_upload)
     const size_t nq = P->g.nq * P->mg, nG = P->g.nG * P->mg;
     switch (field) {
     case IBPM_FIELD_Q: *n = nq; return P->q;
-    case IBPM_FIELD_Q0: *n = nq; return P->q0;
+    case IBPM_FIELD_Q0:
+        *n = nq;
+        if (for_upload) P->q0_uniform = false;
+        return P->q0;
     case IBPM_FIELD_GAMMA: *n = nG; return P->G;
     case IBPM_FIELD_PSI: *n = nG; return P->psi;
     case IBPM_FIELD_NONLIN1:
@@ -779,11 +791,16 @@ int ibpm_base_flux(ibpm_prob* P, const ibpm_step_scalars* s)
         BaseFluxArgs b{};
         b.q0 = P->q0; b.mg = P->mg; b.Ux0 = Ux0; b.Uy0 = Uy0; b.Om = Om; b.offx = P->offx; b.offy = P->offy; b.g = P->g;
         launch_base_flux_moving(b, P->stream);
+        P->q0_uniform = false;
     } else {
         double a = s->inclination;
         double vx = s->Ux * std::cos(a) - s->Uy * std::sin(a);
         double vy = s->Ux * std::sin(a) + s->Uy * std::cos(a);
         launch_base_flux_inertial(P->q0, P->mg, vx, vy, P->g, P->stream);
+        // the same values k_base_flux_inertial writes
+        P->q0_uniform = true;
+        P->q0vx = vx;
+        P->q0vy = vy;
     }
     HIPTRY(P->ctx, hipGetLastError());
     return IBPM_OK;
@@ -945,10 +962,6 @@ int ibpm_op_nonlinear(ibpm_prob* P, int lev, const double* q, const double* q0, 
     if (!dq || !dq0 || !dG || !dout) return fail(P->ctx, IBPM_E_HIP, "alloc");
     NLArgs na{};
     na.q = dq; na.q0 = dq0; na.G = dG; na.out = dout; na.nlev = 1; na.lev0 = lev - 1; na.mg = P->mg; na.g = P->g;
-    for (int l = 0; l < P->mg; l++) {
-        double hc = P->g.h * (double)(1L << l);
-        na.s_lev[l] = 1.0 / (4.0 * (hc * hc));
-    }
     launch_nonlinear(na, P->stream);
     return get_back(P, out, dout, P->g.nG);
 }
@@ -1002,6 +1015,28 @@ int ibpm_op_rot(ibpm_prob* P, const double* q, double* G)
     return get_back(P, G, dG, P->g.nG);
 }
 
+int ibpm_set_option(ibpm_prob* P, int option, int value)
+{
+    if (!P) return IBPM_E_ARG;
+    if (option == IBPM_OPT_YTILE_BATCH || option == IBPM_OPT_YTILE_SINGLE) {
+        if (value != 4 && value != 8) return fail(P->ctx, IBPM_E_ARG, "y tile must be 4 or 8");
+        if (ypass_lds_bytes(P->ty.plan, P->g.n2, value) > 160 * 1024)
+            return fail(P->ctx, IBPM_E_ARG, "y tile does not fit in LDS for this ny");
+        (option == IBPM_OPT_YTILE_BATCH ? P->ytile : P->ytile_single) = value;
+        return IBPM_OK;
+    }
+    return fail(P->ctx, IBPM_E_ARG, "unknown option");
+}
+
+int ibpm_get_option(ibpm_prob* P, int option, int* value)
+{
+    if (!P || !value) return IBPM_E_ARG;
+    if (option == IBPM_OPT_YTILE_BATCH) *value = P->ytile;
+    else if (option == IBPM_OPT_YTILE_SINGLE) *value = P->ytile_single;
+    else return fail(P->ctx, IBPM_E_ARG, "unknown option");
+    return IBPM_OK;
+}
+
 int ibpm_time_kernel(ibpm_prob* P, int which, int reps, double* ms_avg)
 {
     if (!P || !ms_avg || reps < 1) return IBPM_E_ARG;
@@ -1019,18 +1054,26 @@ int ibpm_time_kernel(ibpm_prob* P, int which, int reps, double* ms_avg)
         } else if (which == 2) {
             NLArgs na{};
             na.q = P->q; na.q0 = P->q0; na.G = P->G; na.out = P->T2; na.nlev = L; na.lev0 = 0; na.mg = L; na.g = P->g;
-            for (int l = 0; l < L; l++) {
-                double hc = P->g.h * (double)(1L << l);
-                na.s_lev[l] = 1.0 / (4.0 * (hc * hc));
-            }
+            set_nl_q0(P, na);
             launch_nonlinear(na, P->stream);
+        } else if (which == 4) {
+            XArgs a = xargs(P);
+            a.src = P->G; a.dst = P->T1; a.nlev = 1; a.lev0 = 0;
+            launch_xpass(X_PLAIN, a, P->stream);
+        } else if (which == 5) {
+            launch_ypass(yargs(P, Y_DINV, 0, 1, P->T1), P->ytile_single, P->stream);
+        } else if (which == 6) {
+            CurlArgs ca{};
+            ca.psi = P->psi; ca.q = P->T2; ca.qstride = 0; ca.nlev = 1; ca.lev0 = 0; ca.mg = L; ca.g = P->g;
+            launch_curl(ca, P->stream);
+        } else if (which == 7) {
+            launch_copy(P->T2, P->T1, P->g.nG * L, P->stream);
         }
     };
-    if (which == 3) {
-        // full steps are timed by the caller through ibpm_advance_n
+    if (which < 0 || which > 7 || which == 3) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        return fail(P->ctx, IBPM_E_ARG, "use ibpm_advance_n to time whole steps");
+        return fail(P->ctx, IBPM_E_ARG, "unknown kernel id (whole steps are timed through ibpm_advance_n)");
     }
     one();
     HIPTRY(P->ctx, hipEventRecord(e0, P->stream));

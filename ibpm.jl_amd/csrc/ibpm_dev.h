@@ -38,11 +38,16 @@ struct DstPlan {
     int P;
     int kind;
     int nst;
-    int radix[16];
+    unsigned rad;       // radix of stage k in bits 4k..4k+3 (at most 8 stages)
     const double2* tw;  // exp(-2 pi i e / P), e < P
     const double* sn;   // kind 0: sin(pi j / M), j <= M/2;  kind 1: sin(pi t / M), t < 2M
     const double2* wm;  // exp(-2 pi i k / M), k < P
 };
+
+__host__ __device__ inline int plan_radix(const DstPlan& pl, int k) { return (int)((pl.rad >> (4 * k)) & 15u); }
+
+// grid spacing of level lev (0-based): h 2^lev, exact
+__host__ __device__ inline double level_hc(double h, int lev) { return h * (double)(1L << lev); }
 
 // ---------------------------------------------------------------------------
 // wave-level synchronisation of LDS traffic (all lanes of one wavefront)
@@ -137,8 +142,23 @@ __device__ __forceinline__ void dft8(double2* v)
     v[1] = o[0]; v[3] = o[1]; v[5] = o[2]; v[7] = o[3];
 }
 
-// One in-place Stockham stage of radix R over P complex points in LDS `z`,
-// executed by one wavefront.  Ns = product of the radices of earlier stages.
+// The complex work array of the FFT is padded by one element per 8: element e
+// lives at pz(e).  Stockham writes of the first stages (stride R or R*Ns
+// complex) then fall in distinct LDS banks instead of one.
+__device__ __forceinline__ int pz(int e) { return e + (e >> 3); }
+
+template <int R>
+__device__ __forceinline__ void dft_r(double2* v)
+{
+    if constexpr (R == 2) dft2(v);
+    else if constexpr (R == 3) dft3(v);
+    else if constexpr (R == 4) dft4(v);
+    else if constexpr (R == 5) dft5(v);
+    else if constexpr (R == 8) dft8(v);
+}
+
+// One in-place Stockham stage of radix R over P complex points in LDS `z`
+// (padded), executed by one wavefront.  Ns = product of the earlier radices.
 template <int R>
 __device__ __forceinline__ void fft_stage(double2* z, int P, int Ns, const double2* __restrict__ tw, int lane)
 {
@@ -151,7 +171,7 @@ __device__ __forceinline__ void fft_stage(double2* z, int P, int Ns, const doubl
         int j = lane + WAVE * t;
         if (j < nb) {
 #pragma unroll
-            for (int r = 0; r < R; r++) v[t][r] = z[j + r * nb];
+            for (int r = 0; r < R; r++) v[t][r] = z[pz(j + r * nb)];
         }
     }
     wave_sync();
@@ -160,32 +180,96 @@ __device__ __forceinline__ void fft_stage(double2* z, int P, int Ns, const doubl
         int j = lane + WAVE * t;
         if (j < nb) {
             int k = j % Ns;
-            if (Ns > 1) {
 #pragma unroll
-                for (int r = 1; r < R; r++) v[t][r] = cmul(v[t][r], tw[k * r * tws]);
-            }
-            if constexpr (R == 2) dft2(v[t]);
-            else if constexpr (R == 3) dft3(v[t]);
-            else if constexpr (R == 4) dft4(v[t]);
-            else if constexpr (R == 5) dft5(v[t]);
-            else dft8(v[t]);
+            for (int r = 1; r < R; r++) v[t][r] = cmul(v[t][r], tw[k * r * tws]);
+            dft_r<R>(v[t]);
             int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; r++) z[base + r * Ns] = v[t][r];
+            for (int r = 0; r < R; r++) z[pz(base + r * Ns)] = v[t][r];
         }
     }
     wave_sync();
 }
 
-// Wave-level inclusive prefix sum of one double per lane.
+// y_j = sin(pi j/M)(x_j + x_{M-j}) + (x_j - x_{M-j})/2 with x_j = s[j-1],
+// x_0 = x_M = 0 and sin(pi j/M) = sn[min(j, M-j)]
+__device__ __forceinline__ double pre_y(const double* s, int M, const double* sn, int j)
+{
+    if (j == 0) return 0.0;
+    const int jm = M - j;
+    const double xa = s[j - 1];
+    const double xb = (jm >= 1) ? s[jm - 1] : 0.0;
+    const double sj = sn[j < jm ? j : jm];
+    return fma(sj, xa + xb, 0.5 * (xa - xb));
+}
+
+// First Stockham stage (Ns = 1, no twiddles) fed directly by the
+// pre-processing: butterfly j takes z_m = y_{2m} + i y_{2m+1} for
+// m = j + r P/R.  All reads of the real input s precede all writes of z.
+template <int R>
+__device__ __forceinline__ void fft_first_stage(double* s, double2* z, int P, int M, const double* __restrict__ sn,
+                                                int lane)
+{
+    constexpr int RR = R > 1 ? R : 1;
+    constexpr int TMAX = (MAXP / RR + WAVE - 1) / WAVE;
+    const int nb = P / RR;
+    double2 v[TMAX][RR];
+#pragma unroll
+    for (int t = 0; t < TMAX; t++) {
+        int j = lane + WAVE * t;
+        if (j < nb) {
+#pragma unroll
+            for (int r = 0; r < RR; r++) {
+                int m = j + r * nb;
+                v[t][r] = make_double2(pre_y(s, M, sn, 2 * m), pre_y(s, M, sn, 2 * m + 1));
+            }
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < TMAX; t++) {
+        int j = lane + WAVE * t;
+        if (j < nb) {
+            dft_r<RR>(v[t]);
+#pragma unroll
+            for (int r = 0; r < RR; r++) z[pz(j * RR + r)] = v[t][r];
+        }
+    }
+    wave_sync();
+}
+
+// DPP move of a double (two 32-bit halves); lanes the control does not feed
+// read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xf, true);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// Wave-level inclusive prefix sum of one double per lane with DPP: shifts by
+// 1, 2, 4, 8 within rows of 16 lanes, then the row-15 and row-31 broadcasts
+// carry the partial sums across rows.  No LDS traffic.
 __device__ __forceinline__ double wave_incl_scan(double v, int lane)
 {
-#pragma unroll
-    for (int off = 1; off < WAVE; off <<= 1) {
-        double o = __shfl_up(v, off, WAVE);
-        if (lane >= off) v += o;
-    }
+    (void)lane;
+    v += dpp_d<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_d<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_d<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_d<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_d<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_d<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
     return v;
+}
+
+// value of lane 63 in every lane
+__device__ __forceinline__ double wave_last(double v)
+{
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), WAVE - 1);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), WAVE - 1);
+    return __hiloint2double(hi, lo);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -208,9 +292,69 @@ __device__ __forceinline__ double wave_sum(double v)
 // ---------------------------------------------------------------------------
 constexpr int MAX_PAIRS = (MAXP + WAVE - 1) / WAVE;
 
-__device__ __forceinline__ void dst1_line(double* s, const DstPlan& pl, int lane)
+// The plan's tables staged in LDS: the FFT stages, the pre-processing and the
+// unpacking read them at every butterfly, so they must not cost an L2 round
+// trip each time.
+struct DstTab {
+    const double2* tw;
+    const double* sn;
+    const double2* wm;
+};
+
+// doubles taken by the LDS copy of a plan's tables (kept even: 16 B aligned)
+__host__ __device__ inline int dst_tab_doubles(const DstPlan& pl)
 {
-    const int M = pl.M;
+    return pl.kind == 0 ? 4 * pl.P + ((pl.P + 2) & ~1) : 2 * pl.M;
+}
+
+// doubles of the LDS slot one line is transformed in: the real line (M-1
+// values), the padded complex work array (pz(P-1) + 1 elements) and the
+// linear unpacking arrays (2P = M values) all live in it; +2 staggers the
+// banks of neighbouring slots.
+__host__ __device__ inline int dst_slot_doubles(const DstPlan& pl)
+{
+    int n = pl.M;
+    if (pl.kind == 0) {
+        int c = 2 * (pl.P + (pl.P - 1) / 8);
+        if (c > n) n = c;
+    }
+    return ((n + 1) & ~1) + 2;
+}
+
+// Cooperative copy of the tables into `lds` by threads tid = 0..nthreads-1;
+// the caller synchronises the workgroup before use.
+__device__ inline DstTab load_dst_tab(double* lds, const DstPlan& pl, int tid, int nthreads)
+{
+    DstTab t;
+    if (pl.kind == 0) {
+        double2* tw = reinterpret_cast<double2*>(lds);
+        double2* wm = tw + pl.P;
+        double* sn = reinterpret_cast<double*>(wm + pl.P);
+        for (int i = tid; i < pl.P; i += nthreads) {
+            tw[i] = pl.tw[i];
+            wm[i] = pl.wm[i];
+        }
+        for (int i = tid; i <= pl.P; i += nthreads) sn[i] = pl.sn[i];
+        t.tw = tw;
+        t.wm = wm;
+        t.sn = sn;
+    } else {
+        for (int i = tid; i < 2 * pl.M; i += nthreads) lds[i] = pl.sn[i];
+        t.tw = nullptr;
+        t.wm = nullptr;
+        t.sn = lds;
+    }
+    return t;
+}
+
+__device__ __forceinline__ void dst1_line(double* s, const DstPlan& pl, const DstTab& tb, int lane)
+{
+    // Opaque copies of the lane and sizes: the transform's index arithmetic is
+    // loop-invariant where the caller loops over lines, and hoisting all of it
+    // out of the loop would keep hundreds of addresses live (register spills).
+    asm volatile("" : "+v"(lane));
+    int M = pl.M;
+    asm volatile("" : "+s"(M));
     if (pl.kind == 1) {
         // direct sum: Y_{k} (k = 1..M-1) = 2 sum_j x_j sin(pi j k / M)
         constexpr int OUTMAX = (MAXM + WAVE - 1) / WAVE;
@@ -225,7 +369,7 @@ __device__ __forceinline__ void dst1_line(double* s, const DstPlan& pl, int lane
                 for (int j = 1; j <= M - 1; j++) {
                     e += k;
                     if (e >= 2 * M) e -= 2 * M;
-                    a = fma(s[j - 1], pl.sn[e], a);
+                    a = fma(s[j - 1], tb.sn[e], a);
                 }
                 acc[t] = 2.0 * a;
             }
@@ -239,48 +383,33 @@ __device__ __forceinline__ void dst1_line(double* s, const DstPlan& pl, int lane
         wave_sync();
         return;
     }
-    const int P = pl.P;
-    // --- pre-processing: pairs (j, M-j), j = 1..M/2, into y_j stored at s[j]
-    {
-        double xa[MAX_PAIRS], xb[MAX_PAIRS];
-#pragma unroll
-        for (int t = 0; t < MAX_PAIRS; t++) {
-            int j = lane + 1 + WAVE * t;
-            if (j <= P) {
-                xa[t] = s[j - 1];
-                xb[t] = (j < P) ? s[M - j - 1] : 0.0;
-            }
-        }
-        wave_sync();
-        if (lane == 0) s[0] = 0.0;  // y_0 = 0  (s[0] already consumed above)
-#pragma unroll
-        for (int t = 0; t < MAX_PAIRS; t++) {
-            int j = lane + 1 + WAVE * t;
-            if (j < P) {
-                double sj = pl.sn[j];
-                double a = xa[t] + xb[t];
-                double d = 0.5 * (xa[t] - xb[t]);
-                s[j] = fma(sj, a, d);
-                s[M - j] = fma(sj, a, -d);
-            } else if (j == P) {
-                s[P] = 2.0 * xa[t];  // sin(pi/2) (x + x)
-            }
-        }
-        wave_sync();
-    }
-    // --- complex FFT of P points in place (Stockham, natural order out)
+    int P = pl.P;
+    asm volatile("" : "+s"(P));
     double2* z = reinterpret_cast<double2*>(s);
-    int Ns = 1;
-    for (int st = 0; st < pl.nst; st++) {
-        int R = pl.radix[st];
-        if (R == 8) fft_stage<8>(z, P, Ns, pl.tw, lane);
-        else if (R == 4) fft_stage<4>(z, P, Ns, pl.tw, lane);
-        else if (R == 2) fft_stage<2>(z, P, Ns, pl.tw, lane);
-        else if (R == 5) fft_stage<5>(z, P, Ns, pl.tw, lane);
-        else fft_stage<3>(z, P, Ns, pl.tw, lane);
+    // --- pre-processing fused into the first FFT stage (reads X, writes the
+    //     stage output into the padded complex array)
+    {
+        const int R = plan_radix(pl, 0);
+        if (R == 8) fft_first_stage<8>(s, z, P, M, tb.sn, lane);
+        else if (R == 4) fft_first_stage<4>(s, z, P, M, tb.sn, lane);
+        else if (R == 2) fft_first_stage<2>(s, z, P, M, tb.sn, lane);
+        else if (R == 5) fft_first_stage<5>(s, z, P, M, tb.sn, lane);
+        else if (R == 3) fft_first_stage<3>(s, z, P, M, tb.sn, lane);
+        else fft_first_stage<1>(s, z, P, M, tb.sn, lane);  // P = 1: no stage
+    }
+    // --- remaining Stockham stages (natural order out)
+    int Ns = pl.nst > 0 ? plan_radix(pl, 0) : 1;
+    for (int st = 1; st < pl.nst; st++) {
+        int R = plan_radix(pl, st);
+        if (R == 8) fft_stage<8>(z, P, Ns, tb.tw, lane);
+        else if (R == 4) fft_stage<4>(z, P, Ns, tb.tw, lane);
+        else if (R == 2) fft_stage<2>(z, P, Ns, tb.tw, lane);
+        else if (R == 5) fft_stage<5>(z, P, Ns, tb.tw, lane);
+        else fft_stage<3>(z, P, Ns, tb.tw, lane);
         Ns *= R;
     }
-    // --- real-DFT unpacking: for pairs (k, P-k), k = 0..P/2
+    // --- real-DFT unpacking for pairs (k, P-k), k = 0..P/2, into linear arrays
+    //     R'_k at s[k] (R'_0 = R_0, R'_k = 2 R_k) and Y(2k) = 2 I_k at s[P + k]
     {
         constexpr int KMAX = (MAXP / 2 + 1 + WAVE - 1) / WAVE;
         double2 za[KMAX], zb[KMAX];
@@ -290,8 +419,8 @@ __device__ __forceinline__ void dst1_line(double* s, const DstPlan& pl, int lane
             int k = lane + WAVE * t;
             if (k < nk) {
                 int k2 = (P - k) % P;
-                za[t] = z[k];
-                zb[t] = z[k2];
+                za[t] = z[pz(k)];
+                zb[t] = z[pz(k2)];
             }
         }
         wave_sync();
@@ -305,36 +434,238 @@ __device__ __forceinline__ void dst1_line(double* s, const DstPlan& pl, int lane
                 {
                     double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
                     double2 O = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
-                    double2 F = cadd(E, cmul(pl.wm[k], O));
-                    s[2 * k] = (k == 0) ? F.x : 2.0 * F.x;      // R'_k at the slot of Y(2k+1)
-                    if (k > 0) s[2 * k - 1] = -2.0 * F.y;       // Y(2k) = 2 I_k
+                    double2 F = cadd(E, cmul(tb.wm[k], O));
+                    s[k] = (k == 0) ? F.x : 2.0 * F.x;
+                    if (k > 0) s[P + k] = -2.0 * F.y;
                 }
                 if (k2 != k) {
                     double2 E = make_double2(0.5 * (b.x + a.x), 0.5 * (b.y - a.y));
                     double2 O = make_double2(0.5 * (b.y + a.y), -0.5 * (b.x - a.x));
-                    double2 F = cadd(E, cmul(pl.wm[k2], O));
-                    s[2 * k2] = 2.0 * F.x;
-                    s[2 * k2 - 1] = -2.0 * F.y;
+                    double2 F = cadd(E, cmul(tb.wm[k2], O));
+                    s[k2] = 2.0 * F.x;
+                    s[P + k2] = -2.0 * F.y;
                 }
             }
         }
         wave_sync();
     }
-    // --- prefix sum over s[0], s[2], ..., s[2(P-1)]  ->  Y(1), Y(3), ...
+    // --- prefix sum of R' in rounds of 64 consecutive k: Y(2k+1) = sum_{m<=k} R'_m;
+    //     then both output halves are written in natural order, s[n] = Y(n+1)
     {
-        const int C = (P + WAVE - 1) / WAVE;
-        const int k0 = lane * C;
-        const int k1 = min(k0 + C, P);
-        double part = 0.0;
-        for (int k = k0; k < k1; k++) part += s[2 * k];
-        double incl = wave_incl_scan(part, lane);
-        double run = incl - part;
-        for (int k = k0; k < k1; k++) {
-            run += s[2 * k];
-            s[2 * k] = run;
+        constexpr int NR = (MAXP + WAVE - 1) / WAVE;
+        double sc[NR], ev[NR];
+        double carry = 0.0;
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+            int k = WAVE * t + lane;
+            if (WAVE * t < P) {
+                double v = (k < P) ? s[k] : 0.0;
+                double incl = wave_incl_scan(v, lane);
+                sc[t] = carry + incl;
+                carry += wave_last(incl);
+                ev[t] = (k >= 1 && k < P) ? s[P + k] : 0.0;
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+            int k = WAVE * t + lane;
+            if (k < P) {
+                s[2 * k] = sc[t];
+                if (k >= 1) s[2 * k - 1] = ev[t];
+            }
         }
         wave_sync();
     }
+}
+
+// ---------------------------------------------------------------------------
+// The same transform with P = M/2 known at compile time: the radix plan, the
+// trip counts, the Stockham index arithmetic and the padding all fold into
+// constants, which removes most of the per-lane integer work of dst1_line.
+// ---------------------------------------------------------------------------
+struct RadPlan {
+    int n;
+    int r[8];
+};
+
+// radices of P (8s first, then 4 or 2, then 5s and 3s), as the host plan
+constexpr RadPlan make_radplan(int P)
+{
+    RadPlan rp{0, {0, 0, 0, 0, 0, 0, 0, 0}};
+    int m = P, e2 = 0, e3 = 0, e5 = 0;
+    while (m % 2 == 0) { m /= 2; e2++; }
+    while (m % 3 == 0) { m /= 3; e3++; }
+    while (m % 5 == 0) { m /= 5; e5++; }
+    while (e2 >= 3) { rp.r[rp.n++] = 8; e2 -= 3; }
+    if (e2 == 2) rp.r[rp.n++] = 4;
+    if (e2 == 1) rp.r[rp.n++] = 2;
+    for (int k = 0; k < e5; k++) rp.r[rp.n++] = 5;
+    for (int k = 0; k < e3; k++) rp.r[rp.n++] = 3;
+    return rp;
+}
+
+template <int P, int R, int NS>
+__device__ __forceinline__ void fft_stage_t(double2* z, const double2* __restrict__ tw, int lane)
+{
+    constexpr int NB = P / R;
+    constexpr int TMAX = (NB + WAVE - 1) / WAVE;
+    constexpr int TWS = P / (NS * R);
+    double2 v[TMAX][R];
+#pragma unroll
+    for (int t = 0; t < TMAX; t++) {
+        const int j = lane + WAVE * t;
+        if (NB % WAVE == 0 || j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[t][r] = z[pz(j + r * NB)];
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < TMAX; t++) {
+        const int j = lane + WAVE * t;
+        if (NB % WAVE == 0 || j < NB) {
+            const int k = j % NS;
+#pragma unroll
+            for (int r = 1; r < R; r++) v[t][r] = cmul(v[t][r], tw[k * r * TWS]);
+            dft_r<R>(v[t]);
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) z[pz(base + r * NS)] = v[t][r];
+        }
+    }
+    wave_sync();
+}
+
+template <int P, int ST, int NS>
+__device__ __forceinline__ void fft_stages_t(double2* z, const double2* __restrict__ tw, int lane)
+{
+    constexpr RadPlan rp = make_radplan(P);
+    if constexpr (ST < rp.n) {
+        constexpr int R = rp.r[ST];
+        fft_stage_t<P, R, NS>(z, tw, lane);
+        fft_stages_t<P, ST + 1, NS * R>(z, tw, lane);
+    }
+}
+
+template <int P>
+__device__ __forceinline__ double pre_y_t(const double* s, const double* sn, int j)
+{
+    constexpr int M = 2 * P;
+    const int jm = M - j;
+    const double xa = (j >= 1) ? s[j - 1] : 0.0;
+    const double xb = (jm >= 1 && jm <= M - 1) ? s[jm - 1] : 0.0;
+    const double sj = sn[j < jm ? j : jm];
+    return (j == 0) ? 0.0 : fma(sj, xa + xb, 0.5 * (xa - xb));
+}
+
+template <int P>
+__device__ __forceinline__ void dst1_line_t(double* s, const DstTab& tb, int lane)
+{
+    asm volatile("" : "+v"(lane));
+    constexpr RadPlan rp = make_radplan(P);
+    constexpr int R0 = rp.n > 0 ? rp.r[0] : 1;
+    double2* z = reinterpret_cast<double2*>(s);
+    // --- pre-processing fused into stage 1 (Ns = 1, no twiddles)
+    {
+        constexpr int NB = P / R0;
+        constexpr int TMAX = (NB + WAVE - 1) / WAVE;
+        double2 v[TMAX][R0];
+#pragma unroll
+        for (int t = 0; t < TMAX; t++) {
+            const int j = lane + WAVE * t;
+            if (NB % WAVE == 0 || j < NB) {
+#pragma unroll
+                for (int r = 0; r < R0; r++) {
+                    const int m = j + r * NB;
+                    v[t][r] = make_double2(pre_y_t<P>(s, tb.sn, 2 * m), pre_y_t<P>(s, tb.sn, 2 * m + 1));
+                }
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < TMAX; t++) {
+            const int j = lane + WAVE * t;
+            if (NB % WAVE == 0 || j < NB) {
+                dft_r<R0>(v[t]);
+#pragma unroll
+                for (int r = 0; r < R0; r++) z[pz(j * R0 + r)] = v[t][r];
+            }
+        }
+        wave_sync();
+    }
+    fft_stages_t<P, 1, R0>(z, tb.tw, lane);
+    // --- real-DFT unpacking into R'_k at s[k] and Y(2k) at s[P + k]
+    {
+        constexpr int NK = P / 2 + 1;
+        constexpr int KMAX = (NK + WAVE - 1) / WAVE;
+        double2 za[KMAX], zb[KMAX];
+#pragma unroll
+        for (int t = 0; t < KMAX; t++) {
+            const int k = lane + WAVE * t;
+            if (k < NK) {
+                const int k2 = (P - k) % P;
+                za[t] = z[pz(k)];
+                zb[t] = z[pz(k2)];
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < KMAX; t++) {
+            const int k = lane + WAVE * t;
+            if (k < NK) {
+                const int k2 = (P - k) % P;
+                const double2 a = za[t], b = zb[t];
+                {
+                    double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+                    double2 O = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
+                    double2 F = cadd(E, cmul(tb.wm[k], O));
+                    s[k] = (k == 0) ? F.x : 2.0 * F.x;
+                    if (k > 0) s[P + k] = -2.0 * F.y;
+                }
+                if (k2 != k) {
+                    double2 E = make_double2(0.5 * (b.x + a.x), 0.5 * (b.y - a.y));
+                    double2 O = make_double2(0.5 * (b.y + a.y), -0.5 * (b.x - a.x));
+                    double2 F = cadd(E, cmul(tb.wm[k2], O));
+                    s[k2] = 2.0 * F.x;
+                    s[P + k2] = -2.0 * F.y;
+                }
+            }
+        }
+        wave_sync();
+    }
+    // --- scan of R' in rounds of 64, natural-order output s[n] = Y(n+1)
+    {
+        constexpr int NR = (P + WAVE - 1) / WAVE;
+        double sc[NR], ev[NR];
+        double carry = 0.0;
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+            const int k = WAVE * t + lane;
+            const double v = (P % WAVE == 0 || k < P) ? s[k] : 0.0;
+            const double incl = wave_incl_scan(v, lane);
+            sc[t] = carry + incl;
+            carry += wave_last(incl);
+            ev[t] = (k >= 1 && (P % WAVE == 0 || k < P)) ? s[P + k] : 0.0;
+        }
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+            const int k = WAVE * t + lane;
+            if (P % WAVE == 0 || k < P) {
+                s[2 * k] = sc[t];
+                if (k >= 1) s[2 * k - 1] = ev[t];
+            }
+        }
+        wave_sync();
+    }
+}
+
+// The transform lengths with a compile-time specialisation (P = M/2); any
+// other length runs dst1_line.
+__host__ __device__ inline bool dst_specialised(int P)
+{
+    return P == 512 || P == 400 || P == 256 || P == 200 || P == 128 || P == 100;
 }
 
 // ---------------------------------------------------------------------------

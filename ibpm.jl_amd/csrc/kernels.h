@@ -24,7 +24,7 @@ struct XArgs {
     const double* n0;
     const double* gam;    // state Γ (all levels) for rhsbc of level lev from lev+1
     double c1, c2;
-    double fac_lev[MAXLEV];  // 0.25 dt / (Re hc^2)
+    double dt, Re;        // rhsbc factor 0.25 dt / (Re hc^2) is formed per level in the kernel
     DstPlan pl;
     GridDev g;
 };
@@ -39,12 +39,14 @@ struct YArgs {
     const double* cx;   // cos(pi i / nx), i = 1..nx-1
     const double* cy;   // cos(pi j / ny), j = 1..ny-1
     double dt;
-    double cdiv[MAXLEV];  // (2 Re) hc^2 per level
+    double Re;            // divisor (2 Re) hc^2 is formed per level in the kernel
     double scale;         // 1 / (4 nx ny)
     DstPlan pl;
     GridDev g;
 };
 void launch_ypass(const YArgs& a, int tile, hipStream_t s);
+// LDS bytes one y-pass workgroup needs (tables, cos table, tile slots)
+size_t ypass_lds_bytes(const DstPlan& pl, int n2, int tile);
 
 // ---- field kernels ---------------------------------------------------------
 struct NLArgs {
@@ -53,7 +55,10 @@ struct NLArgs {
     const double* G;
     double* out;
     int nlev, lev0, mg;
-    double s_lev[MAXLEV];  // 1 / (4 hc^2)
+    // q0 uniform per level and component (inertial base flux): q0 = (vx, vy) hc,
+    // so the kernel need not stream q0
+    int q0_uniform;
+    double q0vx, q0vy;
     GridDev g;
 };
 void launch_nonlinear(const NLArgs& a, hipStream_t s);

@@ -13,6 +13,7 @@ namespace ibpm {
 // DST x-pass: one wavefront per x-line (fixed j) of one level.
 // ===========================================================================
 constexpr int XWAVES = 4;  // wavefronts per workgroup
+constexpr int XPL = (MAXM + WAVE - 1) / WAVE;  // line elements per lane
 
 // apply_bc! term of element (i, j) (1-based), mg-utils.jl:43-60: rows first
 // (j = 1 then j = ny-1), then columns (i = 1 then i = nx-1).
@@ -25,7 +26,15 @@ __device__ __forceinline__ double add_bc(double v, const double* __restrict__ r,
     return v;
 }
 
-template <int MODE>
+// one line transform: compile-time length PT > 0, else the plan's runtime length
+template <int PT>
+__device__ __forceinline__ void dst_line(double* s, const DstPlan& pl, const DstTab& tb, int lane)
+{
+    if constexpr (PT > 0) dst1_line_t<PT>(s, tb, lane);
+    else dst1_line(s, pl, tb, lane);
+}
+
+template <int MODE, int PT>
 __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
 {
     extern __shared__ double lds[];
@@ -34,18 +43,32 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
     const int n1 = a.g.n1, n2 = a.g.n2;
     const int line = blockIdx.x * XWAVES + w;  // 0-based j - 1
     const int b = blockIdx.y;
+    const int ntab = dst_tab_doubles(a.pl);
+    const DstTab tb = load_dst_tab(lds, a.pl, threadIdx.x, XWAVES * WAVE);
+    __syncthreads();
     if (line >= n2) return;
-    const int ld = a.pl.M + 2;
-    double* s = lds + w * ld;
+    const int ld = dst_slot_doubles(a.pl);
+    double* s = lds + ntab + w * ld;
     const int j = line + 1;
     const double* src = a.src + b * a.lstride + (long)line * n1;
-    for (int n = lane; n < n1; n += WAVE) {
-        double v = src[n];
-        if constexpr (MODE == X_BC) v = add_bc(v, a.bcr, a.g, a.fac, n + 1, j);
-        s[n] = v;
+    // all loads of the line in flight before the first wait
+    double v[XPL];
+#pragma unroll
+    for (int t = 0; t < XPL; t++) {
+        int n = lane + WAVE * t;
+        if (n < n1) v[t] = src[n];
+    }
+#pragma unroll
+    for (int t = 0; t < XPL; t++) {
+        int n = lane + WAVE * t;
+        if (n < n1) {
+            double x = v[t];
+            if constexpr (MODE == X_BC) x = add_bc(x, a.bcr, a.g, a.fac, n + 1, j);
+            s[n] = x;
+        }
     }
     wave_sync();
-    dst1_line(s, a.pl, lane);
+    dst_line<PT>(s, a.pl, tb, lane);
     if constexpr (MODE == X_TRIAL) {
         // get_trial_state! (timestepping.jl:111-118): rhs = A Γ; rhs += (β1 dt) N1;
         // rhs += (β2 dt) N0; rhs += rhsbc, with rhsbc = apply_bc!(0, get_bc!(Γ[:, lev+1]), fac)
@@ -54,40 +77,95 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
         const double* n0p = a.n0 + b * a.lstride + (long)line * n1;
         const bool hasbc = lev < a.mg - 1;
         const double* gc = a.gam + (long)(lev + 1) * a.lstride;
-        const double fac = a.fac_lev[lev];
-        for (int n = lane; n < n1; n += WAVE) {
-            double rhs = s[n];
-            rhs += a.c1 * n1p[n];
-            rhs += a.c2 * n0p[n];
-            if (hasbc) {
-                const int i = n + 1;
-                if (j == 1 || j == a.g.ny - 1 || i == 1 || i == a.g.nx - 1) {
-                    double rb = 0.0;
-                    rb = add_bc(rb, gc, a.g, fac, i, j);
-                    rhs += rb;
+        const double hc = level_hc(a.g.h, lev);
+        const double fac = 0.25 * a.dt / (a.Re * (hc * hc));  // timestepping.jl:104
+        double u[XPL];
+#pragma unroll
+        for (int t = 0; t < XPL; t++) {
+            int n = lane + WAVE * t;
+            if (n < n1) { v[t] = n1p[n]; u[t] = n0p[n]; }
+        }
+#pragma unroll
+        for (int t = 0; t < XPL; t++) {
+            int n = lane + WAVE * t;
+            if (n < n1) {
+                double rhs = s[n];
+                rhs += a.c1 * v[t];
+                rhs += a.c2 * u[t];
+                if (hasbc) {
+                    const int i = n + 1;
+                    if (j == 1 || j == a.g.ny - 1 || i == 1 || i == a.g.nx - 1) {
+                        double rb = 0.0;
+                        rb = add_bc(rb, gc, a.g, fac, i, j);
+                        rhs += rb;
+                    }
                 }
+                s[n] = rhs;
             }
-            s[n] = rhs;
         }
         wave_sync();
-        dst1_line(s, a.pl, lane);
+        dst_line<PT>(s, a.pl, tb, lane);
     }
     double* dst = a.dst + b * a.lstride + (long)line * n1;
-    for (int n = lane; n < n1; n += WAVE) {
-        if constexpr (MODE == X_SUB) dst[n] = dst[n] - s[n];
-        else dst[n] = s[n];
+    if constexpr (MODE == X_SUB) {
+#pragma unroll
+        for (int t = 0; t < XPL; t++) {
+            int n = lane + WAVE * t;
+            if (n < n1) v[t] = dst[n];
+        }
+#pragma unroll
+        for (int t = 0; t < XPL; t++) {
+            int n = lane + WAVE * t;
+            if (n < n1) dst[n] = v[t] - s[n];
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < XPL; t++) {
+            int n = lane + WAVE * t;
+            if (n < n1) dst[n] = s[n];
+        }
+    }
+}
+
+// dynamic LDS above 64 KiB needs the attribute once per kernel instance
+template <typename K>
+static void allow_big_lds(K kern)
+{
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+template <int MODE, int PT>
+static void launch_x_t(const XArgs& a, hipStream_t s)
+{
+    static bool once = (allow_big_lds(k_xpass<MODE, PT>), true);
+    (void)once;
+    dim3 grid((a.g.n2 + XWAVES - 1) / XWAVES, a.nlev);
+    size_t lds = sizeof(double) * ((size_t)dst_tab_doubles(a.pl) + (size_t)XWAVES * dst_slot_doubles(a.pl));
+    hipLaunchKernelGGL((k_xpass<MODE, PT>), grid, dim3(XWAVES * WAVE), lds, s, a);
+}
+
+template <int MODE>
+static void launch_x_m(const XArgs& a, hipStream_t s)
+{
+    const int P = a.pl.kind == 0 ? a.pl.P : 0;
+    switch (P) {
+    case 512: launch_x_t<MODE, 512>(a, s); break;
+    case 400: launch_x_t<MODE, 400>(a, s); break;
+    case 256: launch_x_t<MODE, 256>(a, s); break;
+    case 200: launch_x_t<MODE, 200>(a, s); break;
+    case 128: launch_x_t<MODE, 128>(a, s); break;
+    case 100: launch_x_t<MODE, 100>(a, s); break;
+    default: launch_x_t<MODE, 0>(a, s); break;
     }
 }
 
 void launch_xpass(int mode, const XArgs& a, hipStream_t s)
 {
-    dim3 grid((a.g.n2 + XWAVES - 1) / XWAVES, a.nlev);
-    size_t lds = sizeof(double) * (size_t)XWAVES * (a.pl.M + 2);
     switch (mode) {
-    case X_PLAIN: hipLaunchKernelGGL(k_xpass<X_PLAIN>, grid, dim3(XWAVES * WAVE), lds, s, a); break;
-    case X_BC: hipLaunchKernelGGL(k_xpass<X_BC>, grid, dim3(XWAVES * WAVE), lds, s, a); break;
-    case X_SUB: hipLaunchKernelGGL(k_xpass<X_SUB>, grid, dim3(XWAVES * WAVE), lds, s, a); break;
-    default: hipLaunchKernelGGL(k_xpass<X_TRIAL>, grid, dim3(XWAVES * WAVE), lds, s, a); break;
+    case X_PLAIN: launch_x_m<X_PLAIN>(a, s); break;
+    case X_BC: launch_x_m<X_BC>(a, s); break;
+    case X_SUB: launch_x_m<X_SUB>(a, s); break;
+    default: launch_x_m<X_TRIAL>(a, s); break;
     }
 }
 
@@ -97,7 +175,7 @@ void launch_xpass(int mode, const XArgs& a, hipStream_t s)
 // ===========================================================================
 constexpr int YWAVES = 4;
 
-template <int TI>
+template <int TI, int PT>
 __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
 {
     extern __shared__ double lds[];
@@ -107,52 +185,114 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
     const int n1 = a.g.n1, n2 = a.g.n2;
     const int b = blockIdx.y;
     const int i0 = blockIdx.x * TI;
-    const int ld = a.pl.M + 2;
+    const int ld = dst_slot_doubles(a.pl);
     double* base = a.buf + b * a.lstride;
-    for (int idx = tid; idx < TI * n2; idx += YWAVES * WAVE) {
-        int r = idx / TI, c = idx % TI;
-        int i = i0 + c;
-        lds[c * ld + r] = (i < n1) ? base[i + (long)n1 * r] : 0.0;
+    // LDS: [DST tables][cos(pi j/ny) table][TI column slots]
+    const int ntab = dst_tab_doubles(a.pl);
+    const DstTab tb = load_dst_tab(lds, a.pl, tid, YWAVES * WAVE);
+    double* cyl = lds + ntab;
+    for (int k = tid; k < n2; k += YWAVES * WAVE) cyl[k] = a.cy[k];
+    double* tile = cyl + ((n2 + 1) & ~1);
+    // tile load: thread (r0, c) moves rows r0, r0 + RPI, ... of column c; CH
+    // loads per thread are in flight before their LDS writes
+    constexpr int RPI = YWAVES * WAVE / TI;
+    constexpr int CH = 16;
+    const int c0 = tid % TI, r0 = tid / TI;
+    const int ic = i0 + c0;
+    const bool colok = ic < n1;
+    for (int rb = 0; rb < n2; rb += RPI * CH) {
+        double v[CH];
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            int r = rb + r0 + RPI * t;
+            v[t] = (colok && r < n2) ? base[ic + (long)n1 * r] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            int r = rb + r0 + RPI * t;
+            if (r < n2) tile[c0 * ld + r] = v[t];
+        }
     }
     __syncthreads();
     const int lev = a.lev0 + b;
-    const double cdiv = a.cdiv[lev];
-    for (int c = w; c < TI; c += YWAVES) {
-        if (i0 + c >= n1) continue;
-        double* s = lds + c * ld;
-        dst1_line(s, a.pl, lane);
-        // dst_inv!: w ./= Λ̃; w .*= scale (dst-inversion.jl:46-48) with
-        // Λ = -2 (cos(pi i/nx) + cos(pi j/ny) - 2) (laplacian.jl:17) and
-        // Λ̃ per operator (laplacian.jl:69, 74-75)
-        const double cxk = a.cx[i0 + c];
-        for (int kj = lane; kj < n2; kj += WAVE) {
-            double lam = -2.0 * ((cxk + a.cy[kj]) - 2.0);
-            double d;
-            if (a.op == Y_DINV) d = lam;
-            else if (a.op == Y_A) d = 1.0 / (1.0 - (lam * a.dt) / cdiv);
-            else d = 1.0 + (lam * a.dt) / cdiv;
-            s[kj] = (s[kj] / d) * a.scale;
+    const double hcl = level_hc(a.g.h, lev);
+    const double cdiv = (2.0 * a.Re) * (hcl * hcl);  // 2 Re h^2 of laplacian.jl:69,74
+    // (column, transform) pairs: transform 0 is followed by the eigenvalue
+    // scaling; one call site keeps a single DST body in the kernel
+    const int ncol = (TI + YWAVES - 1) / YWAVES;
+#pragma unroll 1
+    for (int it = 0; it < 2 * ncol; it++) {
+        const int c = w + YWAVES * (it >> 1);
+        if (c >= TI || i0 + c >= n1) continue;
+        double* s = tile + c * ld;
+        dst_line<PT>(s, a.pl, tb, lane);
+        if ((it & 1) == 0) {
+            // dst_inv!: w ./= Λ̃; w .*= scale (dst-inversion.jl:46-48) with
+            // Λ = -2 (cos(pi i/nx) + cos(pi j/ny) - 2) (laplacian.jl:17) and
+            // Λ̃ per operator (laplacian.jl:69, 74-75)
+            const double cxk = a.cx[i0 + c];
+            for (int kj = lane; kj < n2; kj += WAVE) {
+                double lam = -2.0 * ((cxk + cyl[kj]) - 2.0);
+                double d;
+                if (a.op == Y_DINV) d = lam;
+                else if (a.op == Y_A) d = 1.0 / (1.0 - (lam * a.dt) / cdiv);
+                else d = 1.0 + (lam * a.dt) / cdiv;
+                s[kj] = (s[kj] / d) * a.scale;
+            }
+            wave_sync();
         }
-        wave_sync();
-        dst1_line(s, a.pl, lane);
     }
     __syncthreads();
-    for (int idx = tid; idx < TI * n2; idx += YWAVES * WAVE) {
-        int r = idx / TI, c = idx % TI;
-        int i = i0 + c;
-        if (i < n1) base[i + (long)n1 * r] = lds[c * ld + r];
+    if (colok) {
+        for (int rb = 0; rb < n2; rb += RPI * CH) {
+            double v[CH];
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                int r = rb + r0 + RPI * t;
+                if (r < n2) v[t] = tile[c0 * ld + r];
+            }
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                int r = rb + r0 + RPI * t;
+                if (r < n2) base[ic + (long)n1 * r] = v[t];
+            }
+        }
+    }
+}
+
+size_t ypass_lds_bytes(const DstPlan& pl, int n2, int tile)
+{
+    return sizeof(double) * ((size_t)dst_tab_doubles(pl) + (size_t)((n2 + 1) & ~1) + (size_t)tile * dst_slot_doubles(pl));
+}
+
+template <int TI, int PT>
+static void launch_y_t(const YArgs& a, hipStream_t s)
+{
+    static bool once = (allow_big_lds(k_ypass<TI, PT>), true);
+    (void)once;
+    dim3 grid((a.g.n1 + TI - 1) / TI, a.nlev);
+    hipLaunchKernelGGL((k_ypass<TI, PT>), grid, dim3(YWAVES * WAVE), ypass_lds_bytes(a.pl, a.g.n2, TI), s, a);
+}
+
+template <int TI>
+static void launch_y_m(const YArgs& a, hipStream_t s)
+{
+    const int P = a.pl.kind == 0 ? a.pl.P : 0;
+    switch (P) {
+    case 512: launch_y_t<TI, 512>(a, s); break;
+    case 400: launch_y_t<TI, 400>(a, s); break;
+    case 256: launch_y_t<TI, 256>(a, s); break;
+    case 200: launch_y_t<TI, 200>(a, s); break;
+    case 128: launch_y_t<TI, 128>(a, s); break;
+    case 100: launch_y_t<TI, 100>(a, s); break;
+    default: launch_y_t<TI, 0>(a, s); break;
     }
 }
 
 void launch_ypass(const YArgs& a, int tile, hipStream_t s)
 {
-    size_t ldsb = sizeof(double) * (size_t)tile * (a.pl.M + 2);
-    dim3 grid((a.g.n1 + tile - 1) / tile, a.nlev);
-    switch (tile) {
-    case 4: hipLaunchKernelGGL(k_ypass<4>, grid, dim3(YWAVES * WAVE), ldsb, s, a); break;
-    case 16: hipLaunchKernelGGL(k_ypass<16>, grid, dim3(YWAVES * WAVE), ldsb, s, a); break;
-    default: hipLaunchKernelGGL(k_ypass<8>, grid, dim3(YWAVES * WAVE), ldsb, s, a); break;
-    }
+    if (tile == 4) launch_y_m<4>(a, s);
+    else launch_y_m<8>(a, s);
 }
 
 // ===========================================================================
@@ -180,14 +320,19 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
     auto GA = [&](int ii, int jj) -> double { return G[(ii - 1) + (long)n1 * (jj - 1)]; };
     auto BCV = [&](int k) -> double { return hasbc ? get_bc_val(Gc, g, k) * 0.25 : 0.0; };
     // Qx[ii, jj] (jj >= 2) and Qy[ii, jj] (ii >= 2) of avg_flux
+    const bool u0 = a.q0_uniform != 0;
+    const double hcl = level_hc(g.h, lev);
+    const double cx0 = a.q0vx * hcl, cy0 = a.q0vy * hcl;  // as k_base_flux_inertial writes them
     auto QX = [&](int ii, int jj) -> double {
         long p = (ii - 1) + (long)(nx + 1) * (jj - 1);
         long pm = p - (nx + 1);
+        if (u0) return ((q[p] + cx0) + q[pm]) + cx0;
         return ((q[p] + q0[p]) + q[pm]) + q0[pm];
     };
     auto QY = [&](int ii, int jj) -> double {
         long p = nu + (ii - 1) + (long)nx * (jj - 1);
         long pm = p - 1;
+        if (u0) return ((q[p] + cy0) + q[pm]) + cy0;
         return ((q[p] + q0[p]) + q[pm]) + q0[pm];
     };
     // fqx[ii, jj] for ii = 2..nx, jj = 1..ny
@@ -205,7 +350,8 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
         return v * -1.0;
     };
     double r = (FQX(i, j - 1) - FQX(i, j)) + (FQY(i, j) - FQY(i - 1, j));
-    a.out[(long)b * g.nG + (i - 2) + (long)n1 * (j - 2)] = r * a.s_lev[lev];
+    const double sl = 1.0 / (4.0 * (hcl * hcl));  // nonlinear.jl:197-198
+    a.out[(long)b * g.nG + (i - 2) + (long)n1 * (j - 2)] = r * sl;
 }
 
 void launch_nonlinear(const NLArgs& a, hipStream_t s)
@@ -530,16 +676,6 @@ void launch_absmax(const double* p, long n, double* out, hipStream_t s)
     hipLaunchKernelGGL(k_absmax, dim3(1), dim3(256), 0, s, p, n, out);
 }
 
-void kernels_init()
-{
-    const int big = 160 * 1024;
-    (void)hipFuncSetAttribute((const void*)k_xpass<X_PLAIN>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    (void)hipFuncSetAttribute((const void*)k_xpass<X_BC>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    (void)hipFuncSetAttribute((const void*)k_xpass<X_SUB>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    (void)hipFuncSetAttribute((const void*)k_xpass<X_TRIAL>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    (void)hipFuncSetAttribute((const void*)k_ypass<4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    (void)hipFuncSetAttribute((const void*)k_ypass<8>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    (void)hipFuncSetAttribute((const void*)k_ypass<16>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-}
+void kernels_init() {}
 
 }  // namespace ibpm

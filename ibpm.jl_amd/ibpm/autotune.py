@@ -1,0 +1,41 @@
+"""Launch-shape autotuner for a problem.
+
+Each option is swept over its legal values with the kernel it controls timed by
+HIP events on the problem's stream (ibpm_time_kernel); the fastest value is
+kept.  Results depend only on the grid shape, so one sweep per problem
+suffices.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _capi as C
+
+# option -> (kernel id timed by ibpm_time_kernel, candidate values)
+SWEEPS = {
+    C.OPT_YTILE_BATCH: (1, (4, 8)),
+    C.OPT_YTILE_SINGLE: (5, (4, 8)),
+}
+
+
+def time_kernel(prob, which: int, reps: int = 20) -> float:
+    v = ctypes.c_double(0.0)
+    prob.ctx.check(C.lib().ibpm_time_kernel(prob.h, which, reps, ctypes.byref(v)), "ibpm_time_kernel")
+    return v.value
+
+
+def autotune(prob, reps: int = 20) -> dict:
+    """Sweep every option of SWEEPS on `prob`, apply the fastest values and
+    return {option: (best_value, {value: ms})}."""
+    L = C.lib()
+    out = {}
+    for opt, (which, values) in SWEEPS.items():
+        times = {}
+        for v in values:
+            if L.ibpm_set_option(prob.h, opt, v) != C.IBPM_OK:
+                continue  # does not fit this grid
+            times[v] = time_kernel(prob, which, reps)
+        best = min(times, key=times.get)
+        prob.ctx.check(L.ibpm_set_option(prob.h, opt, best), "ibpm_set_option")
+        out[opt] = (best, times)
+    return out

@@ -166,11 +166,20 @@ int ibpm_op_reg(ibpm_prob* prob, const double* fb, double* q);
 /* rot! (curl.jl:22-34) */
 int ibpm_op_rot(ibpm_prob* prob, const double* q, double* G);
 
-/* --- timing (bench) --- */
+/* --- tuning and timing (bench, autotuner) --- */
+/* Launch-shape options of a problem.  No counterpart in the reference (it has
+ * no kernels); used by the autotuner and the benchmark. */
+#define IBPM_OPT_YTILE_BATCH 0   /* columns per y-pass workgroup, all-level launches: 4 or 8 */
+#define IBPM_OPT_YTILE_SINGLE 1  /* columns per y-pass workgroup, one-level launches: 4 or 8 */
+int ibpm_set_option(ibpm_prob* prob, int option, int value);
+int ibpm_get_option(ibpm_prob* prob, int option, int* value);
+
 /* Average device time in ms of `reps` launches of kernel `which` on the
  * problem's stream, bracketed by HIP events on that stream.  which:
  * 0 = x-pass DST over all levels, 1 = fused y-pass (A operator) over all
- * levels, 2 = nonlinear over all levels, 3 = one full advance! step. */
+ * levels, 2 = nonlinear over all levels, 4 = x-pass on level 1 only,
+ * 5 = fused y-pass (Δinv) on level 1 only, 6 = curl on level 1,
+ * 7 = plain copy of nΓ*mg doubles (HBM-traffic counter calibration). */
 int ibpm_time_kernel(ibpm_prob* prob, int which, int reps, double* ms_avg);
 
 #ifdef __cplusplus

@@ -39,6 +39,10 @@ def case(name: str):
         bodies = [ibpm.make_cylinder(0.25, grid.h, 0.0, 0.0, n=30), ibpm.make_plate(0.5, 0.3, grid.h, 1.0, 0.1, n=12)]
         return dict(grid=grid, bodies=bodies, Re=200.0, dt=0.005,
                     freestream={"Ux": lambda t: 1.0, "Uy": lambda t: 0.1, "inclination": lambda t: 0.05})
+    if name == "spec":  # 256 x 200, 2 levels: compile-time transform lengths P = 128 (x) and 100 (y)
+        grid = ibpm.make_grid(1.0 / 32, (-4.0, 4.0, -3.125, 3.125), mg=2)
+        bodies = [ibpm.make_cylinder(0.5, grid.h, 0.3, -0.2, n=100)]
+        return dict(grid=grid, bodies=bodies, Re=150.0, dt=0.005, freestream=U1)
     if name == "moving":  # MovingGrid pitching plate about its mid-chord at the origin
         grid = ibpm.make_grid(1.0 / 32, (-1.0, 3.0, -1.0, 1.0), mg=3)
         A, f = 15 * math.pi / 180, 0.2
