@@ -7,12 +7,14 @@ of circulation 0.01 and width 0.1 at (1.5, 0.3) (the 5-level 1024^2 precision
 configuration run in FP64).  A step is one advance!(state, prob, t).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py --kernels-only [--kernel-reps R]   (per-kernel timing only)
 
 Prints one JSON line.  Timed region: K consecutive steps enqueued by
 ibpm_advance_n with the state resident in HBM, bracketed by device
 synchronisation (the call returns after its stream has drained).
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -38,6 +40,38 @@ def algorithmic_bytes_per_step(nx, ny, mg, nb):
     return 8.0 * (L * (4 * Q + 24 * N) + 2.5 * (L - 1) * N + 8 * N + (2 * nb) ** 2)
 
 
+def measured_traffic(kernel, tile, nlev):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE passes, each scaled by the copy-kernel
+    calibration): the value on its line `traffic <kernel> <tile> <nlev> <bytes>`,
+    or None when no summary covers this launch shape."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_cylinder_1024x1024x5.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 5 and f[0] == "traffic" and f[1] == kernel and f[2] == str(tile) and f[3] == str(nlev):
+            return float(f[4])
+    return None
+
+
+def time_kernels(prob, tuned, reps):
+    """Average device ms per launch of each kernel kind (ibpm_time_kernel:
+    HIP events on the problem's stream).  Returns ({name: ms}, name of the
+    all-level y-pass)."""
+    from ibpm import _capi as C
+
+    yname = f"k_ypass<{tuned['ytile_batch']}> op=A (all levels)"
+    kb = {}
+    for which, name in ((0, "k_xpass<X_PLAIN> (all levels)"), (1, yname), (2, "k_nonlinear (all levels)"),
+                        (4, "k_xpass<X_PLAIN> (level 1)"), (5, f"k_ypass<{tuned['ytile_single']}> op=Dinv (level 1)"),
+                        (6, "k_curl (level 1)"), (7, "k_copy (nG*mg doubles, counter calibration)")):
+        v = ctypes.c_double(0.0)
+        prob.ctx.check(C.lib().ibpm_time_kernel(prob.h, which, reps, ctypes.byref(v)))
+        kb[name] = v.value
+    return kb, yname
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +80,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only autotune and time each kernel kind (for counter profiling); prints kernel_ms")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -59,14 +95,26 @@ def main():
     c = case("config4")
     g = c["grid"]
     nb = len(c["bodies"][0].ds)
+    dt = c["dt"]
+    from ibpm.autotune import autotune
+
+    if args.kernels_only:
+        # B's values change no kernel's memory traffic: take it as given (identity)
+        prob = ibpm.IBProblem(g, c["bodies"], dt, c["Re"], freestream=c["freestream"], Binv=np.eye(2 * nb))
+        state = ibpm.IBState(prob)
+        state.Γ = gaussian_vortex_gamma(g)
+        ibpm.vort2flux_(state, prob)
+        tuned = {("ytile_batch", "ytile_single")[k]: v[0] for k, v in autotune(prob).items()}
+        kb, _ = time_kernels(prob, tuned, args.kernel_reps)
+        print(json.dumps({"kernels_only": True, "tiles": tuned, "kernel_ms": {k: round(v, 5) for k, v in kb.items()}}),
+              flush=True)
+        return
     t0 = time.time()
-    prob = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
+    prob = ibpm.IBProblem(g, c["bodies"], dt, c["Re"], freestream=c["freestream"])
     setup_s = time.time() - t0
     state = ibpm.IBState(prob)
     state.Γ = gaussian_vortex_gamma(g)
     ibpm.vort2flux_(state, prob)
-    dt = c["dt"]
-    from ibpm.autotune import autotune
 
     tuned = {("ytile_batch", "ytile_single")[k]: v[0] for k, v in autotune(prob).items()}
     print(f"# setup {setup_s:.2f}s (B assembly on GPU {prob.info.setup_seconds:.2f}s, "
@@ -87,18 +135,14 @@ def main():
     # dominant-kernel roofline, HIP events on the problem's stream
     L = g.mg
     N = g.nΓ
-    kb = {}
-    for which, name in ((0, "k_xpass<X_PLAIN> (all levels)"), (1, "k_ypass<8> op=A (all levels)"),
-                        (2, "k_nonlinear (all levels)")):
-        v = __import__("ctypes").c_double(0.0)
-        prob.ctx.check(C.lib().ibpm_time_kernel(prob.h, which, args.kernel_reps, __import__("ctypes").byref(v)))
-        kb[name] = v.value
+    kb, yname = time_kernels(prob, tuned, args.kernel_reps)
     # algorithmic bytes per launch: read N + write N doubles per level for a DST pass
     dst_bytes = 2.0 * 8.0 * N * L
-    ypass_ms = kb["k_ypass<8> op=A (all levels)"]
-    roof = {"bound": "hbm", "kernel": "k_ypass<8> (fused DST-I y-pass, 5 levels)",
+    ypass_ms = kb[yname]
+    roof = {"bound": "hbm", "kernel": f"k_ypass<{tuned['ytile_batch']}> (fused DST-I y-pass, A operator, 5 levels)",
             "achieved": round(dst_bytes / (ypass_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(dst_bytes / (ypass_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(dst_bytes / (ypass_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic("k_ypass", tuned["ytile_batch"], L),
             "bytes_per_launch": dst_bytes, "avg_launch_ms": round(ypass_ms, 5)}
     step_bytes = algorithmic_bytes_per_step(g.nx, g.ny, g.mg, nb)
 

@@ -49,6 +49,15 @@ __host__ __device__ inline int plan_radix(const DstPlan& pl, int k) { return (in
 // grid spacing of level lev (0-based): h 2^lev, exact
 __host__ __device__ inline double level_hc(double h, int lev) { return h * (double)(1L << lev); }
 
+// gfx950 deals the work-groups of a launch round-robin over its 8 XCDs, each
+// with a private L2.  A 1-D launch of NXCD * nper work-groups that reads its
+// work-group index through xcd_block() hands each XCD a contiguous range of
+// nper logical blocks, so neighbouring tiles that share cache lines are
+// fetched into one L2 instead of up to eight.
+constexpr int NXCD = 8;
+__host__ __device__ inline int xcd_nper(int nblocks) { return (nblocks + NXCD - 1) / NXCD; }
+__device__ __forceinline__ int xcd_block(int bid, int nper) { return (bid % NXCD) * nper + bid / NXCD; }
+
 // ---------------------------------------------------------------------------
 // wave-level synchronisation of LDS traffic (all lanes of one wavefront)
 // ---------------------------------------------------------------------------
@@ -667,6 +676,159 @@ __host__ __device__ inline bool dst_specialised(int P)
 {
     return P == 512 || P == 400 || P == 256 || P == 200 || P == 128 || P == 100;
 }
+
+// ---------------------------------------------------------------------------
+// DST-I of lines of 1023 values (M = 1024, P = 512 = 8^3): the transform of
+// grids with 1024 cells along a direction.  Same algorithm as dst1_line, laid
+// out so that nothing branches per lane:
+//  * guarded line layout: element e = 0..M of the line x_e lives at gsl(e)
+//    (two doubles of padding per 16), with guard cells x_0 = x_M = 0, so the
+//    pre-processing reads x_j and x_{M-j} unconditionally;
+//  * stage 1 takes its radix-8 inputs y_{2m}, y_{2m+1} (m = lane + 64 r) from
+//    one 16-byte read of (x_{2m}, x_{2m+1}) plus the mirrored pair;
+//  * the unpacking gives lane L the eight consecutive k = 8L..8L+7, so the
+//    prefix sum of the odd outputs is a serial sum in the lane plus one wave
+//    scan (instead of eight), and the lane owns outputs n = 16L..16L+15.
+// On exit gsl(n) holds Y_n (n = 1..M-1) and the guards are 0 again.
+// ---------------------------------------------------------------------------
+namespace f1024 {
+constexpr int M = 1024, P = 512;
+__host__ __device__ constexpr int gsl(int e) { return e + 2 * (e >> 4); }
+// doubles of one line slot: gsl(M) + 1 = 1153 for the real layout and
+// 2 (pz(P - 1) + 1) = 1150 for the complex work array, rounded to even, + 2
+// to stagger the LDS banks of neighbouring slots
+constexpr int SLOT = 1156;
+// doubles of the LDS tables: twiddles W_P^e (P complex), W_M^k at k + k/8
+// (P + P/8 complex), sin(pi j/M) for j <= M/2 (rounded to even)
+constexpr int TWP = P + P / 8;
+constexpr int TAB = 2 * P + 2 * TWP + (M / 2 + 2);
+}  // namespace f1024
+
+__device__ inline DstTab load_dst_tab_1024(double* lds, const DstPlan& pl, int tid, int nthreads)
+{
+    using namespace f1024;
+    double2* tw = reinterpret_cast<double2*>(lds);
+    double2* wm = tw + P;
+    double* sn = reinterpret_cast<double*>(wm + TWP);
+    for (int i = tid; i < P; i += nthreads) {
+        tw[i] = pl.tw[i];
+        wm[i + (i >> 3)] = pl.wm[i];
+    }
+    for (int i = tid; i <= M / 2; i += nthreads) sn[i] = pl.sn[i];
+    DstTab t;
+    t.tw = tw;
+    t.wm = wm;
+    t.sn = sn;
+    return t;
+}
+
+// exclusive prefix sum over the lanes (lane 0 gets 0): wave_shr:1, then scan
+__device__ __forceinline__ double wave_excl_scan(double v)
+{
+    return wave_incl_scan(dpp_d<0x138, 0xf>(v), 0);
+}
+
+// `out(L, i, ye, yo)` stores the outputs n = 16L + 2i (ye) and n + 1 (yo) of
+// lane L; it may scale them first (ye of n = 0 is the guard and must stay 0).
+template <class OUT>
+__device__ __forceinline__ void dst1024_line(double* S, const DstTab& tb, int lane, OUT out)
+{
+    using namespace f1024;
+    asm volatile("" : "+v"(lane));
+    double2* z = reinterpret_cast<double2*>(S);
+    const int j = lane;
+    double2 v[8];
+    // --- stage 1 (Ns = 1): pre-processing + radix 8, inputs m = j + 64 r
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int m = j + 64 * r;
+        const double2 a = *reinterpret_cast<const double2*>(S + gsl(2 * m));  // x_{2m}, x_{2m+1}
+        const double b0 = S[gsl(M - 2 * m)];                                    // x_{M-2m}
+        const double b1 = S[gsl(M - 2 * m - 1)];                                // x_{M-2m-1}
+        double s0, s1;
+        if (r < 4) {  // 2m + 1 <= M/2: sin(pi (2m)/M), sin(pi (2m+1)/M) straight from the table
+            const double2 t = *reinterpret_cast<const double2*>(tb.sn + 2 * m);
+            s0 = t.x;
+            s1 = t.y;
+        } else {  // mirrored: sin(pi e/M) = sin(pi (M-e)/M)
+            s0 = tb.sn[M - 2 * m];
+            s1 = tb.sn[M - 2 * m - 1];
+        }
+        v[r] = make_double2(fma(s0, a.x + b0, 0.5 * (a.x - b0)), fma(s1, a.y + b1, 0.5 * (a.y - b1)));
+    }
+    wave_sync();
+    dft8(v);
+#pragma unroll
+    for (int r = 0; r < 8; r++) z[9 * j + r] = v[r];  // pz(8 j + r)
+    wave_sync();
+    const int p0 = j + (j >> 3);  // pz(j); pz(j + 64 r) = p0 + 72 r
+    // --- stage 2 (Ns = 8): twiddles W_P^(8 k r), k = j mod 8
+    {
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = z[p0 + 72 * r];
+        const int k = j & 7;
+#pragma unroll
+        for (int r = 1; r < 8; r++) v[r] = cmul(v[r], tb.tw[8 * k * r]);
+        wave_sync();
+        dft8(v);
+        const int q0 = ((j >> 3) << 6) + k + ((j >> 3) << 3);  // pz(64 (j/8) + k); + 9 r per r
+#pragma unroll
+        for (int r = 0; r < 8; r++) z[q0 + 9 * r] = v[r];
+        wave_sync();
+    }
+    // --- stage 3 (Ns = 64): twiddles W_P^(j r), natural-order output
+    {
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = z[p0 + 72 * r];
+#pragma unroll
+        for (int r = 1; r < 8; r++) v[r] = cmul(v[r], tb.tw[j * r]);
+        wave_sync();
+        dft8(v);
+#pragma unroll
+        for (int r = 0; r < 8; r++) z[p0 + 72 * r] = v[r];
+        wave_sync();
+    }
+    // --- unpacking, lane L = j owns k = 8L + i: with a = z_k, b = z_{P-k},
+    //     2 F_k = (a + conj b) + W_M^k (-i)(a - conj b) (exactly twice the F_k
+    //     of dst1_line); R'_k = 2 R_k (R'_0 = R_0) and Y(2k) = 2 I_k = -2 Im F_k
+    double rr[8], ye[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int k = 8 * j + i;
+        const double2 a = z[9 * j + i];
+        const int k2 = (P - k) & (P - 1);
+        const double2 b = z[k2 + (k2 >> 3)];
+        const double2 w = tb.wm[9 * j + i];
+        const double2 E2 = make_double2(a.x + b.x, a.y - b.y);
+        const double2 O2 = make_double2(a.y + b.y, -(a.x - b.x));
+        const double2 F2 = cadd(E2, cmul(w, O2));
+        rr[i] = F2.x;
+        ye[i] = -F2.y;
+    }
+    wave_sync();
+    if (j == 0) {
+        rr[0] = 0.5 * rr[0];  // R'_0 = R_0
+        ye[0] = 0.0;          // n = 0: the guard cell x_0
+    }
+    // --- Y(2k+1) = sum_{m <= k} R'_m: serial in the lane, one wave scan across
+    double cs[8];
+    cs[0] = rr[0];
+#pragma unroll
+    for (int i = 1; i < 8; i++) cs[i] = cs[i - 1] + rr[i];
+    const double off = wave_excl_scan(cs[7]);
+#pragma unroll
+    for (int i = 0; i < 8; i++) out(j, i, ye[i], off + cs[i]);
+    wave_sync();
+}
+
+// plain store of a dst1024_line output pair into the guarded slot
+struct F1024Store {
+    double* S;
+    __device__ __forceinline__ void operator()(int L, int i, double ye, double yo) const
+    {
+        *reinterpret_cast<double2*>(S + f1024::gsl(16 * L + 2 * i)) = make_double2(ye, yo);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // get_bc! evaluated pointwise (src/fluid-operators/mg-utils.jl:7-32).

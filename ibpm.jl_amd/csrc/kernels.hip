@@ -26,11 +26,55 @@ __device__ __forceinline__ double add_bc(double v, const double* __restrict__ r,
     return v;
 }
 
+// Per-length line layout in an LDS slot.  PT = 512 (lines of 1023) uses the
+// guarded layout of dst1024_line; every other length stores element n at n.
+template <int PT>
+__device__ __forceinline__ int lat(int n)
+{
+    if constexpr (PT == 512) return f1024::gsl(n + 1);
+    else return n;
+}
+template <int PT>
+__host__ __device__ inline int slot_doubles_t(const DstPlan& pl)
+{
+    if constexpr (PT == 512) return f1024::SLOT;
+    else return dst_slot_doubles(pl);
+}
+template <int PT>
+__host__ __device__ inline int tab_doubles_t(const DstPlan& pl)
+{
+    if constexpr (PT == 512) return f1024::TAB;
+    else return dst_tab_doubles(pl);
+}
+template <int PT>
+__device__ __forceinline__ DstTab load_tab_t(double* lds, const DstPlan& pl, int tid, int nthreads)
+{
+    if constexpr (PT == 512) return load_dst_tab_1024(lds, pl, tid, nthreads);
+    else return load_dst_tab(lds, pl, tid, nthreads);
+}
+// line length: 1023 for PT = 512 (a compile-time constant), else the grid's
+template <int PT>
+__device__ __forceinline__ int line_len(int n)
+{
+    if constexpr (PT == 512) return f1024::M - 1;
+    else return n;
+}
+// zero guard cells of a slot (guarded layout only)
+template <int PT>
+__device__ __forceinline__ void set_guards(double* s)
+{
+    if constexpr (PT == 512) {
+        s[f1024::gsl(0)] = 0.0;
+        s[f1024::gsl(f1024::M)] = 0.0;
+    }
+}
+
 // one line transform: compile-time length PT > 0, else the plan's runtime length
 template <int PT>
 __device__ __forceinline__ void dst_line(double* s, const DstPlan& pl, const DstTab& tb, int lane)
 {
-    if constexpr (PT > 0) dst1_line_t<PT>(s, tb, lane);
+    if constexpr (PT == 512) dst1024_line(s, tb, lane, F1024Store{s});
+    else if constexpr (PT > 0) dst1_line_t<PT>(s, tb, lane);
     else dst1_line(s, pl, tb, lane);
 }
 
@@ -40,15 +84,16 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
     extern __shared__ double lds[];
     const int lane = threadIdx.x & (WAVE - 1);
     const int w = threadIdx.x / WAVE;
-    const int n1 = a.g.n1, n2 = a.g.n2;
+    const int n1 = line_len<PT>(a.g.n1), n2 = a.g.n2;
     const int line = blockIdx.x * XWAVES + w;  // 0-based j - 1
     const int b = blockIdx.y;
-    const int ntab = dst_tab_doubles(a.pl);
-    const DstTab tb = load_dst_tab(lds, a.pl, threadIdx.x, XWAVES * WAVE);
+    const int ntab = tab_doubles_t<PT>(a.pl);
+    const DstTab tb = load_tab_t<PT>(lds, a.pl, threadIdx.x, XWAVES * WAVE);
     __syncthreads();
     if (line >= n2) return;
-    const int ld = dst_slot_doubles(a.pl);
+    const int ld = slot_doubles_t<PT>(a.pl);
     double* s = lds + ntab + w * ld;
+    if (lane == 0) set_guards<PT>(s);
     const int j = line + 1;
     const double* src = a.src + b * a.lstride + (long)line * n1;
     // all loads of the line in flight before the first wait
@@ -64,7 +109,7 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
         if (n < n1) {
             double x = v[t];
             if constexpr (MODE == X_BC) x = add_bc(x, a.bcr, a.g, a.fac, n + 1, j);
-            s[n] = x;
+            s[lat<PT>(n)] = x;
         }
     }
     wave_sync();
@@ -89,7 +134,7 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
         for (int t = 0; t < XPL; t++) {
             int n = lane + WAVE * t;
             if (n < n1) {
-                double rhs = s[n];
+                double rhs = s[lat<PT>(n)];
                 rhs += a.c1 * v[t];
                 rhs += a.c2 * u[t];
                 if (hasbc) {
@@ -100,7 +145,7 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
                         rhs += rb;
                     }
                 }
-                s[n] = rhs;
+                s[lat<PT>(n)] = rhs;
             }
         }
         wave_sync();
@@ -116,13 +161,13 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
 #pragma unroll
         for (int t = 0; t < XPL; t++) {
             int n = lane + WAVE * t;
-            if (n < n1) dst[n] = v[t] - s[n];
+            if (n < n1) dst[n] = v[t] - s[lat<PT>(n)];
         }
     } else {
 #pragma unroll
         for (int t = 0; t < XPL; t++) {
             int n = lane + WAVE * t;
-            if (n < n1) dst[n] = s[n];
+            if (n < n1) dst[n] = s[lat<PT>(n)];
         }
     }
 }
@@ -140,7 +185,7 @@ static void launch_x_t(const XArgs& a, hipStream_t s)
     static bool once = (allow_big_lds(k_xpass<MODE, PT>), true);
     (void)once;
     dim3 grid((a.g.n2 + XWAVES - 1) / XWAVES, a.nlev);
-    size_t lds = sizeof(double) * ((size_t)dst_tab_doubles(a.pl) + (size_t)XWAVES * dst_slot_doubles(a.pl));
+    size_t lds = sizeof(double) * ((size_t)tab_doubles_t<PT>(a.pl) + (size_t)XWAVES * slot_doubles_t<PT>(a.pl));
     hipLaunchKernelGGL((k_xpass<MODE, PT>), grid, dim3(XWAVES * WAVE), lds, s, a);
 }
 
@@ -175,6 +220,49 @@ void launch_xpass(int mode, const XArgs& a, hipStream_t s)
 // ===========================================================================
 constexpr int YWAVES = 4;
 
+// eigenvalue scaling of dst_inv! (dst-inversion.jl:46-48): w ./= Λ̃; w .*= scale
+// with Λ = -2 (cos(pi i/nx) + cos(pi j/ny) - 2) (laplacian.jl:17) and Λ̃ per
+// operator (laplacian.jl:69, 74-75)
+__device__ __forceinline__ double eig_scale(double v, double cxk, double cyj, int op, double dt, double cdiv,
+                                            double scale)
+{
+    double lam = -2.0 * ((cxk + cyj) - 2.0);
+    double d;
+    if (op == Y_DINV) d = lam;
+    else if (op == Y_A) d = 1.0 / (1.0 - (lam * dt) / cdiv);
+    else d = 1.0 + (lam * dt) / cdiv;
+    return (v / d) * scale;
+}
+
+// dst1024_line output of the y-pass: the first transform of a column stores
+// its outputs scaled (op >= 0), the second stores them as they are (op < 0).
+// cyp holds cos(pi n/ny) at gsl(n) (and 1 at the guard n = 0, whose value 0
+// scales to 0).
+struct YOut1024 {
+    double* S;
+    const double* cyp;
+    double cxk, dt, cdiv, scale;
+    int op;
+    __device__ __forceinline__ void operator()(int L, int i, double ye, double yo) const
+    {
+        const int q = f1024::gsl(16 * L + 2 * i);
+        if (op >= 0) {
+            const double2 cy = *reinterpret_cast<const double2*>(cyp + q);
+            ye = eig_scale(ye, cxk, cy.x, op, dt, cdiv, scale);
+            yo = eig_scale(yo, cxk, cy.y, op, dt, cdiv, scale);
+        }
+        *reinterpret_cast<double2*>(S + q) = make_double2(ye, yo);
+    }
+};
+
+// doubles of the LDS cos(pi j/ny) table
+template <int PT>
+__host__ __device__ inline int cy_doubles_t(int n2)
+{
+    if constexpr (PT == 512) return f1024::gsl(f1024::M) + 2;
+    else return (n2 + 1) & ~1;
+}
+
 template <int TI, int PT>
 __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
 {
@@ -182,17 +270,27 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     const int w = tid / WAVE;
-    const int n1 = a.g.n1, n2 = a.g.n2;
-    const int b = blockIdx.y;
-    const int i0 = blockIdx.x * TI;
-    const int ld = dst_slot_doubles(a.pl);
+    const int n1 = a.g.n1, n2 = line_len<PT>(a.g.n2);
+    // logical block = (level, column tile), consecutive tiles on one XCD
+    const int ntiles = (n1 + TI - 1) / TI;
+    const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
+    if (lin >= ntiles * a.nlev) return;
+    const int b = lin / ntiles;
+    const int i0 = (lin - b * ntiles) * TI;
+    const int ld = slot_doubles_t<PT>(a.pl);
     double* base = a.buf + b * a.lstride;
     // LDS: [DST tables][cos(pi j/ny) table][TI column slots]
-    const int ntab = dst_tab_doubles(a.pl);
-    const DstTab tb = load_dst_tab(lds, a.pl, tid, YWAVES * WAVE);
+    const int ntab = tab_doubles_t<PT>(a.pl);
+    const DstTab tb = load_tab_t<PT>(lds, a.pl, tid, YWAVES * WAVE);
     double* cyl = lds + ntab;
-    for (int k = tid; k < n2; k += YWAVES * WAVE) cyl[k] = a.cy[k];
-    double* tile = cyl + ((n2 + 1) & ~1);
+    if constexpr (PT == 512) {
+        for (int k = tid; k < n2; k += YWAVES * WAVE) cyl[f1024::gsl(k + 1)] = a.cy[k];
+        if (tid == 0) cyl[f1024::gsl(0)] = 1.0;
+    } else {
+        for (int k = tid; k < n2; k += YWAVES * WAVE) cyl[k] = a.cy[k];
+    }
+    double* tile = cyl + cy_doubles_t<PT>(n2);
+    if (tid < TI) set_guards<PT>(tile + tid * ld);
     // tile load: thread (r0, c) moves rows r0, r0 + RPI, ... of column c; CH
     // loads per thread are in flight before their LDS writes
     constexpr int RPI = YWAVES * WAVE / TI;
@@ -210,7 +308,7 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
 #pragma unroll
         for (int t = 0; t < CH; t++) {
             int r = rb + r0 + RPI * t;
-            if (r < n2) tile[c0 * ld + r] = v[t];
+            if (r < n2) tile[c0 * ld + lat<PT>(r)] = v[t];
         }
     }
     __syncthreads();
@@ -225,21 +323,15 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
         const int c = w + YWAVES * (it >> 1);
         if (c >= TI || i0 + c >= n1) continue;
         double* s = tile + c * ld;
-        dst_line<PT>(s, a.pl, tb, lane);
-        if ((it & 1) == 0) {
-            // dst_inv!: w ./= Λ̃; w .*= scale (dst-inversion.jl:46-48) with
-            // Λ = -2 (cos(pi i/nx) + cos(pi j/ny) - 2) (laplacian.jl:17) and
-            // Λ̃ per operator (laplacian.jl:69, 74-75)
-            const double cxk = a.cx[i0 + c];
-            for (int kj = lane; kj < n2; kj += WAVE) {
-                double lam = -2.0 * ((cxk + cyl[kj]) - 2.0);
-                double d;
-                if (a.op == Y_DINV) d = lam;
-                else if (a.op == Y_A) d = 1.0 / (1.0 - (lam * a.dt) / cdiv);
-                else d = 1.0 + (lam * a.dt) / cdiv;
-                s[kj] = (s[kj] / d) * a.scale;
+        const double cxk = a.cx[i0 + c];
+        if constexpr (PT == 512) {
+            dst1024_line(s, tb, lane, YOut1024{s, cyl, cxk, a.dt, cdiv, a.scale, (it & 1) == 0 ? a.op : -1});
+        } else {
+            dst_line<PT>(s, a.pl, tb, lane);
+            if ((it & 1) == 0) {
+                for (int kj = lane; kj < n2; kj += WAVE) s[kj] = eig_scale(s[kj], cxk, cyl[kj], a.op, a.dt, cdiv, a.scale);
+                wave_sync();
             }
-            wave_sync();
         }
     }
     __syncthreads();
@@ -249,7 +341,7 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
 #pragma unroll
             for (int t = 0; t < CH; t++) {
                 int r = rb + r0 + RPI * t;
-                if (r < n2) v[t] = tile[c0 * ld + r];
+                if (r < n2) v[t] = tile[c0 * ld + lat<PT>(r)];
             }
 #pragma unroll
             for (int t = 0; t < CH; t++) {
@@ -260,9 +352,16 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
     }
 }
 
+template <int PT>
+static size_t ypass_lds_bytes_t(const DstPlan& pl, int n2, int tile)
+{
+    return sizeof(double) * ((size_t)tab_doubles_t<PT>(pl) + (size_t)cy_doubles_t<PT>(n2) +
+                             (size_t)tile * slot_doubles_t<PT>(pl));
+}
+
 size_t ypass_lds_bytes(const DstPlan& pl, int n2, int tile)
 {
-    return sizeof(double) * ((size_t)dst_tab_doubles(pl) + (size_t)((n2 + 1) & ~1) + (size_t)tile * dst_slot_doubles(pl));
+    return (pl.kind == 0 && pl.P == 512) ? ypass_lds_bytes_t<512>(pl, n2, tile) : ypass_lds_bytes_t<0>(pl, n2, tile);
 }
 
 template <int TI, int PT>
@@ -270,8 +369,9 @@ static void launch_y_t(const YArgs& a, hipStream_t s)
 {
     static bool once = (allow_big_lds(k_ypass<TI, PT>), true);
     (void)once;
-    dim3 grid((a.g.n1 + TI - 1) / TI, a.nlev);
-    hipLaunchKernelGGL((k_ypass<TI, PT>), grid, dim3(YWAVES * WAVE), ypass_lds_bytes(a.pl, a.g.n2, TI), s, a);
+    const int nblocks = ((a.g.n1 + TI - 1) / TI) * a.nlev;
+    dim3 grid(NXCD * xcd_nper(nblocks));
+    hipLaunchKernelGGL((k_ypass<TI, PT>), grid, dim3(YWAVES * WAVE), ypass_lds_bytes_t<PT>(a.pl, a.g.n2, TI), s, a);
 }
 
 template <int TI>
@@ -305,9 +405,14 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
 {
     const GridDev& g = a.g;
     const int nx = g.nx, ny = g.ny, n1 = g.n1;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;  // 2..nx
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;  // 2..ny
-    const int b = blockIdx.z;
+    // logical block (bx, by, level), XCD-contiguous (halo rows shared in L2)
+    const int nbx = (nx - 1 + 63) / 64, nby = (ny - 1 + 3) / 4;
+    const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
+    const int b = lin / (nbx * nby);
+    if (b >= a.nlev) return;
+    const int rem = lin - b * (nbx * nby);
+    const int i = (rem % nbx) * 64 + threadIdx.x + 2;  // 2..nx
+    const int j = (rem / nbx) * 4 + threadIdx.y + 2;   // 2..ny
     if (i > nx || j > ny) return;
     const int lev = a.lev0 + b;
     const long nq = g.nq, nu = g.nu;
@@ -356,9 +461,9 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
 
 void launch_nonlinear(const NLArgs& a, hipStream_t s)
 {
-    dim3 blk(64, 4);
-    dim3 grid((a.g.nx - 1 + 63) / 64, (a.g.ny - 1 + 3) / 4, a.nlev);
-    hipLaunchKernelGGL(k_nonlinear, grid, blk, 0, s, a);
+    dim3 blk(64, 4);  // k_nonlinear derives its tile from this shape
+    const int nblocks = ((a.g.nx - 1 + 63) / 64) * ((a.g.ny - 1 + 3) / 4) * a.nlev;
+    hipLaunchKernelGGL(k_nonlinear, dim3(NXCD * xcd_nper(nblocks)), blk, 0, s, a);
 }
 
 // ===========================================================================
@@ -369,8 +474,12 @@ __global__ void __launch_bounds__(256) k_curl(CurlArgs a)
 {
     const GridDev& g = a.g;
     const int nx = g.nx, ny = g.ny, n1 = g.n1;
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
+    // logical block (flux block, level), XCD-contiguous (ψ rows j-1, j shared in L2)
+    const int nbq = (int)((g.nq + 255) / 256);
+    const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
+    const int b = lin / nbq;
+    if (b >= a.nlev) return;
+    const long p = (long)(lin - b * nbq) * 256 + threadIdx.x;
     if (p >= g.nq) return;
     const int lev = a.lev0 + b;
     const double* psi = a.psi + (long)lev * g.nG;
@@ -401,8 +510,8 @@ __global__ void __launch_bounds__(256) k_curl(CurlArgs a)
 
 void launch_curl(const CurlArgs& a, hipStream_t s)
 {
-    dim3 grid((unsigned)((a.g.nq + 255) / 256), a.nlev);
-    hipLaunchKernelGGL(k_curl, grid, dim3(256), 0, s, a);
+    const int nblocks = (int)((a.g.nq + 255) / 256) * a.nlev;
+    hipLaunchKernelGGL(k_curl, dim3(NXCD * xcd_nper(nblocks)), dim3(256), 0, s, a);
 }
 
 // ===========================================================================
