@@ -15,17 +15,6 @@ namespace ibpm {
 constexpr int XWAVES = 4;  // wavefronts per workgroup
 constexpr int XPL = (MAXM + WAVE - 1) / WAVE;  // line elements per lane
 
-// apply_bc! term of element (i, j) (1-based), mg-utils.jl:43-60: rows first
-// (j = 1 then j = ny-1), then columns (i = 1 then i = nx-1).
-__device__ __forceinline__ double add_bc(double v, const double* __restrict__ r, const GridDev& g, double fac, int i, int j)
-{
-    if (j == 1) v += fac * get_bc_val(r, g, (g.BOT + 1) + i);
-    if (j == g.ny - 1) v += fac * get_bc_val(r, g, (g.TOP + 1) + i);
-    if (i == 1) v += fac * get_bc_val(r, g, (g.LEFT + 1) + j);
-    if (i == g.nx - 1) v += fac * get_bc_val(r, g, (g.RIGHT + 1) + j);
-    return v;
-}
-
 // Per-length line layout in an LDS slot.  PT = 512 (lines of 1023) uses the
 // guarded layout of dst1024_line; every other length stores element n at n.
 template <int PT>
@@ -69,6 +58,14 @@ __device__ __forceinline__ void set_guards(double* s)
     }
 }
 
+// the row terms of add_bc only (j = 1, then j = ny-1)
+__device__ __forceinline__ double add_bc_rows(double v, const double* __restrict__ r, const GridDev& g, double fac, int i, int j)
+{
+    if (j == 1) v += fac * get_bc_val(r, g, (g.BOT + 1) + i);
+    if (j == g.ny - 1) v += fac * get_bc_val(r, g, (g.TOP + 1) + i);
+    return v;
+}
+
 // one line transform: compile-time length PT > 0, else the plan's runtime length
 template <int PT>
 __device__ __forceinline__ void dst_line(double* s, const DstPlan& pl, const DstTab& tb, int lane)
@@ -103,13 +100,37 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
         int n = lane + WAVE * t;
         if (n < n1) v[t] = src[n];
     }
+    if constexpr (MODE == X_BC) {
+        // apply_bc! (mg-utils.jl:43-60) with its boundary values loaded next to
+        // the line: row terms (j = 1, j = ny-1), then column terms (i = 1, i = nx-1)
+        const bool rb = (j == 1), rt = (j == a.g.ny - 1);
+        const double bl = a.fac * get_bc_val(a.bcr, a.g, (a.g.LEFT + 1) + j);
+        const double br = a.fac * get_bc_val(a.bcr, a.g, (a.g.RIGHT + 1) + j);
+        double w[XPL];
+        if (rb || rt) {
 #pragma unroll
-    for (int t = 0; t < XPL; t++) {
-        int n = lane + WAVE * t;
-        if (n < n1) {
-            double x = v[t];
-            if constexpr (MODE == X_BC) x = add_bc(x, a.bcr, a.g, a.fac, n + 1, j);
-            s[lat<PT>(n)] = x;
+            for (int t = 0; t < XPL; t++) {
+                int n = lane + WAVE * t;
+                if (n < n1) w[t] = a.fac * get_bc_val(a.bcr, a.g, (rb ? a.g.BOT : a.g.TOP) + 1 + (n + 1));
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < XPL; t++) {
+            int n = lane + WAVE * t;
+            if (n < n1) {
+                double x = v[t];
+                if (rb || rt) x += w[t];
+                if (rb && rt) x += a.fac * get_bc_val(a.bcr, a.g, (a.g.TOP + 1) + (n + 1));  // ny = 2 only
+                if (n == 0) x += bl;
+                if (n == n1 - 1) x += br;
+                s[lat<PT>(n)] = x;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < XPL; t++) {
+            int n = lane + WAVE * t;
+            if (n < n1) s[lat<PT>(n)] = v[t];
         }
     }
     wave_sync();
@@ -130,6 +151,13 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
             int n = lane + WAVE * t;
             if (n < n1) { v[t] = n1p[n]; u[t] = n0p[n]; }
         }
+        // column terms of rhsbc for this row, loaded next to N1 and N0
+        double bl = 0.0, br = 0.0;
+        if (hasbc) {
+            bl = fac * get_bc_val(gc, a.g, (a.g.LEFT + 1) + j);
+            br = fac * get_bc_val(gc, a.g, (a.g.RIGHT + 1) + j);
+        }
+        const bool rowbc = hasbc && (j == 1 || j == a.g.ny - 1);
 #pragma unroll
         for (int t = 0; t < XPL; t++) {
             int n = lane + WAVE * t;
@@ -138,10 +166,13 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
                 rhs += a.c1 * v[t];
                 rhs += a.c2 * u[t];
                 if (hasbc) {
+                    // rhsbc = apply_bc!(0, bc, fac): row terms, then column terms
                     const int i = n + 1;
-                    if (j == 1 || j == a.g.ny - 1 || i == 1 || i == a.g.nx - 1) {
+                    if (rowbc || i == 1 || i == a.g.nx - 1) {
                         double rb = 0.0;
-                        rb = add_bc(rb, gc, a.g, fac, i, j);
+                        if (rowbc) rb = add_bc_rows(rb, gc, a.g, fac, i, j);
+                        if (i == 1) rb += bl;
+                        if (i == a.g.nx - 1) rb += br;
                         rhs += rb;
                     }
                 }
@@ -401,117 +432,171 @@ void launch_ypass(const YArgs& a, int tile, hipStream_t s)
 // scaling.  Boundary circulation is get_bc!(Γ[:, lev+1]) .* 0.25.
 // One thread per output Γ point.
 // ===========================================================================
+constexpr int NL_JS = 8;  // output rows per thread (vertical strip)
+
 __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
 {
     const GridDev& g = a.g;
     const int nx = g.nx, ny = g.ny, n1 = g.n1;
-    // logical block (bx, by, level), XCD-contiguous (halo rows shared in L2)
-    const int nbx = (nx - 1 + 63) / 64, nby = (ny - 1 + 3) / 4;
+    // logical block (64 i, 4 strips of NL_JS rows, level), XCD-contiguous
+    const int nbx = (nx - 1 + 63) / 64;
+    const int nbs = ((ny - 1 + NL_JS - 1) / NL_JS + 3) / 4;
     const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
-    const int b = lin / (nbx * nby);
+    const int b = lin / (nbx * nbs);
     if (b >= a.nlev) return;
-    const int rem = lin - b * (nbx * nby);
-    const int i = (rem % nbx) * 64 + threadIdx.x + 2;  // 2..nx
-    const int j = (rem / nbx) * 4 + threadIdx.y + 2;   // 2..ny
-    if (i > nx || j > ny) return;
+    const int rem = lin - b * (nbx * nbs);
+    const int i = (rem % nbx) * 64 + threadIdx.x + 2;                    // 2..nx
+    const int j0 = ((rem / nbx) * 4 + threadIdx.y) * NL_JS + 2;          // first output row, 2..ny
+    if (i > nx || j0 > ny) return;
+    const int j1 = min(j0 + NL_JS - 1, ny);
     const int lev = a.lev0 + b;
     const long nq = g.nq, nu = g.nu;
-    const double* q = a.q + (long)lev * nq;
-    const double* q0 = a.q0 + (long)lev * nq;
-    const double* G = a.G + (long)lev * g.nG;
+    const double* __restrict__ q = a.q + (long)lev * nq;
+    const double* __restrict__ q0 = a.q0 + (long)lev * nq;
+    const double* __restrict__ G = a.G + (long)lev * g.nG;
     const bool hasbc = lev < a.mg - 1;
-    const double* Gc = a.G + (long)(lev + 1) * g.nG;
-
-    auto GA = [&](int ii, int jj) -> double { return G[(ii - 1) + (long)n1 * (jj - 1)]; };
-    auto BCV = [&](int k) -> double { return hasbc ? get_bc_val(Gc, g, k) * 0.25 : 0.0; };
-    // Qx[ii, jj] (jj >= 2) and Qy[ii, jj] (ii >= 2) of avg_flux
+    const double* __restrict__ Gc = a.G + (long)(lev + 1) * g.nG;
     const bool u0 = a.q0_uniform != 0;
     const double hcl = level_hc(g.h, lev);
     const double cx0 = a.q0vx * hcl, cy0 = a.q0vy * hcl;  // as k_base_flux_inertial writes them
-    auto QX = [&](int ii, int jj) -> double {
-        long p = (ii - 1) + (long)(nx + 1) * (jj - 1);
-        long pm = p - (nx + 1);
-        if (u0) return ((q[p] + cx0) + q[pm]) + cx0;
-        return ((q[p] + q0[p]) + q[pm]) + q0[pm];
+
+    // boundary circulation get_bc!(Γ[:, lev+1]) .* 0.25 (nonlinear.jl:186-188)
+    auto BCV = [&](int k) -> double { return hasbc ? get_bc_val(Gc, g, k) * 0.25 : 0.0; };
+    // Γ extended by its boundary ring: GAX(ii, jj), ii = 0..nx, jj = 0..ny; the
+    // ring carries the values direct_product_loops! (nonlinear.jl:73-114) uses
+    // at the edges (corners are never read)
+    auto GAX = [&](int ii, int jj) -> double {
+        if (jj == 0) return BCV(g.BOT + ii + 1);
+        if (jj == ny) return BCV(g.TOP + ii + 1);
+        if (ii == 0) return BCV(g.LEFT + jj + 1);
+        if (ii == nx) return BCV(g.RIGHT + jj + 1);
+        return G[(ii - 1) + (long)n1 * (jj - 1)];
     };
+    // flux components with the base flux: x-flux (ii, jj) and y-flux (ii, jj)
+    auto QXV = [&](int ii, int jj, double& f, double& f0) {
+        const long p = (ii - 1) + (long)(nx + 1) * (jj - 1);
+        f = q[p];
+        f0 = u0 ? cx0 : q0[p];
+    };
+    auto QYV = [&](int ii, int jj, double& f, double& f0) {
+        const long p = nu + (ii - 1) + (long)nx * (jj - 1);
+        f = q[p];
+        f0 = u0 ? cy0 : q0[p];
+    };
+    // avg_flux (nonlinear.jl:17-38): Qy(ii, jj) over (ii, jj), (ii-1, jj); Qx over (ii, jj), (ii, jj-1)
     auto QY = [&](int ii, int jj) -> double {
-        long p = nu + (ii - 1) + (long)nx * (jj - 1);
-        long pm = p - 1;
-        if (u0) return ((q[p] + cy0) + q[pm]) + cy0;
-        return ((q[p] + q0[p]) + q[pm]) + q0[pm];
+        double f, f0, fm, fm0;
+        QYV(ii, jj, f, f0);
+        QYV(ii - 1, jj, fm, fm0);
+        return ((f + f0) + fm) + fm0;
     };
-    // fqx[ii, jj] for ii = 2..nx, jj = 1..ny
-    auto FQX = [&](int ii, int jj) -> double {
-        if (jj == 1) return QY(ii, 2) * GA(ii - 1, 1) + QY(ii, 1) * BCV(g.BOT + ii);
-        if (jj == ny) return QY(ii, ny) * GA(ii - 1, ny - 1) + QY(ii, ny + 1) * BCV(g.TOP + ii);
-        return QY(ii, jj + 1) * GA(ii - 1, jj) + QY(ii, jj) * GA(ii - 1, jj - 1);
-    };
-    // fqy[ii, jj] for ii = 1..nx, jj = 2..ny
-    auto FQY = [&](int ii, int jj) -> double {
-        double v;
-        if (ii == 1) v = QX(2, jj) * GA(1, jj - 1) + QX(1, jj) * BCV(g.LEFT + jj);
-        else if (ii == nx) v = QX(nx, jj) * GA(nx - 1, jj - 1) + QX(nx + 1, jj) * BCV(g.RIGHT + jj);
-        else v = QX(ii + 1, jj) * GA(ii, jj - 1) + QX(ii, jj) * GA(ii - 1, jj - 1);
-        return v * -1.0;
-    };
-    double r = (FQX(i, j - 1) - FQX(i, j)) + (FQY(i, j) - FQY(i - 1, j));
+    // state carried from row j-1 to row j
+    double qxr[3], q0r[3];  // x-flux (i-1..i+1, j-1)
+    double gr[3];           // GAX(i-2..i, j-1)
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        QXV(i - 1 + d, j0 - 1, qxr[d], q0r[d]);
+        gr[d] = GAX(i - 2 + d, j0 - 1);
+    }
+    double qyj = QY(i, j0);  // Qy(i, j)
+    // fqx(i, j-1)
+    double fqxa = qyj * gr[1] + QY(i, j0 - 1) * GAX(i - 1, j0 - 2);
     const double sl = 1.0 / (4.0 * (hcl * hcl));  // nonlinear.jl:197-198
-    a.out[(long)b * g.nG + (i - 2) + (long)n1 * (j - 2)] = r * sl;
+    double* __restrict__ out = a.out + (long)b * g.nG;
+#pragma unroll
+    for (int t = 0; t < NL_JS; t++) {
+        const int j = j0 + t;
+        if (j > j1) break;
+        double qxn[3], q0n[3], gn[3], QX[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            QXV(i - 1 + d, j, qxn[d], q0n[d]);
+            QX[d] = ((qxn[d] + q0n[d]) + qxr[d]) + q0r[d];
+        }
+        gn[1] = GAX(i - 1, j);
+        if (j < ny) {
+            gn[0] = GAX(i - 2, j);
+            gn[2] = GAX(i, j);
+        } else {
+            gn[0] = gn[2] = 0.0;  // row ny is the top ring: only GAX(i-1, ny) is read
+        }
+        const double qyn = QY(i, j + 1);
+        // direct_product_loops!: fqx(i, j) = Qy(i, j+1) Γ(i-1, j) + Qy(i, j) Γ(i-1, j-1);
+        // fqy(ii, j) = -(Qx(ii+1, j) Γ(ii, j-1) + Qx(ii, j) Γ(ii-1, j-1)) for ii = i, i-1
+        const double fqxb = qyn * gn[1] + qyj * gr[1];
+        const double fqya = (QX[2] * gr[2] + QX[1] * gr[1]) * -1.0;
+        const double fqyb = (QX[1] * gr[1] + QX[0] * gr[0]) * -1.0;
+        // rot! (curl.jl:22-34) of fq, scaled by 1/(4 hc^2)
+        const double r = (fqxa - fqxb) + (fqya - fqyb);
+        out[(i - 2) + (long)n1 * (j - 2)] = r * sl;
+        fqxa = fqxb;
+        qyj = qyn;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            qxr[d] = qxn[d];
+            q0r[d] = q0n[d];
+            gr[d] = gn[d];
+        }
+    }
 }
 
 void launch_nonlinear(const NLArgs& a, hipStream_t s)
 {
     dim3 blk(64, 4);  // k_nonlinear derives its tile from this shape
-    const int nblocks = ((a.g.nx - 1 + 63) / 64) * ((a.g.ny - 1 + 3) / 4) * a.nlev;
+    const int nblocks = ((a.g.nx - 1 + 63) / 64) * (((a.g.ny - 1 + NL_JS - 1) / NL_JS + 3) / 4) * a.nlev;
     hipLaunchKernelGGL(k_nonlinear, dim3(NXCD * xcd_nper(nblocks)), blk, 0, s, a);
 }
 
 // ===========================================================================
 // curl! on the multigrid (curl.jl:81-116): q = C ψ with ψbc = get_bc!(ψ[:, lev+1])
-// (zero on the coarsest level).  One thread per flux point.
+// (zero on the coarsest level).  Thread (i, j), i = 1..nx+1, j = 1..ny+1,
+// writes the x-flux (i, j) (j <= ny) and the y-flux (i, j) (i <= nx): both
+// read ψ(i-1, j-1), so each ψ row is fetched by one range of blocks only.
 // ===========================================================================
 __global__ void __launch_bounds__(256) k_curl(CurlArgs a)
 {
     const GridDev& g = a.g;
     const int nx = g.nx, ny = g.ny, n1 = g.n1;
-    // logical block (flux block, level), XCD-contiguous (ψ rows j-1, j shared in L2)
-    const int nbq = (int)((g.nq + 255) / 256);
+    const int nbx = (nx + 1 + 63) / 64, nby = (ny + 1 + 3) / 4;
     const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
-    const int b = lin / nbq;
+    const int b = lin / (nbx * nby);
     if (b >= a.nlev) return;
-    const long p = (long)(lin - b * nbq) * 256 + threadIdx.x;
-    if (p >= g.nq) return;
+    const int rem = lin - b * (nbx * nby);
+    const int i = (rem % nbx) * 64 + threadIdx.x + 1;  // 1..nx+1
+    const int j = (rem / nbx) * 4 + threadIdx.y + 1;   // 1..ny+1
+    if (i > nx + 1 || j > ny + 1) return;
     const int lev = a.lev0 + b;
-    const double* psi = a.psi + (long)lev * g.nG;
+    const double* __restrict__ psi = a.psi + (long)lev * g.nG;
     const bool hasbc = lev < a.mg - 1;
-    const double* pc = a.psi + (long)(lev + 1) * g.nG;
+    const double* __restrict__ pc = a.psi + (long)(lev + 1) * g.nG;
     auto PS = [&](int ii, int jj) -> double { return psi[(ii - 1) + (long)n1 * (jj - 1)]; };
     auto BC = [&](int k) -> double { return hasbc ? get_bc_val(pc, g, k) : 0.0; };
     const int T = g.TOP, B = g.BOT, L = g.LEFT, R = g.RIGHT;
-    double v;
-    if (p < g.nu) {
-        const int i = (int)(p % (nx + 1)) + 1, j = (int)(p / (nx + 1)) + 1;
+    double* __restrict__ q = a.q + (long)b * a.qstride;
+    if (j <= ny) {  // x-flux (i, j)
+        double v;
         if (i == 1) v = BC((L + 1) + j) - BC(L + j);
         else if (i == nx + 1) v = BC((R + 1) + j) - BC(R + j);
         else if (j == 1) v = PS(i - 1, 1) - BC((B - 1) + i);
         else if (j == ny) v = BC(i + T) - PS(i - 1, ny - 1);
         else v = PS(i - 1, j) - PS(i - 1, j - 1);
-    } else {
-        const long pp = p - g.nu;
-        const int i = (int)(pp % nx) + 1, j = (int)(pp / nx) + 1;
+        q[(i - 1) + (long)(nx + 1) * (j - 1)] = v;
+    }
+    if (i <= nx) {  // y-flux (i, j)
+        double v;
         if (j == 1) v = BC(B + i) - BC((B + 1) + i);
         else if (j == ny + 1) v = BC(T + i) - BC((T + 1) + i);
         else if (i == 1) v = BC(L + j) - PS(1, j - 1);
         else if (i == nx) v = PS(nx - 1, j - 1) - BC(R + j);
         else v = PS(i - 1, j - 1) - PS(i, j - 1);
+        q[g.nu + (i - 1) + (long)nx * (j - 1)] = v;
     }
-    a.q[(long)b * a.qstride + p] = v;
 }
 
 void launch_curl(const CurlArgs& a, hipStream_t s)
 {
-    const int nblocks = (int)((a.g.nq + 255) / 256) * a.nlev;
-    hipLaunchKernelGGL(k_curl, dim3(NXCD * xcd_nper(nblocks)), dim3(256), 0, s, a);
+    const int nblocks = ((a.g.nx + 1 + 63) / 64) * ((a.g.ny + 1 + 3) / 4) * a.nlev;
+    hipLaunchKernelGGL(k_curl, dim3(NXCD * xcd_nper(nblocks)), dim3(64, 4), 0, s, a);
 }
 
 // ===========================================================================
