@@ -251,18 +251,28 @@ void launch_xpass(int mode, const XArgs& a, hipStream_t s)
 // ===========================================================================
 constexpr int YWAVES = 4;
 
+// 1/d for the eigenvalue scaling: v_rcp_f64 refined by two Newton steps
+// (each squares the relative error), within an ulp or two of a division
+__device__ __forceinline__ double recip(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
 // eigenvalue scaling of dst_inv! (dst-inversion.jl:46-48): w ./= Λ̃; w .*= scale
 // with Λ = -2 (cos(pi i/nx) + cos(pi j/ny) - 2) (laplacian.jl:17) and Λ̃ per
-// operator (laplacian.jl:69, 74-75)
-__device__ __forceinline__ double eig_scale(double v, double cxk, double cyj, int op, double dt, double cdiv,
-                                            double scale)
+// operator (laplacian.jl:69, 74-75).  Applied as one product by the factor
+// scale / Λ̃: Δinv 1/Λ, A (1 - Λ dt/(2 Re hc^2)) (the reference divides by its
+// reciprocal), Ainv 1/(1 + Λ dt/(2 Re hc^2)); qc = dt / (2 Re hc^2).
+__device__ __forceinline__ double eig_factor(double cxk, double cyj, int op, double qc, double scale)
 {
-    double lam = -2.0 * ((cxk + cyj) - 2.0);
-    double d;
-    if (op == Y_DINV) d = lam;
-    else if (op == Y_A) d = 1.0 / (1.0 - (lam * dt) / cdiv);
-    else d = 1.0 + (lam * dt) / cdiv;
-    return (v / d) * scale;
+    const double lam = -2.0 * ((cxk + cyj) - 2.0);
+    if (op == Y_A) return (1.0 - lam * qc) * scale;
+    const double d = (op == Y_DINV) ? lam : 1.0 + lam * qc;
+    return recip(d) * scale;
 }
 
 // dst1024_line output of the y-pass: the first transform of a column stores
@@ -272,15 +282,15 @@ __device__ __forceinline__ double eig_scale(double v, double cxk, double cyj, in
 struct YOut1024 {
     double* S;
     const double* cyp;
-    double cxk, dt, cdiv, scale;
+    double cxk, qc, scale;
     int op;
     __device__ __forceinline__ void operator()(int L, int i, double ye, double yo) const
     {
-        const int q = f1024::gsl(16 * L + 2 * i);
+        const int q = 18 * L + 2 * i;  // gsl(16 L + 2 i)
         if (op >= 0) {
             const double2 cy = *reinterpret_cast<const double2*>(cyp + q);
-            ye = eig_scale(ye, cxk, cy.x, op, dt, cdiv, scale);
-            yo = eig_scale(yo, cxk, cy.y, op, dt, cdiv, scale);
+            ye *= eig_factor(cxk, cy.x, op, qc, scale);
+            yo *= eig_factor(cxk, cy.y, op, qc, scale);
         }
         *reinterpret_cast<double2*>(S + q) = make_double2(ye, yo);
     }
@@ -322,6 +332,14 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
     }
     double* tile = cyl + cy_doubles_t<PT>(n2);
     if (tid < TI) set_guards<PT>(tile + tid * ld);
+    // cos(pi i/nx) of this wave's columns, loaded with the tile (not after it)
+    constexpr int NCOL = (TI + YWAVES - 1) / YWAVES;
+    double cxw[NCOL];
+#pragma unroll
+    for (int k = 0; k < NCOL; k++) {
+        const int c = w + YWAVES * k;
+        cxw[k] = (c < TI && i0 + c < n1) ? a.cx[i0 + c] : 0.0;
+    }
     // tile load: thread (r0, c) moves rows r0, r0 + RPI, ... of column c; CH
     // loads per thread are in flight before their LDS writes
     constexpr int RPI = YWAVES * WAVE / TI;
@@ -345,22 +363,24 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
     __syncthreads();
     const int lev = a.lev0 + b;
     const double hcl = level_hc(a.g.h, lev);
-    const double cdiv = (2.0 * a.Re) * (hcl * hcl);  // 2 Re h^2 of laplacian.jl:69,74
+    const double qc = a.dt / ((2.0 * a.Re) * (hcl * hcl));  // dt / (2 Re h^2), laplacian.jl:69,74
     // (column, transform) pairs: transform 0 is followed by the eigenvalue
     // scaling; one call site keeps a single DST body in the kernel
-    const int ncol = (TI + YWAVES - 1) / YWAVES;
 #pragma unroll 1
-    for (int it = 0; it < 2 * ncol; it++) {
+    for (int it = 0; it < 2 * NCOL; it++) {
         const int c = w + YWAVES * (it >> 1);
         if (c >= TI || i0 + c >= n1) continue;
         double* s = tile + c * ld;
-        const double cxk = a.cx[i0 + c];
+        double cxk = cxw[0];
+#pragma unroll
+        for (int k = 1; k < NCOL; k++)
+            if ((it >> 1) == k) cxk = cxw[k];
         if constexpr (PT == 512) {
-            dst1024_line(s, tb, lane, YOut1024{s, cyl, cxk, a.dt, cdiv, a.scale, (it & 1) == 0 ? a.op : -1});
+            dst1024_line(s, tb, lane, YOut1024{s, cyl, cxk, qc, a.scale, (it & 1) == 0 ? a.op : -1});
         } else {
             dst_line<PT>(s, a.pl, tb, lane);
             if ((it & 1) == 0) {
-                for (int kj = lane; kj < n2; kj += WAVE) s[kj] = eig_scale(s[kj], cxk, cyl[kj], a.op, a.dt, cdiv, a.scale);
+                for (int kj = lane; kj < n2; kj += WAVE) s[kj] *= eig_factor(cxk, cyl[kj], a.op, qc, a.scale);
                 wave_sync();
             }
         }
