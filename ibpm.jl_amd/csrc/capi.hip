@@ -526,8 +526,8 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
     const int nx = gd->nx, ny = gd->ny, mg = gd->mg;
     if (nx < 4 || ny < 4 || mg < 1 || mg > MAXLEV)
         return fail(ctx, IBPM_E_ARG, "grid needs nx, ny >= 4 and 1 <= mg <= 16");
-    if (mg > 1 && (nx % 4 != 0 || ny % 4 != 0))
-        return fail(ctx, IBPM_E_UNSUPPORTED, "multi-level grids need nx and ny divisible by 4 (get_bc!/coarsify!)");
+    if (mg > 1 && (nx % 4 != 0 || ny % 4 != 0 || nx < 8 || ny < 8))
+        return fail(ctx, IBPM_E_UNSUPPORTED, "multi-level grids need nx, ny >= 8 and divisible by 4 (get_bc!/coarsify!)");
     if (bd->motion == IBPM_MOTION_FUNCTION)
         return fail(ctx, IBPM_E_UNSUPPORTED, "MotionFunction (lab-frame moving body) is not run by this library");
     if (bd->motion == IBPM_MOTION_MOVING_GRID && bd->nbodies != 1)

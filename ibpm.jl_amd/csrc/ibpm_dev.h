@@ -840,25 +840,31 @@ __device__ __forceinline__ double rat(const double* r, const GridDev& g, int i, 
     return r[(i - 1) + (long)g.n1 * (j - 1)];
 }
 
+// One side of get_bc!: sample idx (1-based) of the coarse level along i at
+// row jr (BOT/TOP) or along j at column ir (LEFT/RIGHT).  Odd idx take the
+// coarse value at offset idx/2 from nx/4 (ny/4); even idx the mean of it and
+// its lower neighbour.  Both values are loaded and selected, so no load sits
+// in a lane-divergent branch.
+__device__ __forceinline__ double bc_row(const double* __restrict__ r, const GridDev& g, int jr, int idx)
+{
+    const double* p = r + (g.nx / 4 + idx / 2 - 1) + (long)g.n1 * (jr - 1);
+    const double a = p[0], b = p[-1];
+    return (idx & 1) ? a : 0.5 * (a + b);
+}
+
+__device__ __forceinline__ double bc_col(const double* __restrict__ r, const GridDev& g, int ir, int idx)
+{
+    const double* p = r + (ir - 1) + (long)g.n1 * (g.ny / 4 + idx / 2 - 1);
+    const double a = p[0], b = p[-(long)g.n1];
+    return (idx & 1) ? a : 0.5 * (a + b);
+}
+
 __device__ __forceinline__ double get_bc_val(const double* __restrict__ r, const GridDev& g, int k)
 {
-    const int nx = g.nx, ny = g.ny;
-    int side, idx;  // idx = 1-based position on the side
-    if (k <= g.RIGHT) { side = 0; idx = k - g.LEFT; }
-    else if (k <= g.BOT) { side = 1; idx = k - g.RIGHT; }
-    else if (k <= g.TOP) { side = 2; idx = k - g.BOT; }
-    else { side = 3; idx = k - g.TOP; }
-    if (side >= 2) {  // BOT / TOP : along i
-        int jr = (side == 2) ? ny / 4 : 3 * ny / 4;
-        int m = idx / 2;
-        if (idx & 1) return rat(r, g, nx / 4 + m, jr);
-        return 0.5 * (rat(r, g, nx / 4 + m, jr) + rat(r, g, nx / 4 + m - 1, jr));
-    } else {  // LEFT / RIGHT : along j
-        int ir = (side == 0) ? nx / 4 : 3 * nx / 4;
-        int m = idx / 2;
-        if (idx & 1) return rat(r, g, ir, ny / 4 + m);
-        return 0.5 * (rat(r, g, ir, ny / 4 + m) + rat(r, g, ir, ny / 4 + m - 1));
-    }
+    if (k <= g.RIGHT) return bc_col(r, g, g.nx / 4, k - g.LEFT);
+    if (k <= g.BOT) return bc_col(r, g, 3 * g.nx / 4, k - g.RIGHT);
+    if (k <= g.TOP) return bc_row(r, g, g.ny / 4, k - g.BOT);
+    return bc_row(r, g, 3 * g.ny / 4, k - g.TOP);
 }
 
 }  // namespace ibpm

@@ -58,14 +58,6 @@ __device__ __forceinline__ void set_guards(double* s)
     }
 }
 
-// the row terms of add_bc only (j = 1, then j = ny-1)
-__device__ __forceinline__ double add_bc_rows(double v, const double* __restrict__ r, const GridDev& g, double fac, int i, int j)
-{
-    if (j == 1) v += fac * get_bc_val(r, g, (g.BOT + 1) + i);
-    if (j == g.ny - 1) v += fac * get_bc_val(r, g, (g.TOP + 1) + i);
-    return v;
-}
-
 // one line transform: compile-time length PT > 0, else the plan's runtime length
 template <int PT>
 __device__ __forceinline__ void dst_line(double* s, const DstPlan& pl, const DstTab& tb, int lane)
@@ -84,53 +76,67 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
     const int n1 = line_len<PT>(a.g.n1), n2 = a.g.n2;
     const int line = blockIdx.x * XWAVES + w;  // 0-based j - 1
     const int b = blockIdx.y;
+    const int j = line + 1;
     const int ntab = tab_doubles_t<PT>(a.pl);
     const DstTab tb = load_tab_t<PT>(lds, a.pl, threadIdx.x, XWAVES * WAVE);
     __syncthreads();
     if (line >= n2) return;
-    const int ld = slot_doubles_t<PT>(a.pl);
-    double* s = lds + ntab + w * ld;
-    if (lane == 0) set_guards<PT>(s);
-    const int j = line + 1;
+    // every global load of the line is in flight before the first wait
     const double* src = a.src + b * a.lstride + (long)line * n1;
-    // all loads of the line in flight before the first wait
     double v[XPL];
 #pragma unroll
     for (int t = 0; t < XPL; t++) {
         int n = lane + WAVE * t;
         if (n < n1) v[t] = src[n];
     }
+    // apply_bc! (mg-utils.jl:43-60) terms: row terms (j = 1, j = ny-1), then
+    // column terms (i = 1, i = nx-1); fac * get_bc!(ψ[:, lev+1])
+    const bool rb = (j == 1), rt = (j == a.g.ny - 1);
+    double bl = 0.0, br = 0.0, wr[XPL];
     if constexpr (MODE == X_BC) {
-        // apply_bc! (mg-utils.jl:43-60) with its boundary values loaded next to
-        // the line: row terms (j = 1, j = ny-1), then column terms (i = 1, i = nx-1)
-        const bool rb = (j == 1), rt = (j == a.g.ny - 1);
-        const double bl = a.fac * get_bc_val(a.bcr, a.g, (a.g.LEFT + 1) + j);
-        const double br = a.fac * get_bc_val(a.bcr, a.g, (a.g.RIGHT + 1) + j);
-        double w[XPL];
+        bl = a.fac * bc_col(a.bcr, a.g, a.g.nx / 4, 1 + j);
+        br = a.fac * bc_col(a.bcr, a.g, 3 * a.g.nx / 4, 1 + j);
         if (rb || rt) {
 #pragma unroll
             for (int t = 0; t < XPL; t++) {
                 int n = lane + WAVE * t;
-                if (n < n1) w[t] = a.fac * get_bc_val(a.bcr, a.g, (rb ? a.g.BOT : a.g.TOP) + 1 + (n + 1));
+                if (n < n1) wr[t] = a.fac * bc_row(a.bcr, a.g, rb ? a.g.ny / 4 : 3 * a.g.ny / 4, n + 2);
             }
         }
+    }
+    const int ld = slot_doubles_t<PT>(a.pl);
+    double* s = lds + ntab + w * ld;
+    if (lane == 0) set_guards<PT>(s);
 #pragma unroll
-        for (int t = 0; t < XPL; t++) {
-            int n = lane + WAVE * t;
-            if (n < n1) {
-                double x = v[t];
-                if (rb || rt) x += w[t];
+    for (int t = 0; t < XPL; t++) {
+        int n = lane + WAVE * t;
+        if (n < n1) {
+            double x = v[t];
+            if constexpr (MODE == X_BC) {
+                if (rb || rt) x += wr[t];
                 if (rb && rt) x += a.fac * get_bc_val(a.bcr, a.g, (a.g.TOP + 1) + (n + 1));  // ny = 2 only
                 if (n == 0) x += bl;
                 if (n == n1 - 1) x += br;
-                s[lat<PT>(n)] = x;
             }
+            s[lat<PT>(n)] = x;
         }
-    } else {
+    }
+    if constexpr (MODE == X_TRIAL) {
+        // N1 and N0 of the rhs are in flight during the first transform
+        const double* n1p = a.n1 + b * a.lstride + (long)line * n1;
 #pragma unroll
         for (int t = 0; t < XPL; t++) {
             int n = lane + WAVE * t;
-            if (n < n1) s[lat<PT>(n)] = v[t];
+            if (n < n1) v[t] = n1p[n];
+        }
+    }
+    double u[XPL];
+    if constexpr (MODE == X_TRIAL) {
+        const double* n0p = a.n0 + b * a.lstride + (long)line * n1;
+#pragma unroll
+        for (int t = 0; t < XPL; t++) {
+            int n = lane + WAVE * t;
+            if (n < n1) u[t] = n0p[n];
         }
     }
     wave_sync();
@@ -139,25 +145,16 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
         // get_trial_state! (timestepping.jl:111-118): rhs = A Γ; rhs += (β1 dt) N1;
         // rhs += (β2 dt) N0; rhs += rhsbc, with rhsbc = apply_bc!(0, get_bc!(Γ[:, lev+1]), fac)
         const int lev = a.lev0 + b;
-        const double* n1p = a.n1 + b * a.lstride + (long)line * n1;
-        const double* n0p = a.n0 + b * a.lstride + (long)line * n1;
         const bool hasbc = lev < a.mg - 1;
         const double* gc = a.gam + (long)(lev + 1) * a.lstride;
         const double hc = level_hc(a.g.h, lev);
         const double fac = 0.25 * a.dt / (a.Re * (hc * hc));  // timestepping.jl:104
-        double u[XPL];
-#pragma unroll
-        for (int t = 0; t < XPL; t++) {
-            int n = lane + WAVE * t;
-            if (n < n1) { v[t] = n1p[n]; u[t] = n0p[n]; }
-        }
-        // column terms of rhsbc for this row, loaded next to N1 and N0
-        double bl = 0.0, br = 0.0;
+        double cl = 0.0, cr = 0.0;
         if (hasbc) {
-            bl = fac * get_bc_val(gc, a.g, (a.g.LEFT + 1) + j);
-            br = fac * get_bc_val(gc, a.g, (a.g.RIGHT + 1) + j);
+            cl = fac * bc_col(gc, a.g, a.g.nx / 4, 1 + j);
+            cr = fac * bc_col(gc, a.g, 3 * a.g.nx / 4, 1 + j);
         }
-        const bool rowbc = hasbc && (j == 1 || j == a.g.ny - 1);
+        const bool rowbc = hasbc && (rb || rt);
 #pragma unroll
         for (int t = 0; t < XPL; t++) {
             int n = lane + WAVE * t;
@@ -169,11 +166,14 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
                     // rhsbc = apply_bc!(0, bc, fac): row terms, then column terms
                     const int i = n + 1;
                     if (rowbc || i == 1 || i == a.g.nx - 1) {
-                        double rb = 0.0;
-                        if (rowbc) rb = add_bc_rows(rb, gc, a.g, fac, i, j);
-                        if (i == 1) rb += bl;
-                        if (i == a.g.nx - 1) rb += br;
-                        rhs += rb;
+                        double rbc = 0.0;
+                        if (rowbc) {
+                            if (rb) rbc += fac * bc_row(gc, a.g, a.g.ny / 4, 1 + i);
+                            if (rt) rbc += fac * bc_row(gc, a.g, 3 * a.g.ny / 4, 1 + i);
+                        }
+                        if (i == 1) rbc += cl;
+                        if (i == a.g.nx - 1) rbc += cr;
+                        rhs += rbc;
                     }
                 }
                 s[lat<PT>(n)] = rhs;
@@ -320,6 +320,14 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
     const int i0 = (lin - b * ntiles) * TI;
     const int ld = slot_doubles_t<PT>(a.pl);
     double* base = a.buf + b * a.lstride;
+    // cos(pi i/nx) of this wave's columns, loaded with the tile (not after it)
+    constexpr int NCOL = (TI + YWAVES - 1) / YWAVES;
+    double cxw[NCOL];
+#pragma unroll
+    for (int k = 0; k < NCOL; k++) {
+        const int c = w + YWAVES * k;
+        cxw[k] = (c < TI && i0 + c < n1) ? a.cx[i0 + c] : 0.0;
+    }
     // LDS: [DST tables][cos(pi j/ny) table][TI column slots]
     const int ntab = tab_doubles_t<PT>(a.pl);
     const DstTab tb = load_tab_t<PT>(lds, a.pl, tid, YWAVES * WAVE);
@@ -332,14 +340,6 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
     }
     double* tile = cyl + cy_doubles_t<PT>(n2);
     if (tid < TI) set_guards<PT>(tile + tid * ld);
-    // cos(pi i/nx) of this wave's columns, loaded with the tile (not after it)
-    constexpr int NCOL = (TI + YWAVES - 1) / YWAVES;
-    double cxw[NCOL];
-#pragma unroll
-    for (int k = 0; k < NCOL; k++) {
-        const int c = w + YWAVES * k;
-        cxw[k] = (c < TI && i0 + c < n1) ? a.cx[i0 + c] : 0.0;
-    }
     // tile load: thread (r0, c) moves rows r0, r0 + RPI, ... of column c; CH
     // loads per thread are in flight before their LDS writes
     constexpr int RPI = YWAVES * WAVE / TI;
