@@ -74,6 +74,13 @@ struct ibpm_prob {
     double* hist = nullptr;
     size_t hist_cap = 0;
     std::vector<void*> allocs;
+    // in-step timing of the all-level y-pass launches (IBPM_OPT_PROFILE):
+    // event pairs recorded on the problem's stream, summed after each call
+    bool prof = false;
+    std::vector<hipEvent_t> pev;
+    size_t pev_used = 0;
+    double prof_ms = 0.0;
+    long prof_n = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -355,6 +362,39 @@ static void project_forces(ibpm_prob* P, const double* f, double* Gout, bool sub
     lap_level(P, Y_AINV, 0, P->Gtmp, Gout, nullptr, sub);
 }
 
+// An all-level y-pass of the step, bracketed by a pair of events on the
+// problem's stream when in-step profiling is on.
+static void timed_ypass(ibpm_prob* P, const YArgs& y)
+{
+    hipEvent_t e[2] = {nullptr, nullptr};
+    if (P->prof) {
+        for (int k = 0; k < 2; k++) {
+            if (P->pev_used == P->pev.size()) {
+                hipEvent_t ev;
+                if (hipEventCreate(&ev) != hipSuccess) break;
+                P->pev.push_back(ev);
+            }
+            e[k] = P->pev[P->pev_used++];
+        }
+    }
+    if (e[1]) (void)hipEventRecord(e[0], P->stream);
+    launch_ypass(y, P->ytile, P->stream);
+    if (e[1]) (void)hipEventRecord(e[1], P->stream);
+}
+
+// fold the recorded event pairs into the running sums (stream already drained)
+static void collect_profile(ibpm_prob* P)
+{
+    for (size_t k = 0; k + 1 < P->pev_used; k += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, P->pev[k], P->pev[k + 1]) == hipSuccess) {
+            P->prof_ms += ms;
+            P->prof_n++;
+        }
+    }
+    P->pev_used = 0;
+}
+
 // advance!(state, prob, t) -- timestepping.jl:35-67.  hist: optional CD/CL slot.
 static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, double* clh)
 {
@@ -385,7 +425,7 @@ static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, dou
     XArgs a = xargs(P);
     a.src = P->G; a.dst = P->T1; a.nlev = L; a.lev0 = 0;
     launch_xpass(X_PLAIN, a, s);
-    launch_ypass(yargs(P, Y_A, 0, L, P->T1), P->ytile, s);
+    timed_ypass(P, yargs(P, Y_A, 0, L, P->T1));
     XArgs t = xargs(P);
     t.src = P->T1; t.dst = P->T2; t.nlev = L; t.lev0 = 0;
     t.n1 = P->nl[out]; t.n0 = P->nl[P->nl2]; t.gam = P->G;
@@ -394,7 +434,7 @@ static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, dou
     t.dt = P->dt;
     t.Re = P->Re;
     launch_xpass(X_TRIAL, t, s);
-    launch_ypass(yargs(P, Y_AINV, 0, L, P->T2), P->ytile, s);
+    timed_ypass(P, yargs(P, Y_AINV, 0, L, P->T2));
     XArgs c = xargs(P);
     c.src = P->T2; c.dst = P->Gs; c.nlev = L; c.lev0 = 0;
     launch_xpass(X_PLAIN, c, s);
@@ -700,6 +740,7 @@ int ibpm_problem_destroy(ibpm_prob* P)
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     for (void* p : P->allocs) (void)hipFree(p);
     if (P->hist) (void)hipFree(P->hist);
+    for (hipEvent_t ev : P->pev) (void)hipEventDestroy(ev);
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
     return IBPM_OK;
@@ -851,6 +892,7 @@ int ibpm_advance_n(ibpm_prob* P, int nsteps, const double* t, const ibpm_step_sc
     std::vector<double> hh(2 * (size_t)nsteps * nb);
     HIPTRY(P->ctx, hipMemcpyAsync(hh.data(), P->hist, sizeof(double) * hh.size(), hipMemcpyDeviceToHost, P->stream));
     HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    collect_profile(P);
     int done = nsteps;
     for (int k = 0; k < nsteps && done == nsteps; k++)
         for (int b = 0; b < P->nbodies; b++)
@@ -1025,6 +1067,15 @@ int ibpm_set_option(ibpm_prob* P, int option, int value)
         (option == IBPM_OPT_YTILE_BATCH ? P->ytile : P->ytile_single) = value;
         return IBPM_OK;
     }
+    if (option == IBPM_OPT_PROFILE) {
+        if (value != 0 && value != 1) return fail(P->ctx, IBPM_E_ARG, "profile option is 0 or 1");
+        HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+        collect_profile(P);
+        P->prof = value == 1;
+        P->prof_ms = 0.0;
+        P->prof_n = 0;
+        return IBPM_OK;
+    }
     return fail(P->ctx, IBPM_E_ARG, "unknown option");
 }
 
@@ -1033,7 +1084,18 @@ int ibpm_get_option(ibpm_prob* P, int option, int* value)
     if (!P || !value) return IBPM_E_ARG;
     if (option == IBPM_OPT_YTILE_BATCH) *value = P->ytile;
     else if (option == IBPM_OPT_YTILE_SINGLE) *value = P->ytile_single;
+    else if (option == IBPM_OPT_PROFILE) *value = P->prof ? 1 : 0;
     else return fail(P->ctx, IBPM_E_ARG, "unknown option");
+    return IBPM_OK;
+}
+
+int ibpm_profile_stats(ibpm_prob* P, double* avg_ms, int64_t* count)
+{
+    if (!P || !avg_ms || !count) return IBPM_E_ARG;
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    collect_profile(P);
+    *avg_ms = P->prof_n ? P->prof_ms / (double)P->prof_n : 0.0;
+    *count = P->prof_n;
     return IBPM_OK;
 }
 

@@ -27,7 +27,7 @@ MOTION_FUNCTION = 2
 FIELD_Q, FIELD_Q0, FIELD_GAMMA, FIELD_PSI, FIELD_NONLIN1, FIELD_NONLIN2 = 0, 1, 2, 3, 4, 5
 FIELD_FB_TILDE, FIELD_FB, FIELD_CD, FIELD_CL = 6, 7, 8, 9
 
-OPT_YTILE_BATCH, OPT_YTILE_SINGLE = 0, 1
+OPT_YTILE_BATCH, OPT_YTILE_SINGLE, OPT_PROFILE = 0, 1, 2
 
 EXPORTED = [
     "ibpm_ctx_create", "ibpm_ctx_destroy", "ibpm_last_error", "ibpm_problem_create", "ibpm_problem_destroy",
@@ -35,7 +35,7 @@ EXPORTED = [
     "ibpm_state_download", "ibpm_state_init", "ibpm_base_flux", "ibpm_vort2flux", "ibpm_advance",
     "ibpm_advance_n", "ibpm_get_forces", "ibpm_compute_cfl", "ibpm_op_lapinv", "ibpm_op_dst2",
     "ibpm_op_nonlinear", "ibpm_op_vort2flux", "ibpm_op_regT", "ibpm_op_reg", "ibpm_op_rot", "ibpm_set_option",
-    "ibpm_get_option", "ibpm_time_kernel",
+    "ibpm_get_option", "ibpm_time_kernel", "ibpm_profile_stats",
 ]
 
 
@@ -108,6 +108,7 @@ def lib():
         L.ibpm_op_reg.argtypes = [_vp, _dp, _dp]
         L.ibpm_op_rot.argtypes = [_vp, _dp, _dp]
         L.ibpm_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp]
+        L.ibpm_profile_stats.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_int64)]
         L.ibpm_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.ibpm_get_option.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         _LIB = L

@@ -171,8 +171,14 @@ int ibpm_op_rot(ibpm_prob* prob, const double* q, double* G);
  * no kernels); used by the autotuner and the benchmark. */
 #define IBPM_OPT_YTILE_BATCH 0   /* columns per y-pass workgroup, all-level launches: 4 or 8 */
 #define IBPM_OPT_YTILE_SINGLE 1  /* columns per y-pass workgroup, one-level launches: 4 or 8 */
+#define IBPM_OPT_PROFILE 2       /* 1: time every all-level y-pass inside the steps (HIP events on the
+                                    problem's stream); setting it drains the stream and clears the sums */
 int ibpm_set_option(ibpm_prob* prob, int option, int value);
 int ibpm_get_option(ibpm_prob* prob, int option, int* value);
+/* Average device ms and count of the all-level y-pass launches timed inside
+ * the steps since IBPM_OPT_PROFILE was last set.  No counterpart in the
+ * reference (it has no kernels); the benchmark's roofline reads it. */
+int ibpm_profile_stats(ibpm_prob* prob, double* avg_ms, int64_t* count);
 
 /* Average device time in ms of `reps` launches of kernel `which` on the
  * problem's stream, bracketed by HIP events on that stream.  which:

@@ -210,6 +210,23 @@ def test_advance_equals_advance_n():
     assert np.array_equal(st2.Γ, G1) and np.array_equal(st2.q, q1) and cd[-1, 0] == cd1[0]
 
 
+def test_in_step_profile_counts_and_leaves_results_unchanged():
+    import ctypes
+    from ibpm import _capi as C
+    c, op, gp = build("tiny")
+    _, st = initial_states(c, op, gp)
+    ts = np.arange(6) * c["dt"]
+    gp2 = ibpm.IBProblem(c["grid"], c["bodies"], c["dt"], c["Re"], freestream=c["freestream"], Binv=gp.Binv)
+    _, st2 = initial_states(c, op, gp2)
+    gp2.ctx.check(C.lib().ibpm_set_option(gp2.h, C.OPT_PROFILE, 1))
+    cd, _, _ = ibpm.advance_n_(st, gp, ts)
+    cd2, _, _ = ibpm.advance_n_(st2, gp2, ts)
+    ms, n = ctypes.c_double(0.0), ctypes.c_int64(0)
+    gp2.ctx.check(C.lib().ibpm_profile_stats(gp2.h, ctypes.byref(ms), ctypes.byref(n)))
+    assert n.value == 2 * len(ts) and ms.value > 0.0  # A and Ainv y-passes of every step
+    assert np.array_equal(cd, cd2) and np.array_equal(st.Γ, st2.Γ)
+
+
 def test_config1_steps():
     """BASELINE configs[0] (200 x 200, 3 levels, Re 40 cylinder), 20 steps."""
     c, op, gp = build("config1")
