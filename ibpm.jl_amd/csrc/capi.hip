@@ -266,6 +266,8 @@ static XArgs xargs(ibpm_prob* P)
     a.pl = P->tx.plan;
     a.g = P->g;
     a.fac = 1.0;
+    a.zlo = 1;
+    a.zhi = P->g.n2;
     return a;
 }
 
@@ -296,10 +298,13 @@ static YArgs yargs(ibpm_prob* P, int op, int lev0, int nlev, double* buf)
 
 // x = L b for one level (lev 0-based), L one of Δinv / A / Ainv (laplacian.jl:24-76),
 // with optional apply_bc! prologue (bcr != null) and subtract epilogue.
-static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x, const double* bcr, bool sub)
+static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x, const double* bcr, bool sub,
+                      int zlo = 1, int zhi = 1 << 30)
 {
     hipStream_t s = P->stream;
     XArgs a = xargs(P);
+    a.zlo = zlo;
+    a.zhi = std::min(zhi, P->g.n2);
     a.src = b;
     a.dst = P->T1 + (long)lev * P->g.nG;
     a.nlev = 1;
@@ -359,7 +364,8 @@ static void project_forces(ibpm_prob* P, const double* f, double* Gout, bool sub
     sa.fb = f; sa.q = P->qreg; sa.ncell = P->sc_ncell;
     launch_scatter(sa, s);
     launch_rot(P->qreg, P->Gtmp, P->rb_i0, P->rb_i1, P->rb_j0, P->rb_j1, P->g, s);
-    lap_level(P, Y_AINV, 0, P->Gtmp, Gout, nullptr, sub);
+    // Gtmp is zero outside the rows rb_j0 - 1 .. rb_j1 - 1 of the rot! box
+    lap_level(P, Y_AINV, 0, P->Gtmp, Gout, nullptr, sub, P->rb_j0 - 1, P->rb_j1 - 1);
 }
 
 // An all-level y-pass of the step, bracketed by a pair of events on the
@@ -445,11 +451,10 @@ static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, dou
         GatherArgs ga{};
         ga.q = P->qs1; ga.q0 = P->q0; ga.w = P->w; ga.bidx = P->bidx; ga.out = P->Ftmp; ga.nbt = P->nbt; ga.g = g;
         launch_gather(ga, s);
-        launch_gemv_rows(P->BinvT, P->Ftmp, P->Fb, P->nf, s);
         ForceArgs fa{};
         fa.Fb = P->Fb; fa.ds = P->ds_d; fa.boff = P->boff_d; fa.fb = P->fbv; fa.CD = P->CD; fa.CL = P->CL;
         fa.cd_hist = cdh; fa.cl_hist = clh; fa.nbodies = P->nbodies; fa.nbt = P->nbt; fa.hdt = g.h / P->dt;
-        launch_forces(fa, s);
+        launch_gemv_forces(P->BinvT, P->Ftmp, P->Fb, P->nf, fa, s);
         // ---- project_circ! (timestepping.jl:204-223): Γ = Γs; Γ[:,1] -= Ainv rot(reg(F̃b))
         project_forces(P, P->Fb, P->Gs, true);
     }
@@ -983,6 +988,7 @@ int ibpm_op_dst2(ibpm_prob* P, const double* x, double* y)
     std::swap(gt.n1, gt.n2);
     XArgs b = xargs(P);
     b.g = gt;
+    b.zhi = gt.n2;
     b.pl = P->ty.plan;
     double* dt2 = io.put(ht.data(), n);
     double* dt3 = io.put(nullptr, n);

@@ -25,6 +25,9 @@ struct XArgs {
     const double* gam;    // state Γ (all levels) for rhsbc of level lev from lev+1
     double c1, c2;
     double dt, Re;        // rhsbc factor 0.25 dt / (Re hc^2) is formed per level in the kernel
+    // X_PLAIN: input lines j (1-based) outside [zlo, zhi] are known to be zero;
+    // their output lines are written as zeros without a transform
+    int zlo, zhi;
     DstPlan pl;
     GridDev g;
 };
@@ -106,8 +109,6 @@ struct ScatterArgs {
 };
 void launch_scatter(const ScatterArgs& a, hipStream_t s);
 
-void launch_gemv_rows(const double* AT, const double* x, double* y, int n, hipStream_t s);
-
 struct ForceArgs {
     const double* Fb;
     const double* ds;
@@ -120,7 +121,8 @@ struct ForceArgs {
     int nbodies, nbt;
     double hdt;       // h / dt
 };
-void launch_forces(const ForceArgs& a, hipStream_t s);
+// y = Binv x (AT = Binv transposed, row-major rows), then update_stress! on y
+void launch_gemv_forces(const double* AT, const double* x, double* y, int n, const ForceArgs& fa, hipStream_t s);
 
 struct BaseFluxArgs {
     double* q0;

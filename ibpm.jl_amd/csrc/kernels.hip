@@ -81,6 +81,13 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
     const DstTab tb = load_tab_t<PT>(lds, a.pl, threadIdx.x, XWAVES * WAVE);
     __syncthreads();
     if (line >= n2) return;
+    if constexpr (MODE == X_PLAIN) {
+        if (j < a.zlo || j > a.zhi) {  // zero line: its transform is zero
+            double* dz = a.dst + b * a.lstride + (long)line * n1;
+            for (int n = lane; n < n1; n += WAVE) dz[n] = 0.0;
+            return;
+        }
+    }
     // every global load of the line is in flight before the first wait
     const double* src = a.src + b * a.lstride + (long)line * n1;
     double v[XPL];
@@ -626,8 +633,11 @@ void launch_curl(const CurlArgs& a, hipStream_t s)
 __global__ void __launch_bounds__(256) k_coarsify(CoarsifyArgs a)
 {
     const int n1 = a.g.n1;
-    const int ai = blockIdx.x * blockDim.x + threadIdx.x;
-    const int bj = blockIdx.y * blockDim.y + threadIdx.y;
+    // logical block (64 x 4 coarse points), XCD-contiguous (shared fine rows in one L2)
+    const int nbx = (a.ci + 63) / 64;
+    const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
+    const int ai = (lin % nbx) * 64 + threadIdx.x;
+    const int bj = (lin / nbx) * 4 + threadIdx.y;
     if (ai >= a.ci || bj >= a.cj) return;
     const int i = a.i0 + 2 * ai, ic = a.ic0 + ai;
     const int j = a.j0 + 2 * bj, jc = a.jc0 + bj;
@@ -640,9 +650,8 @@ __global__ void __launch_bounds__(256) k_coarsify(CoarsifyArgs a)
 void launch_coarsify(const CoarsifyArgs& a, hipStream_t s)
 {
     if (a.ci <= 0 || a.cj <= 0) return;
-    dim3 blk(64, 4);
-    dim3 grid((a.ci + 63) / 64, (a.cj + 3) / 4);
-    hipLaunchKernelGGL(k_coarsify, grid, blk, 0, s, a);
+    const int nblocks = ((a.ci + 63) / 64) * ((a.cj + 3) / 4);
+    hipLaunchKernelGGL(k_coarsify, dim3(NXCD * xcd_nper(nblocks)), dim3(64, 4), 0, s, a);
 }
 
 // ===========================================================================
@@ -724,33 +733,13 @@ void launch_scatter(const ScatterArgs& a, hipStream_t s)
 }
 
 // ===========================================================================
-// y = A x with A given row-major (AT = transpose of the column-major matrix);
-// one wavefront per row.  (prob.Binv * F̃b, timestepping.jl:160)
+// F̃b = Binv (E (q + q0)) (boundary_forces!, timestepping.jl:160), then
+// update_stress! (timestepping.jl:330-354) in a second, one-block launch.
+// (A single launch whose last work-group forms the forces needs an
+// agent-scope release per work-group -- an L2 writeback -- and measured
+// slower than the extra launch.)
 // ===========================================================================
-__global__ void __launch_bounds__(256) k_gemv_rows(const double* __restrict__ AT, const double* __restrict__ x,
-                                                   double* __restrict__ y, int n)
-{
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int r = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    if (r >= n) return;
-    const double* row = AT + (long)r * n;
-    double acc = 0.0;
-    for (int c = lane; c < n; c += WAVE) acc = fma(row[c], x[c], acc);
-    acc = wave_sum(acc);
-    if (lane == 0) y[r] = acc;
-}
-
-void launch_gemv_rows(const double* AT, const double* x, double* y, int n, hipStream_t s)
-{
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_gemv_rows, dim3((n + 3) / 4), dim3(256), 0, s, AT, x, y, n);
-}
-
-// ===========================================================================
-// update_stress! (timestepping.jl:330-354): fb = F̃b (h/dt) ./ ds and
-// CD = 2 sum(ds .* fb[:,1]), CL = 2 sum(ds .* fb[:,2]) per body.  One block.
-// ===========================================================================
-__global__ void __launch_bounds__(256) k_forces(ForceArgs a)
+__device__ __forceinline__ void forces_block(const ForceArgs& a)
 {
     __shared__ double red[2][4];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
@@ -781,10 +770,43 @@ __global__ void __launch_bounds__(256) k_forces(ForceArgs a)
     }
 }
 
-void launch_forces(const ForceArgs& a, hipStream_t s)
+// GEMV: one wavefront per row of Binv (given transposed: AT row r = row r of
+// Binv), each lane summing its columns in increasing order
+__global__ void __launch_bounds__(256) k_gemv_rows(const double* __restrict__ AT, const double* __restrict__ x,
+                                                   double* __restrict__ y, int n)
 {
-    if (a.nbodies == 0) return;
-    hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, s, a);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int r = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    if (r >= n) return;
+    const double* row = AT + (long)r * n;
+    double acc = 0.0;
+    // loads of a chunk in flight before its (in-order) fma chain
+    constexpr int U = 8;
+    for (int c0 = 0; c0 < n; c0 += U * WAVE) {
+        double av[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = c0 + lane + u * WAVE;
+            av[u] = c < n ? row[c] : 0.0;
+            xv[u] = c < n ? x[c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (c0 + lane + u * WAVE < n) acc = fma(av[u], xv[u], acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) y[r] = acc;
+}
+
+// update_stress!: fb = F̃b (h/dt) ./ ds, CD = 2 sum(ds .* fb[:,1]),
+// CL = 2 sum(ds .* fb[:,2]) per body; one block
+__global__ void __launch_bounds__(256) k_forces(ForceArgs fa) { forces_block(fa); }
+
+void launch_gemv_forces(const double* AT, const double* x, double* y, int n, const ForceArgs& fa, hipStream_t s)
+{
+    if (n == 0 || fa.nbodies == 0) return;
+    hipLaunchKernelGGL(k_gemv_rows, dim3((n + 3) / 4), dim3(256), 0, s, AT, x, y, n);
+    hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, s, fa);
 }
 
 // ===========================================================================
