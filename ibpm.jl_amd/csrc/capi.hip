@@ -376,8 +376,8 @@ static void timed_ypass(ibpm_prob* P, const YArgs& y)
     if (P->prof) {
         for (int k = 0; k < 2; k++) {
             if (P->pev_used == P->pev.size()) {
-                hipEvent_t ev;
-                if (hipEventCreate(&ev) != hipSuccess) break;
+                hipEvent_t ev;  // timing only: no system-scope fence (no cache writeback per record)
+                if (hipEventCreateWithFlags(&ev, hipEventDisableSystemFence) != hipSuccess) break;
                 P->pev.push_back(ev);
             }
             e[k] = P->pev[P->pev_used++];
