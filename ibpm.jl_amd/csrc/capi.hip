@@ -53,6 +53,10 @@ struct ibpm_prob {
     // timestepping.jl:262-269): kernels then take the values, not the array
     bool q0_uniform = true;
     double q0vx = 0.0, q0vy = 0.0;  // q0 = (q0vx, q0vy) hc on every level
+    // LinearizedIBProblem (problem-types.jl:47-87): base flux QB (avg_flux of
+    // the base state) and base circulation ΓB on the device; q0 stays zero
+    bool lin = false;
+    double *QB = nullptr, *GB = nullptr;
     int ytile = 8;         // y-pass columns per workgroup, all-level launches
     int ytile_single = 4;  // same, one-level launches (vort2flux chain)
 
@@ -276,6 +280,8 @@ static void set_nl_q0(const ibpm_prob* P, NLArgs& na)
     na.q0_uniform = P->q0_uniform ? 1 : 0;
     na.q0vx = P->q0vx;
     na.q0vy = P->q0vy;
+    na.QB = P->lin ? P->QB : nullptr;
+    na.GB = P->lin ? P->GB : nullptr;
 }
 
 static YArgs yargs(ibpm_prob* P, int op, int lev0, int nlev, double* buf)
@@ -751,6 +757,42 @@ int ibpm_problem_destroy(ibpm_prob* P)
     return IBPM_OK;
 }
 
+int ibpm_problem_linearize(ibpm_prob* P, const double* qB, const double* q0B, const double* GammaB)
+{
+    if (!P || !qB || !q0B || !GammaB) return IBPM_E_ARG;
+    if (P->motion != IBPM_MOTION_STATIC)
+        return fail(P->ctx, IBPM_E_UNSUPPORTED, "LinearizedIBProblem is defined for static bodies only");
+    const GridDev& g = P->g;
+    const int nx = g.nx, ny = g.ny, mg = P->mg;
+    const long nq = g.nq, nG = g.nG, nu = g.nu;
+    // QB[:, lev] = avg_flux(base_state, base_prob; lev) -- nonlinear.jl:17-38,
+    // with the reference's order of the four-term sums; other entries zero
+    std::vector<double> QB((size_t)(nq * mg), 0.0);
+    for (int lev = 0; lev < mg; lev++) {
+        const double* q = qB + (long)lev * nq;
+        const double* q0 = q0B + (long)lev * nq;
+        double* Q = QB.data() + (long)lev * nq;
+        for (int j = 2; j <= ny; j++)
+            for (int i = 1; i <= nx + 1; i++) {
+                const long p = (i - 1) + (long)(nx + 1) * (j - 1), pm = p - (nx + 1);
+                Q[p] = ((q[p] + q0[p]) + q[pm]) + q0[pm];
+            }
+        for (int j = 1; j <= ny + 1; j++)
+            for (int i = 2; i <= nx; i++) {
+                const long p = nu + (i - 1) + (long)nx * (j - 1), pm = p - 1;
+                Q[p] = ((q[p] + q0[p]) + q[pm]) + q0[pm];
+            }
+    }
+    if (!P->QB) HIPTRY(P->ctx, dalloc(P, &P->QB, (size_t)(nq * mg)));
+    if (!P->GB) HIPTRY(P->ctx, dalloc(P, &P->GB, (size_t)(nG * mg)));
+    HIPTRY(P->ctx, hipMemcpyAsync(P->QB, QB.data(), sizeof(double) * QB.size(), hipMemcpyHostToDevice, P->stream));
+    HIPTRY(P->ctx, hipMemcpyAsync(P->GB, GammaB, sizeof(double) * (size_t)(nG * mg), hipMemcpyHostToDevice, P->stream));
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    P->lin = true;
+    ibpm_step_scalars z{};
+    return ibpm_base_flux(P, &z);
+}
+
 int ibpm_problem_get_info(const ibpm_prob* P, ibpm_problem_info* info)
 {
     if (!P || !info) return IBPM_E_ARG;
@@ -828,6 +870,14 @@ int ibpm_state_download(ibpm_prob* P, int field, double* host, size_t n)
 int ibpm_base_flux(ibpm_prob* P, const ibpm_step_scalars* s)
 {
     if (!P || !s) return IBPM_E_ARG;
+    if (P->lin) {
+        // static bodies of a linearized problem: q0 = 0 (timestepping.jl:238-250)
+        launch_base_flux_inertial(P->q0, P->mg, 0.0, 0.0, P->g, P->stream);
+        P->q0_uniform = true;
+        P->q0vx = P->q0vy = 0.0;
+        HIPTRY(P->ctx, hipGetLastError());
+        return IBPM_OK;
+    }
     if (P->motion == IBPM_MOTION_MOVING_GRID) {
         double Om = -s->theta_dot, al = -s->theta;
         double Ux0 = s->U * std::cos(al) - s->V * std::sin(al);
@@ -1010,6 +1060,8 @@ int ibpm_op_nonlinear(ibpm_prob* P, int lev, const double* q, const double* q0, 
     if (!dq || !dq0 || !dG || !dout) return fail(P->ctx, IBPM_E_HIP, "alloc");
     NLArgs na{};
     na.q = dq; na.q0 = dq0; na.G = dG; na.out = dout; na.nlev = 1; na.lev0 = lev - 1; na.mg = P->mg; na.g = P->g;
+    na.QB = P->lin ? P->QB : nullptr;  // the given q0 is streamed (q0_uniform stays 0)
+    na.GB = P->lin ? P->GB : nullptr;
     launch_nonlinear(na, P->stream);
     return get_back(P, out, dout, P->g.nG);
 }

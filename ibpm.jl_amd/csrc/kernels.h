@@ -62,6 +62,11 @@ struct NLArgs {
     // so the kernel need not stream q0
     int q0_uniform;
     double q0vx, q0vy;
+    // LinearizedIBProblem (problem-types.jl:47-87): when QB != null the
+    // product is Q x ΓB + QB x Γ with Q = avg_flux of q alone (nonlinear.jl:45-65,
+    // 144-163); QB (nq per level, averaged) and GB (ΓB, nG per level)
+    const double* QB;
+    const double* GB;
     GridDev g;
 };
 void launch_nonlinear(const NLArgs& a, hipStream_t s);

@@ -461,6 +461,7 @@ void launch_ypass(const YArgs& a, int tile, hipStream_t s)
 // ===========================================================================
 constexpr int NL_JS = 8;  // output rows per thread (vertical strip)
 
+template <bool LIN>
 __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
 {
     const GridDev& g = a.g;
@@ -489,70 +490,107 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
 
     // boundary circulation get_bc!(Γ[:, lev+1]) .* 0.25 (nonlinear.jl:186-188)
     auto BCV = [&](int k) -> double { return hasbc ? get_bc_val(Gc, g, k) * 0.25 : 0.0; };
-    // Γ extended by its boundary ring: GAX(ii, jj), ii = 0..nx, jj = 0..ny; the
-    // ring carries the values direct_product_loops! (nonlinear.jl:73-114) uses
-    // at the edges (corners are never read)
-    auto GAX = [&](int ii, int jj) -> double {
+    // a circulation field extended by that boundary ring: X(ii, jj), ii = 0..nx,
+    // jj = 0..ny; the ring carries the values direct_product_loops!
+    // (nonlinear.jl:73-114) uses at the edges (corners are never read)
+    auto EXT = [&](const double* __restrict__ F, int ii, int jj) -> double {
         if (jj == 0) return BCV(g.BOT + ii + 1);
         if (jj == ny) return BCV(g.TOP + ii + 1);
         if (ii == 0) return BCV(g.LEFT + jj + 1);
         if (ii == nx) return BCV(g.RIGHT + jj + 1);
-        return G[(ii - 1) + (long)n1 * (jj - 1)];
+        return F[(ii - 1) + (long)n1 * (jj - 1)];
     };
+    auto GAX = [&](int ii, int jj) -> double { return EXT(G, ii, jj); };
     // flux components with the base flux: x-flux (ii, jj) and y-flux (ii, jj)
     auto QXV = [&](int ii, int jj, double& f, double& f0) {
         const long p = (ii - 1) + (long)(nx + 1) * (jj - 1);
         f = q[p];
-        f0 = u0 ? cx0 : q0[p];
+        f0 = (LIN || u0) ? cx0 : q0[p];
     };
     auto QYV = [&](int ii, int jj, double& f, double& f0) {
         const long p = nu + (ii - 1) + (long)nx * (jj - 1);
         f = q[p];
-        f0 = u0 ? cy0 : q0[p];
+        f0 = (LIN || u0) ? cy0 : q0[p];
     };
-    // avg_flux (nonlinear.jl:17-38): Qy(ii, jj) over (ii, jj), (ii-1, jj); Qx over (ii, jj), (ii, jj-1)
+    // avg_flux (nonlinear.jl:17-38; linearized :45-65 without the base flux):
+    // Qy(ii, jj) over (ii, jj), (ii-1, jj); Qx over (ii, jj), (ii, jj-1)
     auto QY = [&](int ii, int jj) -> double {
         double f, f0, fm, fm0;
         QYV(ii, jj, f, f0);
         QYV(ii - 1, jj, fm, fm0);
-        return ((f + f0) + fm) + fm0;
+        if constexpr (LIN) return f + fm;
+        else return ((f + f0) + fm) + fm0;
     };
+    // linearized problem: base flux QB (already averaged) and ΓB with the ring
+    const double* __restrict__ QBl = LIN ? a.QB + (long)lev * nq : nullptr;
+    const double* __restrict__ GBl = LIN ? a.GB + (long)lev * g.nG : nullptr;
+    auto QBX = [&](int ii, int jj) -> double { return QBl[(ii - 1) + (long)(nx + 1) * (jj - 1)]; };
+    auto QBY = [&](int ii, int jj) -> double { return QBl[nu + (ii - 1) + (long)nx * (jj - 1)]; };
+    auto GBX = [&](int ii, int jj) -> double { return EXT(GBl, ii, jj); };
+
     // state carried from row j-1 to row j
     double qxr[3], q0r[3];  // x-flux (i-1..i+1, j-1)
-    double gr[3];           // GAX(i-2..i, j-1)
+    double gr[3], gbr[3];   // GAX(i-2..i, j-1), GBX(i-2..i, j-1)
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         QXV(i - 1 + d, j0 - 1, qxr[d], q0r[d]);
         gr[d] = GAX(i - 2 + d, j0 - 1);
+        gbr[d] = LIN ? GBX(i - 2 + d, j0 - 1) : 0.0;
     }
     double qyj = QY(i, j0);  // Qy(i, j)
-    // fqx(i, j-1)
-    double fqxa = qyj * gr[1] + QY(i, j0 - 1) * GAX(i - 1, j0 - 2);
+    double qbyj = LIN ? QBY(i, j0) : 0.0;
+    // fq_x(i, j-1): Q x Γ, or Q x ΓB + QB x Γ
+    double fqxa;
+    if constexpr (LIN)
+        fqxa = (qyj * gbr[1] + QY(i, j0 - 1) * GBX(i - 1, j0 - 2)) + (qbyj * gr[1] + QBY(i, j0 - 1) * GAX(i - 1, j0 - 2));
+    else
+        fqxa = qyj * gr[1] + QY(i, j0 - 1) * GAX(i - 1, j0 - 2);
     const double sl = 1.0 / (4.0 * (hcl * hcl));  // nonlinear.jl:197-198
     double* __restrict__ out = a.out + (long)b * g.nG;
 #pragma unroll
     for (int t = 0; t < NL_JS; t++) {
         const int j = j0 + t;
         if (j > j1) break;
-        double qxn[3], q0n[3], gn[3], QX[3];
+        double qxn[3], q0n[3], gn[3], QX[3], gbn[3], qbx[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             QXV(i - 1 + d, j, qxn[d], q0n[d]);
-            QX[d] = ((qxn[d] + q0n[d]) + qxr[d]) + q0r[d];
+            if constexpr (LIN) {
+                QX[d] = qxn[d] + qxr[d];
+                qbx[d] = QBX(i - 1 + d, j);
+            } else {
+                QX[d] = ((qxn[d] + q0n[d]) + qxr[d]) + q0r[d];
+            }
         }
         gn[1] = GAX(i - 1, j);
+        if constexpr (LIN) gbn[1] = GBX(i - 1, j);
         if (j < ny) {
             gn[0] = GAX(i - 2, j);
             gn[2] = GAX(i, j);
+            if constexpr (LIN) {
+                gbn[0] = GBX(i - 2, j);
+                gbn[2] = GBX(i, j);
+            }
         } else {
-            gn[0] = gn[2] = 0.0;  // row ny is the top ring: only GAX(i-1, ny) is read
+            gn[0] = gn[2] = 0.0;  // row ny is the top ring: only X(i-1, ny) is read
+            if constexpr (LIN) gbn[0] = gbn[2] = 0.0;
         }
         const double qyn = QY(i, j + 1);
         // direct_product_loops!: fqx(i, j) = Qy(i, j+1) Γ(i-1, j) + Qy(i, j) Γ(i-1, j-1);
-        // fqy(ii, j) = -(Qx(ii+1, j) Γ(ii, j-1) + Qx(ii, j) Γ(ii-1, j-1)) for ii = i, i-1
-        const double fqxb = qyn * gn[1] + qyj * gr[1];
-        const double fqya = (QX[2] * gr[2] + QX[1] * gr[1]) * -1.0;
-        const double fqyb = (QX[1] * gr[1] + QX[0] * gr[0]) * -1.0;
+        // fqy(ii, j) = -(Qx(ii+1, j) Γ(ii, j-1) + Qx(ii, j) Γ(ii-1, j-1)) for ii = i, i-1;
+        // the linearized fq is the sum of the (Q, ΓB) and (QB, Γ) products
+        double fqxb, fqya, fqyb;
+        if constexpr (LIN) {
+            const double qbyn = QBY(i, j + 1);
+            fqxb = (qyn * gbn[1] + qyj * gbr[1]) + (qbyn * gn[1] + qbyj * gr[1]);
+            fqya = ((QX[2] * gbr[2] + QX[1] * gbr[1]) * -1.0) + ((qbx[2] * gr[2] + qbx[1] * gr[1]) * -1.0);
+            fqyb = ((QX[1] * gbr[1] + QX[0] * gbr[0]) * -1.0) + ((qbx[1] * gr[1] + qbx[0] * gr[0]) * -1.0);
+            qbyj = qbyn;
+        } else {
+            fqxb = qyn * gn[1] + qyj * gr[1];
+            fqya = (QX[2] * gr[2] + QX[1] * gr[1]) * -1.0;
+            fqyb = (QX[1] * gr[1] + QX[0] * gr[0]) * -1.0;
+        }
         // rot! (curl.jl:22-34) of fq, scaled by 1/(4 hc^2)
         const double r = (fqxa - fqxb) + (fqya - fqyb);
         out[(i - 2) + (long)n1 * (j - 2)] = r * sl;
@@ -563,6 +601,7 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
             qxr[d] = qxn[d];
             q0r[d] = q0n[d];
             gr[d] = gn[d];
+            if constexpr (LIN) gbr[d] = gbn[d];
         }
     }
 }
@@ -571,7 +610,8 @@ void launch_nonlinear(const NLArgs& a, hipStream_t s)
 {
     dim3 blk(64, 4);  // k_nonlinear derives its tile from this shape
     const int nblocks = ((a.g.nx - 1 + 63) / 64) * (((a.g.ny - 1 + NL_JS - 1) / NL_JS + 3) / 4) * a.nlev;
-    hipLaunchKernelGGL(k_nonlinear, dim3(NXCD * xcd_nper(nblocks)), blk, 0, s, a);
+    if (a.QB) hipLaunchKernelGGL(k_nonlinear<true>, dim3(NXCD * xcd_nper(nblocks)), blk, 0, s, a);
+    else hipLaunchKernelGGL(k_nonlinear<false>, dim3(NXCD * xcd_nper(nblocks)), blk, 0, s, a);
 }
 
 // ===========================================================================

@@ -31,6 +31,7 @@ OPT_YTILE_BATCH, OPT_YTILE_SINGLE, OPT_PROFILE = 0, 1, 2
 
 EXPORTED = [
     "ibpm_ctx_create", "ibpm_ctx_destroy", "ibpm_last_error", "ibpm_problem_create", "ibpm_problem_destroy",
+    "ibpm_problem_linearize",
     "ibpm_problem_get_info", "ibpm_problem_get_binv", "ibpm_problem_get_body_idx", "ibpm_state_upload",
     "ibpm_state_download", "ibpm_state_init", "ibpm_base_flux", "ibpm_vort2flux", "ibpm_advance",
     "ibpm_advance_n", "ibpm_get_forces", "ibpm_compute_cfl", "ibpm_op_lapinv", "ibpm_op_dst2",
@@ -87,6 +88,7 @@ def lib():
         L.ibpm_problem_create.argtypes = [_vp, ctypes.POINTER(GridDesc), ctypes.POINTER(BodyDesc), _dp,
                                           ctypes.POINTER(_vp)]
         L.ibpm_problem_destroy.argtypes = [_vp]
+        L.ibpm_problem_linearize.argtypes = [_vp, _dp, _dp, _dp]
         L.ibpm_problem_get_info.argtypes = [_vp, ctypes.POINTER(ProblemInfo)]
         L.ibpm_problem_get_binv.argtypes = [_vp, _dp, ctypes.c_size_t]
         L.ibpm_problem_get_body_idx.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t]

@@ -139,6 +139,35 @@ class IBProblem:
         return s
 
 
+class LinearizedIBProblem(IBProblem):
+    """LinearizedIBProblem(base_state, base_prob, dt) -- problem-types.jl:47-87.
+
+    Same grid, bodies and Re as base_prob (static bodies); prob.Binv is
+    base_prob's when dt is the same (get_AB gives the same matrix), else it is
+    assembled for dt.  The library keeps QB = avg_flux(base_state, base_prob)
+    per level and ΓB = base_state.Γ; nonlinear! forms Q x ΓB + QB x Γ and
+    base_flux! sets q0 = 0.  Like the reference, the freestream of the base
+    problem is carried along but not used by the step.
+    """
+
+    def __init__(self, base_state: "IBState", base_prob: IBProblem, dt: float, device: int = 0):
+        m = base_prob.model
+        same_dt = float(dt) == float(base_prob.scheme.dt)
+        super().__init__(m.grid, m.bodies, dt, m.Re, freestream=m.freestream, device=device,
+                         Binv=base_prob.Binv if same_dt else None)
+        if self.motion_kind != C.MOTION_STATIC:
+            raise NotImplementedError("LinearizedIBProblem is defined for static bodies")
+        qB = np.asfortranarray(base_state.q)
+        q0B = np.asfortranarray(base_state.q0)
+        GB = np.asfortranarray(base_state.Γ)
+        self.ctx.check(C.lib().ibpm_problem_linearize(self.h, C.ptr(qB), C.ptr(q0B), C.ptr(GB)),
+                       "ibpm_problem_linearize")
+        self.base_state = snapshot(base_state)
+
+    def scalars(self, t: float) -> C.StepScalars:
+        return C.StepScalars()  # unused by the linearized step (q0 = 0)
+
+
 _FIELDS = {"q": C.FIELD_Q, "q0": C.FIELD_Q0, "Γ": C.FIELD_GAMMA, "ψ": C.FIELD_PSI, "F̃b": C.FIELD_FB_TILDE,
            "CD": C.FIELD_CD, "CL": C.FIELD_CL}
 
@@ -270,28 +299,175 @@ def compute_cfl(state: IBState, prob: IBProblem) -> float:
     return v.value
 
 
-def run_sim_(t, state: IBState, prob: IBProblem, display_freq: int = 25, data=None, verbose: bool = False):
-    """run_sim!(t, state, prob) -- ibpm.jl:90-106: advance over the times t,
-    stop after a step whose CL is non-finite, compute the CFL every
-    display_freq steps.  `data` may be a callable (t, state) -> None called
-    after each chunk of display_freq steps.  Returns (CD, CL) histories."""
+# ---------------------------------------------------------------------------
+# save hooks (save_info.jl) and the run_sim! loop (ibpm.jl:90-106)
+# ---------------------------------------------------------------------------
+class UserVar:
+    """user_var (save_info.jl:1-9): values saved at times `t` by `fcn(t, state)`."""
+
+    def __init__(self, val, t, fcn):
+        self.val = val
+        self.t = np.atleast_1d(np.asarray(t, dtype=np.float64))
+        self.fcn = fcn
+
+
+def init_save_data(t, save_info, state):
+    """init_save_data(t, save_info, state) -- save_info.jl:11-94.
+
+    save_info None: one UserVar that keeps the whole state at t[end] (for a
+    restart).  Otherwise a dict with "save_fcns" (list of fcn(t, state)) and
+    optionally "save_times" (per function: None or 0 -> every step; a float
+    -> every round(s/dt) dt from t[0]; a list -> the step times nearest to
+    each entry) and "save_types" (ignored: values are kept in lists).  The
+    reference's list branch picks argmin(t .- s) and then names an undefined
+    variable; this takes the nearest step time, which is what its comment says.
+    """
     t = np.asarray(t, dtype=np.float64)
+    if save_info is None:
+        return [UserVar([], t[-1], lambda tt, st: snapshot(st))]
+    dt = t[1] - t[0]
+    data = []
+    fcns = save_info["save_fcns"]
+    times = save_info.get("save_times")
+    for j, f in enumerate(fcns):
+        if times is None:
+            tsv = t[-1:]
+        else:
+            sv = times[j]
+            if sv is None or (np.isscalar(sv) and float(sv) == 0.0):
+                tsv = t
+            elif np.isscalar(sv):
+                step = round(float(sv) / dt) * dt
+                n = int(np.floor((t[-1] - t[0]) / step + 1e-9)) + 1
+                tsv = t[0] + step * np.arange(n)
+            else:
+                tsv = np.unique([t[int(np.argmin(np.abs(t - s)))] for s in sv])
+        data.append(UserVar([], tsv, f))
+    return data
+
+
+def save_data_(tnow, t, prob, state, data):
+    """save_data!(tnow, t, prob, state, data) -- save_info.jl:96-106: append
+    fcn(tnow, state) to every UserVar with a save time equal to tnow (to the
+    reference's relative tolerance, |ts - tnow| + 1 ≈ 1)."""
+    for d in data:
+        if np.any(np.isclose(np.abs(d.t - tnow) + 1.0, 1.0, rtol=np.sqrt(np.finfo(float).eps), atol=0.0)):
+            d.val.append(d.fcn(tnow, state))
+
+
+def _save_steps(t, data):
+    """Indices of t at which some UserVar saves."""
+    if not data:
+        return set()
+    idx = set()
+    tol = np.sqrt(np.finfo(float).eps)
+    for d in data:
+        for ts in d.t:
+            k = np.nonzero(np.isclose(np.abs(t - ts) + 1.0, 1.0, rtol=tol, atol=0.0))[0]
+            idx.update(int(i) for i in k)
+    return idx
+
+
+def run_sim_(t, state: IBState, prob: IBProblem, display_freq: int = 25, data=None, verbose: bool = False):
+    """run_sim!(t, state, prob; display_freq, data) -- ibpm.jl:90-106.
+
+    Advances over the times t on the device in chunks of steps, each chunk
+    ending at the next display or save step: stops after the first step
+    whose CL is non-finite; every display_freq steps stores compute_cfl in
+    state.cfl; at the save times of `data` (a list of UserVar from
+    init_save_data) calls save_data!.  Returns the (CD, CL) histories."""
+    t = np.asarray(t, dtype=np.float64)
+    saves = _save_steps(t, data or [])
+    stops = sorted(saves | {k for k in range(len(t)) if (k + 1) % display_freq == 0} | {len(t) - 1})
     cds, cls = [], []
     i = 0
-    while i < len(t):
-        chunk = t[i: i + display_freq]
+    for stop in stops:
+        if stop < i:
+            continue
+        chunk = t[i: stop + 1]
         cd, cl, done = advance_n_(state, prob, chunk)
         cds.append(cd)
         cls.append(cl)
         i += done
         if done < len(chunk):
             break
-        if len(chunk) == display_freq:
+        if (stop + 1) % display_freq == 0:
             state.cfl = compute_cfl(state, prob)
             if verbose:
-                print((float(chunk[-1]), state.CD, state.CL, state.cfl))
-        if data is not None:
-            data(float(chunk[-1]), state)
+                print((float(t[stop]), state.CD, state.CL, state.cfl))
+        if data and stop in saves:
+            save_data_(float(t[stop]), t, prob, state, data)
     cd = np.concatenate(cds) if cds else np.zeros((0, 1))
     cl = np.concatenate(cls) if cls else np.zeros((0, 1))
     return cd, cl
+
+
+# ---------------------------------------------------------------------------
+# restart: a full IBState round trip (state-types.jl:61-98 copy!)
+# ---------------------------------------------------------------------------
+class StateSnapshot:
+    """Host copy of every IBState field (copy! of state-types.jl:62-77):
+    q, q0, Γ, ψ, nonlin[1:2], F̃b, fb, CD, CL, xb, cfl, slip."""
+
+    FIELDS = ("q", "q0", "Γ", "ψ", "F̃b", "CD", "CL")
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def snapshot(state: IBState) -> StateSnapshot:
+    """Host copy of the device state."""
+    d = {k: np.array(state._get(k)) for k in StateSnapshot.FIELDS}
+    d["nonlin"] = [np.array(a) for a in state.nonlin]
+    d["fb"] = [np.array(a) for a in state.fb]
+    d["xb"] = [np.array(x) for x in state.xb]
+    d["cfl"], d["slip"] = state.cfl, state.slip
+    return StateSnapshot(**d)
+
+
+def copy_(w: IBState, v: StateSnapshot) -> IBState:
+    """copy!(w, v) with v a snapshot: uploads every field into w's device
+    state.  Continuing from it gives the same steps, bit for bit, as the
+    state the snapshot was taken from."""
+    # q0 is left alone when it already matches (the device then keeps its
+    # uniform-base-flux fast path for static bodies)
+    if not np.array_equal(w._get("q0"), v.q0):
+        w._set("q0", v.q0)
+    for k in ("q", "Γ", "ψ", "F̃b", "CD", "CL"):
+        w._set(k, getattr(v, k))
+    w.nonlin = v.nonlin
+    w.xb = [np.array(x) for x in v.xb]
+    w.cfl, w.slip = v.cfl, v.slip
+    w.invalidate()
+    return w
+
+
+def save_state(path, state: IBState, t: float = 0.0):
+    """Write a restart file (.npz, arrays only: no pickled objects)."""
+    s = snapshot(state)
+    g = state.prob.model.grid
+    arrays = {"q": s.q, "q0": s.q0, "Gamma": s.Γ, "psi": s.ψ, "Fb_tilde": s.F̃b, "CD": s.CD, "CL": s.CL,
+              "nonlin1": s.nonlin[0], "nonlin2": s.nonlin[1],
+              "meta": np.array([g.nx, g.ny, g.mg, g.h, t, s.cfl, s.slip], dtype=np.float64)}
+    for k, x in enumerate(s.xb):
+        arrays[f"xb{k}"] = x
+    for k, f in enumerate(s.fb):
+        arrays[f"fb{k}"] = f
+    np.savez(path, **arrays)
+
+
+def load_state(path, state: IBState) -> float:
+    """Read a restart file written by save_state into `state` (whose problem
+    must have the same grid); returns the saved time."""
+    with np.load(path, allow_pickle=False) as z:
+        g = state.prob.model.grid
+        nx, ny, mg, h, t, cfl, slip = z["meta"]
+        if (int(nx), int(ny), int(mg), float(h)) != (g.nx, g.ny, g.mg, g.h):
+            raise ValueError("restart file is for another grid")
+        nb = len(state.prob.model.bodies)
+        snap = StateSnapshot(q=z["q"], q0=z["q0"], Γ=z["Gamma"], ψ=z["psi"], F̃b=z["Fb_tilde"], CD=z["CD"],
+                             CL=z["CL"], nonlin=[z["nonlin1"], z["nonlin2"]],
+                             fb=[z[f"fb{k}"] for k in range(nb)], xb=[z[f"xb{k}"] for k in range(nb)],
+                             cfl=float(cfl), slip=float(slip))
+    copy_(state, snap)
+    return float(t)

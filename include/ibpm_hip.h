@@ -117,6 +117,14 @@ const char* ibpm_last_error(const ibpm_ctx* ctx);
 int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* grid, const ibpm_body_desc* bodies,
                         const double* binv, ibpm_prob** out);
 int ibpm_problem_destroy(ibpm_prob* prob);
+/* LinearizedIBProblem(base_state, base_prob, dt) -- problem-types.jl:47-87:
+ * turns `prob` (built with the base problem's grid, bodies, Re and this dt)
+ * into the problem linearized about the base state's q, q0 and Γ (nq x mg,
+ * nq x mg, nG x mg, column-major).  QB = avg_flux(base_state, base_prob) per
+ * level and ΓB = base Γ are kept on the device; nonlinear! then forms
+ * Q x ΓB + QB x Γ (nonlinear.jl:45-65,144-163) and base_flux! sets q0 = 0
+ * (timestepping.jl:238-250, also applied now).  Static bodies only. */
+int ibpm_problem_linearize(ibpm_prob* prob, const double* qB, const double* q0B, const double* GammaB);
 int ibpm_problem_get_info(const ibpm_prob* prob, ibpm_problem_info* info);
 /* prob.Binv (column-major), body_idx (nb_total x 2, column-major, Julia
  * 1-based values as computed by setup_reg, interface-coupling.jl:85-87). */

@@ -406,6 +406,10 @@ typedef struct {
     /* WorkingMemory (models.jl:79-106) */
     double *q1, *G1, *q2, *G2, *q3, *q4, *q5, *q6, *G3, *Gbc, *rhsbc, *Gwork, *wdst;
     double *bt_G, *bt_psi, *bt_q; /* get_Binv scratch */
+    /* LinearizedIBProblem (problem-types.jl:47-87): base flux QB (avg_flux of
+     * the base state, nq x mg) and base circulation ΓB (nG x mg); lin = 1 */
+    int lin;
+    double *QB, *GB;
 } oprob;
 
 /* dst_inv!(x, b, Λ, plan; scale) -- dst-inversion.jl:39-50, wrapped by
@@ -482,7 +486,20 @@ static void o_direct_product_loops(const og_grid* g, double* fq, const double* Q
         QY(fq, nx, j) = (QX(Q, nx, j) * GM(G, nx - 1, j - 1) + QX(Q, nx + 1, j) * BC(Gbc, R + j)) * -1.0;
 }
 
-/* nonlinear! -- nonlinear.jl:168-201.  Scales Gbc by 0.25 in place. */
+/* avg_flux -- nonlinear.jl:45-65 (LinearizedIBProblem method: no base flux) */
+static void o_avg_flux_lin(const og_grid* g, double* Q, const double* q)
+{
+    const int nx = g->nx, ny = g->ny, nu = g->nu;
+    for (long k = 0; k < g->nq; k++) Q[k] *= 0.0;
+    for (int j = 2; j <= ny; j++)
+        for (int i = 1; i <= nx + 1; i++) QX(Q, i, j) = QX(q, i, j) + QX(q, i, j - 1);
+    for (int j = 1; j <= ny + 1; j++)
+        for (int i = 2; i <= nx; i++) QY(Q, i, j) = QY(q, i, j) + QY(q, i - 1, j);
+}
+
+/* nonlinear! -- nonlinear.jl:168-201.  Scales Gbc by 0.25 in place.  For a
+ * LinearizedIBProblem the direct product is Q x ΓB + QB x Γ
+ * (direct_product!, nonlinear.jl:144-163), both with the same Γbc. */
 static void o_nonlinear(oprob* P, double* nonlin, const double* q, const double* q0, const double* G,
                         double* Gbc, int lev)
 {
@@ -491,9 +508,18 @@ static void o_nonlinear(oprob* P, double* nonlin, const double* q, const double*
     for (long k = 0; k < g->nq; k++) fq[k] *= 0.0;
     for (int k = 0; k < g->nbc; k++) Gbc[k] *= 0.25;
     double* Q = P->q2;
-    o_avg_flux(g, Q, q, q0);
+    if (P->lin) o_avg_flux_lin(g, Q, q);
+    else o_avg_flux(g, Q, q, q0);
     for (long k = 0; k < g->nq; k++) fq[k] *= 0.0;
-    o_direct_product_loops(g, fq, Q, G, Gbc);
+    if (P->lin) {
+        double* fq_tmp = P->q4;
+        for (long k = 0; k < g->nq; k++) fq_tmp[k] *= 0.0;
+        o_direct_product_loops(g, fq, Q, P->GB + (long)(lev - 1) * g->nG, Gbc);
+        o_direct_product_loops(g, fq_tmp, P->QB + (long)(lev - 1) * g->nq, G, Gbc);
+        for (long k = 0; k < g->nq; k++) fq[k] += fq_tmp[k];
+    } else {
+        o_direct_product_loops(g, fq, Q, G, Gbc);
+    }
     o_rot(g, nonlin, fq);
     double hc = g->h * (double)(1L << (lev - 1));
     double s = 1.0 / (4.0 * (hc * hc));
@@ -780,6 +806,7 @@ void oracle_problem_free(oprob* P)
     free(P->q1); free(P->G1); free(P->q2); free(P->G2); free(P->q3); free(P->q4); free(P->q5); free(P->q6);
     free(P->G3); free(P->Gbc); free(P->rhsbc); free(P->Gwork); free(P->wdst);
     free(P->bt_G); free(P->bt_psi); free(P->bt_q);
+    free(P->QB); free(P->GB);
     free(P);
 }
 
@@ -837,11 +864,34 @@ static void o_base_flux_moving(const oprob* P, double* q0, double U, double V, d
     }
 }
 
-/* scalars[] = {Ux, Uy, inclination, U, V, theta, theta_dot} evaluated at t */
+/* scalars[] = {Ux, Uy, inclination, U, V, theta, theta_dot} evaluated at t.
+ * A LinearizedIBProblem with static bodies has q0 = 0 (timestepping.jl:238-250). */
 void oracle_base_flux(const oprob* P, ostate* S, const double* sc)
 {
+    if (P->lin) {
+        for (long k = 0; k < P->g.nq * P->g.mg; k++) S->q0[k] = 0.0;
+        return;
+    }
     if (P->motion == 1) o_base_flux_moving(P, S->q0, sc[3], sc[4], sc[5], sc[6], sc[0], sc[1], sc[2]);
     else o_base_flux_inertial(P, S->q0, sc[0], sc[1], sc[2]);
+}
+
+/* LinearizedIBProblem(base_state, base_prob, dt) -- problem-types.jl:66-86:
+ * QB[:, lev] = avg_flux(base_state, base_prob; lev) (the IBProblem method,
+ * with the base q0), ΓB = base_state.Γ.  Static bodies only (returns -1
+ * otherwise). */
+int oracle_problem_linearize(oprob* P, const double* qB, const double* q0B, const double* GB)
+{
+    if (P->motion != 0) return -1;
+    const og_grid* g = &P->g;
+    const long nq = g->nq, nG = g->nG;
+    if (!P->QB) P->QB = (double*)calloc((size_t)(nq * g->mg), sizeof(double));
+    if (!P->GB) P->GB = (double*)calloc((size_t)(nG * g->mg), sizeof(double));
+    for (int lev = 1; lev <= g->mg; lev++)
+        o_avg_flux(g, P->QB + (long)(lev - 1) * nq, qB + (long)(lev - 1) * nq, q0B + (long)(lev - 1) * nq);
+    memcpy(P->GB, GB, sizeof(double) * (size_t)(nG * g->mg));
+    P->lin = 1;
+    return 0;
 }
 
 /* get_trial_state! -- timestepping.jl:86-129 */

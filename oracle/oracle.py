@@ -63,6 +63,7 @@ def lib():
         L.oracle_op_vort2flux.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.oracle_op_nonlinear.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.c_int]
         L.oracle_invert.argtypes = [_dp, ctypes.c_int]
+        L.oracle_problem_linearize.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.oracle_invert.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -194,6 +195,15 @@ class OracleProblem:
         out = np.zeros((self.nx - 1, self.ny - 1), order="F")
         lib().oracle_get_lam(self._h, _p(out))
         return out
+
+    def linearize(self, qB, q0B, GB):
+        """Turn this problem into LinearizedIBProblem(base_state, base_prob, dt)
+        (problem-types.jl:66-86) around the base state (q, q0, Γ)."""
+        qB = np.asfortranarray(qB, dtype=np.float64)
+        q0B = np.asfortranarray(q0B, dtype=np.float64)
+        GB = np.asfortranarray(GB, dtype=np.float64)
+        if lib().oracle_problem_linearize(self._h, _p(qB), _p(q0B), _p(GB)) != 0:
+            raise ValueError("LinearizedIBProblem needs static bodies")
 
     # --- state and stepping ----------------------------------------------
     def new_state(self, sc0):

@@ -227,6 +227,88 @@ def test_in_step_profile_counts_and_leaves_results_unchanged():
     assert np.array_equal(cd, cd2) and np.array_equal(st.Γ, st2.Γ)
 
 
+def test_restart_round_trip_is_bit_exact(tmp_path):
+    """20 steps straight == 10 steps, restart file, fresh problem, 10 steps."""
+    c, op, gp = build("small3", share_binv=True)
+    _, st = initial_states(c, op, gp)
+    ts = np.arange(20) * c["dt"]
+    ibpm.advance_n_(st, gp, ts[:10])
+    path = tmp_path / "restart.npz"
+    ibpm.save_state(path, st, t=float(ts[9]))
+    cd_a, cl_a, _ = ibpm.advance_n_(st, gp, ts[10:])
+    gp2 = ibpm.IBProblem(c["grid"], c["bodies"], c["dt"], c["Re"], freestream=c["freestream"], Binv=gp.Binv)
+    st2 = ibpm.IBState(gp2)
+    assert ibpm.load_state(path, st2) == float(ts[9])
+    cd_b, cl_b, _ = ibpm.advance_n_(st2, gp2, ts[10:])
+    assert np.array_equal(cd_a, cd_b) and np.array_equal(cl_a, cl_b)
+    for name in ("Γ", "ψ", "q"):
+        assert np.array_equal(getattr(st, name), getattr(st2, name)), name
+    assert all(np.array_equal(a, b) for a, b in zip(st.nonlin, st2.nonlin))
+
+
+def test_run_sim_saves_and_cfl():
+    c, op, gp = build("tiny", share_binv=True)
+    _, st = initial_states(c, op, gp)
+    t = np.arange(30) * c["dt"]
+    data = ibpm.init_save_data(t, {"save_fcns": [lambda tt, s: float(s.CD[0])], "save_times": [5 * c["dt"]]}, st)
+    cd, cl = ibpm.run_sim_(t, st, gp, display_freq=25, data=data)
+    assert cd.shape == (30, 1)
+    assert np.allclose(data[0].t, t[::5])
+    assert data[0].val == [float(v) for v in cd[::5, 0]]  # saved right after the step at each save time
+    assert st.cfl > 0  # computed after step 25 (display_freq)
+
+
+# ---------------------------------------------------------------------------
+# LinearizedIBProblem (problem-types.jl:47-87)
+# ---------------------------------------------------------------------------
+def _linearized(name, nbase=6):
+    """GPU base state after a few steps, the linearized problem on both sides."""
+    c, op, gp = build(name, share_binv=True)
+    _, st = initial_states(c, op, gp)
+    ibpm.advance_n_(st, gp, np.arange(nbase) * c["dt"])
+    g = c["grid"]
+    lp = ibpm.LinearizedIBProblem(st, gp, c["dt"])
+    olp = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], c["dt"], oracle_bodies(c), binv=gp.Binv)
+    olp.linearize(st.q, st.q0, st.Γ)
+    return c, lp, olp
+
+
+@pytest.mark.parametrize("name", ["tiny", "small3", "spec"])
+def test_linearized_nonlinear_bit_exact(name, rng):
+    from ibpm import _capi as C
+    c, lp, olp = _linearized(name)
+    g = c["grid"]
+    q = np.asfortranarray(rng.standard_normal((g.nq, g.mg)))
+    q0 = np.zeros((g.nq, g.mg), order="F")
+    G = np.asfortranarray(rng.standard_normal((g.nΓ, g.mg)))
+    for lev in range(1, g.mg + 1):
+        out = np.zeros(g.nΓ)
+        lp.ctx.check(C.lib().ibpm_op_nonlinear(lp.h, lev, C.ptr(q), C.ptr(q0), C.ptr(G), C.ptr(out)))
+        ref = olp.nonlinear(q, q0, G, lev)
+        assert np.array_equal(out, ref), (lev, relerr(out, ref))
+
+
+def test_linearized_steps():
+    c, lp, olp = _linearized("small3")
+    g = c["grid"]
+    st = ibpm.IBState(lp)
+    assert np.all(st.q0 == 0.0)  # base_flux! of the linearized problem
+    Gp = 1e-3 * gaussian_vortex_gamma(g, x0=0.9, y0=0.3, circ=0.5, width=0.3)
+    S = olp.new_state(np.zeros(7))
+    psi, q, G2 = olp.vort2flux(Gp)
+    S.G[:] = G2
+    S.psi[:] = psi
+    S.q[:] = q
+    st.Γ = Gp
+    ibpm.vort2flux_(st, lp)
+    ts = np.arange(20) * c["dt"]
+    cd_o, cl_o = olp.run(S, np.zeros((len(ts), 7)))
+    cd, cl, done = ibpm.advance_n_(st, lp, ts)
+    assert done == len(ts)
+    assert relerr(cd, cd_o) < STEP_TOL
+    compare_state(S, st, STEP_TOL)
+
+
 def test_config1_steps():
     """BASELINE configs[0] (200 x 200, 3 levels, Re 40 cylinder), 20 steps."""
     c, op, gp = build("config1")

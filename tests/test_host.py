@@ -100,3 +100,25 @@ def test_step_scalars_layout():
     s = _capi.StepScalars(1, 2, 3, 4, 5, 6, 7)
     assert ctypes.sizeof(s) == 7 * 8
     assert ctypes.sizeof(_capi.GridDesc) == 3 * 4 + 4 + 7 * 8  # padding after mg
+
+
+def test_init_save_data_default_keeps_final_state():
+    t = np.arange(10) * 0.01
+    data = ibpm.init_save_data(t, None, None)
+    assert len(data) == 1 and np.allclose(data[0].t, [0.09])
+
+
+def test_init_save_data_times():
+    t = np.arange(101) * 0.01
+    f = lambda tt, st: tt
+    data = ibpm.init_save_data(t, {"save_fcns": [f, f, f, f], "save_times": [None, 0.0, 0.05, [0.031, 0.5, 0.5004]]}, None)
+    assert len(data[0].t) == 101 and len(data[1].t) == 101
+    assert np.allclose(data[2].t, np.arange(21) * 0.05)          # every round(0.05/dt) dt from t[0]
+    assert np.allclose(data[3].t, [0.03, 0.5])                   # nearest step times, unique
+    from ibpm.problem import _save_steps
+    assert _save_steps(t, [data[2]]) == set(range(0, 101, 5))
+    assert _save_steps(t, [data[3]]) == {3, 50}
+    ibpm.save_data_(0.5, t, None, None, data)
+    assert [len(d.val) for d in data] == [1, 1, 1, 1]
+    ibpm.save_data_(0.51, t, None, None, data)
+    assert [len(d.val) for d in data] == [2, 2, 1, 1]

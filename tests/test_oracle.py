@@ -21,6 +21,7 @@ import numpy as np
 import pytest
 import scipy.fft as sfft
 
+import ibpm
 import oracle as O
 from cases import boundary_of, case, oracle_bodies, sc7
 from conftest import relerr
@@ -266,3 +267,66 @@ def test_step_conserves_zero_state_without_flow(small):
     S = P.new_state(np.zeros(7))
     P.advance(S, np.zeros(7))
     assert np.all(S.G == 0) and np.all(S.q == 0) and np.all(S.CD == 0)
+
+
+# ---------------------------------------------------------------------------
+# LinearizedIBProblem (problem-types.jl:47-87, nonlinear.jl:45-65,144-163)
+# ---------------------------------------------------------------------------
+def _lin_pair(mg=1, seed=3):
+    """A problem and its linearization about a random base state."""
+    grid = ibpm.make_grid(0.125, (-2.0, 4.0, -2.0, 2.0), mg=mg)
+    bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.1, n=24)]
+    args = (grid.h, boundary_of(grid), mg, 40.0, 0.02, [(b.xb, b.ds) for b in bodies])
+    eye = np.eye(2 * 24)
+    full = O.OracleProblem(*args, binv=eye)
+    lin = O.OracleProblem(*args, binv=eye)
+    rng = np.random.default_rng(seed)
+    qB = np.asfortranarray(rng.standard_normal((grid.nq, mg)))
+    q0B = np.asfortranarray(np.full((grid.nq, mg), 0.3))
+    GB = np.asfortranarray(rng.standard_normal((grid.nΓ, mg)))
+    lin.linearize(qB, q0B, GB)
+    return grid, full, lin, qB, q0B, GB, rng
+
+
+def test_linearized_nonlinear_is_the_cross_term_of_the_bilinear_product():
+    # one level: no boundary ring, so N(Q, Γ) = rot(Q x Γ)/(4h^2) is bilinear
+    grid, full, lin, qB, q0B, GB, rng = _lin_pair()
+    q = np.asfortranarray(rng.standard_normal((grid.nq, 1)))
+    G = np.asfortranarray(rng.standard_normal((grid.nΓ, 1)))
+    z = np.zeros_like(q)
+    tot = full.nonlinear(qB + q, q0B, GB + G, 1)
+    base = full.nonlinear(qB, q0B, GB, 1)
+    pert = full.nonlinear(q, z, G, 1)
+    cross = lin.nonlinear(q, z, G, 1)
+    scale = max(np.abs(tot).max(), np.abs(base).max(), np.abs(pert).max())
+    assert np.abs(cross - (tot - base - pert)).max() < 1e-12 * scale
+    assert np.abs(cross).max() > 1e-3 * scale  # not trivially zero
+
+
+def test_linearized_nonlinear_homogeneous_and_zero_base():
+    grid, full, lin, qB, q0B, GB, rng = _lin_pair(mg=2)
+    q = np.asfortranarray(rng.standard_normal((grid.nq, 2)))
+    G = np.asfortranarray(rng.standard_normal((grid.nΓ, 2)))
+    z = np.zeros_like(q)
+    # Homogeneous wherever the boundary ring is zero: on the coarsest level, and
+    # on level 1 when the coarse level of the perturbation is zero.  With a ring,
+    # the reference multiplies the perturbation's Q by the perturbation's own
+    # Γbc in the Q x ΓB product (direct_product!, nonlinear.jl:155-157), which
+    # is quadratic -- a quirk reproduced here.
+    Gz = G.copy()
+    Gz[:, 1] = 0.0
+    for lev, GG in ((2, G), (1, Gz)):
+        a = lin.nonlinear(q, z, GG, lev)
+        b = lin.nonlinear(2 * q, z, 2 * GG, lev)
+        assert np.array_equal(b, 2 * a)  # every operation is linear; scaling by 2 is exact
+    a = lin.nonlinear(q, z, G, 1)
+    b = lin.nonlinear(2 * q, z, 2 * G, 1)
+    assert not np.allclose(b, 2 * a)  # the quadratic boundary term
+    lin0 = O.OracleProblem(grid.h, boundary_of(grid), 2, 40.0, 0.02, [(np.zeros((1, 2)), np.ones(1))],
+                           binv=np.eye(2))
+    lin0.linearize(np.zeros_like(q), np.zeros_like(q), np.zeros_like(G))
+    G0 = G.copy()
+    G0[:, 1] = 0.0  # no boundary ring from the coarse level either
+    assert np.array_equal(lin0.nonlinear(q, z, G0, 1), np.zeros(grid.nΓ))
+    S = lin.new_state(np.array([1.0, 0.0, 0.0, 0, 0, 0, 0]))
+    assert np.all(S.q0 == 0.0)  # base_flux! of the linearized problem (timestepping.jl:238-250)
