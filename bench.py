@@ -8,6 +8,7 @@ configuration run in FP64).  A step is one advance!(state, prob, t).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
     python bench.py --kernels-only [--kernel-reps R]   (per-kernel timing only)
+    python bench.py --workload naca64 [--steps K]       (batched 64-instance NACA workload)
 
 Prints one JSON line.  Timed region: K consecutive steps enqueued by
 ibpm_advance_n with the state resident in HBM, bracketed by device
@@ -72,6 +73,56 @@ def time_kernels(prob, tuned, reps):
     return kb, yname
 
 
+def bench_naca64(args):
+    """Batched workload: 64 independent NACA 0012 simulations of uneven grids
+    (tests/golden/naca64.json), impulsively started, each on its own stream.
+    value = total timesteps/s over all instances; the same steps run one
+    instance after another are reported beside it."""
+    import ibpm
+    from ibpm.autotune import autotune
+    from cases import naca64_cases
+
+    cs = naca64_cases()
+    t0 = time.time()
+    probs, states = [], []
+    for c in cs:
+        p = ibpm.IBProblem(c["grid"], c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
+        autotune(p, reps=5)
+        probs.append(p)
+        states.append(ibpm.IBState(p))
+    setup_s = time.time() - t0
+    print(f"# setup of {len(cs)} problems (B assembled on the GPU) {setup_s:.1f}s", file=sys.stderr, flush=True)
+    ibpm.advance_batch_(states, probs, [np.arange(args.warmup) * c["dt"] for c in cs])
+    ts = [(args.warmup + np.arange(args.steps)) * c["dt"] for c in cs]
+    t1 = time.perf_counter()
+    res = ibpm.advance_batch_(states, probs, ts)
+    el = time.perf_counter() - t1
+    if any(r[2] != args.steps for r in res):
+        raise RuntimeError("non-finite forces in the batch")
+    ts2 = [(args.warmup + args.steps + np.arange(args.steps)) * c["dt"] for c in cs]
+    t2 = time.perf_counter()
+    for st, p, t in zip(states, probs, ts2):
+        ibpm.advance_n_(st, p, t)
+    el_seq = time.perf_counter() - t2
+    total = len(cs) * args.steps
+    byts = sum(algorithmic_bytes_per_step(c["grid"].nx, c["grid"].ny, c["grid"].mg, len(c["bodies"][0].ds))
+               for c in cs) * args.steps
+    out = {
+        "metric": "timesteps/s summed over 64 NACA 0012 instances (400x200 / 600x300 / 800x400, 3-5 levels)",
+        "value": round(total / el, 1), "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "64 independent NACA 0012 simulations, Re 100-1000, alpha 0-20 deg, dt = 0.25 dx, "
+                               "impulsive start, each problem on its own stream (ibpm_advance_async/_wait)",
+                   "instances": len(cs), "setup_s": round(setup_s, 1)},
+        "one_by_one": {"value": round(total / el_seq, 1), "unit": "timesteps/s"},
+        "step_hbm": {"algorithmic_bytes": byts, "achieved_GBs": round(byts / el / 1e9, 1),
+                     "frac_of_peak": round(byts / el / 1e9 / HBM_PEAK_GBS, 4)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +131,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--workload", choices=("cylinder1024", "naca64"), default="cylinder1024",
+                    help="naca64: the batched 64-instance NACA workload (secondary line)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only autotune and time each kernel kind (for counter profiling); prints kernel_ms")
     args = ap.parse_args()
@@ -87,6 +140,9 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "naca64":
+        return bench_naca64(args)
 
     import ibpm
     from ibpm import _capi as C

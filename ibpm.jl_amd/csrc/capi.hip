@@ -75,8 +75,22 @@ struct ibpm_prob {
     double *cx = nullptr, *cy = nullptr;
     DevTables tx, ty;
     std::vector<CoarsifyRange> crange;
-    double* hist = nullptr;
-    size_t hist_cap = 0;
+    double* hist = nullptr;   // CD history [hist_steps x nbodies], then CL history
+    size_t hist_steps = 0;
+    int pending = 0;          // steps enqueued by ibpm_advance_async, not yet waited for
+    int* slot = nullptr;      // device: history entry of the next step (advanced by k_forces)
+    bool async_err = false;   // a graph launch failed (reported by the next wait)
+    // HIP graphs of one step (static bodies): replayed instead of ~54 launches.
+    // A step's launches depend on the buffer roles only (Γ/Γs and the nonlin
+    // pair alternate), so a graph is kept per role; anything else a captured
+    // launch holds by value invalidates them (graphs_clear).
+    bool graphs = true;
+    struct StepGraph {
+        double* G;
+        int out, nl2;
+        hipGraphExec_t exec;
+    };
+    std::vector<StepGraph> gcache;
     std::vector<void*> allocs;
     // in-step timing of the all-level y-pass launches (IBPM_OPT_PROFILE):
     // event pairs recorded on the problem's stream, summed after each call
@@ -86,6 +100,8 @@ struct ibpm_prob {
     double prof_ms = 0.0;
     long prof_n = 0;
 };
+
+static void graphs_clear(ibpm_prob* P);
 
 // ---------------------------------------------------------------------------
 // error plumbing
@@ -408,7 +424,7 @@ static void collect_profile(ibpm_prob* P)
 }
 
 // advance!(state, prob, t) -- timestepping.jl:35-67.  hist: optional CD/CL slot.
-static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, double* clh)
+static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, double* clh)
 {
     hipStream_t s = P->stream;
     const GridDev& g = P->g;
@@ -460,6 +476,10 @@ static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, dou
         ForceArgs fa{};
         fa.Fb = P->Fb; fa.ds = P->ds_d; fa.boff = P->boff_d; fa.fb = P->fbv; fa.CD = P->CD; fa.CL = P->CL;
         fa.cd_hist = cdh; fa.cl_hist = clh; fa.nbodies = P->nbodies; fa.nbt = P->nbt; fa.hdt = g.h / P->dt;
+        if (cdh) {  // history entry taken from the device slot
+            fa.slot = P->slot;
+            fa.nbh = std::max(P->nbodies, 1);
+        }
         launch_gemv_forces(P->BinvT, P->Ftmp, P->Fb, P->nf, fa, s);
         // ---- project_circ! (timestepping.jl:204-223): Γ = Γs; Γ[:,1] -= Ainv rot(reg(F̃b))
         project_forces(P, P->Fb, P->Gs, true);
@@ -468,6 +488,58 @@ static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, dou
     // ---- vort2flux!(state.ψ, state.q, state.Γ) (timestepping.jl:64)
     vort2flux_dev(P, P->G, P->psi, P->q, false);
     (void)nG;
+}
+
+static void graphs_clear(ibpm_prob* P)
+{
+    for (auto& e : P->gcache) (void)hipGraphExecDestroy(e.exec);
+    P->gcache.clear();
+}
+
+// One advance! step on the problem's stream.  cdh/clh: bases of the CD/CL
+// histories (entry taken from the device slot), or null.  Static bodies are
+// replayed from a captured graph of the step when one exists for the
+// current buffer roles, else the step is captured (and then run) once.
+static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, double* clh)
+{
+    const bool can_graph = P->graphs && P->motion == IBPM_MOTION_STATIC && !P->prof && cdh;
+    if (!can_graph) {
+        step_body(P, sc, cdh, clh);
+        return;
+    }
+    const int out = (P->nl1 != P->nl2) ? P->nl1 : 1 - P->nl2;
+    for (auto& e : P->gcache) {
+        if (e.G == P->G && e.out == out && e.nl2 == P->nl2) {
+            if (hipGraphLaunch(e.exec, P->stream) != hipSuccess) P->async_err = true;
+            // the host side of step_body: nonlin roles, then Γ <-> Γs
+            P->nl1 = P->nl2 = out;
+            std::swap(P->G, P->Gs);
+            return;
+        }
+    }
+    ibpm_prob::StepGraph e{P->G, out, P->nl2, nullptr};
+    double* G0 = P->G;
+    double* Gs0 = P->Gs;
+    const int nl1 = P->nl1, nl2 = P->nl2;
+    hipGraph_t graph = nullptr;
+    bool ok = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        step_body(P, sc, cdh, clh);
+        ok = hipStreamEndCapture(P->stream, &graph) == hipSuccess && graph;
+    }
+    if (ok) ok = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ok) ok = hipGraphLaunch(e.exec, P->stream) == hipSuccess;
+    if (ok) {
+        P->gcache.push_back(e);
+        return;
+    }
+    // capture unavailable: undo the host side of the captured body and run it directly
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    (void)hipGetLastError();
+    P->graphs = false;
+    P->G = G0; P->Gs = Gs0; P->nl1 = nl1; P->nl2 = nl2;
+    step_body(P, sc, cdh, clh);
 }
 
 // get_Binv (laplacian.jl:98-119): B[:, j] = B_times!(e_j) on the device,
@@ -699,7 +771,7 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
     DA(P->nl[0], nG * mg); DA(P->nl[1], nG * mg);
     DA(P->qs1, nq); DA(P->T1, nG * mg); DA(P->T2, nG * mg); DA(P->Gtmp, nG); DA(P->qreg, nq);
     DA(P->Fb, P->nf); DA(P->Ftmp, P->nf); DA(P->fbv, P->nf); DA(P->CD, P->nbodies); DA(P->CL, P->nbodies);
-    DA(P->scal, 16);
+    DA(P->scal, 16); DA(P->slot, 1);
     DA(P->BinvT, (size_t)P->nf * P->nf); DA(P->w, P->weight.size());
     DA(P->ds_d, nbt); DA(P->bidx, 2 * nbt); DA(P->boff_d, P->boff.size());
     DA(P->sc_cell, cell.size()); DA(P->sc_off, off.size()); DA(P->sc_f, ef.size()); DA(P->sc_w, ew.size());
@@ -748,9 +820,10 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
 int ibpm_problem_destroy(ibpm_prob* P)
 {
     if (!P) return IBPM_OK;
+    if (P->hist && P->stream) (void)hipFreeAsync(P->hist, P->stream);
     if (P->stream) (void)hipStreamSynchronize(P->stream);
+    graphs_clear(P);
     for (void* p : P->allocs) (void)hipFree(p);
-    if (P->hist) (void)hipFree(P->hist);
     for (hipEvent_t ev : P->pev) (void)hipEventDestroy(ev);
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -789,6 +862,7 @@ int ibpm_problem_linearize(ibpm_prob* P, const double* qB, const double* q0B, co
     HIPTRY(P->ctx, hipMemcpyAsync(P->GB, GammaB, sizeof(double) * (size_t)(nG * mg), hipMemcpyHostToDevice, P->stream));
     HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
     P->lin = true;
+    graphs_clear(P);
     ibpm_step_scalars z{};
     return ibpm_base_flux(P, &z);
 }
@@ -850,6 +924,7 @@ int ibpm_state_upload(ibpm_prob* P, int field, const double* host, size_t n)
     double* d = field_ptr(P, field, &want, true);
     if (!d) return fail(P->ctx, IBPM_E_ARG, "unknown field");
     if (n != want) return fail(P->ctx, IBPM_E_ARG, "upload size mismatch");
+    graphs_clear(P);
     HIPTRY(P->ctx, hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, P->stream));
     HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
     return IBPM_OK;
@@ -870,6 +945,7 @@ int ibpm_state_download(ibpm_prob* P, int field, double* host, size_t n)
 int ibpm_base_flux(ibpm_prob* P, const ibpm_step_scalars* s)
 {
     if (!P || !s) return IBPM_E_ARG;
+    graphs_clear(P);  // captured nonlinear launches hold the base flux by value
     if (P->lin) {
         // static bodies of a linearized problem: q0 = 0 (timestepping.jl:238-250)
         launch_base_flux_inertial(P->q0, P->mg, 0.0, 0.0, P->g, P->stream);
@@ -921,9 +997,65 @@ int ibpm_advance(ibpm_prob* P, double t, const ibpm_step_scalars* s)
 {
     (void)t;
     if (!P || !s) return IBPM_E_ARG;
-    step_dev(P, *s, nullptr, nullptr);
+    int done = 0;
+    return ibpm_advance_n(P, 1, &t, s, nullptr, nullptr, &done);
+}
+
+int ibpm_advance_async(ibpm_prob* P, int nsteps, const ibpm_step_scalars* s)
+{
+    if (!P || !s || nsteps < 0) return IBPM_E_ARG;
+    const int nb = std::max(P->nbodies, 1);
+    const size_t want = (size_t)P->pending + (size_t)nsteps;
+    if (want > P->hist_steps) {
+        // the history buffer grows only between waits (its halves hold CD and CL)
+        if (P->pending > 0)
+            return fail(P->ctx, IBPM_E_ARG, "history full: call ibpm_advance_wait before enqueuing more steps");
+        // Stream-ordered free and allocation: hipFree would wait for the whole
+        // device, i.e. for the steps other problems have queued on their own
+        // streams, and so serialise a batch of problems.  The capacity grows
+        // geometrically so that a run of growing calls reallocates rarely.
+        const size_t cap = std::max({want, 2 * P->hist_steps, (size_t)64});
+        if (P->hist) (void)hipFreeAsync(P->hist, P->stream);
+        P->hist = nullptr;
+        P->hist_steps = 0;
+        graphs_clear(P);  // captured steps hold the old history buffer
+        HIPTRY(P->ctx, hipMallocAsync((void**)&P->hist, sizeof(double) * 2 * cap * (size_t)nb, P->stream));
+        P->hist_steps = cap;
+    }
+    if (nsteps > 0) launch_set_int(P->slot, P->pending, P->stream);
+    for (int k = 0; k < nsteps; k++) step_dev(P, s[k], P->hist, P->hist + P->hist_steps * nb);
+    P->pending += nsteps;
     HIPTRY(P->ctx, hipGetLastError());
     return IBPM_OK;
+}
+
+int ibpm_advance_wait(ibpm_prob* P, double* cd_hist, double* cl_hist, int* steps_done)
+{
+    if (!P) return IBPM_E_ARG;
+    if (steps_done) *steps_done = 0;
+    const int nb = std::max(P->nbodies, 1);
+    const size_t n = (size_t)P->pending;
+    std::vector<double> cd(n * nb), cl(n * nb);
+    if (n > 0) {
+        HIPTRY(P->ctx, hipMemcpyAsync(cd.data(), P->hist, sizeof(double) * cd.size(), hipMemcpyDeviceToHost, P->stream));
+        HIPTRY(P->ctx, hipMemcpyAsync(cl.data(), P->hist + P->hist_steps * nb, sizeof(double) * cl.size(),
+                                      hipMemcpyDeviceToHost, P->stream));
+    }
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    collect_profile(P);
+    P->pending = 0;
+    if (P->async_err) {
+        P->async_err = false;
+        return fail(P->ctx, IBPM_E_HIP, "launching a captured step failed");
+    }
+    int done = (int)n;
+    for (size_t k = 0; k < n && done == (int)n; k++)
+        for (int b = 0; b < P->nbodies; b++)
+            if (!std::isfinite(cl[k * nb + b])) { done = (int)k + 1; break; }
+    if (cd_hist && P->nbodies > 0) std::copy(cd.begin(), cd.end(), cd_hist);
+    if (cl_hist && P->nbodies > 0) std::copy(cl.begin(), cl.end(), cl_hist);
+    if (steps_done) *steps_done = done;
+    return done < (int)n ? fail(P->ctx, IBPM_E_NONFINITE, "non-finite CL") : IBPM_OK;
 }
 
 int ibpm_advance_n(ibpm_prob* P, int nsteps, const double* t, const ibpm_step_scalars* s, double* cd_hist,
@@ -932,30 +1064,10 @@ int ibpm_advance_n(ibpm_prob* P, int nsteps, const double* t, const ibpm_step_sc
     (void)t;
     if (!P || !s || nsteps < 0) return IBPM_E_ARG;
     if (steps_done) *steps_done = 0;
-    const size_t need = 2 * (size_t)nsteps * (size_t)std::max(P->nbodies, 1);
-    if (need > P->hist_cap) {
-        if (P->hist) (void)hipFree(P->hist);
-        P->hist = nullptr;
-        P->hist_cap = 0;
-        HIPTRY(P->ctx, hipMalloc(&P->hist, sizeof(double) * need));
-        P->hist_cap = need;
-    }
-    const int nb = std::max(P->nbodies, 1);
-    for (int k = 0; k < nsteps; k++)
-        step_dev(P, s[k], P->hist + (size_t)k * nb, P->hist + (size_t)nsteps * nb + (size_t)k * nb);
-    HIPTRY(P->ctx, hipGetLastError());
-    std::vector<double> hh(2 * (size_t)nsteps * nb);
-    HIPTRY(P->ctx, hipMemcpyAsync(hh.data(), P->hist, sizeof(double) * hh.size(), hipMemcpyDeviceToHost, P->stream));
-    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
-    collect_profile(P);
-    int done = nsteps;
-    for (int k = 0; k < nsteps && done == nsteps; k++)
-        for (int b = 0; b < P->nbodies; b++)
-            if (!std::isfinite(hh[(size_t)nsteps * nb + (size_t)k * nb + b])) { done = k + 1; break; }
-    if (cd_hist && P->nbodies > 0) std::copy(hh.begin(), hh.begin() + (size_t)nsteps * nb, cd_hist);
-    if (cl_hist && P->nbodies > 0) std::copy(hh.begin() + (size_t)nsteps * nb, hh.end(), cl_hist);
-    if (steps_done) *steps_done = done;
-    return done < nsteps ? fail(P->ctx, IBPM_E_NONFINITE, "non-finite CL") : IBPM_OK;
+    if (P->pending > 0) return fail(P->ctx, IBPM_E_ARG, "steps enqueued by ibpm_advance_async are not waited for");
+    const int rc = ibpm_advance_async(P, nsteps, s);
+    if (rc != IBPM_OK) return rc;
+    return ibpm_advance_wait(P, cd_hist, cl_hist, steps_done);
 }
 
 int ibpm_get_forces(ibpm_prob* P, double* CD, double* CL, double* Fb)
@@ -1118,11 +1230,17 @@ int ibpm_op_rot(ibpm_prob* P, const double* q, double* G)
 int ibpm_set_option(ibpm_prob* P, int option, int value)
 {
     if (!P) return IBPM_E_ARG;
+    graphs_clear(P);
     if (option == IBPM_OPT_YTILE_BATCH || option == IBPM_OPT_YTILE_SINGLE) {
         if (value != 4 && value != 8) return fail(P->ctx, IBPM_E_ARG, "y tile must be 4 or 8");
         if (ypass_lds_bytes(P->ty.plan, P->g.n2, value) > 160 * 1024)
             return fail(P->ctx, IBPM_E_ARG, "y tile does not fit in LDS for this ny");
         (option == IBPM_OPT_YTILE_BATCH ? P->ytile : P->ytile_single) = value;
+        return IBPM_OK;
+    }
+    if (option == IBPM_OPT_GRAPHS) {
+        if (value != 0 && value != 1) return fail(P->ctx, IBPM_E_ARG, "graphs option is 0 or 1");
+        P->graphs = value == 1;
         return IBPM_OK;
     }
     if (option == IBPM_OPT_PROFILE) {
@@ -1143,6 +1261,7 @@ int ibpm_get_option(ibpm_prob* P, int option, int* value)
     if (option == IBPM_OPT_YTILE_BATCH) *value = P->ytile;
     else if (option == IBPM_OPT_YTILE_SINGLE) *value = P->ytile_single;
     else if (option == IBPM_OPT_PROFILE) *value = P->prof ? 1 : 0;
+    else if (option == IBPM_OPT_GRAPHS) *value = P->graphs ? 1 : 0;
     else return fail(P->ctx, IBPM_E_ARG, "unknown option");
     return IBPM_OK;
 }

@@ -123,6 +123,11 @@ struct ForceArgs {
     double* CL;
     double* cd_hist;  // may be null
     double* cl_hist;
+    // with slot != null the history entry is [(*slot) * nbh + body] of cd_hist
+    // and cl_hist, and the kernel advances *slot by one (graph replays of a
+    // step then need no per-step arguments)
+    int* slot;
+    int nbh;
     int nbodies, nbt;
     double hdt;       // h / dt
 };
@@ -140,6 +145,7 @@ void launch_base_flux_moving(const BaseFluxArgs& a, hipStream_t s);
 void launch_base_flux_inertial(double* q0, int mg, double vx, double vy, const GridDev& g, hipStream_t s);
 
 void launch_fill(double* p, long n, double v, hipStream_t s);
+void launch_set_int(int* p, int v, hipStream_t s);
 void launch_copy(double* dst, const double* src, long n, hipStream_t s);
 void launch_set_unit(double* p, int n, int j, hipStream_t s);
 void launch_absmax(const double* p, long n, double* out, hipStream_t s);  // out: single double (>= 0)

@@ -783,6 +783,7 @@ __device__ __forceinline__ void forces_block(const ForceArgs& a)
 {
     __shared__ double red[2][4];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
+    const long ho = a.slot ? (long)(*a.slot) * a.nbh : 0;  // history entry of this step
     for (int bd = 0; bd < a.nbodies; bd++) {
         const int o = a.boff[bd], n = a.boff[bd + 1] - o;
         double sx = 0.0, sy = 0.0;
@@ -804,10 +805,11 @@ __device__ __forceinline__ void forces_block(const ForceArgs& a)
             double Y = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
             a.CD[bd] = 2.0 * X;
             a.CL[bd] = 2.0 * Y;
-            if (a.cd_hist) { a.cd_hist[bd] = 2.0 * X; a.cl_hist[bd] = 2.0 * Y; }
+            if (a.cd_hist) { a.cd_hist[ho + bd] = 2.0 * X; a.cl_hist[ho + bd] = 2.0 * Y; }
         }
         __syncthreads();
     }
+    if (a.slot && tid == 0) *a.slot = (int)(ho / a.nbh) + 1;
 }
 
 // GEMV: one wavefront per row of Binv (given transposed: AT row r = row r of
@@ -907,6 +909,9 @@ __global__ void k_fill(double* p, long n, double v)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
+__global__ void k_set_int(int* p, int v) { *p = v; }
+void launch_set_int(int* p, int v, hipStream_t s) { hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, p, v); }
+
 void launch_fill(double* p, long n, double v, hipStream_t s)
 {
     if (n <= 0) return;

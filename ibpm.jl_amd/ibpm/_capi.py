@@ -27,11 +27,11 @@ MOTION_FUNCTION = 2
 FIELD_Q, FIELD_Q0, FIELD_GAMMA, FIELD_PSI, FIELD_NONLIN1, FIELD_NONLIN2 = 0, 1, 2, 3, 4, 5
 FIELD_FB_TILDE, FIELD_FB, FIELD_CD, FIELD_CL = 6, 7, 8, 9
 
-OPT_YTILE_BATCH, OPT_YTILE_SINGLE, OPT_PROFILE = 0, 1, 2
+OPT_YTILE_BATCH, OPT_YTILE_SINGLE, OPT_PROFILE, OPT_GRAPHS = 0, 1, 2, 3
 
 EXPORTED = [
     "ibpm_ctx_create", "ibpm_ctx_destroy", "ibpm_last_error", "ibpm_problem_create", "ibpm_problem_destroy",
-    "ibpm_problem_linearize",
+    "ibpm_problem_linearize", "ibpm_advance_async", "ibpm_advance_wait",
     "ibpm_problem_get_info", "ibpm_problem_get_binv", "ibpm_problem_get_body_idx", "ibpm_state_upload",
     "ibpm_state_download", "ibpm_state_init", "ibpm_base_flux", "ibpm_vort2flux", "ibpm_advance",
     "ibpm_advance_n", "ibpm_get_forces", "ibpm_compute_cfl", "ibpm_op_lapinv", "ibpm_op_dst2",
@@ -89,6 +89,8 @@ def lib():
                                           ctypes.POINTER(_vp)]
         L.ibpm_problem_destroy.argtypes = [_vp]
         L.ibpm_problem_linearize.argtypes = [_vp, _dp, _dp, _dp]
+        L.ibpm_advance_async.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(StepScalars)]
+        L.ibpm_advance_wait.argtypes = [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
         L.ibpm_problem_get_info.argtypes = [_vp, ctypes.POINTER(ProblemInfo)]
         L.ibpm_problem_get_binv.argtypes = [_vp, _dp, ctypes.c_size_t]
         L.ibpm_problem_get_body_idx.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t]

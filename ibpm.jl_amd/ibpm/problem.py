@@ -285,6 +285,33 @@ def advance_n_(state: IBState, prob: IBProblem, ts):
     return cd[: done.value], cl[: done.value], done.value
 
 
+def advance_batch_(states, probs, ts_list):
+    """advance! of several independent problems at once: each problem's steps
+    are enqueued on its own stream (ibpm_advance_async) before any is waited
+    for (ibpm_advance_wait), so the GPU runs them concurrently.  Returns one
+    (CD, CL, steps_done) per problem, as advance_n_ does."""
+    L = C.lib()
+    for st, prob, ts in zip(states, probs, ts_list):
+        ts = np.asarray(ts, dtype=np.float64)
+        arr = (C.StepScalars * max(len(ts), 1))()
+        for k, t in enumerate(ts):
+            arr[k] = prob.scalars(float(t))
+        prob.ctx.check(L.ibpm_advance_async(prob.h, len(ts), arr), "ibpm_advance_async")
+    out = []
+    for st, prob, ts in zip(states, probs, ts_list):
+        n = len(ts)
+        nbod = max(len(prob.model.bodies), 1)
+        cd = np.zeros((n, nbod))
+        cl = np.zeros((n, nbod))
+        done = ctypes.c_int(0)
+        rc = L.ibpm_advance_wait(prob.h, C.ptr(cd), C.ptr(cl), ctypes.byref(done))
+        st.invalidate()
+        if rc not in (C.IBPM_OK, C.IBPM_E_NONFINITE):
+            prob.ctx.check(rc, "ibpm_advance_wait")
+        out.append((cd[: done.value], cl[: done.value], done.value))
+    return out
+
+
 def advance(x: IBState, prob: IBProblem, t: float):
     """advance(x, prob, t) -- timestepping.jl:22-28.  The device state is
     single-copy per problem, so this advances `x` in place and returns it."""

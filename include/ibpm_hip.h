@@ -144,8 +144,9 @@ int ibpm_base_flux(ibpm_prob* prob, const ibpm_step_scalars* at_t);
 int ibpm_vort2flux(ibpm_prob* prob);
 
 /* advance!(state, prob, t) -- timestepping.jl:35-67.  `s` holds the problem's
- * closures evaluated at t (the time at the start of the step).  Returns
- * IBPM_E_NONFINITE if a CD/CL came out non-finite (state is still updated). */
+ * closures evaluated at t (the time at the start of the step).  Returns when
+ * the step is done; IBPM_E_NONFINITE if its CL came out non-finite (the state
+ * is still updated). */
 int ibpm_advance(ibpm_prob* prob, double t, const ibpm_step_scalars* s);
 /* nsteps calls of advance! with per-step scalars s[k] (times t[k]) without a
  * host round trip in between: the time loop of run_sim! (ibpm.jl:93-105).
@@ -154,6 +155,14 @@ int ibpm_advance(ibpm_prob* prob, double t, const ibpm_step_scalars* s);
  * IBPM_E_NONFINITE with *steps_done set. */
 int ibpm_advance_n(ibpm_prob* prob, int nsteps, const double* t, const ibpm_step_scalars* s,
                    double* cd_hist, double* cl_hist, int* steps_done);
+/* ibpm_advance_n split in two, so that several problems (each with its own
+ * stream) can step concurrently: ibpm_advance_async enqueues nsteps steps and
+ * returns at once; ibpm_advance_wait drains the problem's stream and returns
+ * the CD/CL histories of every step enqueued since the last wait (same
+ * layout and non-finite rule as ibpm_advance_n).  No counterpart in the
+ * reference: it is the batched form of run_sim!'s loop (ibpm.jl:93-105). */
+int ibpm_advance_async(ibpm_prob* prob, int nsteps, const ibpm_step_scalars* s);
+int ibpm_advance_wait(ibpm_prob* prob, double* cd_hist, double* cl_hist, int* steps_done);
 /* state.CD, state.CL (nbodies each) and state.F̃b (2 nb_total); any may be NULL. */
 int ibpm_get_forces(ibpm_prob* prob, double* CD, double* CL, double* Fb);
 /* compute_cfl(state, prob) = max|q[:,1]| dt / h (ibpm.jl:83-88). */
@@ -181,6 +190,8 @@ int ibpm_op_rot(ibpm_prob* prob, const double* q, double* G);
 #define IBPM_OPT_YTILE_SINGLE 1  /* columns per y-pass workgroup, one-level launches: 4 or 8 */
 #define IBPM_OPT_PROFILE 2       /* 1: time every all-level y-pass inside the steps (HIP events on the
                                     problem's stream); setting it drains the stream and clears the sums */
+#define IBPM_OPT_GRAPHS 3        /* 1 (default): static-body steps are captured once per buffer role in a
+                                    HIP graph and replayed; 0: every step launches its kernels */
 int ibpm_set_option(ibpm_prob* prob, int option, int value);
 int ibpm_get_option(ibpm_prob* prob, int option, int* value);
 /* Average device ms and count of the all-level y-pass launches timed inside

@@ -76,6 +76,45 @@ def case(name: str):
     raise KeyError(name)
 
 
+def naca_arclength(dx, spec="0012", chord=1.0, x_le=0.0):
+    """NACA surface points equispaced in arc length (spacing close to dx),
+    from the thickness law of naca() (sample-bodies.jl:104-134), leading edge
+    at (x_le, 0), clockwise from the trailing edge over the upper surface.
+    Deterministic: no random numbers."""
+    n_fine = 20001
+    xs = 0.5 * (1 - np.cos(np.linspace(0.0, np.pi, n_fine)))  # fine stations, dense at both edges
+    _, xU, xL, yU, yL = ibpm.naca(xs, spec)
+    # closed contour TE -> upper -> LE -> lower -> TE
+    X = np.concatenate([xU[::-1], xL[1:]]) * chord + x_le
+    Y = np.concatenate([yU[::-1], yL[1:]]) * chord
+    seg = np.hypot(np.diff(X), np.diff(Y))
+    s = np.concatenate([[0.0], np.cumsum(seg)])
+    n = int(round(s[-1] / dx))
+    sk = (np.arange(n) + 0.5) * (s[-1] / n)  # midpoints of n equal arcs
+    xb = np.stack([np.interp(sk, s, X), np.interp(sk, s, Y)], axis=1)
+    ds = np.full(n, s[-1] / n)
+    return xb, ds
+
+
+def naca64_cases(path=None):
+    """The 64 instances of the batched NACA 0012 workload (tests/golden/naca64.json)."""
+    import json
+    import os
+    path = path or os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "naca64.json")
+    spec = json.load(open(path))
+    x0, x1, y0, y1 = spec["box"]
+    out = []
+    for inst in spec["instances"]:
+        grid = ibpm.make_grid(inst["dx"], (x0, x1, y0, y1), mg=inst["mg"])
+        assert (grid.nx, grid.ny) == (inst["nx"], inst["ny"])
+        xb, ds = naca_arclength(inst["dx"], spec["airfoil"], spec["chord"])
+        body = ibpm.RigidBody(ibpm.Static(), xb, xb.copy(), np.zeros_like(xb), ds)
+        a = math.radians(inst["alpha_deg"])
+        out.append(dict(grid=grid, bodies=[body], Re=inst["Re"], dt=inst["dt"],
+                        freestream={"Ux": lambda t: 1.0, "Uy": lambda t: 0.0, "inclination": (lambda a: lambda t: a)(a)}))
+    return out
+
+
 def oracle_bodies(c):
     return [(b.xb, b.ds) for b in c["bodies"]]
 

@@ -258,6 +258,51 @@ def test_run_sim_saves_and_cfl():
     assert st.cfl > 0  # computed after step 25 (display_freq)
 
 
+def test_graph_replays_match_direct_launches():
+    """Captured steps (default) against a twin problem that launches every
+    kernel: several rounds of replays with eager work and syncs in between,
+    every round compared bit for bit."""
+    from ibpm import _capi as C
+    pairs = [build("small3", share_binv=True) for _ in range(2)]
+    gps = [gp for _, _, gp in pairs]
+    gps[1].ctx.check(C.lib().ibpm_set_option(gps[1].h, C.OPT_GRAPHS, 0))
+    sts = [initial_states(c, op, gp)[1] for c, op, gp in pairs]
+    c = pairs[0][0]
+    g = c["grid"]
+    ts = np.arange(31) * c["dt"]
+    for k in range(3):
+        cds = [ibpm.advance_n_(st, gp, ts[10 * k: 10 * k + 10])[0] for st, gp in zip(sts, gps)]
+        assert np.array_equal(cds[0], cds[1]), k
+        for name in ("Γ", "q", "ψ"):
+            assert np.array_equal(getattr(sts[0], name), getattr(sts[1], name)), (k, name)
+        # eager work on the same streams between the replays
+        for st, gp in zip(sts, gps):
+            ibpm.compute_cfl(st, gp)
+            _op(gp, "ibpm_op_dst2", st.Γ[:, 0], out_sizes=[g.nΓ])
+    for st, gp in zip(sts, gps):
+        ibpm.advance_(st, gp, ts[-1])
+    assert np.array_equal(sts[0].Γ, sts[1].Γ) and np.array_equal(sts[0].CD, sts[1].CD)
+
+
+def test_batch_of_problems_equals_one_by_one():
+    """Concurrent streams (ibpm_advance_async / _wait) change nothing."""
+    names = ["tiny", "mixed", "small3"]
+    runs = []
+    for batched in (False, True):
+        pairs = [build(n, share_binv=True) for n in names]
+        states = [initial_states(c, op, gp)[1] for c, op, gp in pairs]
+        tss = [np.arange(12) * c["dt"] for c, _, _ in pairs]
+        if batched:
+            res = ibpm.advance_batch_(states, [gp for _, _, gp in pairs], tss)
+        else:
+            res = [ibpm.advance_n_(st, gp, ts) for st, (_, _, gp), ts in zip(states, pairs, tss)]
+        runs.append((res, [st.Γ.copy() for st in states]))
+    (ra, ga), (rb, gb) = runs
+    for (cda, cla, na), (cdb, clb, nb) in zip(ra, rb):
+        assert na == nb == 12 and np.array_equal(cda, cdb) and np.array_equal(cla, clb)
+    assert all(np.array_equal(a, b) for a, b in zip(ga, gb))
+
+
 # ---------------------------------------------------------------------------
 # LinearizedIBProblem (problem-types.jl:47-87)
 # ---------------------------------------------------------------------------
