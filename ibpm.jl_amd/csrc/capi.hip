@@ -33,6 +33,13 @@ struct CoarsifyRange {
     int i0, ic0, ci, j0, jc0, cj;
 };
 
+// E' of the current body points as a segmented gather: for every touched flux
+// cell (ordered), the contributions (F̃b index, weight) in body-point order
+struct RegLists {
+    std::vector<int> cell, off, ef;
+    std::vector<double> ew;
+};
+
 }  // namespace
 
 struct ibpm_prob {
@@ -75,6 +82,16 @@ struct ibpm_prob {
     double *cx = nullptr, *cy = nullptr;
     DevTables tx, ty;
     std::vector<CoarsifyRange> crange;
+    RegLists reg;
+    // MotionFunction (lab-frame moving bodies): reference points x0 and point
+    // velocities ub (nbt x 2 column-major, like xb); device output of B_times!
+    // and pinned staging of its input and output for the host-side CG; the
+    // last CG solve's products, final |r| and tolerance
+    std::vector<double> x0, ub;
+    double* Fw = nullptr;
+    double* pin = nullptr;
+    int cg_iters = 0;
+    double cg_res = 0.0, cg_tol = 0.0;
     double* hist = nullptr;   // CD history [hist_steps x nbodies], then CL history
     size_t hist_steps = 0;
     int pending = 0;          // steps enqueued by ibpm_advance_async, not yet waited for
@@ -318,6 +335,88 @@ static YArgs yargs(ibpm_prob* P, int op, int lev0, int nlev, double* buf)
     return a;
 }
 
+// setup_reg (interface-coupling.jl:72-96) for the points in P->xb: body_idx,
+// the 13 x 13 weights, the E' segmented-gather lists (flux cell, F̃b index,
+// weight ordered by cell, then by point) and the rot! box of the E' support.
+// Returns false when a point is closer than 6 cells to the domain edge.
+static bool setup_reg_host(ibpm_prob* P)
+{
+    const GridDev& g = P->g;
+    const int nx = g.nx, ny = g.ny, nbt = P->nbt;
+    const double h = g.h;
+    P->body_idx.resize(2 * (size_t)nbt);
+    for (int k = 0; k < nbt; k++) {
+        P->body_idx[k] = (int)std::floor((P->xb[k] + P->offx) / h);
+        P->body_idx[nbt + k] = (int)std::floor((P->xb[nbt + k] + P->offy) / h);
+        int bx = P->body_idx[k], by = P->body_idx[nbt + k];
+        if (bx - 6 < 1 || bx + 6 > nx || by - 6 < 1 || by + 6 > ny) return false;
+    }
+    P->weight.assign((size_t)2 * nbt * 169, 0.0);
+    for (int k = 0; k < nbt; k++) {
+        double xk = P->xb[k], yk = P->xb[nbt + k];
+        int bx = P->body_idx[k], by = P->body_idx[nbt + k];
+        for (int m = 0; m < 13; m++) {
+            double y = h * (double)(by - 1 + (m - 6)) - P->offy;
+            for (int l = 0; l < 13; l++) {
+                double x = h * (double)(bx - 1 + (l - 6)) - P->offx;
+                P->weight[(size_t)(2 * k) * 169 + l + 13 * m] = delta_h(x, xk, h) * delta_h(y + h / 2, yk, h);
+                P->weight[(size_t)(2 * k + 1) * 169 + l + 13 * m] = delta_h(x + h / 2, xk, h) * delta_h(y, yk, h);
+            }
+        }
+    }
+    struct Ent { long p; int f; double w; };
+    std::vector<Ent> ents;
+    ents.reserve((size_t)2 * nbt * 169);
+    int ib0 = nx + 1, ib1 = 0, jb0 = ny + 2, jb1 = 0;
+    for (int c = 0; c < 2; c++)
+        for (int k = 0; k < nbt; k++) {
+            int bx = P->body_idx[k], by = P->body_idx[nbt + k];
+            for (int m = 0; m < 13; m++)
+                for (int l = 0; l < 13; l++) {
+                    int i = bx + l - 6, j = by + m - 6;
+                    long p = (c == 0) ? (i - 1) + (long)(nx + 1) * (j - 1) : g.nu + (i - 1) + (long)nx * (j - 1);
+                    ents.push_back({p, k + c * nbt, P->weight[(size_t)(2 * k + c) * 169 + l + 13 * m]});
+                    // rot! outputs touched: x-flux (i,j) -> Γ(i,j),(i,j+1); y-flux -> Γ(i,j),(i+1,j)
+                    ib0 = std::min(ib0, i); jb0 = std::min(jb0, j);
+                    ib1 = std::max(ib1, i + 1); jb1 = std::max(jb1, j + 1);
+                }
+        }
+    std::stable_sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) { return a.p < b.p; });
+    RegLists& R = P->reg;
+    R.cell.clear(); R.off.clear(); R.ef.clear(); R.ew.clear();
+    for (size_t e = 0; e < ents.size(); e++) {
+        if (e == 0 || ents[e].p != ents[e - 1].p) {
+            R.cell.push_back((int)ents[e].p);
+            R.off.push_back((int)e);
+        }
+        R.ef.push_back(ents[e].f);
+        R.ew.push_back(ents[e].w);
+    }
+    R.off.push_back((int)ents.size());
+    P->sc_ncell = (int)R.cell.size();
+    P->rb_i0 = std::max(ib0, 2); P->rb_i1 = std::min(ib1, nx);
+    P->rb_j0 = std::max(jb0, 2); P->rb_j1 = std::min(jb1, ny);
+    return true;
+}
+
+// E tables of setup_reg_host to the device, on the problem's stream.  The
+// device arrays hold the largest lists (every window entry its own cell).
+static hipError_t upload_reg(ibpm_prob* P)
+{
+    hipStream_t s = P->stream;
+    const RegLists& R = P->reg;
+    auto cp = [&](void* d, const void* h, size_t bytes) {
+        return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
+    };
+    hipError_t e = cp(P->w, P->weight.data(), sizeof(double) * P->weight.size());
+    if (e == hipSuccess) e = cp(P->bidx, P->body_idx.data(), sizeof(int) * P->body_idx.size());
+    if (e == hipSuccess) e = cp(P->sc_cell, R.cell.data(), sizeof(int) * R.cell.size());
+    if (e == hipSuccess) e = cp(P->sc_off, R.off.data(), sizeof(int) * R.off.size());
+    if (e == hipSuccess) e = cp(P->sc_f, R.ef.data(), sizeof(int) * R.ef.size());
+    if (e == hipSuccess) e = cp(P->sc_w, R.ew.data(), sizeof(double) * R.ew.size());
+    return e;
+}
+
 // x = L b for one level (lev 0-based), L one of Δinv / A / Ainv (laplacian.jl:24-76),
 // with optional apply_bc! prologue (bcr != null) and subtract epilogue.
 static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x, const double* bcr, bool sub,
@@ -423,6 +522,8 @@ static void collect_profile(ibpm_prob* P)
     P->pev_used = 0;
 }
 
+static void trial_dev(ibpm_prob* P);
+
 // advance!(state, prob, t) -- timestepping.jl:35-67.  hist: optional CD/CL slot.
 static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, double* clh)
 {
@@ -443,7 +544,36 @@ static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, do
         launch_base_flux_moving(b, s);
         P->q0_uniform = false;
     }
-    // ---- get_trial_state! (timestepping.jl:86-129), all levels at once
+    trial_dev(P);
+    // ---- boundary_forces! (timestepping.jl:154-163) and update_stress!
+    if (P->nbt > 0) {
+        GatherArgs ga{};
+        ga.q = P->qs1; ga.q0 = P->q0; ga.w = P->w; ga.bidx = P->bidx; ga.out = P->Ftmp; ga.nbt = P->nbt; ga.g = g;
+        launch_gather(ga, s);
+        ForceArgs fa{};
+        fa.Fb = P->Fb; fa.ds = P->ds_d; fa.boff = P->boff_d; fa.fb = P->fbv; fa.CD = P->CD; fa.CL = P->CL;
+        fa.cd_hist = cdh; fa.cl_hist = clh; fa.nbodies = P->nbodies; fa.nbt = P->nbt; fa.hdt = g.h / P->dt;
+        if (cdh) {  // history entry taken from the device slot
+            fa.slot = P->slot;
+            fa.nbh = std::max(P->nbodies, 1);
+        }
+        launch_gemv_forces(P->BinvT, P->Ftmp, P->Fb, P->nf, fa, s);
+        // ---- project_circ! (timestepping.jl:204-223): Γ = Γs; Γ[:,1] -= Ainv rot(reg(F̃b))
+        project_forces(P, P->Fb, P->Gs, true);
+    }
+    std::swap(P->G, P->Gs);
+    // ---- vort2flux!(state.ψ, state.q, state.Γ) (timestepping.jl:64)
+    vort2flux_dev(P, P->G, P->psi, P->q, false);
+    (void)nG;
+}
+
+// get_trial_state! (timestepping.jl:86-129), all levels at once: Γs, the
+// trial flux qs[:,1], the AB2 history shift and the trial ψ in state.ψ
+static void trial_dev(ibpm_prob* P)
+{
+    hipStream_t s = P->stream;
+    const GridDev& g = P->g;
+    const int L = P->mg;
     const int out = (P->nl1 != P->nl2) ? P->nl1 : 1 - P->nl2;
     NLArgs na{};
     na.q = P->q; na.q0 = P->q0; na.G = P->G; na.out = P->nl[out];
@@ -468,26 +598,151 @@ static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, do
     launch_xpass(X_PLAIN, c, s);
     P->nl1 = P->nl2 = out;  // state.nonlin[2] .= state.nonlin[1]
     vort2flux_dev(P, P->Gs, P->psi, P->qs1, true);
-    // ---- boundary_forces! (timestepping.jl:154-163) and update_stress!
-    if (P->nbt > 0) {
+}
+
+// move_body!(::Type{MotionFunction}, body, t) for every body (move-body-utils.jl:
+// 123-133, with get_transformation :73-97); kin = {x, y, θ, ẋ, ẏ, θ̇} per body:
+//   xb = pos + Rx x0, Rx = [cθ -sθ; sθ cθ];  ub = vel + Ru x0, Ru = θ̇ [-sθ -cθ; cθ -sθ]
+static void move_bodies(ibpm_prob* P, const double* kin)
+{
+    const int nbt = P->nbt;
+    for (int b = 0; b < P->nbodies; b++) {
+        const double* k = kin + 6L * b;
+        const double c = std::cos(k[2]), sn = std::sin(k[2]);
+        const double Rx11 = c, Rx12 = -sn, Rx21 = sn, Rx22 = c;
+        const double Ru11 = k[5] * -sn, Ru12 = k[5] * -c, Ru21 = k[5] * c, Ru22 = k[5] * -sn;
+        for (int i = P->boff[b]; i < P->boff[b + 1]; i++) {
+            const double x0 = P->x0[i], y0 = P->x0[nbt + i];
+            P->xb[i] = k[0] + (Rx11 * x0 + Rx12 * y0);
+            P->xb[nbt + i] = k[1] + (Rx21 * x0 + Rx22 * y0);
+            P->ub[i] = k[3] + (Ru11 * x0 + Ru12 * y0);
+            P->ub[nbt + i] = k[4] + (Ru21 * x0 + Ru22 * y0);
+        }
+    }
+}
+
+// update_bodies! for MotionFunction (timestepping.jl:386-398): move the bodies,
+// then E = setup_reg on the device.  The E' support moved, so the scratch that
+// the scatter and rot! leave untouched outside it is cleared again.
+static int update_bodies_dev(ibpm_prob* P, const double* kin)
+{
+    const std::vector<double> xb_prev = P->xb, ub_prev = P->ub;
+    move_bodies(P, kin);
+    if (!setup_reg_host(P)) {
+        // keep the bodies and E of the last good position
+        P->xb = xb_prev;
+        P->ub = ub_prev;
+        (void)setup_reg_host(P);
+        return fail(P->ctx, IBPM_E_ARG, "a moving body came within 6 cells of the domain edge");
+    }
+    HIPTRY(P->ctx, upload_reg(P));
+    launch_fill(P->qreg, P->g.nq, 0.0, P->stream);
+    launch_fill(P->Gtmp, P->g.nG, 0.0, P->stream);
+    return IBPM_OK;
+}
+
+// B_times! (laplacian.jl:143-169): out = E q1(vort2flux(Ainv rot(E' z))) for z
+// in P->pin[0:nf); the product lands in P->pin[nf:2nf).  T2 is the Γ work
+// array (its coarse-level ring was cleared before the solve), ψ and qs1 are
+// free during the force solve.
+static hipError_t b_times_dev(ibpm_prob* P)
+{
+    hipStream_t s = P->stream;
+    const int nf = P->nf;
+    hipError_t e = hipMemcpyAsync(P->Ftmp, P->pin, sizeof(double) * nf, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    project_forces(P, P->Ftmp, P->T2, false);
+    vort2flux_dev(P, P->T2, P->psi, P->qs1, true);
+    GatherArgs ga{};
+    ga.q = P->qs1; ga.q0 = nullptr; ga.w = P->w; ga.bidx = P->bidx; ga.out = P->Fw; ga.nbt = P->nbt; ga.g = P->g;
+    launch_gather(ga, s);
+    e = hipMemcpyAsync(P->pin + nf, P->Fw, sizeof(double) * nf, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
+static double dot_n(const double* a, const double* b, int n)
+{
+    double acc = 0.0;
+    for (int k = 0; k < n; k++) acc += a[k] * b[k];
+    return acc;
+}
+
+// advance! for MotionFunction bodies (timestepping.jl:35-67): update_bodies!,
+// get_trial_state!, boundary_forces! for V <: Motion (timestepping.jl:176-186:
+// F̃work = E(qs + q0) - ub h, F̃b = Binv F̃work with Binv the CG map of
+// laplacian.jl:137-138), update_stress!, project_circ!, vort2flux!.  The fluid
+// operators run on the device; the CG recurrences of cg! (IterativeSolvers.jl
+// 0.9.1: initial guess = the state's F̃b, tolerance max(1e-12 |b|, 0),
+// maxiter 5000) run on the host over the 2 nb_total-long vectors, one B_times!
+// on the device per product.  Synchronous.
+static int step_mf(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, double* clh)
+{
+    ibpm_ctx* C = P->ctx;
+    hipStream_t s = P->stream;
+    const GridDev& g = P->g;
+    const int nf = P->nf;
+    if (!sc.body_kin) return fail(C, IBPM_E_ARG, "MotionFunction steps need body_kin (6 values per body)");
+    int rc = update_bodies_dev(P, sc.body_kin);
+    if (rc != IBPM_OK) return rc;
+    trial_dev(P);
+    if (nf > 0) {
+        // b = E(qs + q0) - ub h
         GatherArgs ga{};
-        ga.q = P->qs1; ga.q0 = P->q0; ga.w = P->w; ga.bidx = P->bidx; ga.out = P->Ftmp; ga.nbt = P->nbt; ga.g = g;
+        ga.q = P->qs1; ga.q0 = P->q0; ga.w = P->w; ga.bidx = P->bidx; ga.out = P->Fw; ga.nbt = P->nbt; ga.g = g;
         launch_gather(ga, s);
+        std::vector<double> b(nf), x(nf), r(nf), u(nf, 0.0), c(nf);
+        HIPTRY(C, hipMemcpyAsync(P->pin + nf, P->Fw, sizeof(double) * nf, hipMemcpyDeviceToHost, s));
+        HIPTRY(C, hipMemcpyAsync(P->pin, P->Fb, sizeof(double) * nf, hipMemcpyDeviceToHost, s));
+        launch_fill(P->T2, g.nG * P->mg, 0.0, s);
+        HIPTRY(C, hipStreamSynchronize(s));
+        for (int k = 0; k < nf; k++) b[k] = P->pin[nf + k] - P->ub[k] * g.h;
+        std::copy(P->pin, P->pin + nf, x.begin());
+        // cg_iterator! with initially_zero = false: r = b - B x
+        HIPTRY(C, b_times_dev(P));
+        for (int k = 0; k < nf; k++) r[k] = b[k] - P->pin[nf + k];
+        double res = std::sqrt(dot_n(r.data(), r.data(), nf));
+        const double tol = std::max(1e-12 * std::sqrt(dot_n(b.data(), b.data(), nf)), 0.0);
+        double prev = 1.0;
+        int it = 0;
+        // a non-finite residual ends the solve (the reference would run all
+        // maxiter products to the same non-finite F̃b)
+        while (!(it >= 5000 || res <= tol) && std::isfinite(res)) {
+            const double beta = (res * res) / (prev * prev);
+            for (int k = 0; k < nf; k++) u[k] = r[k] + beta * u[k];
+            std::copy(u.begin(), u.end(), P->pin);
+            HIPTRY(C, b_times_dev(P));
+            std::copy(P->pin + nf, P->pin + 2 * nf, c.begin());
+            const double alpha = (res * res) / dot_n(u.data(), c.data(), nf);
+            for (int k = 0; k < nf; k++) x[k] += alpha * u[k];
+            for (int k = 0; k < nf; k++) r[k] -= alpha * c[k];
+            prev = res;
+            res = std::sqrt(dot_n(r.data(), r.data(), nf));
+            it++;
+        }
+        P->cg_iters = it;
+        P->cg_res = res;
+        P->cg_tol = tol;
+        std::copy(x.begin(), x.end(), P->pin);
+        HIPTRY(C, hipMemcpyAsync(P->Fb, P->pin, sizeof(double) * nf, hipMemcpyHostToDevice, s));
+        // ---- update_stress!
         ForceArgs fa{};
         fa.Fb = P->Fb; fa.ds = P->ds_d; fa.boff = P->boff_d; fa.fb = P->fbv; fa.CD = P->CD; fa.CL = P->CL;
         fa.cd_hist = cdh; fa.cl_hist = clh; fa.nbodies = P->nbodies; fa.nbt = P->nbt; fa.hdt = g.h / P->dt;
-        if (cdh) {  // history entry taken from the device slot
+        if (cdh) {
             fa.slot = P->slot;
             fa.nbh = std::max(P->nbodies, 1);
         }
-        launch_gemv_forces(P->BinvT, P->Ftmp, P->Fb, P->nf, fa, s);
-        // ---- project_circ! (timestepping.jl:204-223): Γ = Γs; Γ[:,1] -= Ainv rot(reg(F̃b))
+        launch_forces(fa, s);
+        // ---- project_circ!
         project_forces(P, P->Fb, P->Gs, true);
     }
     std::swap(P->G, P->Gs);
-    // ---- vort2flux!(state.ψ, state.q, state.Γ) (timestepping.jl:64)
     vort2flux_dev(P, P->G, P->psi, P->q, false);
-    (void)nG;
+    HIPTRY(C, hipGetLastError());
+    // the pinned staging is reused by the next step: drain before returning
+    HIPTRY(C, hipStreamSynchronize(s));
+    return IBPM_OK;
 }
 
 static void graphs_clear(ibpm_prob* P)
@@ -651,8 +906,8 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
         return fail(ctx, IBPM_E_ARG, "grid needs nx, ny >= 4 and 1 <= mg <= 16");
     if (mg > 1 && (nx % 4 != 0 || ny % 4 != 0 || nx < 8 || ny < 8))
         return fail(ctx, IBPM_E_UNSUPPORTED, "multi-level grids need nx, ny >= 8 and divisible by 4 (get_bc!/coarsify!)");
-    if (bd->motion == IBPM_MOTION_FUNCTION)
-        return fail(ctx, IBPM_E_UNSUPPORTED, "MotionFunction (lab-frame moving body) is not run by this library");
+    if (bd->motion == IBPM_MOTION_FUNCTION && binv)
+        return fail(ctx, IBPM_E_ARG, "MotionFunction problems have no Binv matrix (CG every step, laplacian.jl:123-141)");
     if (bd->motion == IBPM_MOTION_MOVING_GRID && bd->nbodies != 1)
         return fail(ctx, IBPM_E_ARG, "MovingGrid needs exactly one body (timestepping.jl:277)");
     ibpm_prob* P = new ibpm_prob();
@@ -678,66 +933,15 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
     const int nbt = P->nbt;
     P->xb.assign(bd->xb, bd->xb + 2 * (size_t)nbt);
     P->ds.assign(bd->ds, bd->ds + (size_t)nbt);
+    P->x0 = P->xb;  // RigidBody.x0 = copy(xb) (sample-bodies.jl:32)
+    P->ub.assign(2 * (size_t)nbt, 0.0);
+    const bool mf = P->motion == IBPM_MOTION_FUNCTION;
     auto bail = [&](int code, const std::string& m) {
         ibpm_problem_destroy(P);
         return fail(ctx, code, m);
     };
     // ---- setup_reg (interface-coupling.jl:72-96)
-    const double h = g.h;
-    P->body_idx.resize(2 * (size_t)nbt);
-    for (int k = 0; k < nbt; k++) {
-        P->body_idx[k] = (int)std::floor((P->xb[k] + P->offx) / h);
-        P->body_idx[nbt + k] = (int)std::floor((P->xb[nbt + k] + P->offy) / h);
-        int bx = P->body_idx[k], by = P->body_idx[nbt + k];
-        if (bx - 6 < 1 || bx + 6 > nx || by - 6 < 1 || by + 6 > ny)
-            return bail(IBPM_E_ARG, "body point closer than 6 cells to the domain edge");
-    }
-    P->weight.assign((size_t)2 * nbt * 169, 0.0);
-    for (int k = 0; k < nbt; k++) {
-        double xk = P->xb[k], yk = P->xb[nbt + k];
-        int bx = P->body_idx[k], by = P->body_idx[nbt + k];
-        for (int m = 0; m < 13; m++) {
-            double y = h * (double)(by - 1 + (m - 6)) - P->offy;
-            for (int l = 0; l < 13; l++) {
-                double x = h * (double)(bx - 1 + (l - 6)) - P->offx;
-                P->weight[(size_t)(2 * k) * 169 + l + 13 * m] = delta_h(x, xk, h) * delta_h(y + h / 2, yk, h);
-                P->weight[(size_t)(2 * k + 1) * 169 + l + 13 * m] = delta_h(x + h / 2, xk, h) * delta_h(y, yk, h);
-            }
-        }
-    }
-    // E' as a segmented gather: (flux cell, F̃b index, weight), ordered by cell then point
-    struct Ent { long p; int f; double w; };
-    std::vector<Ent> ents;
-    ents.reserve((size_t)2 * nbt * 169);
-    int ib0 = nx + 1, ib1 = 0, jb0 = ny + 2, jb1 = 0;
-    for (int c = 0; c < 2; c++)
-        for (int k = 0; k < nbt; k++) {
-            int bx = P->body_idx[k], by = P->body_idx[nbt + k];
-            for (int m = 0; m < 13; m++)
-                for (int l = 0; l < 13; l++) {
-                    int i = bx + l - 6, j = by + m - 6;
-                    long p = (c == 0) ? (i - 1) + (long)(nx + 1) * (j - 1) : g.nu + (i - 1) + (long)nx * (j - 1);
-                    ents.push_back({p, k + c * nbt, P->weight[(size_t)(2 * k + c) * 169 + l + 13 * m]});
-                    // rot! outputs touched: x-flux (i,j) -> Γ(i,j),(i,j+1); y-flux -> Γ(i,j),(i+1,j)
-                    ib0 = std::min(ib0, i); jb0 = std::min(jb0, j);
-                    ib1 = std::max(ib1, i + 1); jb1 = std::max(jb1, j + 1);
-                }
-        }
-    std::stable_sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) { return a.p < b.p; });
-    std::vector<int> cell, off, ef;
-    std::vector<double> ew;
-    for (size_t e = 0; e < ents.size(); e++) {
-        if (e == 0 || ents[e].p != ents[e - 1].p) {
-            cell.push_back((int)ents[e].p);
-            off.push_back((int)e);
-        }
-        ef.push_back(ents[e].f);
-        ew.push_back(ents[e].w);
-    }
-    off.push_back((int)ents.size());
-    P->sc_ncell = (int)cell.size();
-    P->rb_i0 = std::max(ib0, 2); P->rb_i1 = std::min(ib1, nx);
-    P->rb_j0 = std::max(jb0, 2); P->rb_j1 = std::min(jb1, ny);
+    if (!setup_reg_host(P)) return bail(IBPM_E_ARG, "body point closer than 6 cells to the domain edge");
     // ---- coarsify! ranges (mg-utils.jl:296-299, zip truncates to the shorter range)
     {
         CoarsifyRange r{};
@@ -772,9 +976,11 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
     DA(P->qs1, nq); DA(P->T1, nG * mg); DA(P->T2, nG * mg); DA(P->Gtmp, nG); DA(P->qreg, nq);
     DA(P->Fb, P->nf); DA(P->Ftmp, P->nf); DA(P->fbv, P->nf); DA(P->CD, P->nbodies); DA(P->CL, P->nbodies);
     DA(P->scal, 16); DA(P->slot, 1);
-    DA(P->BinvT, (size_t)P->nf * P->nf); DA(P->w, P->weight.size());
+    DA(P->BinvT, mf ? 1 : (size_t)P->nf * P->nf); DA(P->w, P->weight.size()); DA(P->Fw, P->nf);
     DA(P->ds_d, nbt); DA(P->bidx, 2 * nbt); DA(P->boff_d, P->boff.size());
-    DA(P->sc_cell, cell.size()); DA(P->sc_off, off.size()); DA(P->sc_f, ef.size()); DA(P->sc_w, ew.size());
+    // E' lists sized for any body position: at most one cell per window entry
+    const size_t nent = (size_t)2 * nbt * 169;
+    DA(P->sc_cell, nent); DA(P->sc_off, nent + 1); DA(P->sc_f, nent); DA(P->sc_w, nent);
     DA(P->cx, nx - 1); DA(P->cy, ny - 1);
     DA(P->tx.tw, twx.size()); DA(P->tx.sn, snx.size()); DA(P->tx.wm, wmx.size());
     DA(P->ty.tw, twy.size()); DA(P->ty.sn, sny.size()); DA(P->ty.wm, wmy.size());
@@ -782,12 +988,9 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
     auto up = [&](void* d, const void* h_, size_t bytes) -> bool {
         return bytes == 0 || hipMemcpy(d, h_, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
-    bool ok = up(P->w, P->weight.data(), sizeof(double) * P->weight.size()) &&
+    bool ok = upload_reg(P) == hipSuccess && hipStreamSynchronize(P->stream) == hipSuccess &&
               up(P->ds_d, P->ds.data(), sizeof(double) * P->ds.size()) &&
-              up(P->bidx, P->body_idx.data(), sizeof(int) * P->body_idx.size()) &&
               up(P->boff_d, P->boff.data(), sizeof(int) * P->boff.size()) &&
-              up(P->sc_cell, cell.data(), sizeof(int) * cell.size()) && up(P->sc_off, off.data(), sizeof(int) * off.size()) &&
-              up(P->sc_f, ef.data(), sizeof(int) * ef.size()) && up(P->sc_w, ew.data(), sizeof(double) * ew.size()) &&
               up(P->cx, cxh.data(), sizeof(double) * cxh.size()) && up(P->cy, cyh.data(), sizeof(double) * cyh.size()) &&
               up(P->tx.tw, twx.data(), sizeof(double2) * twx.size()) && up(P->tx.sn, snx.data(), sizeof(double) * snx.size()) &&
               up(P->tx.wm, wmx.data(), sizeof(double2) * wmx.size()) && up(P->ty.tw, twy.data(), sizeof(double2) * twy.size()) &&
@@ -799,8 +1002,13 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
     launch_fill(P->qreg, nq, 0.0, P->stream);
     launch_fill(P->Gtmp, nG, 0.0, P->stream);
     zero_state(P);
-    // ---- Binv
-    if (P->nf > 0) {
+    if (mf && P->nf > 0) {
+        // pinned staging of the CG's B_times! input and output
+        if (hipHostMalloc((void**)&P->pin, sizeof(double) * 2 * (size_t)P->nf, hipHostMallocDefault) != hipSuccess)
+            return bail(IBPM_E_HIP, "hipHostMalloc failed for the CG staging");
+    }
+    // ---- Binv (none for MotionFunction: CG every step)
+    if (P->nf > 0 && !mf) {
         if (binv) {
             P->binv.assign(binv, binv + (size_t)P->nf * P->nf);
         } else {
@@ -824,6 +1032,7 @@ int ibpm_problem_destroy(ibpm_prob* P)
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     graphs_clear(P);
     for (void* p : P->allocs) (void)hipFree(p);
+    if (P->pin) (void)hipHostFree(P->pin);
     for (hipEvent_t ev : P->pev) (void)hipEventDestroy(ev);
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -878,6 +1087,8 @@ int ibpm_problem_get_info(const ibpm_prob* P, ibpm_problem_info* info)
 
 int ibpm_problem_get_binv(ibpm_prob* P, double* out, size_t n)
 {
+    if (P && P->motion == IBPM_MOTION_FUNCTION)
+        return fail(P->ctx, IBPM_E_UNSUPPORTED, "MotionFunction problems have no Binv matrix (CG every step)");
     if (!P || !out || n != (size_t)P->nf * P->nf) return fail(P ? P->ctx : nullptr, IBPM_E_ARG, "binv size");
     std::copy(P->binv.begin(), P->binv.end(), out);
     return IBPM_OK;
@@ -1023,6 +1234,15 @@ int ibpm_advance_async(ibpm_prob* P, int nsteps, const ibpm_step_scalars* s)
         P->hist_steps = cap;
     }
     if (nsteps > 0) launch_set_int(P->slot, P->pending, P->stream);
+    if (P->motion == IBPM_MOTION_FUNCTION) {
+        // each step solves on the host: run them now, keep what finished
+        for (int k = 0; k < nsteps; k++) {
+            int rc = step_mf(P, s[k], P->hist, P->hist + P->hist_steps * nb);
+            if (rc != IBPM_OK) return rc;
+            P->pending++;
+        }
+        return IBPM_OK;
+    }
     for (int k = 0; k < nsteps; k++) step_dev(P, s[k], P->hist, P->hist + P->hist_steps * nb);
     P->pending += nsteps;
     HIPTRY(P->ctx, hipGetLastError());
@@ -1076,6 +1296,35 @@ int ibpm_get_forces(ibpm_prob* P, double* CD, double* CL, double* Fb)
     if (CD && P->nbodies) HIPTRY(P->ctx, hipMemcpyAsync(CD, P->CD, sizeof(double) * P->nbodies, hipMemcpyDeviceToHost, P->stream));
     if (CL && P->nbodies) HIPTRY(P->ctx, hipMemcpyAsync(CL, P->CL, sizeof(double) * P->nbodies, hipMemcpyDeviceToHost, P->stream));
     if (Fb && P->nf) HIPTRY(P->ctx, hipMemcpyAsync(Fb, P->Fb, sizeof(double) * P->nf, hipMemcpyDeviceToHost, P->stream));
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    return IBPM_OK;
+}
+
+int ibpm_get_body_state(ibpm_prob* P, double* xb, double* ub, size_t n)
+{
+    if (!P || n != 2 * (size_t)P->nbt) return fail(P ? P->ctx : nullptr, IBPM_E_ARG, "body state size");
+    if (xb) std::copy(P->xb.begin(), P->xb.end(), xb);
+    if (ub) std::copy(P->ub.begin(), P->ub.end(), ub);
+    return IBPM_OK;
+}
+
+int ibpm_cg_stats(ibpm_prob* P, int32_t* iterations, double* residual, double* tolerance)
+{
+    if (!P) return IBPM_E_ARG;
+    if (iterations) *iterations = P->cg_iters;
+    if (residual) *residual = P->cg_res;
+    if (tolerance) *tolerance = P->cg_tol;
+    return IBPM_OK;
+}
+
+int ibpm_op_update_bodies(ibpm_prob* P, const double* kin)
+{
+    if (!P || !kin) return IBPM_E_ARG;
+    if (P->motion != IBPM_MOTION_FUNCTION)
+        return fail(P->ctx, IBPM_E_ARG, "update_bodies! moves MotionFunction bodies only");
+    graphs_clear(P);
+    int rc = update_bodies_dev(P, kin);
+    if (rc != IBPM_OK) return rc;
     HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
     return IBPM_OK;
 }

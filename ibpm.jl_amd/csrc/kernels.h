@@ -133,6 +133,8 @@ struct ForceArgs {
 };
 // y = Binv x (AT = Binv transposed, row-major rows), then update_stress! on y
 void launch_gemv_forces(const double* AT, const double* x, double* y, int n, const ForceArgs& fa, hipStream_t s);
+// update_stress! alone, on an F̃b already in place (MotionFunction: F̃b comes from the CG solve)
+void launch_forces(const ForceArgs& fa, hipStream_t s);
 
 struct BaseFluxArgs {
     double* q0;

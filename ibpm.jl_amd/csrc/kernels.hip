@@ -851,6 +851,12 @@ void launch_gemv_forces(const double* AT, const double* x, double* y, int n, con
     hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, s, fa);
 }
 
+void launch_forces(const ForceArgs& fa, hipStream_t s)
+{
+    if (fa.nbodies == 0) return;
+    hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, s, fa);
+}
+
 // ===========================================================================
 // base_flux! for MovingGrid (timestepping.jl:273-318) with XX, YY of
 // models.jl:140-153 (xc = yc = 0) generated from indices.

@@ -36,7 +36,8 @@ EXPORTED = [
     "ibpm_state_download", "ibpm_state_init", "ibpm_base_flux", "ibpm_vort2flux", "ibpm_advance",
     "ibpm_advance_n", "ibpm_get_forces", "ibpm_compute_cfl", "ibpm_op_lapinv", "ibpm_op_dst2",
     "ibpm_op_nonlinear", "ibpm_op_vort2flux", "ibpm_op_regT", "ibpm_op_reg", "ibpm_op_rot", "ibpm_set_option",
-    "ibpm_get_option", "ibpm_time_kernel", "ibpm_profile_stats",
+    "ibpm_get_option", "ibpm_time_kernel", "ibpm_profile_stats", "ibpm_get_body_state", "ibpm_cg_stats",
+    "ibpm_op_update_bodies",
 ]
 
 
@@ -53,7 +54,8 @@ class BodyDesc(ctypes.Structure):
 
 
 class StepScalars(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_double) for n in ("Ux", "Uy", "inclination", "U", "V", "theta", "theta_dot")]
+    _fields_ = [(n, ctypes.c_double) for n in ("Ux", "Uy", "inclination", "U", "V", "theta", "theta_dot")] + [
+        ("body_kin", ctypes.POINTER(ctypes.c_double))]
 
 
 class ProblemInfo(ctypes.Structure):
@@ -113,6 +115,9 @@ def lib():
         L.ibpm_op_rot.argtypes = [_vp, _dp, _dp]
         L.ibpm_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp]
         L.ibpm_profile_stats.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_int64)]
+        L.ibpm_get_body_state.argtypes = [_vp, _dp, _dp, ctypes.c_size_t]
+        L.ibpm_cg_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32), _dp, _dp]
+        L.ibpm_op_update_bodies.argtypes = [_vp, _dp]
         L.ibpm_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.ibpm_get_option.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         _LIB = L

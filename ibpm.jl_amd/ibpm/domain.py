@@ -98,10 +98,28 @@ class Static(InertialMotion):
 
 @dataclass
 class MotionFunction(InertialMotion):
-    """Body moving in the lab frame (motion-types.jl:14-17).  Not run on the GPU."""
+    """Body moving in the lab frame (motion-types.jl:14-17): centre position
+    [x, y, θ] and velocity [ẋ, ẏ, θ̇], each a list of three closures of t
+    (read as motion.xc[k](t), move-body-utils.jl:77-87).  A single closure
+    returning the three values is accepted as well."""
 
     xc: Any
     uc: Any
+
+    @staticmethod
+    def _eval3(f, t):
+        if callable(f):
+            v = f(t)
+        else:
+            v = [g(t) for g in f]
+        v = [float(x) for x in v]
+        if len(v) != 3:
+            raise ValueError("MotionFunction xc/uc give three values (x, y, θ) / (ẋ, ẏ, θ̇)")
+        return v
+
+    def kinematics(self, t: float):
+        """[x, y, θ, ẋ, ẏ, θ̇] at t (get_transformation, move-body-utils.jl:73-97)."""
+        return self._eval3(self.xc, t) + self._eval3(self.uc, t)
 
 
 class MovingGrid(Motion):

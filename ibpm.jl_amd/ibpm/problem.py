@@ -78,7 +78,9 @@ class IBProblem:
             if len(self.model.bodies) != 1:
                 raise ValueError("MovingGrid needs exactly one body (timestepping.jl:277)")
         elif mt is MotionFunction:
-            raise NotImplementedError("MotionFunction (lab-frame moving body) is outside the GPU path")
+            self.motion_kind = C.MOTION_FUNCTION
+            if Binv is not None:
+                raise ValueError("MotionFunction problems have no Binv matrix (CG every step, laplacian.jl:123-141)")
         else:
             raise NotImplementedError("bodies with different motion types")
         self.ctx = C.Context.get(device)
@@ -130,13 +132,73 @@ class IBProblem:
         return out
 
     def scalars(self, t: float) -> C.StepScalars:
-        """The problem's closures evaluated at time t."""
+        """The problem's closures evaluated at time t.  For MotionFunction
+        bodies the kinematics buffer stays valid until the next call."""
+        arr, keep = self.step_scalars([t])
+        self._keep = keep
+        return arr[0]
+
+    def step_scalars(self, ts):
+        """ibpm_step_scalars for each time in ts, and the buffer that the
+        MotionFunction kinematics pointers refer to (keep it alive for the
+        call that takes the array)."""
+        n = len(ts)
+        arr = (C.StepScalars * max(n, 1))()
         fs = self.model.freestream
-        s = C.StepScalars(Ux=float(fs["Ux"](t)), Uy=float(fs["Uy"](t)), inclination=float(fs["inclination"](t)))
-        if self.motion_kind == C.MOTION_MOVING_GRID:
-            m = self.model.bodies[0].motion
-            s.U, s.V, s.theta, s.theta_dot = float(m.U(t)), float(m.V(t)), float(m.θ(t)), float(m.θ̇(t))
-        return s
+        kin = None
+        if self.motion_kind == C.MOTION_FUNCTION:
+            kin = np.zeros((max(n, 1), len(self.model.bodies), 6))
+        for k, t in enumerate(ts):
+            t = float(t)
+            s = arr[k]
+            s.Ux, s.Uy, s.inclination = float(fs["Ux"](t)), float(fs["Uy"](t)), float(fs["inclination"](t))
+            if self.motion_kind == C.MOTION_MOVING_GRID:
+                m = self.model.bodies[0].motion
+                s.U, s.V, s.theta, s.theta_dot = float(m.U(t)), float(m.V(t)), float(m.θ(t)), float(m.θ̇(t))
+            elif kin is not None:
+                for j, b in enumerate(self.model.bodies):
+                    kin[k, j, :] = b.motion.kinematics(t)
+                s.body_kin = C.ptr(kin[k])
+        return arr, kin
+
+    def body_state(self):
+        """(xb, ub), each (nb_total, 2): the body points after the last
+        update_bodies! and their velocities."""
+        xb = np.zeros((self.nb_total, 2), order="F")
+        ub = np.zeros((self.nb_total, 2), order="F")
+        self.ctx.check(C.lib().ibpm_get_body_state(self.h, C.ptr(xb), C.ptr(ub), xb.size), "ibpm_get_body_state")
+        return xb, ub
+
+    def _sync_bodies(self, state=None):
+        """update_bodies! mutates bodies[j].xb and .ub (move-body-utils.jl:128-132)
+        and sets state.xb[j] = bodies[j].xb (timestepping.jl:392)."""
+        if self.motion_kind != C.MOTION_FUNCTION:
+            return
+        xb, ub = self.body_state()
+        k = 0
+        for b in self.model.bodies:
+            n = len(b.ds)
+            b.xb[...] = xb[k:k + n]
+            b.ub[...] = ub[k:k + n]
+            k += n
+        if state is not None:
+            state.xb = [b.xb.copy() for b in self.model.bodies]
+
+    def update_bodies(self, t: float):
+        """update_bodies!(state, prob, t) alone (timestepping.jl:386-398)."""
+        if self.motion_kind != C.MOTION_FUNCTION:
+            return
+        arr, keep = self.step_scalars([t])
+        self.ctx.check(C.lib().ibpm_op_update_bodies(self.h, arr[0].body_kin), "ibpm_op_update_bodies")
+        self._sync_bodies()
+
+    def cg_stats(self):
+        """(products, final |r|, tolerance) of the last step's CG solve (MotionFunction)."""
+        it = ctypes.c_int32(0)
+        r = ctypes.c_double(0.0)
+        tol = ctypes.c_double(0.0)
+        self.ctx.check(C.lib().ibpm_cg_stats(self.h, ctypes.byref(it), ctypes.byref(r), ctypes.byref(tol)))
+        return it.value, r.value, tol.value
 
 
 class LinearizedIBProblem(IBProblem):
@@ -259,9 +321,12 @@ def vort2flux_(state: IBState, prob: IBProblem):
 
 def advance_(state: IBState, prob: IBProblem, t: float):
     """advance!(state, prob, t) -- timestepping.jl:35-67.  Mutates `state`."""
-    s = prob.scalars(t)
-    prob.ctx.check(C.lib().ibpm_advance(prob.h, float(t), ctypes.byref(s)), "ibpm_advance")
-    state.invalidate()
+    arr, keep = prob.step_scalars([t])
+    try:
+        prob.ctx.check(C.lib().ibpm_advance(prob.h, float(t), arr), "ibpm_advance")
+    finally:
+        state.invalidate()
+        prob._sync_bodies(state)
     return None
 
 
@@ -271,15 +336,14 @@ def advance_n_(state: IBState, prob: IBProblem, ts):
     steps taken (fewer than len(ts) if a CL became non-finite)."""
     ts = np.asarray(ts, dtype=np.float64)
     n = len(ts)
-    arr = (C.StepScalars * max(n, 1))()
-    for k, t in enumerate(ts):
-        arr[k] = prob.scalars(float(t))
+    arr, keep = prob.step_scalars(ts)
     nbod = max(len(prob.model.bodies), 1)
     cd = np.zeros((n, nbod))
     cl = np.zeros((n, nbod))
     done = ctypes.c_int(0)
     rc = C.lib().ibpm_advance_n(prob.h, n, C.ptr(ts), arr, C.ptr(cd), C.ptr(cl), ctypes.byref(done))
     state.invalidate()
+    prob._sync_bodies(state)
     if rc not in (C.IBPM_OK, C.IBPM_E_NONFINITE):
         prob.ctx.check(rc, "ibpm_advance_n")
     return cd[: done.value], cl[: done.value], done.value
@@ -293,9 +357,7 @@ def advance_batch_(states, probs, ts_list):
     L = C.lib()
     for st, prob, ts in zip(states, probs, ts_list):
         ts = np.asarray(ts, dtype=np.float64)
-        arr = (C.StepScalars * max(len(ts), 1))()
-        for k, t in enumerate(ts):
-            arr[k] = prob.scalars(float(t))
+        arr, keep = prob.step_scalars(ts)
         prob.ctx.check(L.ibpm_advance_async(prob.h, len(ts), arr), "ibpm_advance_async")
     out = []
     for st, prob, ts in zip(states, probs, ts_list):
@@ -306,6 +368,7 @@ def advance_batch_(states, probs, ts_list):
         done = ctypes.c_int(0)
         rc = L.ibpm_advance_wait(prob.h, C.ptr(cd), C.ptr(cl), ctypes.byref(done))
         st.invalidate()
+        prob._sync_bodies(st)
         if rc not in (C.IBPM_OK, C.IBPM_E_NONFINITE):
             prob.ctx.check(rc, "ibpm_advance_wait")
         out.append((cd[: done.value], cl[: done.value], done.value))

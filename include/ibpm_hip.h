@@ -50,7 +50,8 @@ typedef struct ibpm_prob ibpm_prob;
 /* motion kinds: MotionType(bodies) (src/structure-domain/move-body-utils.jl:15-26) */
 #define IBPM_MOTION_STATIC 0
 #define IBPM_MOTION_MOVING_GRID 1
-#define IBPM_MOTION_FUNCTION 2   /* lab-frame moving body: rejected (IBPM_E_UNSUPPORTED) */
+#define IBPM_MOTION_FUNCTION 2   /* lab-frame moving body (motion-types.jl:14-17): E is rebuilt and the
+                                    Schur system solved by CG every step */
 
 /* state fields for upload/download (IBState, src/pre-processing/state-types.jl:7-20) */
 #define IBPM_FIELD_Q 0
@@ -75,7 +76,9 @@ typedef struct {
     double beta[2];      /* AdamsBashforth β; AB2 = {1.5, -0.5} */
 } ibpm_grid_desc;
 
-/* Bodies: vcat of RigidBody.xb / .ds (src/structure-domain/body-types.jl:10-16). */
+/* Bodies: vcat of RigidBody.xb / .ds (src/structure-domain/body-types.jl:10-16).
+ * For IBPM_MOTION_FUNCTION, xb are also the reference points x0 that
+ * move_body! transforms (x0 = copy(xb), sample-bodies.jl:32). */
 typedef struct {
     int32_t nbodies;
     const int32_t* nb;   /* points per body */
@@ -90,6 +93,10 @@ typedef struct {
 typedef struct {
     double Ux, Uy, inclination;      /* freestream.Ux(t), .Uy(t), .inclination(t) */
     double U, V, theta, theta_dot;   /* motion.U(t), .V(t), .θ(t), .θ̇(t) (MovingGrid only) */
+    /* MotionFunction only: 6 values per body, {x, y, θ, ẋ, ẏ, θ̇} =
+     * motion.xc[1:3](t), motion.uc[1:3](t) (move-body-utils.jl:73-97); read
+     * during the call that takes these scalars.  NULL otherwise. */
+    const double* body_kin;
 } ibpm_step_scalars;
 
 /* Sizes and setup products of a problem (for host mirrors and tests). */
@@ -116,6 +123,9 @@ const char* ibpm_last_error(const ibpm_ctx* ctx);
  * (LU with partial pivoting, as inv() does). */
 int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* grid, const ibpm_body_desc* bodies,
                         const double* binv, ibpm_prob** out);
+/* For IBPM_MOTION_FUNCTION bodies there is no matrix: prob.Binv is the CG map
+ * of get_Binv (laplacian.jl:123-141), so `binv` must be NULL and
+ * ibpm_problem_get_binv returns IBPM_E_UNSUPPORTED. */
 int ibpm_problem_destroy(ibpm_prob* prob);
 /* LinearizedIBProblem(base_state, base_prob, dt) -- problem-types.jl:47-87:
  * turns `prob` (built with the base problem's grid, bodies, Re and this dt)
@@ -163,6 +173,22 @@ int ibpm_advance_n(ibpm_prob* prob, int nsteps, const double* t, const ibpm_step
  * reference: it is the batched form of run_sim!'s loop (ibpm.jl:93-105). */
 int ibpm_advance_async(ibpm_prob* prob, int nsteps, const ibpm_step_scalars* s);
 int ibpm_advance_wait(ibpm_prob* prob, double* cd_hist, double* cl_hist, int* steps_done);
+/* MotionFunction problems step synchronously inside these calls (the CG
+ * solve reads its residual every iteration): ibpm_advance_async then returns
+ * after its steps are done, and ibpm_advance_wait only collects them.  A step
+ * fails with IBPM_E_ARG, leaving the state and the bodies as they were
+ * before that step, when a body point would come within 6 cells of the
+ * domain edge (where the reference's E window raises a BoundsError). */
+
+/* state.xb and RigidBody.ub after the last update_bodies! (timestepping.jl:386-398):
+ * nb_total x 2 column-major each (n = 2 nb_total); either may be NULL.  For
+ * Static and MovingGrid bodies, xb are the points given at creation, ub = 0. */
+int ibpm_get_body_state(ibpm_prob* prob, double* xb, double* ub, size_t n);
+/* MotionFunction: products B u used and |r| reached by the last step's CG
+ * solve, and its tolerance 1e-12 |b| (cg!, laplacian.jl:137-138).  No
+ * counterpart in the reference (cg! is called without log = true there). */
+int ibpm_cg_stats(ibpm_prob* prob, int32_t* iterations, double* residual, double* tolerance);
+
 /* state.CD, state.CL (nbodies each) and state.F̃b (2 nb_total); any may be NULL. */
 int ibpm_get_forces(ibpm_prob* prob, double* CD, double* CL, double* Fb);
 /* compute_cfl(state, prob) = max|q[:,1]| dt / h (ibpm.jl:83-88). */
@@ -182,6 +208,9 @@ int ibpm_op_regT(ibpm_prob* prob, const double* q, double* fb);
 int ibpm_op_reg(ibpm_prob* prob, const double* fb, double* q);
 /* rot! (curl.jl:22-34) */
 int ibpm_op_rot(ibpm_prob* prob, const double* q, double* G);
+/* MotionFunction: update_bodies! alone (timestepping.jl:386-398): move_body!
+ * of every body to kin (6 values per body, as body_kin) and E = setup_reg. */
+int ibpm_op_update_bodies(ibpm_prob* prob, const double* kin);
 
 /* --- tuning and timing (bench, autotuner) --- */
 /* Launch-shape options of a problem.  No counterpart in the reference (it has

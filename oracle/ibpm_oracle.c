@@ -391,7 +391,12 @@ typedef struct {
     int* nb;
     double* xb; /* nbt x 2 col-major */
     double* ds;
-    int motion; /* 0 = Static, 1 = MovingGrid */
+    int motion; /* 0 = Static, 1 = MovingGrid, 2 = MotionFunction */
+    /* MotionFunction: reference points x0 (RigidBody.x0, body-types.jl:13),
+     * point velocities ub (nbt x 2 col-major) and the last CG solve */
+    double *x0, *ub;
+    int cg_iters;
+    double cg_res, cg_tol;
     /* precomputed operators */
     double* Lam;     /* mats.Λ */
     double* divA;    /* mg x nG: 1 ./ (1 .- Λ dt/(2 Re hc^2)) */
@@ -719,6 +724,9 @@ oprob* oracle_problem_create(double h, double x0, double x1, double y0, double y
     memcpy(P->xb, xb, sizeof(double) * 2 * (size_t)nbt);
     memcpy(P->ds, ds, sizeof(double) * (size_t)nbt);
     P->motion = motion;
+    P->x0 = (double*)malloc(sizeof(double) * 2 * (size_t)(nbt > 0 ? nbt : 1));
+    P->ub = (double*)calloc(2 * (size_t)(nbt > 0 ? nbt : 1), sizeof(double));
+    memcpy(P->x0, xb, sizeof(double) * 2 * (size_t)nbt); /* x0 = copy(xb), sample-bodies.jl:32 */
     const long nG = g->nG, nq = g->nq;
     P->Lam = (double*)malloc(sizeof(double) * nG);
     o_lap_eigs(g, P->Lam);
@@ -781,8 +789,11 @@ oprob* oracle_problem_create(double h, double x0, double x1, double y0, double y
         }
     }
     const int nf = 2 * nbt;
-    P->Binv = (double*)malloc(sizeof(double) * (size_t)(nf > 0 ? nf : 1) * (nf > 0 ? nf : 1));
-    if (binv) {
+    P->Binv = (double*)calloc((size_t)(nf > 0 ? nf : 1) * (nf > 0 ? nf : 1), sizeof(double));
+    if (motion == 2) {
+        /* get_Binv for MotionFunction (laplacian.jl:123-141) forms no matrix:
+         * prob.Binv is a CG solve against B_times!, done per step */
+    } else if (binv) {
         memcpy(P->Binv, binv, sizeof(double) * (size_t)nf * nf);
     } else {
         double* e = (double*)calloc(nf, sizeof(double));
@@ -801,7 +812,7 @@ void oracle_problem_free(oprob* P)
 {
     if (!P) return;
     odst_plan_free(&P->p1); odst_plan_free(&P->p2);
-    free(P->nb); free(P->xb); free(P->ds); free(P->Lam); free(P->divA); free(P->divAinv);
+    free(P->nb); free(P->xb); free(P->ds); free(P->x0); free(P->ub); free(P->Lam); free(P->divA); free(P->divAinv);
     free(P->body_idx); free(P->weight); free(P->Binv); free(P->XX); free(P->YY);
     free(P->q1); free(P->G1); free(P->q2); free(P->G2); free(P->q3); free(P->q4); free(P->q5); free(P->q6);
     free(P->G3); free(P->Gbc); free(P->rhsbc); free(P->Gwork); free(P->wdst);
@@ -944,6 +955,114 @@ static void o_boundary_forces(oprob* P, double* Fb, const double* qs1, const dou
     free(tmp);
 }
 
+/* move_body!(::Type{MotionFunction}, body, t) -- move-body-utils.jl:123-133,
+ * with get_transformation -- :73-97.  kin = {x, y, θ, ẋ, ẏ, θ̇} per body, the
+ * closures motion.xc[1:3](t), motion.uc[1:3](t):
+ *   xb = pos + Rx x0,  Rx = [cθ -sθ; sθ cθ]
+ *   ub = vel + Ru x0,  Ru = θ̇ [-sθ -cθ; cθ -sθ]                               */
+static void o_move_bodies(oprob* P, const double* kin)
+{
+    const int nbt = P->nbt;
+    int tally = 0;
+    for (int b = 0; b < P->nbodies; b++) {
+        const double* k = kin + 6L * b;
+        const double c = cos(k[2]), s = sin(k[2]);
+        const double Rx11 = c, Rx12 = -s, Rx21 = s, Rx22 = c;
+        const double Ru11 = k[5] * -s, Ru12 = k[5] * -c, Ru21 = k[5] * c, Ru22 = k[5] * -s;
+        for (int i = 0; i < P->nb[b]; i++) {
+            const long ix = tally + i, iy = (long)nbt + tally + i;
+            const double x0 = P->x0[ix], y0 = P->x0[iy];
+            P->xb[ix] = k[0] + (Rx11 * x0 + Rx12 * y0);
+            P->xb[iy] = k[1] + (Rx21 * x0 + Rx22 * y0);
+            P->ub[ix] = k[3] + (Ru11 * x0 + Ru12 * y0);
+            P->ub[iy] = k[4] + (Ru21 * x0 + Ru22 * y0);
+        }
+        tally += P->nb[b];
+    }
+}
+
+/* update_bodies!(::Type{MotionFunction}, state, prob, t) --
+ * timestepping.jl:386-398: move every body, then E = setup_reg(grid, bodies)
+ * (the new Binv is the CG map, nothing to form).  Returns -2 when a body
+ * point comes within 6 cells of the domain edge (the 13 x 13 window of E
+ * would leave the grid; Julia raises a BoundsError there). */
+static int o_update_bodies(oprob* P, const double* kin)
+{
+    o_move_bodies(P, kin);
+    o_setup_reg(P);
+    const int nx = P->g.nx, ny = P->g.ny;
+    for (int k = 0; k < P->nbt; k++) {
+        int bx = P->body_idx[k], by = P->body_idx[k + P->nbt];
+        if (bx - SUPP < 1 || bx + SUPP > nx || by - SUPP < 1 || by + SUPP > ny) return -2;
+    }
+    return 0;
+}
+
+static double o_dot(const double* a, const double* b, int n)
+{
+    double s = 0.0;
+    for (int k = 0; k < n; k++) s += a[k] * b[k];
+    return s;
+}
+
+/* cg!(x, B, b; maxiter, reltol) of IterativeSolvers.jl 0.9.1 (the reference's
+ * Manifest.toml:266-270; the package is not in the reference tree), as called
+ * by prob.Binv of get_Binv (laplacian.jl:137-138) with B = B_times!:
+ * cg_iterator! with initially_zero = false (x is the initial guess; the
+ * mutating LinearMap hands cg! the output array, i.e. the state's F̃b of the
+ * previous step), tolerance = max(reltol |b|, abstol = 0), then the loop
+ *   β = |r|² / |r_prev|²  (|r_prev| = 1 at the start, u = 0),  u = r + β u,
+ *   c = B u,  α = |r|² / (u·c),  x += α u,  r -= α c
+ * until |r| <= tolerance or maxiter products.  |·| and · are plain sums here
+ * (Julia uses BLAS nrm2 and dot: equal up to rounding). */
+static void o_cg(oprob* P, double* x, const double* b, double reltol, int maxiter)
+{
+    const int n = 2 * P->nbt;
+    double* u = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    double* r = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* c = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    memcpy(r, b, sizeof(double) * (size_t)n);
+    o_B_times(P, c, x);
+    for (int k = 0; k < n; k++) r[k] -= c[k];
+    double res = sqrt(o_dot(r, r, n));
+    const double tol = fmax(reltol * sqrt(o_dot(b, b, n)), 0.0);
+    double prev = 1.0;
+    int it = 0;
+    while (!(it >= maxiter || res <= tol)) {
+        const double beta = (res * res) / (prev * prev);
+        for (int k = 0; k < n; k++) u[k] = r[k] + beta * u[k];
+        o_B_times(P, c, u);
+        const double alpha = (res * res) / o_dot(u, c, n);
+        for (int k = 0; k < n; k++) x[k] += alpha * u[k];
+        for (int k = 0; k < n; k++) r[k] -= alpha * c[k];
+        prev = res;
+        res = sqrt(o_dot(r, r, n));
+        it++;
+    }
+    P->cg_iters = it;
+    P->cg_res = res;
+    P->cg_tol = tol;
+    free(u); free(r); free(c);
+}
+
+/* boundary_forces! for V <: Motion -- timestepping.jl:176-186:
+ * F̃work = E (qs + q0) - ub h (ub stacked [ub; vb], get_ub move-body-utils.jl:35-48),
+ * F̃b = Binv F̃work with Binv the CG map (laplacian.jl:137-138: maxiter 5000,
+ * reltol 1e-12). */
+static void o_boundary_forces_mf(oprob* P, double* Fb, const double* qs1, const double* q01)
+{
+    const og_grid* g = &P->g;
+    const int nf = 2 * P->nbt;
+    const double h = g->h;
+    double* Q = P->q2;
+    for (long k = 0; k < g->nq; k++) Q[k] = qs1[k] + q01[k];
+    double* Fw = (double*)malloc(sizeof(double) * (size_t)(nf > 0 ? nf : 1));
+    o_regT(P, Fw, Q);
+    for (int k = 0; k < nf; k++) Fw[k] -= P->ub[k] * h;
+    o_cg(P, Fb, Fw, 1e-12, 5000);
+    free(Fw);
+}
+
 /* update_stress! -- timestepping.jl:330-354 */
 static void o_update_stress(const oprob* P, ostate* S)
 {
@@ -980,26 +1099,40 @@ static void o_project_circ(oprob* P, double* Gs, ostate* S)
     for (long k = 0; k < nG; k++) S->G[k] -= Gwork[k];
 }
 
-/* advance!(state, prob, t) -- timestepping.jl:35-67.  sc: closures at t. */
+/* advance!(state, prob, t) -- timestepping.jl:35-67.  sc: closures at t
+ * (oracle_base_flux's 7 scalars; for MotionFunction followed by the 6
+ * kinematic values {x, y, θ, ẋ, ẏ, θ̇} of every body).  Returns 0, or -2 when
+ * a moving body leaves the grid. */
 int oracle_advance(oprob* P, ostate* S, const double* sc)
 {
+    if (P->motion == 2) { /* update_bodies!, timestepping.jl:39 */
+        int rc = o_update_bodies(P, sc + 7);
+        if (rc) return rc;
+    }
     if (P->motion == 1) oracle_base_flux(P, S, sc); /* timestepping.jl:41-43 */
     double* qs = P->q1;
     double* Gs = P->G1;
     o_get_trial_state(P, qs, Gs, S);
-    o_boundary_forces(P, S->Fb, qs, S->q0);
+    if (P->motion == 2) o_boundary_forces_mf(P, S->Fb, qs, S->q0);
+    else o_boundary_forces(P, S->Fb, qs, S->q0);
     o_update_stress(P, S);
     o_project_circ(P, Gs, S);
     o_vort2flux(P, S->psi, S->q, S->G, P->g.mg);
     return 0;
 }
 
-/* run nsteps from times t[0..nsteps-1]; sc: 7 scalars per step; CD/CL
- * history (nsteps x nbodies) is recorded when hist != NULL. */
+/* scalars per step of oracle_run: 7, plus 6 per body for MotionFunction */
+int oracle_step_stride(const oprob* P) { return 7 + (P->motion == 2 ? 6 * P->nbodies : 0); }
+
+/* run nsteps from times t[0..nsteps-1]; sc: oracle_step_stride scalars per
+ * step; CD/CL history (nsteps x nbodies) is recorded when hist != NULL.
+ * Returns 0, or the first failing step's status. */
 int oracle_run(oprob* P, ostate* S, int nsteps, const double* sc, double* cd_hist, double* cl_hist)
 {
+    const long stride = oracle_step_stride(P);
     for (int s = 0; s < nsteps; s++) {
-        oracle_advance(P, S, sc + 7L * s);
+        int rc = oracle_advance(P, S, sc + stride * s);
+        if (rc) return rc;
         if (cd_hist) for (int b = 0; b < P->nbodies; b++) {
             cd_hist[(long)s * P->nbodies + b] = S->CD[b];
             cl_hist[(long)s * P->nbodies + b] = S->CL[b];
@@ -1049,3 +1182,21 @@ void oracle_op_nonlinear(oprob* P, double* out, const double* q, const double* q
 }
 void oracle_op_B_times(oprob* P, double* x, const double* z) { o_B_times(P, x, z); }
 int oracle_invert(double* A, int n) { return o_invert(A, n); }
+/* MotionFunction: current body points and velocities (nbt x 2 col-major
+ * each, state.xb and RigidBody.ub), and the last CG solve (iterations,
+ * final |r|, tolerance). */
+void oracle_get_body_state(const oprob* P, double* xb, double* ub)
+{
+    if (xb) memcpy(xb, P->xb, sizeof(double) * 2 * (size_t)P->nbt);
+    if (ub) memcpy(ub, P->ub, sizeof(double) * 2 * (size_t)P->nbt);
+}
+void oracle_cg_stats(const oprob* P, int* iters, double* res_tol)
+{
+    *iters = P->cg_iters;
+    res_tol[0] = P->cg_res;
+    res_tol[1] = P->cg_tol;
+}
+/* update_bodies! alone (kin: 6 per body) and one CG solve B x = b from the
+ * initial guess in x (reltol, maxiter as given) */
+int oracle_op_update_bodies(oprob* P, const double* kin) { return o_update_bodies(P, kin); }
+void oracle_op_cg(oprob* P, double* x, const double* b, double reltol, int maxiter) { o_cg(P, x, b, reltol, maxiter); }

@@ -64,6 +64,11 @@ def lib():
         L.oracle_op_nonlinear.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.c_int]
         L.oracle_invert.argtypes = [_dp, ctypes.c_int]
         L.oracle_problem_linearize.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.oracle_step_stride.argtypes = [ctypes.c_void_p]
+        L.oracle_get_body_state.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.oracle_cg_stats.argtypes = [ctypes.c_void_p, _ip, _dp]
+        L.oracle_op_update_bodies.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_op_cg.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_int]
         L.oracle_invert.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -134,9 +139,15 @@ class OState:
 class OracleProblem:
     """IBProblem + IBModel restated on the CPU (problem-types.jl:188-200).
 
-    bodies: list of (xb (n,2), ds (n,)) arrays.  motion: 0 Static, 1 MovingGrid.
+    bodies: list of (xb (n,2), ds (n,)) arrays.  motion: 0 Static, 1 MovingGrid,
+    2 MotionFunction (lab-frame moving bodies; xb are the reference points x0).
     binv: optional precomputed inverse Schur matrix (nf x nf); computed by
-    get_Binv (laplacian.jl:98-119) when None.
+    get_Binv (laplacian.jl:98-119) when None (no matrix for MotionFunction:
+    laplacian.jl:123-141 solves by CG every step).
+
+    Step scalars: 7 per step (Ux, Uy, inclination, U, V, θ, θ̇); for
+    MotionFunction followed by {x, y, θ, ẋ, ẏ, θ̇} = motion.xc(t), motion.uc(t)
+    of every body (``step_stride`` values in all).
     """
 
     def __init__(self, h, boundary, mg, Re, dt, bodies, motion=0, binv=None):
@@ -167,6 +178,7 @@ class OracleProblem:
         self.nbodies = len(bodies)
         self.nb = nb
         self.Re, self.dt, self.motion = float(Re), float(dt), int(motion)
+        self.step_stride = int(L.oracle_step_stride(self._h))
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -219,18 +231,52 @@ class OracleProblem:
 
     def advance(self, S, sc):
         sc = np.ascontiguousarray(sc, dtype=np.float64)
+        assert sc.size == self.step_stride
         cs = S.cstruct()
-        lib().oracle_advance(self._h, ctypes.byref(cs), _p(sc))
+        rc = lib().oracle_advance(self._h, ctypes.byref(cs), _p(sc))
+        if rc != 0:
+            raise ValueError(f"oracle_advance failed with status {rc}")
 
     def run(self, S, sc_steps, record=True):
-        sc_steps = np.ascontiguousarray(sc_steps, dtype=np.float64).reshape(-1, 7)
+        sc_steps = np.ascontiguousarray(sc_steps, dtype=np.float64).reshape(-1, self.step_stride)
         n = sc_steps.shape[0]
         cd = np.zeros((n, max(self.nbodies, 1)))
         cl = np.zeros((n, max(self.nbodies, 1)))
         cs = S.cstruct()
-        lib().oracle_run(self._h, ctypes.byref(cs), n, _p(sc_steps), _p(cd) if record else None,
-                         _p(cl) if record else None)
+        rc = lib().oracle_run(self._h, ctypes.byref(cs), n, _p(sc_steps), _p(cd) if record else None,
+                              _p(cl) if record else None)
+        if rc != 0:
+            raise ValueError(f"oracle_run failed with status {rc}")
         return cd, cl
+
+    # --- MotionFunction ----------------------------------------------------
+    def body_state(self):
+        """(xb, ub), each (nbt, 2): the moved points and their velocities."""
+        xb = np.zeros((self.nbt, 2), order="F")
+        ub = np.zeros((self.nbt, 2), order="F")
+        lib().oracle_get_body_state(self._h, _p(xb), _p(ub))
+        return xb, ub
+
+    def cg_stats(self):
+        """(iterations, final residual norm, tolerance) of the last CG solve."""
+        it = ctypes.c_int(0)
+        rt = np.zeros(2)
+        lib().oracle_cg_stats(self._h, ctypes.byref(it), _p(rt))
+        return it.value, float(rt[0]), float(rt[1])
+
+    def update_bodies(self, kin):
+        """update_bodies! alone: move the bodies to kin (6 per body) and rebuild E."""
+        kin = np.ascontiguousarray(kin, dtype=np.float64)
+        rc = lib().oracle_op_update_bodies(self._h, _p(kin))
+        if rc != 0:
+            raise ValueError(f"update_bodies failed with status {rc}")
+
+    def cg(self, b, x0=None, reltol=1e-12, maxiter=5000):
+        """cg!(x, B, b) from the initial guess x0 (zeros when None)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
+        lib().oracle_op_cg(self._h, _p(x), _p(b), float(reltol), int(maxiter))
+        return x
 
     # --- operators ---------------------------------------------------------
     def _g(self, n):

@@ -23,6 +23,19 @@ def sc7(prob_like, t):
     return np.array(s, dtype=np.float64)
 
 
+def oracle_steps(prob_like, ts):
+    """Per-step scalars of the oracle: sc7, then for MotionFunction bodies
+    their kinematics {x, y, θ, ẋ, ẏ, θ̇} at t."""
+    rows = []
+    for t in ts:
+        r = [sc7(prob_like, t)]
+        for b in prob_like["bodies"]:
+            if isinstance(b.motion, ibpm.MotionFunction):
+                r.append(np.array(b.motion.kinematics(t)))
+        rows.append(np.concatenate(r))
+    return np.array(rows)
+
+
 def case(name: str):
     """Named configurations.  Sizes are small unless the name says otherwise."""
     U1 = {"Ux": lambda t: 1.0}
@@ -50,6 +63,15 @@ def case(name: str):
                               lambda t: 2 * math.pi * f * A * math.cos(2 * math.pi * f * t))
         bodies = [ibpm.make_plate(1.0, 0.0, grid.h, -0.5, 0.0, motion=mot, n=33)]
         return dict(grid=grid, bodies=bodies, Re=200.0, dt=0.004, freestream={}, motion=mot)
+    if name == "lab":  # MotionFunction: a cylinder translating and spinning through a 3-level grid
+        grid = ibpm.make_grid(1.0 / 16, (-1.5, 2.5, -2.0, 2.0), mg=3)
+        w = 2 * math.pi
+        mot = ibpm.MotionFunction([lambda t: 0.1 * math.sin(w * t), lambda t: 0.05 * (1 - math.cos(w * t)),
+                                   lambda t: 0.4 * t],
+                                  [lambda t: 0.1 * w * math.cos(w * t), lambda t: 0.05 * w * math.sin(w * t),
+                                   lambda t: 0.4])
+        bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, motion=mot, n=40)]
+        return dict(grid=grid, bodies=bodies, Re=100.0, dt=0.01, freestream=U1)
     if name == "config1":  # Re 40 cylinder, 200 x 200, 3 levels
         grid = ibpm.make_grid(0.02, (-1.0, 3.0, -2.0, 2.0), mg=3)
         bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, n=157)]

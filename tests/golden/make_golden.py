@@ -10,6 +10,12 @@ Outputs (data only -- no reference source is copied):
                           MAT v7.3 = HDF5; the three 10001-double datasets are
                           stored contiguously and uncompressed at the byte
                           offsets below (checked: t runs 0:0.001:10).
+  eldredge_force_jl1.npz  t, CL, CD of force_jl1.mat, the Julia lab-frame
+                          (MotionFunction) run of the same maneuver
+                          (pitch_maneuver_motion.jl): raw state.CD/CL, which
+                          are lab-frame forces there; same layout and offsets.
+                          Its last sample is 0 (the animation loop runs
+                          100 x 100 of the 10001 steps).
   eldredge_force_fort.npz step, CD, CL columns (4, 5) of force_fort.dat, the
                           Fortran run of the same maneuver (plot_results.jl:46-49).
   reference_numbers.json  published scalar results with their source lines.
@@ -31,6 +37,10 @@ def main(ref):
     arrs = {k: np.frombuffer(raw, dtype="<f8", count=N, offset=o).copy() for k, o in OFFSETS.items()}
     assert np.allclose(arrs["t"], np.arange(N) * 1e-3, atol=1e-12)
     np.savez_compressed(os.path.join(here, "eldredge_force_jl2.npz"), **arrs)
+    raw1 = open(os.path.join(res, "force_jl1.mat"), "rb").read()
+    arrs1 = {k: np.frombuffer(raw1, dtype="<f8", count=N, offset=o).copy() for k, o in OFFSETS.items()}
+    assert np.allclose(arrs1["t"], np.arange(N) * 1e-3, atol=1e-12)
+    np.savez_compressed(os.path.join(here, "eldredge_force_jl1.npz"), **arrs1)
     fort = np.loadtxt(os.path.join(res, "force_fort.dat"))
     np.savez_compressed(os.path.join(here, "eldredge_force_fort.npz"), step=fort[:, 0], CD=fort[:, 3], CL=fort[:, 4])
     numbers = {

@@ -98,7 +98,8 @@ def test_make_body_specs():
 
 def test_step_scalars_layout():
     s = _capi.StepScalars(1, 2, 3, 4, 5, 6, 7)
-    assert ctypes.sizeof(s) == 7 * 8
+    assert ctypes.sizeof(s) == 7 * 8 + 8  # seven doubles, then the body_kin pointer
+    assert _capi.StepScalars.body_kin.offset == 7 * 8 and not s.body_kin
     assert ctypes.sizeof(_capi.GridDesc) == 3 * 4 + 4 + 7 * 8  # padding after mg
 
 
