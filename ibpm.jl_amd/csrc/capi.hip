@@ -90,6 +90,13 @@ struct ibpm_prob {
     std::vector<double> x0, ub;
     double* Fw = nullptr;
     double* pin = nullptr;
+    // B assembled every step in batches of bk unit-force fields: E' e_f (bq,
+    // also the level-1 flux out), rot! (bgt), transform scratch (bt1), Γ and ψ
+    // of every level (bG, bpsi: level l of field b at (l bk + b) nG), B on the
+    // device (nf x nf column-major) and on the host
+    int bk = 0;
+    double *bq = nullptr, *bgt = nullptr, *bt1 = nullptr, *bG = nullptr, *bpsi = nullptr, *Bd = nullptr;
+    std::vector<double> Bh;
     int cg_iters = 0;
     double cg_res = 0.0, cg_tol = 0.0;
     double* hist = nullptr;   // CD history [hist_steps x nbodies], then CL history
@@ -641,24 +648,85 @@ static int update_bodies_dev(ibpm_prob* P, const double* kin)
     return IBPM_OK;
 }
 
-// B_times! (laplacian.jl:143-169): out = E q1(vort2flux(Ainv rot(E' z))) for z
-// in P->pin[0:nf); the product lands in P->pin[nf:2nf).  T2 is the Γ work
-// array (its coarse-level ring was cleared before the solve), ψ and qs1 are
-// free during the force solve.
-static hipError_t b_times_dev(ibpm_prob* P)
+// B = [B_times!(e_1) ... B_times!(e_nf)] (laplacian.jl:143-169) for the
+// current E, in batches of bk unit forces; every field of a batch goes through
+// the operators of one B_times!: E' e_f, rot!, Ainv on level 1, vort2flux!
+// (coarsify chain, Δinv coarse to fine with the ψ boundary values, curl on
+// level 1) and E.  The result lands in P->Bh.
+static int assemble_b(ibpm_prob* P)
 {
     hipStream_t s = P->stream;
-    const int nf = P->nf;
-    hipError_t e = hipMemcpyAsync(P->Ftmp, P->pin, sizeof(double) * nf, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return e;
-    project_forces(P, P->Ftmp, P->T2, false);
-    vort2flux_dev(P, P->T2, P->psi, P->qs1, true);
-    GatherArgs ga{};
-    ga.q = P->qs1; ga.q0 = nullptr; ga.w = P->w; ga.bidx = P->bidx; ga.out = P->Fw; ga.nbt = P->nbt; ga.g = P->g;
-    launch_gather(ga, s);
-    e = hipMemcpyAsync(P->pin + nf, P->Fw, sizeof(double) * nf, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    return e;
+    const GridDev& g = P->g;
+    const long nG = g.nG, nq = g.nq;
+    const int L = P->mg, nf = P->nf, K = P->bk;
+    for (int f0 = 0; f0 < nf; f0 += K) {
+        const int kc = std::min(K, nf - f0);
+        launch_fill(P->bq, nq * kc, 0.0, s);
+        launch_unit_reg(P->bq, nq, kc, f0, P->bidx, P->w, P->nbt, g, s);
+        launch_fill(P->bgt, nG * kc, 0.0, s);  // rot! writes the E' support box only
+        launch_rot_batch(P->bq, nq, P->bgt, nG, kc, P->rb_i0, P->rb_i1, P->rb_j0, P->rb_j1, g, s);
+        // Γ[:, 1] = Ainv rot(E' e_f); the rows outside the box are zero
+        XArgs a = xargs(P);
+        a.src = P->bgt; a.dst = P->bt1; a.nlev = kc; a.lev0 = 0;
+        a.zlo = P->rb_j0 - 1; a.zhi = std::min(P->rb_j1 - 1, g.n2);
+        launch_xpass(X_PLAIN, a, s);
+        YArgs y = yargs(P, Y_AINV, 0, kc, P->bt1);
+        y.fixlev = 1;
+        launch_ypass(y, P->ytile, s);
+        XArgs c = xargs(P);
+        c.src = P->bt1; c.dst = P->bG; c.nlev = kc; c.lev0 = 0;
+        launch_xpass(X_PLAIN, c, s);
+        // vort2flux!: coarsify into the centres of the coarse levels (their
+        // rings stay zero), then Δinv from the coarsest level down
+        for (int lev = 2; lev <= L; lev++) {
+            CoarsifyArgs cr{};
+            const CoarsifyRange& r = P->crange[0];
+            cr.Gc = P->bG + (long)(lev - 1) * K * nG;
+            cr.G = P->bG + (long)(lev - 2) * K * nG;
+            cr.i0 = r.i0; cr.ic0 = r.ic0; cr.ci = r.ci; cr.j0 = r.j0; cr.jc0 = r.jc0; cr.cj = r.cj;
+            cr.nb = kc; cr.bstride = nG; cr.g = g;
+            launch_coarsify(cr, s);
+        }
+        for (int l = L - 1; l >= 0; l--) {
+            const bool bc = l < L - 1;
+            XArgs a1 = xargs(P);
+            a1.src = P->bG + (long)l * K * nG; a1.dst = P->bt1; a1.nlev = kc; a1.lev0 = l;
+            if (bc) {
+                a1.bcr = P->bpsi + (long)(l + 1) * K * nG;
+                a1.bcstride = nG;
+            }
+            launch_xpass(bc ? X_BC : X_PLAIN, a1, s);
+            YArgs y1 = yargs(P, Y_DINV, l, kc, P->bt1);
+            y1.fixlev = 1;
+            launch_ypass(y1, P->ytile, s);
+            XArgs c1 = xargs(P);
+            c1.src = P->bt1; c1.dst = P->bpsi + (long)l * K * nG; c1.nlev = kc; c1.lev0 = l;
+            launch_xpass(X_PLAIN, c1, s);
+        }
+        CurlArgs ca{};
+        ca.psi = P->bpsi; ca.q = P->bq; ca.qstride = nq; ca.nlev = kc; ca.lev0 = 0; ca.mg = L;
+        ca.colbatch = 1; ca.psl = (long)K * nG; ca.g = g;
+        launch_curl(ca, s);
+        GatherArgs ga{};
+        ga.q = P->bq; ga.q0 = nullptr; ga.w = P->w; ga.bidx = P->bidx; ga.out = P->Bd + (long)f0 * nf;
+        ga.nbt = P->nbt; ga.nb = kc; ga.qstride = nq; ga.ostride = nf; ga.g = g;
+        launch_gather(ga, s);
+    }
+    HIPTRY(P->ctx, hipGetLastError());
+    HIPTRY(P->ctx, hipMemcpyAsync(P->Bh.data(), P->Bd, sizeof(double) * P->Bh.size(), hipMemcpyDeviceToHost, s));
+    HIPTRY(P->ctx, hipStreamSynchronize(s));
+    return IBPM_OK;
+}
+
+// c = B u with the assembled B (column-major)
+static void b_matvec(const std::vector<double>& B, int nf, const double* u, double* c)
+{
+    std::fill(c, c + nf, 0.0);
+    for (int j = 0; j < nf; j++) {
+        const double uj = u[j];
+        const double* col = B.data() + (size_t)j * nf;
+        for (int i = 0; i < nf; i++) c[i] += col[i] * uj;
+    }
 }
 
 static double dot_n(const double* a, const double* b, int n)
@@ -671,11 +739,13 @@ static double dot_n(const double* a, const double* b, int n)
 // advance! for MotionFunction bodies (timestepping.jl:35-67): update_bodies!,
 // get_trial_state!, boundary_forces! for V <: Motion (timestepping.jl:176-186:
 // F̃work = E(qs + q0) - ub h, F̃b = Binv F̃work with Binv the CG map of
-// laplacian.jl:137-138), update_stress!, project_circ!, vort2flux!.  The fluid
-// operators run on the device; the CG recurrences of cg! (IterativeSolvers.jl
-// 0.9.1: initial guess = the state's F̃b, tolerance max(1e-12 |b|, 0),
-// maxiter 5000) run on the host over the 2 nb_total-long vectors, one B_times!
-// on the device per product.  Synchronous.
+// laplacian.jl:137-138), update_stress!, project_circ!, vort2flux!.  The CG
+// recurrences of cg! (IterativeSolvers.jl 0.9.1: initial guess = the state's
+// F̃b, tolerance max(1e-12 |b|, 0), maxiter 5000) run on the host over the
+// 2 nb_total-long vectors; their products B u use B as assembled on the
+// device for this step's E (one batched pass of all unit forces through
+// B_times!'s operators), which equals the reference's matrix-free product
+// up to rounding.  Synchronous.
 static int step_mf(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, double* clh)
 {
     ibpm_ctx* C = P->ctx;
@@ -687,20 +757,20 @@ static int step_mf(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, doubl
     if (rc != IBPM_OK) return rc;
     trial_dev(P);
     if (nf > 0) {
-        // b = E(qs + q0) - ub h
+        // b = E(qs + q0) - ub h, and the initial guess: the state's F̃b
         GatherArgs ga{};
         ga.q = P->qs1; ga.q0 = P->q0; ga.w = P->w; ga.bidx = P->bidx; ga.out = P->Fw; ga.nbt = P->nbt; ga.g = g;
         launch_gather(ga, s);
-        std::vector<double> b(nf), x(nf), r(nf), u(nf, 0.0), c(nf);
         HIPTRY(C, hipMemcpyAsync(P->pin + nf, P->Fw, sizeof(double) * nf, hipMemcpyDeviceToHost, s));
         HIPTRY(C, hipMemcpyAsync(P->pin, P->Fb, sizeof(double) * nf, hipMemcpyDeviceToHost, s));
-        launch_fill(P->T2, g.nG * P->mg, 0.0, s);
-        HIPTRY(C, hipStreamSynchronize(s));
+        rc = assemble_b(P);  // synchronises the stream
+        if (rc != IBPM_OK) return rc;
+        std::vector<double> b(nf), x(nf), r(nf), u(nf, 0.0), c(nf);
         for (int k = 0; k < nf; k++) b[k] = P->pin[nf + k] - P->ub[k] * g.h;
         std::copy(P->pin, P->pin + nf, x.begin());
         // cg_iterator! with initially_zero = false: r = b - B x
-        HIPTRY(C, b_times_dev(P));
-        for (int k = 0; k < nf; k++) r[k] = b[k] - P->pin[nf + k];
+        b_matvec(P->Bh, nf, x.data(), c.data());
+        for (int k = 0; k < nf; k++) r[k] = b[k] - c[k];
         double res = std::sqrt(dot_n(r.data(), r.data(), nf));
         const double tol = std::max(1e-12 * std::sqrt(dot_n(b.data(), b.data(), nf)), 0.0);
         double prev = 1.0;
@@ -710,9 +780,7 @@ static int step_mf(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, doubl
         while (!(it >= 5000 || res <= tol) && std::isfinite(res)) {
             const double beta = (res * res) / (prev * prev);
             for (int k = 0; k < nf; k++) u[k] = r[k] + beta * u[k];
-            std::copy(u.begin(), u.end(), P->pin);
-            HIPTRY(C, b_times_dev(P));
-            std::copy(P->pin + nf, P->pin + 2 * nf, c.begin());
+            b_matvec(P->Bh, nf, u.data(), c.data());
             const double alpha = (res * res) / dot_n(u.data(), c.data(), nf);
             for (int k = 0; k < nf; k++) x[k] += alpha * u[k];
             for (int k = 0; k < nf; k++) r[k] -= alpha * c[k];
@@ -1003,9 +1071,22 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
     launch_fill(P->Gtmp, nG, 0.0, P->stream);
     zero_state(P);
     if (mf && P->nf > 0) {
-        // pinned staging of the CG's B_times! input and output
+        // pinned staging of the force solve's right-hand side and F̃b
         if (hipHostMalloc((void**)&P->pin, sizeof(double) * 2 * (size_t)P->nf, hipHostMallocDefault) != hipSuccess)
             return bail(IBPM_E_HIP, "hipHostMalloc failed for the CG staging");
+        // batches of unit-force fields for the per-step B: as many as fit in 4 GiB
+        const size_t per = sizeof(double) * ((size_t)nq + 2 * (size_t)nG + 2 * (size_t)mg * nG);
+        // (at most 32768 fields: the batch is a grid dimension of launch_rot_batch)
+        P->bk = (int)std::max<size_t>(1, std::min<size_t>({(size_t)P->nf, ((size_t)4 << 30) / per, (size_t)32768}));
+        const size_t K = (size_t)P->bk;
+#define DB(ptr, n)                                                              \
+        if ((e = dalloc(P, &(ptr), (size_t)(n))) != hipSuccess)                 \
+            return bail(IBPM_E_HIP, std::string("hipMalloc failed for " #ptr));
+        DB(P->bq, K * nq); DB(P->bgt, K * nG); DB(P->bt1, K * nG); DB(P->bG, K * mg * nG); DB(P->bpsi, K * mg * nG);
+        DB(P->Bd, (size_t)P->nf * P->nf);
+#undef DB
+        P->Bh.assign((size_t)P->nf * P->nf, 0.0);
+        launch_fill(P->bG, (long)(K * mg * nG), 0.0, P->stream);  // the coarse-level rings stay zero
     }
     // ---- Binv (none for MotionFunction: CG every step)
     if (P->nf > 0 && !mf) {

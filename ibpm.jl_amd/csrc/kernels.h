@@ -18,6 +18,7 @@ struct XArgs {
     int mg;
     // X_BC: apply_bc!(line, get_bc!(bcr), fac) prologue (one level)
     const double* bcr;
+    long bcstride;        // X_BC: element stride of bcr between batch entries (0: one bcr for all)
     double fac;
     // X_TRIAL: rhs = ((A Γ + c1 N1) + c2 N0) + rhsbc, then a second x-DST
     const double* n1;
@@ -38,6 +39,7 @@ struct YArgs {
     double* buf;
     long lstride;
     int nlev, lev0;
+    int fixlev;           // 1: every batch entry is level lev0 (a batch of fields); 0: entry b is level lev0 + b
     int op;
     const double* cx;   // cos(pi i / nx), i = 1..nx-1
     const double* cy;   // cos(pi j / ny), j = 1..ny-1
@@ -76,6 +78,10 @@ struct CurlArgs {
     double* q;          // output: entry b is level lev0 + b
     long qstride;
     int nlev, lev0, mg;
+    // colbatch = 1: the nlev entries are fields at level lev0, field b's level
+    // l at psi + l * psl + b * nG (levels of a batch laid out level-major)
+    int colbatch;
+    long psl;
     GridDev g;
 };
 void launch_curl(const CurlArgs& a, hipStream_t s);
@@ -84,6 +90,8 @@ struct CoarsifyArgs {
     double* Gc;
     const double* G;
     int i0, ic0, ci, j0, jc0, cj;
+    int nb;               // batch of fields (0 or 1: one), field b at Gc/G + b * bstride
+    long bstride;
     GridDev g;
 };
 void launch_coarsify(const CoarsifyArgs& a, hipStream_t s);
@@ -99,9 +107,20 @@ struct GatherArgs {
     const int* bidx;   // [k] = bx, [nbt + k] = by (1-based)
     double* out;       // 2 nbt
     int nbt;
+    int nb;            // batch of fields (0 or 1: one): q + b * qstride -> out + b * ostride
+    long qstride, ostride;
     GridDev g;
 };
 void launch_gather(const GatherArgs& a, hipStream_t s);
+
+// E' e_f for f = f0 .. f0 + nb - 1 (unit force vectors, reg! of
+// interface-coupling.jl:99-119): field b (at q + b * qstride, zero on entry)
+// receives the 13 x 13 weights of body point f mod nbt, component f / nbt
+void launch_unit_reg(double* q, long qstride, int nb, int f0, const int* bidx, const double* w, int nbt,
+                     const GridDev& g, hipStream_t s);
+// rot! over a rectangle for nb fields: q + b * qstride -> G + b * gstride
+void launch_rot_batch(const double* q, long qstride, double* G, long gstride, int nb, int ib0, int ib1, int jb0,
+                      int jb1, const GridDev& g, hipStream_t s);
 
 struct ScatterArgs {
     const int* cell;   // flux index of each touched cell

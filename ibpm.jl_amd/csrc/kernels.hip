@@ -100,14 +100,15 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
     // column terms (i = 1, i = nx-1); fac * get_bc!(ψ[:, lev+1])
     const bool rb = (j == 1), rt = (j == a.g.ny - 1);
     double bl = 0.0, br = 0.0, wr[XPL];
+    const double* bcr = a.bcr + b * a.bcstride;
     if constexpr (MODE == X_BC) {
-        bl = a.fac * bc_col(a.bcr, a.g, a.g.nx / 4, 1 + j);
-        br = a.fac * bc_col(a.bcr, a.g, 3 * a.g.nx / 4, 1 + j);
+        bl = a.fac * bc_col(bcr, a.g, a.g.nx / 4, 1 + j);
+        br = a.fac * bc_col(bcr, a.g, 3 * a.g.nx / 4, 1 + j);
         if (rb || rt) {
 #pragma unroll
             for (int t = 0; t < XPL; t++) {
                 int n = lane + WAVE * t;
-                if (n < n1) wr[t] = a.fac * bc_row(a.bcr, a.g, rb ? a.g.ny / 4 : 3 * a.g.ny / 4, n + 2);
+                if (n < n1) wr[t] = a.fac * bc_row(bcr, a.g, rb ? a.g.ny / 4 : 3 * a.g.ny / 4, n + 2);
             }
         }
     }
@@ -121,7 +122,7 @@ __global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
             double x = v[t];
             if constexpr (MODE == X_BC) {
                 if (rb || rt) x += wr[t];
-                if (rb && rt) x += a.fac * get_bc_val(a.bcr, a.g, (a.g.TOP + 1) + (n + 1));  // ny = 2 only
+                if (rb && rt) x += a.fac * get_bc_val(bcr, a.g, (a.g.TOP + 1) + (n + 1));  // ny = 2 only
                 if (n == 0) x += bl;
                 if (n == n1 - 1) x += br;
             }
@@ -368,7 +369,7 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
         }
     }
     __syncthreads();
-    const int lev = a.lev0 + b;
+    const int lev = a.fixlev ? a.lev0 : a.lev0 + b;
     const double hcl = level_hc(a.g.h, lev);
     const double qc = a.dt / ((2.0 * a.Re) * (hcl * hcl));  // dt / (2 Re h^2), laplacian.jl:69,74
     // (column, transform) pairs: transform 0 is followed by the eigenvalue
@@ -632,10 +633,12 @@ __global__ void __launch_bounds__(256) k_curl(CurlArgs a)
     const int i = (rem % nbx) * 64 + threadIdx.x + 1;  // 1..nx+1
     const int j = (rem / nbx) * 4 + threadIdx.y + 1;   // 1..ny+1
     if (i > nx + 1 || j > ny + 1) return;
-    const int lev = a.lev0 + b;
-    const double* __restrict__ psi = a.psi + (long)lev * g.nG;
+    const int lev = a.colbatch ? a.lev0 : a.lev0 + b;
+    const long psl = a.colbatch ? a.psl : g.nG;
+    const long psb = a.colbatch ? (long)b * g.nG : 0;
+    const double* __restrict__ psi = a.psi + (long)lev * psl + psb;
     const bool hasbc = lev < a.mg - 1;
-    const double* __restrict__ pc = a.psi + (long)(lev + 1) * g.nG;
+    const double* __restrict__ pc = a.psi + (long)(lev + 1) * psl + psb;
     auto PS = [&](int ii, int jj) -> double { return psi[(ii - 1) + (long)n1 * (jj - 1)]; };
     auto BC = [&](int k) -> double { return hasbc ? get_bc_val(pc, g, k) : 0.0; };
     const int T = g.TOP, B = g.BOT, L = g.LEFT, R = g.RIGHT;
@@ -674,23 +677,28 @@ __global__ void __launch_bounds__(256) k_coarsify(CoarsifyArgs a)
 {
     const int n1 = a.g.n1;
     // logical block (64 x 4 coarse points), XCD-contiguous (shared fine rows in one L2)
-    const int nbx = (a.ci + 63) / 64;
-    const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
+    const int nbx = (a.ci + 63) / 64, nby = (a.cj + 3) / 4;
+    const int nbt = a.nb > 1 ? a.nb : 1;
+    const int lin0 = xcd_block(blockIdx.x, gridDim.x / NXCD);
+    const int b = lin0 / (nbx * nby);
+    if (b >= nbt) return;
+    const int lin = lin0 - b * (nbx * nby);
     const int ai = (lin % nbx) * 64 + threadIdx.x;
     const int bj = (lin / nbx) * 4 + threadIdx.y;
     if (ai >= a.ci || bj >= a.cj) return;
     const int i = a.i0 + 2 * ai, ic = a.ic0 + ai;
     const int j = a.j0 + 2 * bj, jc = a.jc0 + bj;
-    auto F = [&](int ii, int jj) -> double { return a.G[(ii - 1) + (long)n1 * (jj - 1)]; };
+    const double* __restrict__ G = a.G + (long)b * a.bstride;
+    auto F = [&](int ii, int jj) -> double { return G[(ii - 1) + (long)n1 * (jj - 1)]; };
     double s1 = ((F(i + 1, j) + F(i, j + 1)) + F(i - 1, j)) + F(i, j - 1);
     double s2 = ((F(i + 1, j + 1) + F(i + 1, j - 1)) + F(i - 1, j - 1)) + F(i - 1, j + 1);
-    a.Gc[(ic - 1) + (long)n1 * (jc - 1)] = (F(i, j) + 0.5 * s1) + 0.25 * s2;
+    a.Gc[(long)b * a.bstride + (ic - 1) + (long)n1 * (jc - 1)] = (F(i, j) + 0.5 * s1) + 0.25 * s2;
 }
 
 void launch_coarsify(const CoarsifyArgs& a, hipStream_t s)
 {
     if (a.ci <= 0 || a.cj <= 0) return;
-    const int nblocks = ((a.ci + 63) / 64) * ((a.cj + 3) / 4);
+    const int nblocks = ((a.ci + 63) / 64) * ((a.cj + 3) / 4) * (a.nb > 1 ? a.nb : 1);
     hipLaunchKernelGGL(k_coarsify, dim3(NXCD * xcd_nper(nblocks)), dim3(64, 4), 0, s, a);
 }
 
@@ -698,11 +706,13 @@ void launch_coarsify(const CoarsifyArgs& a, hipStream_t s)
 // rot! (curl.jl:22-34) over a rectangle of output points
 // ===========================================================================
 __global__ void __launch_bounds__(256) k_rot(const double* __restrict__ q, double* __restrict__ G, int ib0, int ib1,
-                                             int jb0, int jb1, GridDev g)
+                                             int jb0, int jb1, GridDev g, long qstride, long gstride)
 {
     const int i = ib0 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = jb0 + blockIdx.y * blockDim.y + threadIdx.y;
     if (i > ib1 || j > jb1) return;
+    q += (long)blockIdx.z * qstride;
+    G += (long)blockIdx.z * gstride;
     const int nx = g.nx;
     auto QXv = [&](int ii, int jj) -> double { return q[(ii - 1) + (long)(nx + 1) * (jj - 1)]; };
     auto QYv = [&](int ii, int jj) -> double { return q[g.nu + (ii - 1) + (long)nx * (jj - 1)]; };
@@ -716,7 +726,16 @@ void launch_rot(const double* q, double* G, int ib0, int ib1, int jb0, int jb1, 
     if (ib1 < ib0 || jb1 < jb0) return;
     dim3 blk(64, 4);
     dim3 grid((ib1 - ib0 + 1 + 63) / 64, (jb1 - jb0 + 1 + 3) / 4);
-    hipLaunchKernelGGL(k_rot, grid, blk, 0, s, q, G, ib0, ib1, jb0, jb1, g);
+    hipLaunchKernelGGL(k_rot, grid, blk, 0, s, q, G, ib0, ib1, jb0, jb1, g, 0L, 0L);
+}
+
+void launch_rot_batch(const double* q, long qstride, double* G, long gstride, int nb, int ib0, int ib1, int jb0,
+                      int jb1, const GridDev& g, hipStream_t s)
+{
+    if (ib1 < ib0 || jb1 < jb0 || nb <= 0) return;
+    dim3 blk(64, 4);
+    dim3 grid((ib1 - ib0 + 1 + 63) / 64, (jb1 - jb0 + 1 + 3) / 4, nb);
+    hipLaunchKernelGGL(k_rot, grid, blk, 0, s, q, G, ib0, ib1, jb0, jb1, g, qstride, gstride);
 }
 
 // ===========================================================================
@@ -727,8 +746,11 @@ void launch_rot(const double* q, double* G, int ib0, int ib1, int jb0, int jb1, 
 __global__ void __launch_bounds__(256) k_gather(GatherArgs a)
 {
     const int lane = threadIdx.x & (WAVE - 1);
-    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    if (wid >= 2 * a.nbt) return;
+    const int wid0 = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int bq = wid0 / (2 * a.nbt);
+    if (bq >= (a.nb > 1 ? a.nb : 1)) return;
+    const int wid = wid0 - bq * (2 * a.nbt);
+    const double* __restrict__ qf = a.q + (long)bq * a.qstride;
     const int c = wid / a.nbt, k = wid % a.nbt;
     const int bx = a.bidx[k], by = a.bidx[a.nbt + k];
     const double* w = a.w + (long)(2 * k + c) * 169;
@@ -738,18 +760,45 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs a)
         const int l = e % 13, m = e / 13;
         const int i = bx + l - 6, j = by + m - 6;
         long p = (c == 0) ? (i - 1) + (long)(nx + 1) * (j - 1) : a.g.nu + (i - 1) + (long)nx * (j - 1);
-        double v = a.q0 ? (a.q[p] + a.q0[p]) : a.q[p];
+        double v = a.q0 ? (qf[p] + a.q0[p]) : qf[p];
         acc += w[e] * v;
     }
     acc = wave_sum(acc);
-    if (lane == 0) a.out[c * a.nbt + k] = acc;
+    if (lane == 0) a.out[(long)bq * a.ostride + c * a.nbt + k] = acc;
 }
 
 void launch_gather(const GatherArgs& a, hipStream_t s)
 {
-    int waves = 2 * a.nbt;
-    if (waves == 0) return;
-    hipLaunchKernelGGL(k_gather, dim3((waves + 3) / 4), dim3(256), 0, s, a);
+    long waves = 2L * a.nbt * (a.nb > 1 ? a.nb : 1);
+    if (a.nbt == 0) return;
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, a);
+}
+
+// E' e_f: field b gets the weights of F̃b entry f = f0 + b at its window cells
+// (the only non-zero terms of reg! for a unit force, interface-coupling.jl:99-119)
+__global__ void __launch_bounds__(256) k_unit_reg(double* q, long qstride, int nb, int f0, const int* bidx,
+                                                  const double* w, int nbt, GridDev g)
+{
+    const int b = blockIdx.x;
+    if (b >= nb) return;
+    const int f = f0 + b;
+    const int c = f / nbt, k = f - c * nbt;
+    const int bx = bidx[k], by = bidx[nbt + k];
+    const double* wk = w + (long)(2 * k + c) * 169;
+    double* qb = q + (long)b * qstride;
+    for (int e = threadIdx.x; e < 169; e += blockDim.x) {
+        const int l = e % 13, m = e / 13;
+        const int i = bx + l - 6, j = by + m - 6;
+        long p = (c == 0) ? (i - 1) + (long)(g.nx + 1) * (j - 1) : g.nu + (i - 1) + (long)g.nx * (j - 1);
+        qb[p] = wk[e];
+    }
+}
+
+void launch_unit_reg(double* q, long qstride, int nb, int f0, const int* bidx, const double* w, int nbt,
+                     const GridDev& g, hipStream_t s)
+{
+    if (nb <= 0 || nbt == 0) return;
+    hipLaunchKernelGGL(k_unit_reg, dim3(nb), dim3(256), 0, s, q, qstride, nb, f0, bidx, w, nbt, g);
 }
 
 // ===========================================================================
