@@ -9,6 +9,7 @@ configuration run in FP64).  A step is one advance!(state, prob, t).
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
     python bench.py --kernels-only [--kernel-reps R]   (per-kernel timing only)
     python bench.py --workload naca64 [--steps K]       (batched 64-instance NACA workload)
+    python bench.py --workload configs                  (every configuration, CPU oracle beside it)
 
 Prints one JSON line.  Timed region: K consecutive steps enqueued by
 ibpm_advance_n with the state resident in HBM, bracketed by device
@@ -123,6 +124,62 @@ def bench_naca64(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_configs(args):
+    """One line per configuration: GPU FP64 timesteps/s (state resident, one
+    ibpm_advance_n call), the same rate as a fraction of 8 TB/s by the bytes
+    formula, and the C oracle's timesteps/s on 1 and on 16 host threads
+    (a few seconds of steps each, after one untimed step)."""
+    import ibpm
+    import oracle as O
+    from ibpm.autotune import autotune
+    from cases import boundary_of, case, gaussian_vortex_gamma, oracle_bodies, oracle_steps
+
+    runs = [("config1", "cylinder Re=40, 200x200, 3 levels", 600, False),
+            ("config2", "cylinder Re=100, 800x400, 5 levels, Gaussian vortex", 300, True),
+            ("config4", "cylinder Re=200, 1024x1024, 5 levels, Gaussian vortex", 300, True),
+            ("config5", "MovingGrid pitching plate Re=200, 512x256, 4 levels", 300, False),
+            ("eldredge_lab", "MotionFunction pitching plate Re=100, 400x200, 5 levels (lab frame)", 200, False)]
+    for name, desc, steps, vortex in runs:
+        c = case(name)
+        g = c["grid"]
+        nb = sum(len(b.ds) for b in c["bodies"])
+        motion = 2 if isinstance(c["bodies"][0].motion, ibpm.MotionFunction) else (1 if "motion" in c else 0)
+        prob = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
+        autotune(prob, reps=5)
+        st = ibpm.IBState(prob)
+        if vortex:
+            st.Γ = gaussian_vortex_gamma(g)
+            ibpm.vort2flux_(st, prob)
+        w = 20
+        ibpm.advance_n_(st, prob, np.arange(w) * c["dt"])
+        ts = (w + np.arange(steps)) * c["dt"]
+        t0 = time.perf_counter()
+        _, _, done = ibpm.advance_n_(st, prob, ts)
+        el = time.perf_counter() - t0
+        sps = done / el
+        byts = algorithmic_bytes_per_step(g.nx, g.ny, g.mg, nb) + (8.0 * g.mg * g.nq if motion == 1 else 0.0)
+        cpu = {}
+        for threads in (1, 16):
+            O.set_threads(threads)
+            op = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], c["dt"], oracle_bodies(c), motion=motion,
+                                 binv=None if motion == 2 else prob.Binv)
+            S = op.new_state(oracle_steps(c, [0.0])[0])
+            if vortex:
+                S.G[:] = gaussian_vortex_gamma(g)
+            rows = oracle_steps(c, np.arange(12) * c["dt"])
+            op.advance(S, rows[0])
+            k, tc = 0, time.perf_counter()
+            while k < 10 and (time.perf_counter() - tc) < 3.0:
+                k += 1
+                op.advance(S, rows[k])
+            cpu[threads] = k / (time.perf_counter() - tc)
+        print(json.dumps({"workload": name, "desc": desc, "grid": [g.nx, g.ny], "levels": g.mg, "body_points": nb,
+                          "gpu_timesteps_per_s": round(sps, 1), "steps": done,
+                          "hbm_frac_by_bytes_formula": round(byts * sps / 1e9 / HBM_PEAK_GBS, 4),
+                          "cpu_oracle_1_thread": round(cpu[1], 3), "cpu_oracle_16_threads": round(cpu[16], 3)}),
+              flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,8 +188,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--kernel-reps", type=int, default=50)
-    ap.add_argument("--workload", choices=("cylinder1024", "naca64"), default="cylinder1024",
-                    help="naca64: the batched 64-instance NACA workload (secondary line)")
+    ap.add_argument("--workload", choices=("cylinder1024", "naca64", "configs"), default="cylinder1024",
+                    help="naca64: the batched 64-instance NACA workload (secondary line); configs: one line "
+                         "per configuration with the CPU oracle beside it")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only autotune and time each kernel kind (for counter profiling); prints kernel_ms")
     args = ap.parse_args()
@@ -143,6 +201,8 @@ def main():
 
     if args.workload == "naca64":
         return bench_naca64(args)
+    if args.workload == "configs":
+        return bench_configs(args)
 
     import ibpm
     from ibpm import _capi as C

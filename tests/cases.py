@@ -72,6 +72,25 @@ def case(name: str):
                                    lambda t: 0.4])
         bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, motion=mot, n=40)]
         return dict(grid=grid, bodies=bodies, Re=100.0, dt=0.01, freestream=U1)
+    if name == "config5":  # pitching plate in the body-fixed frame: 512 x 256, 4 levels, nominal A = 15 deg, f = 0.2
+        grid = ibpm.make_grid(1.0 / 128, (-1.0, 3.0, -1.0, 1.0), mg=4)
+        A, f = 15 * math.pi / 180, 0.2
+        mot = ibpm.MovingGrid(lambda t: 1.0, lambda t: 0.0, lambda t: A * math.sin(2 * math.pi * f * t),
+                              lambda t: 2 * math.pi * f * A * math.cos(2 * math.pi * f * t))
+        bodies = [ibpm.make_plate(1.0, 0.0, grid.h, -0.5, 0.0, motion=mot, n=101)]
+        return dict(grid=grid, bodies=bodies, Re=200.0, dt=0.002, freestream={}, motion=mot)
+    if name == "eldredge_lab":  # pitch_maneuver_motion.jl: plate pitching about its leading edge in the lab frame
+        grid = ibpm.make_grid(0.02, (-1.5, 6.5, -2.0, 2.0), mg=5)
+        a, t1, t2, t3, t4, amax = 11.0, 1.0, 3.0, 4.0, 6.0, 45 * math.pi / 180
+        G = lambda t: math.log(math.cosh(a * (t - t1)) * math.cosh(a * (t - t4))  # noqa: E731
+                               / (math.cosh(a * (t - t2)) * math.cosh(a * (t - t3))))
+        Gd = lambda t: a * (math.tanh(a * (t - t1)) - math.tanh(a * (t - t2))  # noqa: E731
+                            - math.tanh(a * (t - t3)) + math.tanh(a * (t - t4)))
+        Gmax = max(G(k * 1e-3) for k in range(10001))
+        mot = ibpm.MotionFunction([lambda t: 0.0, lambda t: 0.0, lambda t: -amax * G(t) / Gmax],
+                                  [lambda t: 0.0, lambda t: 0.0, lambda t: -amax * Gd(t) / Gmax])
+        bodies = [ibpm.make_plate(1.0, 0.0, grid.h, 0.0, 0.0, motion=mot, n=51)]
+        return dict(grid=grid, bodies=bodies, Re=100.0, dt=1e-3, freestream=U1)
     if name == "config1":  # Re 40 cylinder, 200 x 200, 3 levels
         grid = ibpm.make_grid(0.02, (-1.0, 3.0, -2.0, 2.0), mg=3)
         bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, n=157)]
