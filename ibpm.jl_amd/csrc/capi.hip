@@ -66,6 +66,7 @@ struct ibpm_prob {
     double *QB = nullptr, *GB = nullptr;
     int ytile = 8;         // y-pass columns per workgroup, all-level launches
     int ytile_single = 4;  // same, one-level launches (vort2flux chain)
+    int xwaves = 4;        // lines per work-group of the plain x-passes over all levels (or a batch)
 
     // device state (IBState) and work arrays
     double *q = nullptr, *q0 = nullptr, *G = nullptr, *Gs = nullptr, *psi = nullptr;
@@ -312,6 +313,7 @@ static XArgs xargs(ibpm_prob* P)
     a.fac = 1.0;
     a.zlo = 1;
     a.zhi = P->g.n2;
+    a.waves = P->xwaves;
     return a;
 }
 
@@ -1561,8 +1563,16 @@ int ibpm_set_option(ibpm_prob* P, int option, int value)
 {
     if (!P) return IBPM_E_ARG;
     graphs_clear(P);
+    if (option == IBPM_OPT_XWAVES_BATCH) {
+        if (value != 4 && value != 8 && value != 12 && value != 14)
+            return fail(P->ctx, IBPM_E_ARG, "x-pass work-group lines must be 4, 8, 12 or 14");
+        if (xpass_lds_bytes(P->tx.plan, value) > 160 * 1024)
+            return fail(P->ctx, IBPM_E_ARG, "x-pass work-group does not fit in LDS for this nx");
+        P->xwaves = value;
+        return IBPM_OK;
+    }
     if (option == IBPM_OPT_YTILE_BATCH || option == IBPM_OPT_YTILE_SINGLE) {
-        if (value != 4 && value != 8) return fail(P->ctx, IBPM_E_ARG, "y tile must be 4 or 8");
+        if (value != 4 && value != 8 && value != 12) return fail(P->ctx, IBPM_E_ARG, "y tile must be 4, 8 or 12");
         if (ypass_lds_bytes(P->ty.plan, P->g.n2, value) > 160 * 1024)
             return fail(P->ctx, IBPM_E_ARG, "y tile does not fit in LDS for this ny");
         (option == IBPM_OPT_YTILE_BATCH ? P->ytile : P->ytile_single) = value;
@@ -1592,6 +1602,7 @@ int ibpm_get_option(ibpm_prob* P, int option, int* value)
     else if (option == IBPM_OPT_YTILE_SINGLE) *value = P->ytile_single;
     else if (option == IBPM_OPT_PROFILE) *value = P->prof ? 1 : 0;
     else if (option == IBPM_OPT_GRAPHS) *value = P->graphs ? 1 : 0;
+    else if (option == IBPM_OPT_XWAVES_BATCH) *value = P->xwaves;
     else return fail(P->ctx, IBPM_E_ARG, "unknown option");
     return IBPM_OK;
 }

@@ -29,10 +29,15 @@ struct XArgs {
     // X_PLAIN: input lines j (1-based) outside [zlo, zhi] are known to be zero;
     // their output lines are written as zeros without a transform
     int zlo, zhi;
+    // X_PLAIN over a batch (nlev > 1): waves (lines) per work-group, 8, 12 or
+    // 14; anything else (0) gives the default 4
+    int waves;
     DstPlan pl;
     GridDev g;
 };
 void launch_xpass(int mode, const XArgs& a, hipStream_t s);
+// LDS bytes of an x-pass work-group of `waves` lines
+size_t xpass_lds_bytes(const DstPlan& pl, int waves);
 
 enum YOp { Y_DINV = 0, Y_A = 1, Y_AINV = 2 };
 struct YArgs {

@@ -67,18 +67,18 @@ __device__ __forceinline__ void dst_line(double* s, const DstPlan& pl, const Dst
     else dst1_line(s, pl, tb, lane);
 }
 
-template <int MODE, int PT>
-__global__ void __launch_bounds__(XWAVES * WAVE) k_xpass(XArgs a)
+template <int MODE, int PT, int XW>
+__global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
 {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & (WAVE - 1);
     const int w = threadIdx.x / WAVE;
     const int n1 = line_len<PT>(a.g.n1), n2 = a.g.n2;
-    const int line = blockIdx.x * XWAVES + w;  // 0-based j - 1
+    const int line = blockIdx.x * XW + w;  // 0-based j - 1
     const int b = blockIdx.y;
     const int j = line + 1;
     const int ntab = tab_doubles_t<PT>(a.pl);
-    const DstTab tb = load_tab_t<PT>(lds, a.pl, threadIdx.x, XWAVES * WAVE);
+    const DstTab tb = load_tab_t<PT>(lds, a.pl, threadIdx.x, XW * WAVE);
     __syncthreads();
     if (line >= n2) return;
     if constexpr (MODE == X_PLAIN) {
@@ -218,14 +218,32 @@ static void allow_big_lds(K kern)
     (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 
-template <int MODE, int PT>
+template <int MODE, int PT, int XW>
 static void launch_x_t(const XArgs& a, hipStream_t s)
 {
-    static bool once = (allow_big_lds(k_xpass<MODE, PT>), true);
+    static bool once = (allow_big_lds(k_xpass<MODE, PT, XW>), true);
     (void)once;
-    dim3 grid((a.g.n2 + XWAVES - 1) / XWAVES, a.nlev);
-    size_t lds = sizeof(double) * ((size_t)tab_doubles_t<PT>(a.pl) + (size_t)XWAVES * slot_doubles_t<PT>(a.pl));
-    hipLaunchKernelGGL((k_xpass<MODE, PT>), grid, dim3(XWAVES * WAVE), lds, s, a);
+    dim3 grid((a.g.n2 + XW - 1) / XW, a.nlev);
+    size_t lds = sizeof(double) * ((size_t)tab_doubles_t<PT>(a.pl) + (size_t)XW * slot_doubles_t<PT>(a.pl));
+    hipLaunchKernelGGL((k_xpass<MODE, PT, XW>), grid, dim3(XW * WAVE), lds, s, a);
+}
+
+// plain x-passes over a batch may run in larger work-groups (one copy of the
+// tables in LDS for more lines: more waves per CU); the others use XWAVES
+template <int MODE, int PT>
+static void launch_x_w(const XArgs& a, hipStream_t s)
+{
+    if constexpr (MODE == X_PLAIN) {
+        if (a.nlev > 1) {
+            switch (a.waves) {
+            case 8: launch_x_t<MODE, PT, 8>(a, s); return;
+            case 12: launch_x_t<MODE, PT, 12>(a, s); return;
+            case 14: launch_x_t<MODE, PT, 14>(a, s); return;
+            default: break;
+            }
+        }
+    }
+    launch_x_t<MODE, PT, XWAVES>(a, s);
 }
 
 template <int MODE>
@@ -233,14 +251,21 @@ static void launch_x_m(const XArgs& a, hipStream_t s)
 {
     const int P = a.pl.kind == 0 ? a.pl.P : 0;
     switch (P) {
-    case 512: launch_x_t<MODE, 512>(a, s); break;
-    case 400: launch_x_t<MODE, 400>(a, s); break;
-    case 256: launch_x_t<MODE, 256>(a, s); break;
-    case 200: launch_x_t<MODE, 200>(a, s); break;
-    case 128: launch_x_t<MODE, 128>(a, s); break;
-    case 100: launch_x_t<MODE, 100>(a, s); break;
-    default: launch_x_t<MODE, 0>(a, s); break;
+    case 512: launch_x_w<MODE, 512>(a, s); break;
+    case 400: launch_x_w<MODE, 400>(a, s); break;
+    case 256: launch_x_w<MODE, 256>(a, s); break;
+    case 200: launch_x_w<MODE, 200>(a, s); break;
+    case 128: launch_x_w<MODE, 128>(a, s); break;
+    case 100: launch_x_w<MODE, 100>(a, s); break;
+    default: launch_x_w<MODE, 0>(a, s); break;
     }
+}
+
+size_t xpass_lds_bytes(const DstPlan& pl, int waves)
+{
+    return (pl.kind == 0 && pl.P == 512)
+               ? sizeof(double) * ((size_t)tab_doubles_t<512>(pl) + (size_t)waves * slot_doubles_t<512>(pl))
+               : sizeof(double) * ((size_t)tab_doubles_t<0>(pl) + (size_t)waves * slot_doubles_t<0>(pl));
 }
 
 void launch_xpass(int mode, const XArgs& a, hipStream_t s)
@@ -312,8 +337,8 @@ __host__ __device__ inline int cy_doubles_t(int n2)
     else return (n2 + 1) & ~1;
 }
 
-template <int TI, int PT>
-__global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
+template <int TI, int PT, int YW>
+__global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
 {
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
@@ -329,32 +354,32 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
     const int ld = slot_doubles_t<PT>(a.pl);
     double* base = a.buf + b * a.lstride;
     // cos(pi i/nx) of this wave's columns, loaded with the tile (not after it)
-    constexpr int NCOL = (TI + YWAVES - 1) / YWAVES;
+    constexpr int NCOL = (TI + YW - 1) / YW;
     double cxw[NCOL];
 #pragma unroll
     for (int k = 0; k < NCOL; k++) {
-        const int c = w + YWAVES * k;
+        const int c = w + YW * k;
         cxw[k] = (c < TI && i0 + c < n1) ? a.cx[i0 + c] : 0.0;
     }
-    // LDS: [DST tables][cos(pi j/ny) table][TI column slots]
-    const int ntab = tab_doubles_t<PT>(a.pl);
-    const DstTab tb = load_tab_t<PT>(lds, a.pl, tid, YWAVES * WAVE);
-    double* cyl = lds + ntab;
-    if constexpr (PT == 512) {
-        for (int k = tid; k < n2; k += YWAVES * WAVE) cyl[f1024::gsl(k + 1)] = a.cy[k];
-        if (tid == 0) cyl[f1024::gsl(0)] = 1.0;
-    } else {
-        for (int k = tid; k < n2; k += YWAVES * WAVE) cyl[k] = a.cy[k];
-    }
-    double* tile = cyl + cy_doubles_t<PT>(n2);
-    if (tid < TI) set_guards<PT>(tile + tid * ld);
     // tile load: thread (r0, c) moves rows r0, r0 + RPI, ... of column c; CH
     // loads per thread are in flight before their LDS writes
-    constexpr int RPI = YWAVES * WAVE / TI;
+    constexpr int RPI = YW * WAVE / TI;
     constexpr int CH = 16;
     const int c0 = tid % TI, r0 = tid / TI;
     const int ic = i0 + c0;
     const bool colok = ic < n1;
+    // LDS: [DST tables][cos(pi j/ny) table][TI column slots]
+    const int ntab = tab_doubles_t<PT>(a.pl);
+    const DstTab tb = load_tab_t<PT>(lds, a.pl, tid, YW * WAVE);
+    double* cyl = lds + ntab;
+    if constexpr (PT == 512) {
+        for (int k = tid; k < n2; k += YW * WAVE) cyl[f1024::gsl(k + 1)] = a.cy[k];
+        if (tid == 0) cyl[f1024::gsl(0)] = 1.0;
+    } else {
+        for (int k = tid; k < n2; k += YW * WAVE) cyl[k] = a.cy[k];
+    }
+    double* tile = cyl + cy_doubles_t<PT>(n2);
+    if (tid < TI) set_guards<PT>(tile + tid * ld);
     for (int rb = 0; rb < n2; rb += RPI * CH) {
         double v[CH];
 #pragma unroll
@@ -376,7 +401,7 @@ __global__ void __launch_bounds__(YWAVES * WAVE) k_ypass(YArgs a)
     // scaling; one call site keeps a single DST body in the kernel
 #pragma unroll 1
     for (int it = 0; it < 2 * NCOL; it++) {
-        const int c = w + YWAVES * (it >> 1);
+        const int c = w + YW * (it >> 1);
         if (c >= TI || i0 + c >= n1) continue;
         double* s = tile + c * ld;
         double cxk = cxw[0];
@@ -423,35 +448,39 @@ size_t ypass_lds_bytes(const DstPlan& pl, int n2, int tile)
     return (pl.kind == 0 && pl.P == 512) ? ypass_lds_bytes_t<512>(pl, n2, tile) : ypass_lds_bytes_t<0>(pl, n2, tile);
 }
 
-template <int TI, int PT>
+template <int TI, int PT, int YW>
 static void launch_y_t(const YArgs& a, hipStream_t s)
 {
-    static bool once = (allow_big_lds(k_ypass<TI, PT>), true);
+    static bool once = (allow_big_lds(k_ypass<TI, PT, YW>), true);
     (void)once;
     const int nblocks = ((a.g.n1 + TI - 1) / TI) * a.nlev;
     dim3 grid(NXCD * xcd_nper(nblocks));
-    hipLaunchKernelGGL((k_ypass<TI, PT>), grid, dim3(YWAVES * WAVE), ypass_lds_bytes_t<PT>(a.pl, a.g.n2, TI), s, a);
+    hipLaunchKernelGGL((k_ypass<TI, PT, YW>), grid, dim3(YW * WAVE), ypass_lds_bytes_t<PT>(a.pl, a.g.n2, TI), s, a);
 }
 
-template <int TI>
+template <int TI, int YW>
 static void launch_y_m(const YArgs& a, hipStream_t s)
 {
     const int P = a.pl.kind == 0 ? a.pl.P : 0;
     switch (P) {
-    case 512: launch_y_t<TI, 512>(a, s); break;
-    case 400: launch_y_t<TI, 400>(a, s); break;
-    case 256: launch_y_t<TI, 256>(a, s); break;
-    case 200: launch_y_t<TI, 200>(a, s); break;
-    case 128: launch_y_t<TI, 128>(a, s); break;
-    case 100: launch_y_t<TI, 100>(a, s); break;
-    default: launch_y_t<TI, 0>(a, s); break;
+    case 512: launch_y_t<TI, 512, YW>(a, s); break;
+    case 400: launch_y_t<TI, 400, YW>(a, s); break;
+    case 256: launch_y_t<TI, 256, YW>(a, s); break;
+    case 200: launch_y_t<TI, 200, YW>(a, s); break;
+    case 128: launch_y_t<TI, 128, YW>(a, s); break;
+    case 100: launch_y_t<TI, 100, YW>(a, s); break;
+    default: launch_y_t<TI, 0, YW>(a, s); break;
     }
 }
 
+// tile 4: four waves, a column each; 8: four waves, two columns each; 12:
+// twelve waves, a column each (one copy of the tables for three times the
+// columns of tile 4)
 void launch_ypass(const YArgs& a, int tile, hipStream_t s)
 {
-    if (tile == 4) launch_y_m<4>(a, s);
-    else launch_y_m<8>(a, s);
+    if (tile == 4) launch_y_m<4, YWAVES>(a, s);
+    else if (tile == 12) launch_y_m<12, 12>(a, s);
+    else launch_y_m<8, YWAVES>(a, s);
 }
 
 // ===========================================================================

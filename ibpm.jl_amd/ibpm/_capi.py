@@ -27,7 +27,7 @@ MOTION_FUNCTION = 2
 FIELD_Q, FIELD_Q0, FIELD_GAMMA, FIELD_PSI, FIELD_NONLIN1, FIELD_NONLIN2 = 0, 1, 2, 3, 4, 5
 FIELD_FB_TILDE, FIELD_FB, FIELD_CD, FIELD_CL = 6, 7, 8, 9
 
-OPT_YTILE_BATCH, OPT_YTILE_SINGLE, OPT_PROFILE, OPT_GRAPHS = 0, 1, 2, 3
+OPT_YTILE_BATCH, OPT_YTILE_SINGLE, OPT_PROFILE, OPT_GRAPHS, OPT_XWAVES_BATCH = 0, 1, 2, 3, 4
 
 EXPORTED = [
     "ibpm_ctx_create", "ibpm_ctx_destroy", "ibpm_last_error", "ibpm_problem_create", "ibpm_problem_destroy",

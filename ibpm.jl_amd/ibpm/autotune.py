@@ -13,8 +13,9 @@ from . import _capi as C
 
 # option -> (kernel id timed by ibpm_time_kernel, candidate values)
 SWEEPS = {
-    C.OPT_YTILE_BATCH: (1, (4, 8)),
-    C.OPT_YTILE_SINGLE: (5, (4, 8)),
+    C.OPT_YTILE_BATCH: (1, (4, 8, 12)),
+    C.OPT_YTILE_SINGLE: (5, (4, 8, 12)),
+    C.OPT_XWAVES_BATCH: (0, (4, 8, 12, 14)),
 }
 
 

@@ -258,6 +258,42 @@ def test_run_sim_saves_and_cfl():
     assert st.cfl > 0  # computed after step 25 (display_freq)
 
 
+@pytest.mark.parametrize("name", ["small3", "spec", "config1"])
+def test_launch_shapes_give_identical_steps(name):
+    """Every legal work-group shape of the x and y passes (the autotuner's
+    options) transforms each line with the same arithmetic: the steps are
+    bit-identical to the default shape's."""
+    from ibpm import _capi as C
+    c = case(name)
+    g = c["grid"]
+    ts = np.arange(3) * c["dt"]
+
+    def run(opts):
+        gp = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"], Binv=ref_binv)
+        for o, v in opts.items():
+            if C.lib().ibpm_set_option(gp.h, o, v) != C.IBPM_OK:
+                return None  # does not fit this grid
+        st = ibpm.IBState(gp)
+        st.Γ = gaussian_vortex_gamma(g, x0=0.8, y0=0.2, circ=0.5, width=0.3)
+        ibpm.vort2flux_(st, gp)
+        cd, cl, _ = ibpm.advance_n_(st, gp, ts)
+        return cd, st.Γ.copy(), st.q.copy()
+
+    ref_binv = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"]).Binv
+    base = run({})
+    n = 0
+    for yb in (4, 8, 12):
+        for ys in (4, 8, 12):
+            for xw in (4, 8, 12, 14):
+                r = run({C.OPT_YTILE_BATCH: yb, C.OPT_YTILE_SINGLE: ys, C.OPT_XWAVES_BATCH: xw})
+                if r is None:
+                    continue
+                n += 1
+                for a, b in zip(r, base):
+                    assert np.array_equal(a, b), (yb, ys, xw)
+    assert n >= 12
+
+
 def test_graph_replays_match_direct_launches():
     """Captured steps (default) against a twin problem that launches every
     kernel: several rounds of replays with eager work and syncs in between,
