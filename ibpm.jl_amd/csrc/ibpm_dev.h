@@ -698,10 +698,14 @@ __host__ __device__ constexpr int gsl(int e) { return e + 2 * (e >> 4); }
 // 2 (pz(P - 1) + 1) = 1150 for the complex work array, rounded to even, + 2
 // to stagger the LDS banks of neighbouring slots
 constexpr int SLOT = 1156;
-// doubles of the LDS tables: twiddles W_P^e (P complex), W_M^k at k + k/8
-// (P + P/8 complex), sin(pi j/M) for j <= M/2 (rounded to even)
+// doubles of the LDS tables: the stage twiddles (P complex: W_P^(8kr) of
+// stage 2 at 8r + k, W_P^(jr) of stage 3 at 64r + j, so that the 64 lanes of
+// a read hit consecutive entries), W_M^k at k + k/8 (P + P/8 complex),
+// sin(pi j/M) for j <= M/2 (rounded to even), then the mirrored pairs
+// {sin(pi (M-2m)/M), sin(pi (M-2m-1)/M)} of stage 1 for m = P/2 .. P-1 (P doubles)
 constexpr int TWP = P + P / 8;
-constexpr int TAB = 2 * P + 2 * TWP + (M / 2 + 2);
+constexpr int SNM = M / 2 + 2;
+constexpr int TAB = 2 * P + 2 * TWP + SNM + P;
 }  // namespace f1024
 
 __device__ inline DstTab load_dst_tab_1024(double* lds, const DstPlan& pl, int tid, int nthreads)
@@ -710,11 +714,18 @@ __device__ inline DstTab load_dst_tab_1024(double* lds, const DstPlan& pl, int t
     double2* tw = reinterpret_cast<double2*>(lds);
     double2* wm = tw + P;
     double* sn = reinterpret_cast<double*>(wm + TWP);
+    double* snm = sn + SNM;
     for (int i = tid; i < P; i += nthreads) {
-        tw[i] = pl.tw[i];
+        // i < 64: stage 2, r = i / 8, k = i % 8 -> W_P^(8 k r); else stage 3, r = i / 64, j = i % 64 -> W_P^(j r)
+        const int e = i < 64 ? 8 * (i & 7) * (i >> 3) : (i & 63) * (i >> 6);
+        tw[i] = pl.tw[e];
         wm[i + (i >> 3)] = pl.wm[i];
     }
     for (int i = tid; i <= M / 2; i += nthreads) sn[i] = pl.sn[i];
+    for (int i = tid; i < P; i += nthreads) {  // m = P/2 + i/2: sin(pi (M - 2m - (i & 1))/M)
+        const int m = P / 2 + (i >> 1);
+        snm[i] = pl.sn[M - 2 * m - (i & 1)];
+    }
     DstTab t;
     t.tw = tw;
     t.wm = wm;
@@ -750,9 +761,10 @@ __device__ __forceinline__ void dst1024_line(double* S, const DstTab& tb, int la
             const double2 t = *reinterpret_cast<const double2*>(tb.sn + 2 * m);
             s0 = t.x;
             s1 = t.y;
-        } else {  // mirrored: sin(pi e/M) = sin(pi (M-e)/M)
-            s0 = tb.sn[M - 2 * m];
-            s1 = tb.sn[M - 2 * m - 1];
+        } else {  // mirrored, sin(pi e/M) = sin(pi (M-e)/M): the pair stored for m
+            const double2 t = *reinterpret_cast<const double2*>(tb.sn + SNM + 2 * (m - P / 2));
+            s0 = t.x;
+            s1 = t.y;
         }
         v[r] = make_double2(fma(s0, a.x + b0, 0.5 * (a.x - b0)), fma(s1, a.y + b1, 0.5 * (a.y - b1)));
     }
@@ -768,7 +780,7 @@ __device__ __forceinline__ void dst1024_line(double* S, const DstTab& tb, int la
         for (int r = 0; r < 8; r++) v[r] = z[p0 + 72 * r];
         const int k = j & 7;
 #pragma unroll
-        for (int r = 1; r < 8; r++) v[r] = cmul(v[r], tb.tw[8 * k * r]);
+        for (int r = 1; r < 8; r++) v[r] = cmul(v[r], tb.tw[8 * r + k]);  // W_P^(8 k r)
         wave_sync();
         dft8(v);
         const int q0 = ((j >> 3) << 6) + k + ((j >> 3) << 3);  // pz(64 (j/8) + k); + 9 r per r
@@ -781,7 +793,7 @@ __device__ __forceinline__ void dst1024_line(double* S, const DstTab& tb, int la
 #pragma unroll
         for (int r = 0; r < 8; r++) v[r] = z[p0 + 72 * r];
 #pragma unroll
-        for (int r = 1; r < 8; r++) v[r] = cmul(v[r], tb.tw[j * r]);
+        for (int r = 1; r < 8; r++) v[r] = cmul(v[r], tb.tw[64 * r + j]);  // W_P^(j r)
         wave_sync();
         dft8(v);
 #pragma unroll
