@@ -20,6 +20,8 @@ struct ibpm_ctx {
     std::string err;
 };
 
+struct ibpm_group;
+
 namespace {
 
 struct DevTables {
@@ -105,6 +107,7 @@ struct ibpm_prob {
     int pending = 0;          // steps enqueued by ibpm_advance_async, not yet waited for
     int* slot = nullptr;      // device: history entry of the next step (advanced by k_forces)
     bool async_err = false;   // a graph launch failed (reported by the next wait)
+    ibpm_group* group = nullptr;  // set while the problem's state lives in a group's arrays
     // HIP graphs of one step (static bodies): replayed instead of ~54 launches.
     // A step's launches depend on the buffer roles only (Γ/Γs and the nonlin
     // pair alternate), so a graph is kept per role; anything else a captured
@@ -707,7 +710,7 @@ static int assemble_b(ibpm_prob* P)
         }
         CurlArgs ca{};
         ca.psi = P->bpsi; ca.q = P->bq; ca.qstride = nq; ca.nlev = kc; ca.lev0 = 0; ca.mg = L;
-        ca.colbatch = 1; ca.psl = (long)K * nG; ca.g = g;
+        ca.colbatch = 1; ca.psl = (long)K * nG; ca.psb = nG; ca.g = g;
         launch_curl(ca, s);
         GatherArgs ga{};
         ga.q = P->bq; ga.q0 = nullptr; ga.w = P->w; ga.bidx = P->bidx; ga.out = P->Bd + (long)f0 * nf;
@@ -1111,6 +1114,7 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
 int ibpm_problem_destroy(ibpm_prob* P)
 {
     if (!P) return IBPM_OK;
+    if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: destroy the group first");
     if (P->hist && P->stream) (void)hipFreeAsync(P->hist, P->stream);
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     graphs_clear(P);
@@ -1125,6 +1129,7 @@ int ibpm_problem_destroy(ibpm_prob* P)
 int ibpm_problem_linearize(ibpm_prob* P, const double* qB, const double* q0B, const double* GammaB)
 {
     if (!P || !qB || !q0B || !GammaB) return IBPM_E_ARG;
+    if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: step or change it through the group");
     if (P->motion != IBPM_MOTION_STATIC)
         return fail(P->ctx, IBPM_E_UNSUPPORTED, "LinearizedIBProblem is defined for static bodies only");
     const GridDev& g = P->g;
@@ -1214,6 +1219,7 @@ static double* field_ptr(ibpm_prob* P, int field, size_t* n, bool for_upload)
 int ibpm_state_upload(ibpm_prob* P, int field, const double* host, size_t n)
 {
     if (!P || !host) return IBPM_E_ARG;
+    if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: step or change it through the group");
     size_t want = 0;
     double* d = field_ptr(P, field, &want, true);
     if (!d) return fail(P->ctx, IBPM_E_ARG, "unknown field");
@@ -1239,6 +1245,7 @@ int ibpm_state_download(ibpm_prob* P, int field, double* host, size_t n)
 int ibpm_base_flux(ibpm_prob* P, const ibpm_step_scalars* s)
 {
     if (!P || !s) return IBPM_E_ARG;
+    if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: step or change it through the group");
     graphs_clear(P);  // captured nonlinear launches hold the base flux by value
     if (P->lin) {
         // static bodies of a linearized problem: q0 = 0 (timestepping.jl:238-250)
@@ -1275,6 +1282,7 @@ int ibpm_base_flux(ibpm_prob* P, const ibpm_step_scalars* s)
 int ibpm_state_init(ibpm_prob* P, const ibpm_step_scalars* s)
 {
     if (!P || !s) return IBPM_E_ARG;
+    if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: step or change it through the group");
     zero_state(P);
     return ibpm_base_flux(P, s);
 }
@@ -1282,6 +1290,7 @@ int ibpm_state_init(ibpm_prob* P, const ibpm_step_scalars* s)
 int ibpm_vort2flux(ibpm_prob* P)
 {
     if (!P) return IBPM_E_ARG;
+    if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: step or change it through the group");
     vort2flux_dev(P, P->G, P->psi, P->q, false);
     HIPTRY(P->ctx, hipGetLastError());
     return IBPM_OK;
@@ -1298,6 +1307,7 @@ int ibpm_advance(ibpm_prob* P, double t, const ibpm_step_scalars* s)
 int ibpm_advance_async(ibpm_prob* P, int nsteps, const ibpm_step_scalars* s)
 {
     if (!P || !s || nsteps < 0) return IBPM_E_ARG;
+    if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: step or change it through the group");
     const int nb = std::max(P->nbodies, 1);
     const size_t want = (size_t)P->pending + (size_t)nsteps;
     if (want > P->hist_steps) {
@@ -1410,6 +1420,484 @@ int ibpm_op_update_bodies(ibpm_prob* P, const double* kin)
     if (rc != IBPM_OK) return rc;
     HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
     return IBPM_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Groups: same-shape static problems stepped together, every pass of the step
+// one launch over all instances (fields laid out instance-major: level l of
+// instance i at (i L + l) nG, resp. nq)
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct ibpm_group {
+    ibpm_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<ibpm_prob*> probs;
+    int n = 0, L = 1;
+    GridDev g{};
+    DevTables tx, ty;  // the first member's plans (identical for all members)
+    double *cx = nullptr, *cy = nullptr;
+    std::vector<CoarsifyRange> crange;
+    double *G = nullptr, *Gs = nullptr, *psi = nullptr, *nl[2] = {nullptr, nullptr}, *q = nullptr, *q0 = nullptr;
+    double *qs1 = nullptr, *T1 = nullptr, *T2 = nullptr, *Gt = nullptr, *qreg = nullptr;
+    int nl1 = 0, nl2 = 1;
+    bool q0_uniform = true;
+    // bodies: all instances' points and F̃b entries concatenated
+    int NBT = 0, NF = 0;
+    double *Fb = nullptr, *Ftmp = nullptr, *BinvT = nullptr, *w = nullptr;
+    int *bidx = nullptr, *pinst = nullptr, *plocal = nullptr, *inbt = nullptr, *ifoff = nullptr, *inf = nullptr;
+    int *rinst = nullptr, *rloc = nullptr, *slots = nullptr;
+    long* aoff = nullptr;
+    InstPar* ipar = nullptr;
+    ForceArgs* fargs = nullptr;
+    std::vector<ForceArgs> fargs_h;
+    int *sc_cell = nullptr, *sc_off = nullptr, *sc_f = nullptr;
+    double* sc_w = nullptr;
+    int sc_ncell = 0;
+    int rb_i0 = 2, rb_i1 = 1, rb_j0 = 2, rb_j1 = 1;
+    std::vector<int> nbh, hoff;  // history: instance i's CD block at hist + hoff[i], CL block after it
+    double* hist = nullptr;
+    size_t hist_steps = 0;
+    int pending = 0;
+    int ytile = 4, ytile_single = 4, xwaves = 4;
+    std::vector<void*> allocs;
+    // the members' own buffers, restored when the group is destroyed
+    struct Own {
+        double *G, *Gs, *psi, *nl0, *nl1b, *q, *q0, *Fb;
+        int nl1, nl2;
+    };
+    std::vector<Own> own;
+};
+
+namespace {
+
+template <typename T>
+hipError_t galloc(ibpm_group* Gp, T** p, size_t n)
+{
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, sizeof(T) * (n > 0 ? n : 1));
+    if (e == hipSuccess) {
+        Gp->allocs.push_back(v);
+        *p = static_cast<T*>(v);
+    }
+    return e;
+}
+
+XArgs gxargs(const ibpm_group* Gp)
+{
+    XArgs a{};
+    a.lstride = Gp->g.nG;
+    a.mg = Gp->L;
+    a.pl = Gp->tx.plan;
+    a.g = Gp->g;
+    a.fac = 1.0;
+    a.zlo = 1;
+    a.zhi = Gp->g.n2;
+    a.waves = Gp->xwaves;
+    return a;
+}
+
+YArgs gyargs(const ibpm_group* Gp, int op, double* buf)
+{
+    YArgs a{};
+    a.buf = buf;
+    a.op = op;
+    a.cx = Gp->cx;
+    a.cy = Gp->cy;
+    a.scale = 1.0 / (double)(4L * Gp->g.nx * Gp->g.ny);
+    a.ipar = Gp->ipar;
+    a.pl = Gp->ty.plan;
+    a.g = Gp->g;
+    return a;
+}
+
+// vort2flux! (curl.jl:133-165) of every instance: the level chain runs over
+// all instances at once (one launch per pass and level)
+void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, bool level1_only)
+{
+    hipStream_t s = Gp->stream;
+    const long nG = Gp->g.nG, ist = (long)Gp->L * nG;
+    const int L = Gp->L, n = Gp->n;
+    for (int lev = 2; lev <= L; lev++) {
+        CoarsifyArgs c{};
+        const CoarsifyRange& r = Gp->crange[0];
+        c.Gc = Gb + (long)(lev - 1) * nG;
+        c.G = Gb + (long)(lev - 2) * nG;
+        c.i0 = r.i0; c.ic0 = r.ic0; c.ci = r.ci; c.j0 = r.j0; c.jc0 = r.jc0; c.cj = r.cj;
+        c.nb = n; c.bstride = ist; c.g = Gp->g;
+        launch_coarsify(c, s);
+    }
+    for (int l = L - 1; l >= 0; l--) {
+        const bool bc = l < L - 1;
+        XArgs a = gxargs(Gp);
+        a.src = Gb + (long)l * nG; a.dst = Gp->T1 + (long)l * nG; a.lstride = ist; a.nlev = n; a.lev0 = l;
+        if (bc) {
+            a.bcr = psib + (long)(l + 1) * nG;
+            a.bcstride = ist;
+        }
+        launch_xpass(bc ? X_BC : X_PLAIN, a, s);
+        YArgs y = gyargs(Gp, Y_DINV, Gp->T1 + (long)l * nG);
+        y.lstride = ist; y.nlev = n; y.lev0 = l; y.fixlev = 1;
+        launch_ypass(y, Gp->ytile_single, s);
+        XArgs c = gxargs(Gp);
+        c.src = Gp->T1 + (long)l * nG; c.dst = psib + (long)l * nG; c.lstride = ist; c.nlev = n; c.lev0 = l;
+        launch_xpass(X_PLAIN, c, s);
+    }
+    CurlArgs ca{};
+    ca.psi = psib; ca.q = qout; ca.qstride = Gp->g.nq; ca.mg = L; ca.g = Gp->g; ca.lev0 = 0;
+    if (level1_only) {
+        ca.nlev = n; ca.colbatch = 1; ca.psl = nG; ca.psb = ist;
+    } else {
+        ca.nlev = n * L; ca.levmod = L;
+    }
+    launch_curl(ca, s);
+}
+
+// advance!(state, prob, t) of every instance (timestepping.jl:35-67), static bodies
+void group_step(ibpm_group* Gp)
+{
+    hipStream_t s = Gp->stream;
+    const GridDev& g = Gp->g;
+    const long nG = g.nG, ist = (long)Gp->L * nG;
+    const int L = Gp->L, n = Gp->n, NE = n * L;
+    // ---- get_trial_state!: all levels of all instances per launch
+    const int out = (Gp->nl1 != Gp->nl2) ? Gp->nl1 : 1 - Gp->nl2;
+    NLArgs na{};
+    na.q = Gp->q; na.q0 = Gp->q0; na.G = Gp->G; na.out = Gp->nl[out];
+    na.nlev = NE; na.lev0 = 0; na.mg = L; na.g = g; na.levmod = L; na.ipar = Gp->ipar;
+    na.q0_uniform = Gp->q0_uniform ? 1 : 0;
+    launch_nonlinear(na, s);
+    XArgs a = gxargs(Gp);
+    a.src = Gp->G; a.dst = Gp->T1; a.nlev = NE; a.lev0 = 0;
+    launch_xpass(X_PLAIN, a, s);
+    YArgs ya = gyargs(Gp, Y_A, Gp->T1);
+    ya.lstride = nG; ya.nlev = NE; ya.lev0 = 0; ya.levmod = L;
+    launch_ypass(ya, Gp->ytile, s);
+    XArgs t = gxargs(Gp);
+    t.src = Gp->T1; t.dst = Gp->T2; t.nlev = NE; t.lev0 = 0;
+    t.n1 = Gp->nl[out]; t.n0 = Gp->nl[Gp->nl2]; t.gam = Gp->G; t.levmod = L; t.ipar = Gp->ipar;
+    launch_xpass(X_TRIAL, t, s);
+    YArgs yi = gyargs(Gp, Y_AINV, Gp->T2);
+    yi.lstride = nG; yi.nlev = NE; yi.lev0 = 0; yi.levmod = L;
+    launch_ypass(yi, Gp->ytile, s);
+    XArgs c = gxargs(Gp);
+    c.src = Gp->T2; c.dst = Gp->Gs; c.nlev = NE; c.lev0 = 0;
+    launch_xpass(X_PLAIN, c, s);
+    Gp->nl1 = Gp->nl2 = out;
+    group_vort2flux(Gp, Gp->Gs, Gp->psi, Gp->qs1, true);
+    // ---- boundary_forces! and update_stress! of every instance
+    GatherArgs ga{};
+    ga.q = Gp->qs1; ga.q0 = Gp->q0; ga.w = Gp->w; ga.bidx = Gp->bidx; ga.out = Gp->Ftmp; ga.nbt = Gp->NBT; ga.g = g;
+    ga.qstride = g.nq; ga.q0stride = (long)L * g.nq;
+    ga.pinst = Gp->pinst; ga.plocal = Gp->plocal; ga.inbt = Gp->inbt; ga.ifoff = Gp->ifoff;
+    launch_gather(ga, s);
+    launch_gemv_forces_group(Gp->BinvT, Gp->aoff, Gp->Ftmp, Gp->Fb, Gp->rinst, Gp->rloc, Gp->ifoff, Gp->inf, Gp->NF,
+                             Gp->fargs, n, s);
+    // ---- project_circ!: Γs[:, 1] -= Ainv rot(E' F̃b) of every instance
+    ScatterArgs sa{};
+    sa.cell = Gp->sc_cell; sa.off = Gp->sc_off; sa.ent_f = Gp->sc_f; sa.ent_w = Gp->sc_w;
+    sa.fb = Gp->Fb; sa.q = Gp->qreg; sa.ncell = Gp->sc_ncell;
+    launch_scatter(sa, s);
+    launch_rot_batch(Gp->qreg, g.nq, Gp->Gt, ist, n, Gp->rb_i0, Gp->rb_i1, Gp->rb_j0, Gp->rb_j1, g, s);
+    XArgs p1 = gxargs(Gp);
+    p1.src = Gp->Gt; p1.dst = Gp->T1; p1.lstride = ist; p1.nlev = n; p1.lev0 = 0;
+    p1.zlo = Gp->rb_j0 - 1; p1.zhi = std::min(Gp->rb_j1 - 1, g.n2);
+    launch_xpass(X_PLAIN, p1, s);
+    YArgs yp = gyargs(Gp, Y_AINV, Gp->T1);
+    yp.lstride = ist; yp.nlev = n; yp.lev0 = 0; yp.fixlev = 1;
+    launch_ypass(yp, Gp->ytile_single, s);
+    XArgs p2 = gxargs(Gp);
+    p2.src = Gp->T1; p2.dst = Gp->Gs; p2.lstride = ist; p2.nlev = n; p2.lev0 = 0;
+    launch_xpass(X_SUB, p2, s);
+    std::swap(Gp->G, Gp->Gs);
+    group_vort2flux(Gp, Gp->G, Gp->psi, Gp->q, false);
+    // the members follow the group's buffer roles
+    for (int i = 0; i < n; i++) {
+        ibpm_prob* P = Gp->probs[i];
+        P->G = Gp->G + (long)i * ist;
+        P->Gs = Gp->Gs + (long)i * ist;
+        P->nl1 = Gp->nl1;
+        P->nl2 = Gp->nl2;
+    }
+}
+
+// history blocks of every instance for `steps` steps, and their pointers in
+// the per-instance force arguments
+int group_hist(ibpm_group* Gp, size_t steps)
+{
+    if (steps <= Gp->hist_steps) return IBPM_OK;
+    const size_t cap = std::max({steps, 2 * Gp->hist_steps, (size_t)64});
+    if (Gp->hist) (void)hipFreeAsync(Gp->hist, Gp->stream);
+    Gp->hist = nullptr;
+    Gp->hist_steps = 0;
+    size_t tot = 0;
+    for (int i = 0; i < Gp->n; i++) {
+        Gp->hoff[i] = (int)tot;
+        tot += 2 * cap * (size_t)Gp->nbh[i];
+    }
+    HIPTRY(Gp->ctx, hipMallocAsync((void**)&Gp->hist, sizeof(double) * tot, Gp->stream));
+    Gp->hist_steps = cap;
+    for (int i = 0; i < Gp->n; i++) {
+        Gp->fargs_h[i].cd_hist = Gp->hist + Gp->hoff[i];
+        Gp->fargs_h[i].cl_hist = Gp->hist + Gp->hoff[i] + cap * (size_t)Gp->nbh[i];
+    }
+    HIPTRY(Gp->ctx, hipMemcpyAsync(Gp->fargs, Gp->fargs_h.data(), sizeof(ForceArgs) * Gp->n, hipMemcpyHostToDevice,
+                                   Gp->stream));
+    // the staging copy must complete before fargs_h changes again
+    HIPTRY(Gp->ctx, hipStreamSynchronize(Gp->stream));
+    return IBPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* probs, ibpm_group** out)
+{
+    if (!ctx || !probs || !out || n < 1) return IBPM_E_ARG;
+    *out = nullptr;
+    const ibpm_prob* P0 = probs[0];
+    for (int i = 0; i < n; i++) {
+        const ibpm_prob* P = probs[i];
+        if (!P || P->ctx != ctx) return fail(ctx, IBPM_E_ARG, "group members must be problems of this context");
+        if (P->group) return fail(ctx, IBPM_E_ARG, "a problem can belong to one group only");
+        if (P->motion != IBPM_MOTION_STATIC || P->lin || P->nbodies < 1)
+            return fail(ctx, IBPM_E_UNSUPPORTED, "groups take problems with static bodies (not linearized)");
+        if (P->g.nx != P0->g.nx || P->g.ny != P0->g.ny || P->mg != P0->mg || P->g.h != P0->g.h ||
+            P->offx != P0->offx || P->offy != P0->offy)
+            return fail(ctx, IBPM_E_ARG, "group members must share the grid (nx, ny, mg, h, offset)");
+        if (P->pending > 0) return fail(ctx, IBPM_E_ARG, "wait for a member's queued steps before grouping it");
+        for (int k = 0; k < i; k++)
+            if (probs[k] == P) return fail(ctx, IBPM_E_ARG, "a problem appears twice in the group");
+    }
+    ibpm_group* Gp = new ibpm_group();
+    Gp->ctx = ctx;
+    Gp->n = n;
+    Gp->L = P0->mg;
+    Gp->g = P0->g;
+    Gp->tx = P0->tx; Gp->ty = P0->ty; Gp->cx = P0->cx; Gp->cy = P0->cy; Gp->crange = P0->crange;
+    Gp->ytile = P0->ytile; Gp->ytile_single = P0->ytile_single; Gp->xwaves = P0->xwaves;
+    Gp->probs.assign(probs, probs + n);
+    auto bail = [&](int code, const std::string& m) {
+        for (void* p : Gp->allocs) (void)hipFree(p);
+        if (Gp->stream) (void)hipStreamDestroy(Gp->stream);
+        delete Gp;
+        return fail(ctx, code, m);
+    };
+    for (int i = 0; i < n; i++)
+        if (hipStreamSynchronize(probs[i]->stream) != hipSuccess) return bail(IBPM_E_HIP, "draining a member failed");
+    if (hipStreamCreate(&Gp->stream) != hipSuccess) return bail(IBPM_E_HIP, "hipStreamCreate failed");
+    const long nG = Gp->g.nG, nq = Gp->g.nq, L = Gp->L;
+    const size_t fG = (size_t)n * L * nG, fq = (size_t)n * L * nq;
+    // instance maps of the bodies
+    std::vector<int> pinst, plocal, inbt(n), ifoff(n), inf(n), rinst, rloc;
+    std::vector<long> aoff(n);
+    long atot = 0;
+    for (int i = 0; i < n; i++) {
+        const ibpm_prob* P = probs[i];
+        inbt[i] = P->nbt;
+        ifoff[i] = Gp->NF;
+        inf[i] = P->nf;
+        aoff[i] = atot;
+        atot += (long)P->nf * P->nf;
+        for (int k = 0; k < P->nbt; k++) { pinst.push_back(i); plocal.push_back(k); }
+        for (int r = 0; r < P->nf; r++) { rinst.push_back(i); rloc.push_back(r); }
+        Gp->NBT += P->nbt;
+        Gp->NF += P->nf;
+        Gp->nbh.push_back(std::max(P->nbodies, 1));
+    }
+    Gp->hoff.assign(n, 0);
+    // E' of all instances: one segmented-gather list over the concatenated
+    // fields (instances in order, each in its own cell order)
+    std::vector<int> cell, off, ef;
+    std::vector<double> ew;
+    int ib0 = Gp->g.nx + 1, ib1 = 0, jb0 = Gp->g.ny + 2, jb1 = 0;
+    for (int i = 0; i < n; i++) {
+        const ibpm_prob* P = probs[i];
+        const RegLists& R = P->reg;
+        for (size_t c = 0; c < R.cell.size(); c++) {
+            cell.push_back((int)(R.cell[c] + (long)i * nq));
+            off.push_back((int)ef.size() + (R.off[c] - R.off[0]));
+        }
+        for (size_t e = 0; e < R.ef.size(); e++) {
+            ef.push_back(R.ef[e] + ifoff[i]);
+            ew.push_back(R.ew[e]);
+        }
+        ib0 = std::min(ib0, P->rb_i0); ib1 = std::max(ib1, P->rb_i1);
+        jb0 = std::min(jb0, P->rb_j0); jb1 = std::max(jb1, P->rb_j1);
+    }
+    off.push_back((int)ef.size());
+    Gp->sc_ncell = (int)cell.size();
+    Gp->rb_i0 = ib0; Gp->rb_i1 = ib1; Gp->rb_j0 = jb0; Gp->rb_j1 = jb1;
+    if ((size_t)n * nq > (size_t)INT32_MAX) return bail(IBPM_E_ARG, "group too large for 32-bit cell indices");
+    hipError_t e;
+#define GA(ptr, cnt)                                                             \
+    if ((e = galloc(Gp, &(ptr), (size_t)(cnt))) != hipSuccess)                   \
+        return bail(IBPM_E_HIP, std::string("hipMalloc failed for group " #ptr));
+    GA(Gp->G, fG); GA(Gp->Gs, fG); GA(Gp->psi, fG); GA(Gp->nl[0], fG); GA(Gp->nl[1], fG);
+    GA(Gp->q, fq); GA(Gp->q0, fq); GA(Gp->qs1, (size_t)n * nq); GA(Gp->T1, fG); GA(Gp->T2, fG); GA(Gp->Gt, fG);
+    GA(Gp->qreg, (size_t)n * nq);
+    GA(Gp->Fb, Gp->NF); GA(Gp->Ftmp, Gp->NF); GA(Gp->BinvT, atot); GA(Gp->w, (size_t)2 * Gp->NBT * 169);
+    GA(Gp->bidx, 2 * (size_t)Gp->NBT); GA(Gp->pinst, Gp->NBT); GA(Gp->plocal, Gp->NBT); GA(Gp->inbt, n);
+    GA(Gp->ifoff, n); GA(Gp->inf, n); GA(Gp->rinst, Gp->NF); GA(Gp->rloc, Gp->NF); GA(Gp->slots, n);
+    GA(Gp->aoff, n); GA(Gp->ipar, n); GA(Gp->fargs, n);
+    GA(Gp->sc_cell, cell.size()); GA(Gp->sc_off, off.size()); GA(Gp->sc_f, ef.size()); GA(Gp->sc_w, ew.size());
+#undef GA
+    hipStream_t s = Gp->stream;
+    auto h2d = [&](void* d, const void* h, size_t bytes) {
+        return bytes == 0 || hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+    };
+    auto d2d = [&](void* d, const void* src, size_t bytes) {
+        return bytes == 0 || hipMemcpyAsync(d, src, bytes, hipMemcpyDeviceToDevice, s) == hipSuccess;
+    };
+    std::vector<InstPar> ip(n);
+    Gp->fargs_h.resize(n);
+    bool ok = true;
+    for (int i = 0; i < n && ok; i++) {
+        ibpm_prob* P = probs[i];
+        const long io = (long)i * L;
+        ip[i] = InstPar{P->beta[0] * P->dt, P->beta[1] * P->dt, P->dt, P->Re, P->q0vx, P->q0vy};
+        Gp->q0_uniform = Gp->q0_uniform && P->q0_uniform;
+        // the member's state into its slices; nonlin[1] and nonlin[2] as roles 0 and 1
+        ok = d2d(Gp->G + io * nG, P->G, sizeof(double) * L * nG) && d2d(Gp->psi + io * nG, P->psi, sizeof(double) * L * nG) &&
+             d2d(Gp->nl[0] + io * nG, P->nl[P->nl1], sizeof(double) * L * nG) &&
+             d2d(Gp->nl[1] + io * nG, P->nl[P->nl2], sizeof(double) * L * nG) &&
+             d2d(Gp->q + io * nq, P->q, sizeof(double) * L * nq) && d2d(Gp->q0 + io * nq, P->q0, sizeof(double) * L * nq) &&
+             d2d(Gp->Fb + ifoff[i], P->Fb, sizeof(double) * P->nf) &&
+             d2d(Gp->BinvT + aoff[i], P->BinvT, sizeof(double) * P->nf * P->nf);
+        // weights and indices of instance i's points at its global point offset
+        const long po = ifoff[i] / 2;  // points before instance i
+        ok = ok && d2d(Gp->w + po * 2 * 169, P->w, sizeof(double) * 2 * 169 * P->nbt) &&
+             d2d(Gp->bidx + po, P->bidx, sizeof(int) * P->nbt) &&
+             d2d(Gp->bidx + Gp->NBT + po, P->bidx + P->nbt, sizeof(int) * P->nbt);
+        ForceArgs& fa = Gp->fargs_h[i];
+        fa = ForceArgs{};
+        fa.Fb = Gp->Fb + ifoff[i]; fa.ds = P->ds_d; fa.boff = P->boff_d; fa.fb = P->fbv; fa.CD = P->CD; fa.CL = P->CL;
+        fa.slot = Gp->slots + i; fa.nbh = Gp->nbh[i]; fa.nbodies = P->nbodies; fa.nbt = P->nbt; fa.hdt = Gp->g.h / P->dt;
+    }
+    ok = ok && h2d(Gp->pinst, pinst.data(), sizeof(int) * pinst.size()) &&
+         h2d(Gp->plocal, plocal.data(), sizeof(int) * plocal.size()) && h2d(Gp->inbt, inbt.data(), sizeof(int) * n) &&
+         h2d(Gp->ifoff, ifoff.data(), sizeof(int) * n) && h2d(Gp->inf, inf.data(), sizeof(int) * n) &&
+         h2d(Gp->rinst, rinst.data(), sizeof(int) * rinst.size()) && h2d(Gp->rloc, rloc.data(), sizeof(int) * rloc.size()) &&
+         h2d(Gp->aoff, aoff.data(), sizeof(long) * n) && h2d(Gp->ipar, ip.data(), sizeof(InstPar) * n) &&
+         h2d(Gp->fargs, Gp->fargs_h.data(), sizeof(ForceArgs) * n) &&
+         h2d(Gp->sc_cell, cell.data(), sizeof(int) * cell.size()) && h2d(Gp->sc_off, off.data(), sizeof(int) * off.size()) &&
+         h2d(Gp->sc_f, ef.data(), sizeof(int) * ef.size()) && h2d(Gp->sc_w, ew.data(), sizeof(double) * ew.size());
+    if (!ok || hipStreamSynchronize(s) != hipSuccess) return bail(IBPM_E_HIP, "building the group failed");
+    // scratch that the scatter and rot! leave untouched outside the E' support stays zero
+    launch_fill(Gp->qreg, (long)n * nq, 0.0, s);
+    launch_fill(Gp->Gt, (long)fG, 0.0, s);
+    if (hipStreamSynchronize(s) != hipSuccess) return bail(IBPM_E_HIP, "building the group failed");
+    // the members now live in the group's arrays
+    for (int i = 0; i < n; i++) {
+        ibpm_prob* P = probs[i];
+        const long io = (long)i * L;
+        Gp->own.push_back({P->G, P->Gs, P->psi, P->nl[0], P->nl[1], P->q, P->q0, P->Fb, P->nl1, P->nl2});
+        graphs_clear(P);
+        P->G = Gp->G + io * nG; P->Gs = Gp->Gs + io * nG; P->psi = Gp->psi + io * nG;
+        P->nl[0] = Gp->nl[0] + io * nG; P->nl[1] = Gp->nl[1] + io * nG; P->nl1 = 0; P->nl2 = 1;
+        P->q = Gp->q + io * nq; P->q0 = Gp->q0 + io * nq; P->Fb = Gp->Fb + ifoff[i];
+        P->group = Gp;
+    }
+    *out = Gp;
+    return IBPM_OK;
+}
+
+int ibpm_group_destroy(ibpm_group* Gp)
+{
+    if (!Gp) return IBPM_OK;
+    (void)hipStreamSynchronize(Gp->stream);
+    const long nG = Gp->g.nG, nq = Gp->g.nq, L = Gp->L;
+    int rc = IBPM_OK;
+    for (int i = 0; i < Gp->n; i++) {
+        ibpm_prob* P = Gp->probs[i];
+        const ibpm_group::Own& o = Gp->own[i];
+        // current state back into the member's own buffers, same roles
+        const bool ok =
+            hipMemcpy(o.G, P->G, sizeof(double) * L * nG, hipMemcpyDeviceToDevice) == hipSuccess &&
+            hipMemcpy(o.Gs, P->Gs, sizeof(double) * L * nG, hipMemcpyDeviceToDevice) == hipSuccess &&
+            hipMemcpy(o.psi, P->psi, sizeof(double) * L * nG, hipMemcpyDeviceToDevice) == hipSuccess &&
+            hipMemcpy(o.nl0, P->nl[0], sizeof(double) * L * nG, hipMemcpyDeviceToDevice) == hipSuccess &&
+            hipMemcpy(o.nl1b, P->nl[1], sizeof(double) * L * nG, hipMemcpyDeviceToDevice) == hipSuccess &&
+            hipMemcpy(o.q, P->q, sizeof(double) * L * nq, hipMemcpyDeviceToDevice) == hipSuccess &&
+            hipMemcpy(o.q0, P->q0, sizeof(double) * L * nq, hipMemcpyDeviceToDevice) == hipSuccess &&
+            hipMemcpy(o.Fb, P->Fb, sizeof(double) * P->nf, hipMemcpyDeviceToDevice) == hipSuccess;
+        if (!ok) rc = IBPM_E_HIP;
+        P->G = o.G; P->Gs = o.Gs; P->psi = o.psi; P->nl[0] = o.nl0; P->nl[1] = o.nl1b; P->q = o.q; P->q0 = o.q0;
+        P->Fb = o.Fb;
+        P->group = nullptr;
+    }
+    if (Gp->hist) (void)hipFreeAsync(Gp->hist, Gp->stream);
+    (void)hipStreamSynchronize(Gp->stream);
+    for (void* p : Gp->allocs) (void)hipFree(p);
+    if (Gp->stream) (void)hipStreamDestroy(Gp->stream);
+    ibpm_ctx* ctx = Gp->ctx;
+    delete Gp;
+    return rc == IBPM_OK ? IBPM_OK : fail(ctx, rc, "copying a member's state back failed");
+}
+
+int ibpm_group_set_option(ibpm_group* Gp, int option, int value)
+{
+    if (!Gp) return IBPM_E_ARG;
+    // the members' checks hold for the shared grid
+    int rc = ibpm_set_option(Gp->probs[0], option, value);
+    if (rc != IBPM_OK) return rc;
+    if (option == IBPM_OPT_YTILE_BATCH) Gp->ytile = value;
+    else if (option == IBPM_OPT_YTILE_SINGLE) Gp->ytile_single = value;
+    else if (option == IBPM_OPT_XWAVES_BATCH) Gp->xwaves = value;
+    return IBPM_OK;
+}
+
+int ibpm_group_advance_async(ibpm_group* Gp, int nsteps)
+{
+    if (!Gp || nsteps < 0) return IBPM_E_ARG;
+    const size_t want = (size_t)Gp->pending + (size_t)nsteps;
+    if (want > Gp->hist_steps) {
+        if (Gp->pending > 0)
+            return fail(Gp->ctx, IBPM_E_ARG, "history full: call ibpm_group_wait before enqueuing more steps");
+        int rc = group_hist(Gp, want);
+        if (rc != IBPM_OK) return rc;
+    }
+    if (nsteps > 0)
+        for (int i = 0; i < Gp->n; i++) launch_set_int(Gp->slots + i, Gp->pending, Gp->stream);
+    for (int k = 0; k < nsteps; k++) group_step(Gp);
+    Gp->pending += nsteps;
+    HIPTRY(Gp->ctx, hipGetLastError());
+    return IBPM_OK;
+}
+
+int ibpm_group_wait(ibpm_group* Gp, double* cd_hist, double* cl_hist, int32_t* steps_done)
+{
+    if (!Gp) return IBPM_E_ARG;
+    const size_t m = (size_t)Gp->pending;
+    std::vector<double> h;
+    if (m > 0 && Gp->hist) {
+        size_t tot = 0;
+        for (int i = 0; i < Gp->n; i++) tot = std::max(tot, Gp->hoff[i] + 2 * Gp->hist_steps * (size_t)Gp->nbh[i]);
+        h.resize(tot);
+        HIPTRY(Gp->ctx, hipMemcpyAsync(h.data(), Gp->hist, sizeof(double) * tot, hipMemcpyDeviceToHost, Gp->stream));
+    }
+    HIPTRY(Gp->ctx, hipStreamSynchronize(Gp->stream));
+    Gp->pending = 0;
+    bool nonfinite = false;
+    size_t o = 0;
+    for (int i = 0; i < Gp->n; i++) {
+        const ibpm_prob* P = Gp->probs[i];
+        const size_t nb = (size_t)Gp->nbh[i];
+        const double* cd = m ? h.data() + Gp->hoff[i] : nullptr;
+        const double* cl = m ? cd + Gp->hist_steps * nb : nullptr;
+        int done = (int)m;
+        for (size_t k = 0; k < m && done == (int)m; k++)
+            for (int b = 0; b < P->nbodies; b++)
+                if (!std::isfinite(cl[k * nb + b])) { done = (int)k + 1; break; }
+        if (done < (int)m) nonfinite = true;
+        if (steps_done) steps_done[i] = done;
+        if (cd_hist && m) std::copy(cd, cd + m * nb, cd_hist + o);
+        if (cl_hist && m) std::copy(cl, cl + m * nb, cl_hist + o);
+        o += m * nb;
+    }
+    return nonfinite ? fail(Gp->ctx, IBPM_E_NONFINITE, "non-finite CL in a group member") : IBPM_OK;
 }
 
 int ibpm_compute_cfl(ibpm_prob* P, double* cfl)

@@ -6,6 +6,16 @@ namespace ibpm {
 
 constexpr int MAXLEV = 16;
 
+// Per-instance parameters of a group of same-shape problems stepped together.
+// A batch over all levels of all instances has entries e = inst * L + lev
+// (levmod = L); its kernels take dt, Re, the AB2 coefficients and the uniform
+// base flux of entry e from ipar[e / L].
+struct InstPar {
+    double c1, c2;      // β1 dt, β2 dt
+    double dt, Re;
+    double q0vx, q0vy;  // inertial base flux per unit hc
+};
+
 // ---- DST passes -----------------------------------------------------------
 enum XMode { X_PLAIN = 0, X_BC = 1, X_SUB = 2, X_TRIAL = 3 };
 
@@ -32,6 +42,10 @@ struct XArgs {
     // X_PLAIN over a batch (nlev > 1): waves (lines) per work-group, 8, 12 or
     // 14; anything else (0) gives the default 4
     int waves;
+    // X_TRIAL over a group: entry e = lev0 + b is level e % levmod of
+    // instance e / levmod, whose c1, c2, dt, Re come from ipar (0 / null: one problem)
+    int levmod;
+    const InstPar* ipar;
     DstPlan pl;
     GridDev g;
 };
@@ -51,6 +65,11 @@ struct YArgs {
     double dt;
     double Re;            // divisor (2 Re) hc^2 is formed per level in the kernel
     double scale;         // 1 / (4 nx ny)
+    // over a group: entry e = lev0 + b is level e % levmod of instance
+    // e / levmod (fixlev: entry b is instance b at level lev0); dt and Re of
+    // the instance from ipar (0 / null: one problem)
+    int levmod;
+    const InstPar* ipar;
     DstPlan pl;
     GridDev g;
 };
@@ -74,6 +93,10 @@ struct NLArgs {
     // 144-163); QB (nq per level, averaged) and GB (ΓB, nG per level)
     const double* QB;
     const double* GB;
+    // over a group: entry e = lev0 + b is level e % levmod of instance
+    // e / levmod, with that instance's base flux from ipar (0 / null: one problem)
+    int levmod;
+    const InstPar* ipar;
     GridDev g;
 };
 void launch_nonlinear(const NLArgs& a, hipStream_t s);
@@ -84,9 +107,11 @@ struct CurlArgs {
     long qstride;
     int nlev, lev0, mg;
     // colbatch = 1: the nlev entries are fields at level lev0, field b's level
-    // l at psi + l * psl + b * nG (levels of a batch laid out level-major)
+    // l at psi + l * psl + b * psb
     int colbatch;
-    long psl;
+    long psl, psb;
+    // over a group (colbatch = 0): entry e = lev0 + b is level e % levmod
+    int levmod;
     GridDev g;
 };
 void launch_curl(const CurlArgs& a, hipStream_t s);
@@ -114,6 +139,15 @@ struct GatherArgs {
     int nbt;
     int nb;            // batch of fields (0 or 1: one): q + b * qstride -> out + b * ostride
     long qstride, ostride;
+    // over a group (pinst != null): nbt is the total of all instances' points;
+    // point k belongs to instance pinst[k] as its point plocal[k]; that
+    // instance reads q + inst * qstride, q0 + inst * q0stride and writes its
+    // F̃b at out + ifoff[inst] (inbt[inst] points)
+    const int* pinst;
+    const int* plocal;
+    const int* inbt;
+    const int* ifoff;
+    long q0stride;
     GridDev g;
 };
 void launch_gather(const GatherArgs& a, hipStream_t s);
@@ -159,6 +193,13 @@ struct ForceArgs {
 void launch_gemv_forces(const double* AT, const double* x, double* y, int n, const ForceArgs& fa, hipStream_t s);
 // update_stress! alone, on an F̃b already in place (MotionFunction: F̃b comes from the CG solve)
 void launch_forces(const ForceArgs& fa, hipStream_t s);
+// a group: y_i = Binv_i x_i for every instance (rows of all instances: row r
+// is row rloc[r] of instance rinst[r], whose Binv (transposed) starts at AT +
+// aoff[inst], has inf[inst] rows and reads/writes x, y at ifoff[inst]), then
+// update_stress! per instance (fa: device array of n argument blocks)
+void launch_gemv_forces_group(const double* AT, const long* aoff, const double* x, double* y, const int* rinst,
+                              const int* rloc, const int* ifoff, const int* inf, int nrows, const ForceArgs* fa,
+                              int n, hipStream_t s);
 
 struct BaseFluxArgs {
     double* q0;

@@ -152,11 +152,17 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
     if constexpr (MODE == X_TRIAL) {
         // get_trial_state! (timestepping.jl:111-118): rhs = A Γ; rhs += (β1 dt) N1;
         // rhs += (β2 dt) N0; rhs += rhsbc, with rhsbc = apply_bc!(0, get_bc!(Γ[:, lev+1]), fac)
-        const int lev = a.lev0 + b;
+        const int e = a.lev0 + b;
+        const int lev = a.levmod ? e % a.levmod : e;
         const bool hasbc = lev < a.mg - 1;
-        const double* gc = a.gam + (long)(lev + 1) * a.lstride;
+        const double* gc = a.gam + (long)(e + 1) * a.lstride;
         const double hc = level_hc(a.g.h, lev);
-        const double fac = 0.25 * a.dt / (a.Re * (hc * hc));  // timestepping.jl:104
+        double dt = a.dt, Re = a.Re, c1 = a.c1, c2 = a.c2;
+        if (a.ipar) {
+            const InstPar& ip = a.ipar[e / a.levmod];
+            dt = ip.dt; Re = ip.Re; c1 = ip.c1; c2 = ip.c2;
+        }
+        const double fac = 0.25 * dt / (Re * (hc * hc));  // timestepping.jl:104
         double cl = 0.0, cr = 0.0;
         if (hasbc) {
             cl = fac * bc_col(gc, a.g, a.g.nx / 4, 1 + j);
@@ -168,8 +174,8 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
             int n = lane + WAVE * t;
             if (n < n1) {
                 double rhs = s[lat<PT>(n)];
-                rhs += a.c1 * v[t];
-                rhs += a.c2 * u[t];
+                rhs += c1 * v[t];
+                rhs += c2 * u[t];
                 if (hasbc) {
                     // rhsbc = apply_bc!(0, bc, fac): row terms, then column terms
                     const int i = n + 1;
@@ -394,9 +400,16 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
         }
     }
     __syncthreads();
-    const int lev = a.fixlev ? a.lev0 : a.lev0 + b;
+    const int e = a.lev0 + b;
+    const int lev = a.fixlev ? a.lev0 : (a.levmod ? e % a.levmod : e);
     const double hcl = level_hc(a.g.h, lev);
-    const double qc = a.dt / ((2.0 * a.Re) * (hcl * hcl));  // dt / (2 Re h^2), laplacian.jl:69,74
+    double dt = a.dt, Re = a.Re;
+    if (a.ipar) {
+        const int inst = a.fixlev ? b : e / a.levmod;
+        dt = a.ipar[inst].dt;
+        Re = a.ipar[inst].Re;
+    }
+    const double qc = dt / ((2.0 * Re) * (hcl * hcl));  // dt / (2 Re h^2), laplacian.jl:69,74
     // (column, transform) pairs: transform 0 is followed by the eigenvalue
     // scaling; one call site keeps a single DST body in the kernel
 #pragma unroll 1
@@ -507,16 +520,22 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
     const int j0 = ((rem / nbx) * 4 + threadIdx.y) * NL_JS + 2;          // first output row, 2..ny
     if (i > nx || j0 > ny) return;
     const int j1 = min(j0 + NL_JS - 1, ny);
-    const int lev = a.lev0 + b;
+    const int e = a.lev0 + b;                     // batch entry: its fields at e * nq, e * nG
+    const int lev = a.levmod ? e % a.levmod : e;  // its grid level
     const long nq = g.nq, nu = g.nu;
-    const double* __restrict__ q = a.q + (long)lev * nq;
-    const double* __restrict__ q0 = a.q0 + (long)lev * nq;
-    const double* __restrict__ G = a.G + (long)lev * g.nG;
+    const double* __restrict__ q = a.q + (long)e * nq;
+    const double* __restrict__ q0 = a.q0 + (long)e * nq;
+    const double* __restrict__ G = a.G + (long)e * g.nG;
     const bool hasbc = lev < a.mg - 1;
-    const double* __restrict__ Gc = a.G + (long)(lev + 1) * g.nG;
+    const double* __restrict__ Gc = a.G + (long)(e + 1) * g.nG;
     const bool u0 = a.q0_uniform != 0;
     const double hcl = level_hc(g.h, lev);
-    const double cx0 = a.q0vx * hcl, cy0 = a.q0vy * hcl;  // as k_base_flux_inertial writes them
+    double q0vx = a.q0vx, q0vy = a.q0vy;
+    if (a.ipar) {
+        q0vx = a.ipar[e / a.levmod].q0vx;
+        q0vy = a.ipar[e / a.levmod].q0vy;
+    }
+    const double cx0 = q0vx * hcl, cy0 = q0vy * hcl;  // as k_base_flux_inertial writes them
 
     // boundary circulation get_bc!(Γ[:, lev+1]) .* 0.25 (nonlinear.jl:186-188)
     auto BCV = [&](int k) -> double { return hasbc ? get_bc_val(Gc, g, k) * 0.25 : 0.0; };
@@ -662,12 +681,15 @@ __global__ void __launch_bounds__(256) k_curl(CurlArgs a)
     const int i = (rem % nbx) * 64 + threadIdx.x + 1;  // 1..nx+1
     const int j = (rem / nbx) * 4 + threadIdx.y + 1;   // 1..ny+1
     if (i > nx + 1 || j > ny + 1) return;
-    const int lev = a.colbatch ? a.lev0 : a.lev0 + b;
-    const long psl = a.colbatch ? a.psl : g.nG;
-    const long psb = a.colbatch ? (long)b * g.nG : 0;
-    const double* __restrict__ psi = a.psi + (long)lev * psl + psb;
+    // entry e = lev0 + b: level lev of its field; colbatch: fields at level
+    // lev0 with levels psl and fields psb apart
+    const int e = a.lev0 + b;
+    const int lev = a.colbatch ? a.lev0 : (a.levmod ? e % a.levmod : e);
+    const long pe = a.colbatch ? (long)lev * a.psl + (long)b * a.psb : (long)e * g.nG;
+    const long pl1 = a.colbatch ? a.psl : g.nG;
+    const double* __restrict__ psi = a.psi + pe;
     const bool hasbc = lev < a.mg - 1;
-    const double* __restrict__ pc = a.psi + (long)(lev + 1) * psl + psb;
+    const double* __restrict__ pc = a.psi + pe + pl1;
     auto PS = [&](int ii, int jj) -> double { return psi[(ii - 1) + (long)n1 * (jj - 1)]; };
     auto BC = [&](int k) -> double { return hasbc ? get_bc_val(pc, g, k) : 0.0; };
     const int T = g.TOP, B = g.BOT, L = g.LEFT, R = g.RIGHT;
@@ -779,8 +801,16 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs a)
     const int bq = wid0 / (2 * a.nbt);
     if (bq >= (a.nb > 1 ? a.nb : 1)) return;
     const int wid = wid0 - bq * (2 * a.nbt);
-    const double* __restrict__ qf = a.q + (long)bq * a.qstride;
     const int c = wid / a.nbt, k = wid % a.nbt;
+    const double* __restrict__ qf = a.q + (long)bq * a.qstride;
+    const double* __restrict__ q0f = a.q0;
+    long o = (long)bq * a.ostride + c * a.nbt + k;
+    if (a.pinst) {  // a group: point k is point plocal[k] of instance pinst[k]
+        const int inst = a.pinst[k];
+        qf = a.q + (long)inst * a.qstride;
+        if (q0f) q0f += (long)inst * a.q0stride;
+        o = a.ifoff[inst] + (long)c * a.inbt[inst] + a.plocal[k];
+    }
     const int bx = a.bidx[k], by = a.bidx[a.nbt + k];
     const double* w = a.w + (long)(2 * k + c) * 169;
     const int nx = a.g.nx;
@@ -789,11 +819,11 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs a)
         const int l = e % 13, m = e / 13;
         const int i = bx + l - 6, j = by + m - 6;
         long p = (c == 0) ? (i - 1) + (long)(nx + 1) * (j - 1) : a.g.nu + (i - 1) + (long)nx * (j - 1);
-        double v = a.q0 ? (qf[p] + a.q0[p]) : qf[p];
+        double v = q0f ? (qf[p] + q0f[p]) : qf[p];
         acc += w[e] * v;
     }
     acc = wave_sum(acc);
-    if (lane == 0) a.out[(long)bq * a.ostride + c * a.nbt + k] = acc;
+    if (lane == 0) a.out[o] = acc;
 }
 
 void launch_gather(const GatherArgs& a, hipStream_t s)
@@ -933,6 +963,51 @@ void launch_forces(const ForceArgs& fa, hipStream_t s)
 {
     if (fa.nbodies == 0) return;
     hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, s, fa);
+}
+
+// a group's GEMVs: row r of all instances' rows is row rloc[r] of instance
+// rinst[r]; the same wavefront-per-row sum as k_gemv_rows
+__global__ void __launch_bounds__(256) k_gemv_rows_group(const double* __restrict__ AT, const long* __restrict__ aoff,
+                                                         const double* __restrict__ x, double* __restrict__ y,
+                                                         const int* __restrict__ rinst, const int* __restrict__ rloc,
+                                                         const int* __restrict__ ifoff, const int* __restrict__ inf,
+                                                         int nrows)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int rr = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    if (rr >= nrows) return;
+    const int inst = rinst[rr], r = rloc[rr], n = inf[inst];
+    const double* row = AT + aoff[inst] + (long)r * n;
+    const double* xi = x + ifoff[inst];
+    double acc = 0.0;
+    constexpr int U = 8;
+    for (int c0 = 0; c0 < n; c0 += U * WAVE) {
+        double av[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = c0 + lane + u * WAVE;
+            av[u] = c < n ? row[c] : 0.0;
+            xv[u] = c < n ? xi[c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (c0 + lane + u * WAVE < n) acc = fma(av[u], xv[u], acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) y[ifoff[inst] + r] = acc;
+}
+
+// update_stress! of every instance of a group: block i takes instance i
+__global__ void __launch_bounds__(256) k_forces_group(const ForceArgs* __restrict__ fa) { forces_block(fa[blockIdx.x]); }
+
+void launch_gemv_forces_group(const double* AT, const long* aoff, const double* x, double* y, const int* rinst,
+                              const int* rloc, const int* ifoff, const int* inf, int nrows, const ForceArgs* fa,
+                              int n, hipStream_t s)
+{
+    if (nrows == 0 || n == 0) return;
+    hipLaunchKernelGGL(k_gemv_rows_group, dim3((nrows + 3) / 4), dim3(256), 0, s, AT, aoff, x, y, rinst, rloc, ifoff,
+                       inf, nrows);
+    hipLaunchKernelGGL(k_forces_group, dim3(n), dim3(256), 0, s, fa);
 }
 
 // ===========================================================================

@@ -16,13 +16,15 @@ GPU, constructing an IBProblem needs one.
 """
 from .domain import (InertialMotion, Motion, MotionFunction, MotionType, MovingGrid, MultiGrid, RigidBody, Static,
                      get_body_info, make_body, make_cylinder, make_grid, make_naca, make_plate, naca)
-from .problem import (AB2, AdamsBashforth, IBModel, IBProblem, IBState, LinearizedIBProblem, StateSnapshot, UserVar,
+from .problem import (AB2, AdamsBashforth, Group, IBModel, IBProblem, IBState, LinearizedIBProblem, StateSnapshot,
+                      UserVar,
                       advance, advance_, advance_batch_,
                       advance_n_, base_flux_, compute_cfl, copy_, init_save_data, load_state, run_sim_, save_data_,
                       save_state, snapshot, vort2flux_)
 from . import _capi
 
 __all__ = [
+    "Group",
     "MultiGrid", "make_grid", "Motion", "InertialMotion", "Static", "MovingGrid", "MotionFunction", "RigidBody",
     "make_plate", "make_cylinder", "make_naca", "naca", "make_body", "MotionType", "get_body_info", "AB2",
     "AdamsBashforth", "IBModel", "IBProblem", "LinearizedIBProblem", "IBState", "advance_", "advance", "advance_n_", "advance_batch_", "base_flux_",

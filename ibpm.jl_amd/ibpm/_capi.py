@@ -37,7 +37,8 @@ EXPORTED = [
     "ibpm_advance_n", "ibpm_get_forces", "ibpm_compute_cfl", "ibpm_op_lapinv", "ibpm_op_dst2",
     "ibpm_op_nonlinear", "ibpm_op_vort2flux", "ibpm_op_regT", "ibpm_op_reg", "ibpm_op_rot", "ibpm_set_option",
     "ibpm_get_option", "ibpm_time_kernel", "ibpm_profile_stats", "ibpm_get_body_state", "ibpm_cg_stats",
-    "ibpm_op_update_bodies",
+    "ibpm_op_update_bodies", "ibpm_group_create", "ibpm_group_destroy", "ibpm_group_set_option",
+    "ibpm_group_advance_async", "ibpm_group_wait",
 ]
 
 
@@ -118,6 +119,11 @@ def lib():
         L.ibpm_get_body_state.argtypes = [_vp, _dp, _dp, ctypes.c_size_t]
         L.ibpm_cg_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32), _dp, _dp]
         L.ibpm_op_update_bodies.argtypes = [_vp, _dp]
+        L.ibpm_group_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]
+        L.ibpm_group_destroy.argtypes = [_vp]
+        L.ibpm_group_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.ibpm_group_advance_async.argtypes = [_vp, ctypes.c_int]
+        L.ibpm_group_wait.argtypes = [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int32)]
         L.ibpm_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.ibpm_get_option.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         _LIB = L

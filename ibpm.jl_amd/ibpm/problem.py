@@ -375,6 +375,71 @@ def advance_batch_(states, probs, ts_list):
     return out
 
 
+class Group:
+    """Problems with static bodies on one grid, stepped together: every pass
+    of advance! is one launch over all of them (ibpm_group_*).  While the
+    group exists the members' states live in its arrays; they are read as
+    usual (IBState fields, forces) and stepped only through the group.
+    close() (or garbage collection) hands every member its state back."""
+
+    def __init__(self, probs, states=None):
+        self.probs = list(probs)
+        self.states = list(states) if states is not None else []
+        self.ctx = self.probs[0].ctx
+        arr = (ctypes.c_void_p * len(self.probs))(*[p.h.value for p in self.probs])
+        h = ctypes.c_void_p()
+        self.ctx.check(C.lib().ibpm_group_create(self.ctx.h, len(self.probs), arr, ctypes.byref(h)),
+                       "ibpm_group_create")
+        self.h = h
+
+    def set_option(self, option: int, value: int):
+        self.ctx.check(C.lib().ibpm_group_set_option(self.h, option, value), "ibpm_group_set_option")
+
+    def advance_async(self, nsteps: int):
+        self.ctx.check(C.lib().ibpm_group_advance_async(self.h, int(nsteps)), "ibpm_group_advance_async")
+        self._pending = getattr(self, "_pending", 0) + int(nsteps)
+
+    def wait(self):
+        """[(CD, CL, steps_done)] per member for the steps since the last wait."""
+        m = getattr(self, "_pending", 0)
+        nb = [max(len(p.model.bodies), 1) for p in self.probs]
+        cd = np.zeros(max(m * sum(nb), 1))
+        cl = np.zeros_like(cd)
+        done = (ctypes.c_int32 * len(self.probs))()
+        rc = C.lib().ibpm_group_wait(self.h, C.ptr(cd), C.ptr(cl), done)
+        self._pending = 0
+        for st in self.states:
+            st.invalidate()
+        if rc not in (C.IBPM_OK, C.IBPM_E_NONFINITE):
+            self.ctx.check(rc, "ibpm_group_wait")
+        out, o = [], 0
+        for i, k in enumerate(nb):
+            blk = m * k
+            d = int(done[i])
+            out.append((cd[o:o + blk].reshape(m, k)[:d], cl[o:o + blk].reshape(m, k)[:d], d))
+            o += blk
+        return out
+
+    def advance_n_(self, nsteps: int):
+        self.advance_async(nsteps)
+        return self.wait()
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.h = None
+            rc = C.lib().ibpm_group_destroy(h)
+            for st in self.states:
+                st.invalidate()
+            self.ctx.check(rc, "ibpm_group_destroy")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def advance(x: IBState, prob: IBProblem, t: float):
     """advance(x, prob, t) -- timestepping.jl:22-28.  The device state is
     single-copy per problem, so this advances `x` in place and returns it."""

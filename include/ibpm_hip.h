@@ -38,6 +38,7 @@ extern "C" {
 
 typedef struct ibpm_ctx ibpm_ctx;
 typedef struct ibpm_prob ibpm_prob;
+typedef struct ibpm_group ibpm_group;
 
 /* status codes */
 #define IBPM_OK 0
@@ -188,6 +189,29 @@ int ibpm_get_body_state(ibpm_prob* prob, double* xb, double* ub, size_t n);
  * solve, and its tolerance 1e-12 |b| (cg!, laplacian.jl:137-138).  No
  * counterpart in the reference (cg! is called without log = true there). */
 int ibpm_cg_stats(ibpm_prob* prob, int32_t* iterations, double* residual, double* tolerance);
+
+/* Groups: n problems with static bodies on the same grid (nx, ny, mg, h,
+ * offsets; dt, Re, freestream and bodies may differ) stepped together, each
+ * pass of advance! one launch over all instances.  No counterpart in the
+ * reference: it is the batched form of many run_sim! loops (ibpm.jl:93-105)
+ * over independent simulations.  While grouped, a member's state lives in the
+ * group's arrays: it is read with ibpm_state_download / ibpm_get_forces (after
+ * ibpm_group_wait), and the member's own stepping, upload, state_init,
+ * base_flux, vort2flux, linearize and destroy calls return IBPM_E_ARG.
+ * ibpm_group_destroy copies every member's state back and releases them.
+ * Every instance takes the same steps: the results are those of stepping each
+ * member alone. */
+int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* members, ibpm_group** out);
+int ibpm_group_destroy(ibpm_group* group);
+/* IBPM_OPT_YTILE_* and IBPM_OPT_XWAVES_BATCH for the group's launches */
+int ibpm_group_set_option(ibpm_group* group, int option, int value);
+/* Enqueue nsteps steps of every member (static bodies: no per-step scalars). */
+int ibpm_group_advance_async(ibpm_group* group, int nsteps);
+/* Drain; cd_hist / cl_hist (may be NULL) receive, member after member, each
+ * member's (steps x nbodies) history of the steps since the last wait;
+ * steps_done (n entries, may be NULL) as in ibpm_advance_wait per member;
+ * IBPM_E_NONFINITE if any member's CL became non-finite. */
+int ibpm_group_wait(ibpm_group* group, double* cd_hist, double* cl_hist, int32_t* steps_done);
 
 /* state.CD, state.CL (nbodies each) and state.F̃b (2 nb_total); any may be NULL. */
 int ibpm_get_forces(ibpm_prob* prob, double* CD, double* CL, double* Fb);

@@ -1,0 +1,95 @@
+"""Groups (ibpm_group_*): problems on one grid stepped together, every pass of
+advance! one launch over all of them.  Each member's arithmetic is the one of
+stepping it alone, so the results must be identical bit for bit: forces,
+every state field, and the continuation after the group is dissolved.
+"""
+import numpy as np
+import pytest
+
+import ibpm
+from cases import gaussian_vortex_gamma, naca64_cases
+from ibpm import _capi as C
+
+pytestmark = pytest.mark.gpu
+
+
+def members(binvs=None):
+    """Three problems on the 64 x 64, 3-level grid with different bodies (one
+    with two bodies), Reynolds numbers, time steps and freestreams."""
+    g = ibpm.make_grid(1.0 / 16, (-1.5, 2.5, -2.0, 2.0), mg=3)
+    specs = [
+        ([ibpm.make_cylinder(0.5, g.h, 0.0, 0.0, n=40)], 100.0, 0.01, {"Ux": lambda t: 1.0}),
+        ([ibpm.make_plate(1.0, 0.3, g.h, -0.4, 0.1, n=17)], 200.0, 0.005,
+         {"Ux": lambda t: 1.0, "inclination": lambda t: 0.2}),
+        ([ibpm.make_cylinder(0.25, g.h, -0.3, 0.2, n=20), ibpm.make_plate(0.5, 0.0, g.h, 0.4, -0.3, n=9)], 50.0,
+         0.008, {"Ux": lambda t: 0.8, "Uy": lambda t: 0.1}),
+    ]
+    probs, states = [], []
+    for k, (bodies, Re, dt, fs) in enumerate(specs):
+        p = ibpm.IBProblem(g, bodies, dt, Re, freestream=fs, Binv=None if binvs is None else binvs[k])
+        st = ibpm.IBState(p)
+        st.Γ = gaussian_vortex_gamma(g, x0=0.8, y0=0.2 * k, circ=0.5, width=0.3)
+        ibpm.vort2flux_(st, p)
+        probs.append(p)
+        states.append(st)
+    return probs, states
+
+
+def fields(st):
+    return [st.Γ.copy(), st.q.copy(), st.ψ.copy(), st.F̃b.copy(), st.nonlin[0].copy(), st.nonlin[1].copy()]
+
+
+def test_group_steps_equal_single_steps():
+    pa, sa = members()
+    pg, sg = members([p.Binv for p in pa])
+    n1, n2 = 12, 5
+    alone = [ibpm.advance_n_(st, p, np.arange(n1) * p.scheme.dt) for p, st in zip(pa, sa)]
+    grp = ibpm.Group(pg, sg)
+    res = grp.advance_n_(n1)
+    for (cd, cl, d), (cdg, clg, dg), st, stg in zip(alone, res, sa, sg):
+        assert d == dg == n1
+        assert np.array_equal(cd, cdg) and np.array_equal(cl, clg)
+        for a, b in zip(fields(st), fields(stg)):
+            assert np.array_equal(a, b)
+    # members step through the group only
+    with pytest.raises(C.IBPMError):
+        ibpm.advance_(sg[0], pg[0], 0.0)
+    # dissolved: every member carries on alone from the same state
+    grp.close()
+    for p, st, pgi, sgi in zip(pa, sa, pg, sg):
+        cd, cl, _ = ibpm.advance_n_(st, p, np.arange(n2) * p.scheme.dt)
+        cdg, clg, _ = ibpm.advance_n_(sgi, pgi, np.arange(n2) * pgi.scheme.dt)
+        assert np.array_equal(cd, cdg) and np.array_equal(cl, clg)
+        assert np.array_equal(st.Γ, sgi.Γ) and np.array_equal(st.q, sgi.q)
+
+
+def test_group_of_naca_instances_equals_single():
+    allc = naca64_cases()
+    key = (allc[0]["grid"].nx, allc[0]["grid"].ny, allc[0]["grid"].mg)
+    cs = [c for c in allc if (c["grid"].nx, c["grid"].ny, c["grid"].mg) == key][:4]
+    assert len(cs) >= 2
+
+    def build():
+        out = []
+        for c in cs:
+            p = ibpm.IBProblem(c["grid"], c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
+            out.append((p, ibpm.IBState(p)))
+        return out
+
+    a, b = build(), build()
+    alone = [ibpm.advance_n_(st, p, np.arange(6) * p.scheme.dt) for p, st in a]
+    grp = ibpm.Group([p for p, _ in b], [st for _, st in b])
+    res = grp.advance_n_(3)
+    res2 = grp.advance_n_(3)  # two calls: the history slots restart at each wait
+    for (cd, cl, _), (c1, l1, _), (c2, l2, _), (_, sta), (_, stb) in zip(alone, res, res2, a, b):
+        assert np.array_equal(cd, np.vstack([c1, c2])) and np.array_equal(cl, np.vstack([l1, l2]))
+        assert np.array_equal(sta.Γ, stb.Γ)
+    grp.close()
+
+
+def test_group_rejects_mixed_grids():
+    pa, _ = members()
+    g2 = ibpm.make_grid(1.0 / 16, (-1.5, 2.5, -1.0, 1.0), mg=3)
+    other = ibpm.IBProblem(g2, [ibpm.make_cylinder(0.3, g2.h, 0.0, 0.0, n=24)], 0.01, 100.0)
+    with pytest.raises(C.IBPMError):
+        ibpm.Group([pa[0], other])
