@@ -672,14 +672,14 @@ static int assemble_b(ibpm_prob* P)
         launch_rot_batch(P->bq, nq, P->bgt, nG, kc, P->rb_i0, P->rb_i1, P->rb_j0, P->rb_j1, g, s);
         // Γ[:, 1] = Ainv rot(E' e_f); the rows outside the box are zero
         XArgs a = xargs(P);
-        a.src = P->bgt; a.dst = P->bt1; a.nlev = kc; a.lev0 = 0;
+        a.src = P->bgt; a.dst = P->bt1; a.nlev = kc; a.lev0 = 0; a.fixlev = 1;
         a.zlo = P->rb_j0 - 1; a.zhi = std::min(P->rb_j1 - 1, g.n2);
         launch_xpass(X_PLAIN, a, s);
         YArgs y = yargs(P, Y_AINV, 0, kc, P->bt1);
         y.fixlev = 1;
         launch_ypass(y, P->ytile, s);
         XArgs c = xargs(P);
-        c.src = P->bt1; c.dst = P->bG; c.nlev = kc; c.lev0 = 0;
+        c.src = P->bt1; c.dst = P->bG; c.nlev = kc; c.lev0 = 0; c.fixlev = 1;
         launch_xpass(X_PLAIN, c, s);
         // vort2flux!: coarsify into the centres of the coarse levels (their
         // rings stay zero), then Δinv from the coarsest level down
@@ -695,7 +695,7 @@ static int assemble_b(ibpm_prob* P)
         for (int l = L - 1; l >= 0; l--) {
             const bool bc = l < L - 1;
             XArgs a1 = xargs(P);
-            a1.src = P->bG + (long)l * K * nG; a1.dst = P->bt1; a1.nlev = kc; a1.lev0 = l;
+            a1.src = P->bG + (long)l * K * nG; a1.dst = P->bt1; a1.nlev = kc; a1.lev0 = l; a1.fixlev = 1;
             if (bc) {
                 a1.bcr = P->bpsi + (long)(l + 1) * K * nG;
                 a1.bcstride = nG;
@@ -705,7 +705,7 @@ static int assemble_b(ibpm_prob* P)
             y1.fixlev = 1;
             launch_ypass(y1, P->ytile, s);
             XArgs c1 = xargs(P);
-            c1.src = P->bt1; c1.dst = P->bpsi + (long)l * K * nG; c1.nlev = kc; c1.lev0 = l;
+            c1.src = P->bt1; c1.dst = P->bpsi + (long)l * K * nG; c1.nlev = kc; c1.lev0 = l; c1.fixlev = 1;
             launch_xpass(X_PLAIN, c1, s);
         }
         CurlArgs ca{};
@@ -1495,6 +1495,7 @@ XArgs gxargs(const ibpm_group* Gp)
     a.zlo = 1;
     a.zhi = Gp->g.n2;
     a.waves = Gp->xwaves;
+    a.ipar = Gp->ipar;
     return a;
 }
 
@@ -1525,13 +1526,14 @@ void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, boo
         c.Gc = Gb + (long)(lev - 1) * nG;
         c.G = Gb + (long)(lev - 2) * nG;
         c.i0 = r.i0; c.ic0 = r.ic0; c.ci = r.ci; c.j0 = r.j0; c.jc0 = r.jc0; c.cj = r.cj;
-        c.nb = n; c.bstride = ist; c.g = Gp->g;
+        c.nb = n; c.bstride = ist; c.g = Gp->g; c.ipar = Gp->ipar; c.clev = lev;
         launch_coarsify(c, s);
     }
     for (int l = L - 1; l >= 0; l--) {
         const bool bc = l < L - 1;
         XArgs a = gxargs(Gp);
         a.src = Gb + (long)l * nG; a.dst = Gp->T1 + (long)l * nG; a.lstride = ist; a.nlev = n; a.lev0 = l;
+        a.fixlev = 1;
         if (bc) {
             a.bcr = psib + (long)(l + 1) * nG;
             a.bcstride = ist;
@@ -1542,10 +1544,11 @@ void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, boo
         launch_ypass(y, Gp->ytile_single, s);
         XArgs c = gxargs(Gp);
         c.src = Gp->T1 + (long)l * nG; c.dst = psib + (long)l * nG; c.lstride = ist; c.nlev = n; c.lev0 = l;
+        c.fixlev = 1;
         launch_xpass(X_PLAIN, c, s);
     }
     CurlArgs ca{};
-    ca.psi = psib; ca.q = qout; ca.qstride = Gp->g.nq; ca.mg = L; ca.g = Gp->g; ca.lev0 = 0;
+    ca.psi = psib; ca.q = qout; ca.qstride = Gp->g.nq; ca.mg = L; ca.g = Gp->g; ca.lev0 = 0; ca.ipar = Gp->ipar;
     if (level1_only) {
         ca.nlev = n; ca.colbatch = 1; ca.psl = nG; ca.psb = ist;
     } else {
@@ -1569,7 +1572,7 @@ void group_step(ibpm_group* Gp)
     na.q0_uniform = Gp->q0_uniform ? 1 : 0;
     launch_nonlinear(na, s);
     XArgs a = gxargs(Gp);
-    a.src = Gp->G; a.dst = Gp->T1; a.nlev = NE; a.lev0 = 0;
+    a.src = Gp->G; a.dst = Gp->T1; a.nlev = NE; a.lev0 = 0; a.levmod = L;
     launch_xpass(X_PLAIN, a, s);
     YArgs ya = gyargs(Gp, Y_A, Gp->T1);
     ya.lstride = nG; ya.nlev = NE; ya.lev0 = 0; ya.levmod = L;
@@ -1582,7 +1585,7 @@ void group_step(ibpm_group* Gp)
     yi.lstride = nG; yi.nlev = NE; yi.lev0 = 0; yi.levmod = L;
     launch_ypass(yi, Gp->ytile, s);
     XArgs c = gxargs(Gp);
-    c.src = Gp->T2; c.dst = Gp->Gs; c.nlev = NE; c.lev0 = 0;
+    c.src = Gp->T2; c.dst = Gp->Gs; c.nlev = NE; c.lev0 = 0; c.levmod = L;
     launch_xpass(X_PLAIN, c, s);
     Gp->nl1 = Gp->nl2 = out;
     group_vort2flux(Gp, Gp->Gs, Gp->psi, Gp->qs1, true);
@@ -1601,14 +1604,14 @@ void group_step(ibpm_group* Gp)
     launch_scatter(sa, s);
     launch_rot_batch(Gp->qreg, g.nq, Gp->Gt, ist, n, Gp->rb_i0, Gp->rb_i1, Gp->rb_j0, Gp->rb_j1, g, s);
     XArgs p1 = gxargs(Gp);
-    p1.src = Gp->Gt; p1.dst = Gp->T1; p1.lstride = ist; p1.nlev = n; p1.lev0 = 0;
+    p1.src = Gp->Gt; p1.dst = Gp->T1; p1.lstride = ist; p1.nlev = n; p1.lev0 = 0; p1.fixlev = 1;
     p1.zlo = Gp->rb_j0 - 1; p1.zhi = std::min(Gp->rb_j1 - 1, g.n2);
     launch_xpass(X_PLAIN, p1, s);
     YArgs yp = gyargs(Gp, Y_AINV, Gp->T1);
     yp.lstride = ist; yp.nlev = n; yp.lev0 = 0; yp.fixlev = 1;
     launch_ypass(yp, Gp->ytile_single, s);
     XArgs p2 = gxargs(Gp);
-    p2.src = Gp->T1; p2.dst = Gp->Gs; p2.lstride = ist; p2.nlev = n; p2.lev0 = 0;
+    p2.src = Gp->T1; p2.dst = Gp->Gs; p2.lstride = ist; p2.nlev = n; p2.lev0 = 0; p2.fixlev = 1;
     launch_xpass(X_SUB, p2, s);
     std::swap(Gp->G, Gp->Gs);
     group_vort2flux(Gp, Gp->G, Gp->psi, Gp->q, false);
@@ -1664,9 +1667,9 @@ int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* probs, ibpm_group*
         if (P->group) return fail(ctx, IBPM_E_ARG, "a problem can belong to one group only");
         if (P->motion != IBPM_MOTION_STATIC || P->lin || P->nbodies < 1)
             return fail(ctx, IBPM_E_UNSUPPORTED, "groups take problems with static bodies (not linearized)");
-        if (P->g.nx != P0->g.nx || P->g.ny != P0->g.ny || P->mg != P0->mg || P->g.h != P0->g.h ||
-            P->offx != P0->offx || P->offy != P0->offy)
-            return fail(ctx, IBPM_E_ARG, "group members must share the grid (nx, ny, mg, h, offset)");
+        if (P->g.nx != P0->g.nx || P->g.ny != P0->g.ny || P->g.h != P0->g.h || P->offx != P0->offx ||
+            P->offy != P0->offy)
+            return fail(ctx, IBPM_E_ARG, "group members must share the finest grid (nx, ny, h, offset)");
         if (P->pending > 0) return fail(ctx, IBPM_E_ARG, "wait for a member's queued steps before grouping it");
         for (int k = 0; k < i; k++)
             if (probs[k] == P) return fail(ctx, IBPM_E_ARG, "a problem appears twice in the group");
@@ -1674,7 +1677,8 @@ int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* probs, ibpm_group*
     ibpm_group* Gp = new ibpm_group();
     Gp->ctx = ctx;
     Gp->n = n;
-    Gp->L = P0->mg;
+    Gp->L = 0;
+    for (int i = 0; i < n; i++) Gp->L = std::max(Gp->L, probs[i]->mg);
     Gp->g = P0->g;
     Gp->tx = P0->tx; Gp->ty = P0->ty; Gp->cx = P0->cx; Gp->cy = P0->cy; Gp->crange = P0->crange;
     Gp->ytile = P0->ytile; Gp->ytile_single = P0->ytile_single; Gp->xwaves = P0->xwaves;
@@ -1745,6 +1749,9 @@ int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* probs, ibpm_group*
     GA(Gp->sc_cell, cell.size()); GA(Gp->sc_off, off.size()); GA(Gp->sc_f, ef.size()); GA(Gp->sc_w, ew.size());
 #undef GA
     hipStream_t s = Gp->stream;
+    // levels a member does not have stay zero (they are never read)
+    for (double* f : {Gp->G, Gp->Gs, Gp->psi, Gp->nl[0], Gp->nl[1]}) launch_fill(f, (long)fG, 0.0, s);
+    for (double* f : {Gp->q, Gp->q0}) launch_fill(f, (long)fq, 0.0, s);
     auto h2d = [&](void* d, const void* h, size_t bytes) {
         return bytes == 0 || hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
     };
@@ -1757,13 +1764,14 @@ int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* probs, ibpm_group*
     for (int i = 0; i < n && ok; i++) {
         ibpm_prob* P = probs[i];
         const long io = (long)i * L;
-        ip[i] = InstPar{P->beta[0] * P->dt, P->beta[1] * P->dt, P->dt, P->Re, P->q0vx, P->q0vy};
+        ip[i] = InstPar{P->beta[0] * P->dt, P->beta[1] * P->dt, P->dt, P->Re, P->q0vx, P->q0vy, P->mg, 0};
+        const long Li = P->mg;
         Gp->q0_uniform = Gp->q0_uniform && P->q0_uniform;
         // the member's state into its slices; nonlin[1] and nonlin[2] as roles 0 and 1
-        ok = d2d(Gp->G + io * nG, P->G, sizeof(double) * L * nG) && d2d(Gp->psi + io * nG, P->psi, sizeof(double) * L * nG) &&
-             d2d(Gp->nl[0] + io * nG, P->nl[P->nl1], sizeof(double) * L * nG) &&
-             d2d(Gp->nl[1] + io * nG, P->nl[P->nl2], sizeof(double) * L * nG) &&
-             d2d(Gp->q + io * nq, P->q, sizeof(double) * L * nq) && d2d(Gp->q0 + io * nq, P->q0, sizeof(double) * L * nq) &&
+        ok = d2d(Gp->G + io * nG, P->G, sizeof(double) * Li * nG) && d2d(Gp->psi + io * nG, P->psi, sizeof(double) * Li * nG) &&
+             d2d(Gp->nl[0] + io * nG, P->nl[P->nl1], sizeof(double) * Li * nG) &&
+             d2d(Gp->nl[1] + io * nG, P->nl[P->nl2], sizeof(double) * Li * nG) &&
+             d2d(Gp->q + io * nq, P->q, sizeof(double) * Li * nq) && d2d(Gp->q0 + io * nq, P->q0, sizeof(double) * Li * nq) &&
              d2d(Gp->Fb + ifoff[i], P->Fb, sizeof(double) * P->nf) &&
              d2d(Gp->BinvT + aoff[i], P->BinvT, sizeof(double) * P->nf * P->nf);
         // weights and indices of instance i's points at its global point offset
@@ -1808,11 +1816,12 @@ int ibpm_group_destroy(ibpm_group* Gp)
 {
     if (!Gp) return IBPM_OK;
     (void)hipStreamSynchronize(Gp->stream);
-    const long nG = Gp->g.nG, nq = Gp->g.nq, L = Gp->L;
+    const long nG = Gp->g.nG, nq = Gp->g.nq;
     int rc = IBPM_OK;
     for (int i = 0; i < Gp->n; i++) {
         ibpm_prob* P = Gp->probs[i];
         const ibpm_group::Own& o = Gp->own[i];
+        const long L = P->mg;  // the member's own levels
         // current state back into the member's own buffers, same roles
         const bool ok =
             hipMemcpy(o.G, P->G, sizeof(double) * L * nG, hipMemcpyDeviceToDevice) == hipSuccess &&

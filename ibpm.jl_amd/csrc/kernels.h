@@ -14,6 +14,8 @@ struct InstPar {
     double c1, c2;      // β1 dt, β2 dt
     double dt, Re;
     double q0vx, q0vy;  // inertial base flux per unit hc
+    int L;              // levels of the instance (a group keeps every instance at its largest count;
+    int pad;            //   entries of levels an instance does not have are skipped)
 };
 
 // ---- DST passes -----------------------------------------------------------
@@ -43,8 +45,10 @@ struct XArgs {
     // 14; anything else (0) gives the default 4
     int waves;
     // X_TRIAL over a group: entry e = lev0 + b is level e % levmod of
-    // instance e / levmod, whose c1, c2, dt, Re come from ipar (0 / null: one problem)
+    // instance e / levmod, whose c1, c2, dt, Re come from ipar (0 / null: one problem);
+    // fixlev: every entry is at level lev0 (a batch of fields, or instance b of a group)
     int levmod;
+    int fixlev;
     const InstPar* ipar;
     DstPlan pl;
     GridDev g;
@@ -110,8 +114,10 @@ struct CurlArgs {
     // l at psi + l * psl + b * psb
     int colbatch;
     long psl, psb;
-    // over a group (colbatch = 0): entry e = lev0 + b is level e % levmod
+    // over a group (colbatch = 0): entry e = lev0 + b is level e % levmod;
+    // ipar: the instances' level counts (colbatch: entry b is instance b)
     int levmod;
+    const InstPar* ipar;
     GridDev g;
 };
 void launch_curl(const CurlArgs& a, hipStream_t s);
@@ -122,6 +128,10 @@ struct CoarsifyArgs {
     int i0, ic0, ci, j0, jc0, cj;
     int nb;               // batch of fields (0 or 1: one), field b at Gc/G + b * bstride
     long bstride;
+    // a group: field b is instance b, which has level clev (1-based, the level
+    // written) only if clev <= ipar[b].L
+    const InstPar* ipar;
+    int clev;
     GridDev g;
 };
 void launch_coarsify(const CoarsifyArgs& a, hipStream_t s);

@@ -13,6 +13,26 @@ namespace ibpm {
 // DST x-pass: one wavefront per x-line (fixed j) of one level.
 // ===========================================================================
 constexpr int XWAVES = 4;  // wavefronts per workgroup
+
+// Batch entry b of a launch -> (instance, level).  One problem: entry b is
+// level lev0 + b (fixlev: level lev0 for every entry, a batch of fields).  A
+// group: entry e = lev0 + b is level e % levmod of instance e / levmod
+// (fixlev: instance b at level lev0).  `active` is false for a level the
+// instance does not have; `L` is its level count.
+struct Entry {
+    int inst, lev, L;
+    bool active;
+};
+__device__ __forceinline__ Entry batch_entry(int b, int lev0, int levmod, int fixlev, int mg, const InstPar* ip)
+{
+    Entry r;
+    const int e = lev0 + b;
+    r.inst = fixlev ? b : (levmod ? e / levmod : 0);
+    r.lev = fixlev ? lev0 : (levmod ? e % levmod : e);
+    r.L = ip ? ip[r.inst].L : mg;
+    r.active = r.lev < r.L;
+    return r;
+}
 constexpr int XPL = (MAXM + WAVE - 1) / WAVE;  // line elements per lane
 
 // Per-length line layout in an LDS slot.  PT = 512 (lines of 1023) uses the
@@ -77,6 +97,8 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
     const int line = blockIdx.x * XW + w;  // 0-based j - 1
     const int b = blockIdx.y;
     const int j = line + 1;
+    const Entry en = batch_entry(b, a.lev0, a.levmod, a.fixlev, a.mg, a.ipar);
+    if (!en.active) return;  // the whole work-group: a level its instance does not have
     const int ntab = tab_doubles_t<PT>(a.pl);
     const DstTab tb = load_tab_t<PT>(lds, a.pl, threadIdx.x, XW * WAVE);
     __syncthreads();
@@ -101,7 +123,9 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
     const bool rb = (j == 1), rt = (j == a.g.ny - 1);
     double bl = 0.0, br = 0.0, wr[XPL];
     const double* bcr = a.bcr + b * a.bcstride;
-    if constexpr (MODE == X_BC) {
+    // (in a group an instance's coarsest level has no boundary terms)
+    const bool xbc = (MODE == X_BC) && en.lev < en.L - 1;
+    if (xbc) {
         bl = a.fac * bc_col(bcr, a.g, a.g.nx / 4, 1 + j);
         br = a.fac * bc_col(bcr, a.g, 3 * a.g.nx / 4, 1 + j);
         if (rb || rt) {
@@ -120,7 +144,7 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
         int n = lane + WAVE * t;
         if (n < n1) {
             double x = v[t];
-            if constexpr (MODE == X_BC) {
+            if (xbc) {
                 if (rb || rt) x += wr[t];
                 if (rb && rt) x += a.fac * get_bc_val(bcr, a.g, (a.g.TOP + 1) + (n + 1));  // ny = 2 only
                 if (n == 0) x += bl;
@@ -153,13 +177,13 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
         // get_trial_state! (timestepping.jl:111-118): rhs = A Γ; rhs += (β1 dt) N1;
         // rhs += (β2 dt) N0; rhs += rhsbc, with rhsbc = apply_bc!(0, get_bc!(Γ[:, lev+1]), fac)
         const int e = a.lev0 + b;
-        const int lev = a.levmod ? e % a.levmod : e;
-        const bool hasbc = lev < a.mg - 1;
+        const int lev = en.lev;
+        const bool hasbc = lev < en.L - 1;
         const double* gc = a.gam + (long)(e + 1) * a.lstride;
         const double hc = level_hc(a.g.h, lev);
         double dt = a.dt, Re = a.Re, c1 = a.c1, c2 = a.c2;
         if (a.ipar) {
-            const InstPar& ip = a.ipar[e / a.levmod];
+            const InstPar& ip = a.ipar[en.inst];
             dt = ip.dt; Re = ip.Re; c1 = ip.c1; c2 = ip.c2;
         }
         const double fac = 0.25 * dt / (Re * (hc * hc));  // timestepping.jl:104
@@ -356,6 +380,8 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
     if (lin >= ntiles * a.nlev) return;
     const int b = lin / ntiles;
+    const Entry en = batch_entry(b, a.lev0, a.levmod, a.fixlev, 1 << 30, a.ipar);
+    if (!en.active) return;  // the whole work-group: a level its instance does not have
     const int i0 = (lin - b * ntiles) * TI;
     const int ld = slot_doubles_t<PT>(a.pl);
     double* base = a.buf + b * a.lstride;
@@ -400,14 +426,12 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
         }
     }
     __syncthreads();
-    const int e = a.lev0 + b;
-    const int lev = a.fixlev ? a.lev0 : (a.levmod ? e % a.levmod : e);
+    const int lev = en.lev;
     const double hcl = level_hc(a.g.h, lev);
     double dt = a.dt, Re = a.Re;
     if (a.ipar) {
-        const int inst = a.fixlev ? b : e / a.levmod;
-        dt = a.ipar[inst].dt;
-        Re = a.ipar[inst].Re;
+        dt = a.ipar[en.inst].dt;
+        Re = a.ipar[en.inst].Re;
     }
     const double qc = dt / ((2.0 * Re) * (hcl * hcl));  // dt / (2 Re h^2), laplacian.jl:69,74
     // (column, transform) pairs: transform 0 is followed by the eigenvalue
@@ -520,20 +544,22 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
     const int j0 = ((rem / nbx) * 4 + threadIdx.y) * NL_JS + 2;          // first output row, 2..ny
     if (i > nx || j0 > ny) return;
     const int j1 = min(j0 + NL_JS - 1, ny);
-    const int e = a.lev0 + b;                     // batch entry: its fields at e * nq, e * nG
-    const int lev = a.levmod ? e % a.levmod : e;  // its grid level
+    const int e = a.lev0 + b;  // batch entry: its fields at e * nq, e * nG
+    const Entry en = batch_entry(b, a.lev0, a.levmod, 0, a.mg, a.ipar);
+    if (!en.active) return;
+    const int lev = en.lev;    // its grid level
     const long nq = g.nq, nu = g.nu;
     const double* __restrict__ q = a.q + (long)e * nq;
     const double* __restrict__ q0 = a.q0 + (long)e * nq;
     const double* __restrict__ G = a.G + (long)e * g.nG;
-    const bool hasbc = lev < a.mg - 1;
+    const bool hasbc = lev < en.L - 1;
     const double* __restrict__ Gc = a.G + (long)(e + 1) * g.nG;
     const bool u0 = a.q0_uniform != 0;
     const double hcl = level_hc(g.h, lev);
     double q0vx = a.q0vx, q0vy = a.q0vy;
     if (a.ipar) {
-        q0vx = a.ipar[e / a.levmod].q0vx;
-        q0vy = a.ipar[e / a.levmod].q0vy;
+        q0vx = a.ipar[en.inst].q0vx;
+        q0vy = a.ipar[en.inst].q0vy;
     }
     const double cx0 = q0vx * hcl, cy0 = q0vy * hcl;  // as k_base_flux_inertial writes them
 
@@ -684,11 +710,13 @@ __global__ void __launch_bounds__(256) k_curl(CurlArgs a)
     // entry e = lev0 + b: level lev of its field; colbatch: fields at level
     // lev0 with levels psl and fields psb apart
     const int e = a.lev0 + b;
-    const int lev = a.colbatch ? a.lev0 : (a.levmod ? e % a.levmod : e);
+    const Entry en = batch_entry(b, a.lev0, a.levmod, a.colbatch, a.mg, a.ipar);
+    if (!en.active) return;
+    const int lev = en.lev;
     const long pe = a.colbatch ? (long)lev * a.psl + (long)b * a.psb : (long)e * g.nG;
     const long pl1 = a.colbatch ? a.psl : g.nG;
     const double* __restrict__ psi = a.psi + pe;
-    const bool hasbc = lev < a.mg - 1;
+    const bool hasbc = lev < en.L - 1;
     const double* __restrict__ pc = a.psi + pe + pl1;
     auto PS = [&](int ii, int jj) -> double { return psi[(ii - 1) + (long)n1 * (jj - 1)]; };
     auto BC = [&](int k) -> double { return hasbc ? get_bc_val(pc, g, k) : 0.0; };
@@ -733,6 +761,7 @@ __global__ void __launch_bounds__(256) k_coarsify(CoarsifyArgs a)
     const int lin0 = xcd_block(blockIdx.x, gridDim.x / NXCD);
     const int b = lin0 / (nbx * nby);
     if (b >= nbt) return;
+    if (a.ipar && a.clev > a.ipar[b].L) return;  // instance b has no level clev
     const int lin = lin0 - b * (nbx * nby);
     const int ai = (lin % nbx) * 64 + threadIdx.x;
     const int bj = (lin / nbx) * 4 + threadIdx.y;

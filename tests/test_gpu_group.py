@@ -64,10 +64,16 @@ def test_group_steps_equal_single_steps():
 
 
 def test_group_of_naca_instances_equals_single():
+    """Instances on the same finest grid with 3, 4 and 5 levels in one group:
+    the group keeps every instance at 5 levels and skips the levels an
+    instance does not have."""
     allc = naca64_cases()
-    key = (allc[0]["grid"].nx, allc[0]["grid"].ny, allc[0]["grid"].mg)
-    cs = [c for c in allc if (c["grid"].nx, c["grid"].ny, c["grid"].mg) == key][:4]
-    assert len(cs) >= 2
+    nx0 = allc[0]["grid"].nx
+    same = [c for c in allc if c["grid"].nx == nx0]
+    cs = []
+    for mg in (3, 4, 5):
+        cs += [c for c in same if c["grid"].mg == mg][:2]
+    assert sorted({c["grid"].mg for c in cs}) == [3, 4, 5]
 
     def build():
         out = []
@@ -84,6 +90,32 @@ def test_group_of_naca_instances_equals_single():
     for (cd, cl, _), (c1, l1, _), (c2, l2, _), (_, sta), (_, stb) in zip(alone, res, res2, a, b):
         assert np.array_equal(cd, np.vstack([c1, c2])) and np.array_equal(cl, np.vstack([l1, l2]))
         assert np.array_equal(sta.Γ, stb.Γ)
+    grp.close()
+
+
+def test_group_with_fewer_levels_member():
+    """A 2-level member beside 3-level ones (same finest grid)."""
+    pa, sa = members()
+    pg, sg = members([p.Binv for p in pa])
+    g2 = ibpm.make_grid(1.0 / 16, (-1.5, 2.5, -2.0, 2.0), mg=2)
+
+    def two_level():
+        p = ibpm.IBProblem(g2, [ibpm.make_cylinder(0.4, g2.h, 0.1, 0.0, n=30)], 0.01, 80.0,
+                           freestream={"Ux": lambda t: 1.0})
+        st = ibpm.IBState(p)
+        st.Γ = gaussian_vortex_gamma(g2, x0=0.7, y0=0.1, circ=0.5, width=0.3)
+        ibpm.vort2flux_(st, p)
+        return p, st
+
+    p2a, s2a = two_level()
+    p2g, s2g = two_level()
+    alone = [ibpm.advance_n_(st, p, np.arange(8) * p.scheme.dt) for p, st in zip(pa + [p2a], sa + [s2a])]
+    grp = ibpm.Group(pg + [p2g], sg + [s2g])
+    res = grp.advance_n_(8)
+    for (cd, cl, _), (cdg, clg, _), st, stg in zip(alone, res, sa + [s2a], sg + [s2g]):
+        assert np.array_equal(cd, cdg) and np.array_equal(cl, clg)
+        for x, y in zip(fields(st), fields(stg)):
+            assert np.array_equal(x, y)
     grp.close()
 
 
