@@ -2145,9 +2145,13 @@ int ibpm_time_kernel(ibpm_prob* P, int which, int reps, double* ms_avg)
             launch_curl(ca, P->stream);
         } else if (which == 7) {
             launch_copy(P->T2, P->T1, P->g.nG * L, P->stream);
+        } else if (which == 8 && L > 1) {
+            XArgs a = xargs(P);
+            a.src = P->G; a.dst = P->T1; a.nlev = 1; a.lev0 = 0; a.bcr = P->psi + P->g.nG;
+            launch_xpass(X_BC, a, P->stream);
         }
     };
-    if (which < 0 || which > 7 || which == 3) {
+    if (which < 0 || which > 8 || which == 3 || (which == 8 && L < 2)) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         return fail(P->ctx, IBPM_E_ARG, "unknown kernel id (whole steps are timed through ibpm_advance_n)");

@@ -129,10 +129,25 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
         bl = a.fac * bc_col(bcr, a.g, a.g.nx / 4, 1 + j);
         br = a.fac * bc_col(bcr, a.g, 3 * a.g.nx / 4, 1 + j);
         if (rb || rt) {
+            // bc_row of every element: all loads issued before any is used
+            const int jr = rb ? a.g.ny / 4 : 3 * a.g.ny / 4;
+            const double* rowp = bcr + (a.g.nx / 4 - 1) + (long)a.g.n1 * (jr - 1);
+            double wa[XPL], wb[XPL];
 #pragma unroll
             for (int t = 0; t < XPL; t++) {
-                int n = lane + WAVE * t;
-                if (n < n1) wr[t] = a.fac * bc_row(bcr, a.g, rb ? a.g.ny / 4 : 3 * a.g.ny / 4, n + 2);
+                const int n = lane + WAVE * t;
+                wa[t] = 0.0;
+                wb[t] = 0.0;
+                if (n < n1) {
+                    const double* p = rowp + (n + 2) / 2;  // bc_row(bcr, g, jr, n + 2)
+                    wa[t] = p[0];
+                    wb[t] = p[-1];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < XPL; t++) {
+                const int n = lane + WAVE * t;
+                wr[t] = a.fac * (((n + 2) & 1) ? wa[t] : 0.5 * (wa[t] + wb[t]));
             }
         }
     }
