@@ -75,6 +75,9 @@ struct ibpm_prob {
     double* nl[2] = {nullptr, nullptr};
     int nl1 = 0, nl2 = 0;  // buffers holding state.nonlin[1], state.nonlin[2]
     double *qs1 = nullptr, *T1 = nullptr, *T2 = nullptr, *Gtmp = nullptr, *qreg = nullptr;
+    // columns nx/4 and 3nx/4 of the last ψ level of the vort2flux! chain, as
+    // (L, R) pairs per row: the get_bc! input of the next finer level
+    double* edge = nullptr;
     double *Fb = nullptr, *Ftmp = nullptr, *fbv = nullptr, *CD = nullptr, *CL = nullptr, *scal = nullptr;
     double *BinvT = nullptr, *w = nullptr, *ds_d = nullptr;
     int *bidx = nullptr, *boff_d = nullptr;
@@ -99,6 +102,7 @@ struct ibpm_prob {
     // device (nf x nf column-major) and on the host
     int bk = 0;
     double *bq = nullptr, *bgt = nullptr, *bt1 = nullptr, *bG = nullptr, *bpsi = nullptr, *Bd = nullptr;
+    double* bedge = nullptr;  // edge columns of every field of a batch
     std::vector<double> Bh;
     int cg_iters = 0;
     double cg_res = 0.0, cg_tol = 0.0;
@@ -313,7 +317,6 @@ static XArgs xargs(ibpm_prob* P)
     a.mg = P->mg;
     a.pl = P->tx.plan;
     a.g = P->g;
-    a.fac = 1.0;
     a.zlo = 1;
     a.zhi = P->g.n2;
     a.waves = P->xwaves;
@@ -342,6 +345,8 @@ static YArgs yargs(ibpm_prob* P, int op, int lev0, int nlev, double* buf)
     a.dt = P->dt;
     a.Re = P->Re;
     a.scale = 1.0 / (double)(4L * P->g.nx * P->g.ny);
+    a.sinx = P->tx.sn;  // sin(pi m / nx), m <= nx/2, leads either plan kind's table
+    a.bcfac = 1.0;
     a.pl = P->ty.plan;
     a.g = P->g;
     return a;
@@ -430,9 +435,9 @@ static hipError_t upload_reg(ibpm_prob* P)
 }
 
 // x = L b for one level (lev 0-based), L one of Δinv / A / Ainv (laplacian.jl:24-76),
-// with optional apply_bc! prologue (bcr != null) and subtract epilogue.
-static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x, const double* bcr, bool sub,
-                      int zlo = 1, int zhi = 1 << 30)
+// with an optional subtract epilogue.
+static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x, bool sub, int zlo = 1,
+                      int zhi = 1 << 30)
 {
     hipStream_t s = P->stream;
     XArgs a = xargs(P);
@@ -442,8 +447,7 @@ static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x,
     a.dst = P->T1 + (long)lev * P->g.nG;
     a.nlev = 1;
     a.lev0 = lev;
-    a.bcr = bcr;
-    launch_xpass(bcr ? X_BC : X_PLAIN, a, s);
+    launch_xpass(X_PLAIN, a, s);
     launch_ypass(yargs(P, op, lev, 1, P->T1 + (long)lev * P->g.nG), P->ytile_single, s);
     XArgs c = xargs(P);
     c.src = P->T1 + (long)lev * P->g.nG;
@@ -470,10 +474,25 @@ static void vort2flux_dev(ibpm_prob* P, double* Gb, double* psib, double* qout, 
         c.g = P->g;
         launch_coarsify(c, s);
     }
-    lap_level(P, Y_DINV, L - 1, Gb + (long)(L - 1) * nG, psib + (long)(L - 1) * nG, nullptr, false);
-    for (int lev = L - 1; lev >= 1; lev--) {
-        int l = lev - 1;
-        lap_level(P, Y_DINV, l, Gb + (long)l * nG, psib + (long)l * nG, psib + (long)(l + 1) * nG, false);
+    // Δinv of every level, coarse to fine.  The x-transform of Γ does not
+    // depend on the chain, so all levels take it in one launch; the boundary
+    // terms of apply_bc! (from ψ of the level above) enter, x-transformed, in
+    // the y-pass; the inverse x-pass of a level stores the columns the next
+    // finer level's get_bc! reads.
+    XArgs a = xargs(P);
+    a.src = Gb; a.dst = P->T1; a.nlev = L; a.lev0 = 0;
+    launch_xpass(X_PLAIN, a, s);
+    for (int l = L - 1; l >= 0; l--) {
+        YArgs y = yargs(P, Y_DINV, l, 1, P->T1 + (long)l * nG);
+        if (l < L - 1) {
+            y.bcr = psib + (long)(l + 1) * nG;
+            y.edge = P->edge;
+        }
+        launch_ypass(y, P->ytile_single, s);
+        XArgs c = xargs(P);
+        c.src = P->T1 + (long)l * nG; c.dst = psib + (long)l * nG; c.nlev = 1; c.lev0 = l;
+        if (l > 0) c.edge = P->edge;
+        launch_xpass(X_PLAIN, c, s);
     }
     CurlArgs ca{};
     ca.psi = psib;
@@ -498,7 +517,7 @@ static void project_forces(ibpm_prob* P, const double* f, double* Gout, bool sub
     launch_scatter(sa, s);
     launch_rot(P->qreg, P->Gtmp, P->rb_i0, P->rb_i1, P->rb_j0, P->rb_j1, P->g, s);
     // Gtmp is zero outside the rows rb_j0 - 1 .. rb_j1 - 1 of the rot! box
-    lap_level(P, Y_AINV, 0, P->Gtmp, Gout, nullptr, sub, P->rb_j0 - 1, P->rb_j1 - 1);
+    lap_level(P, Y_AINV, 0, P->Gtmp, Gout, sub, P->rb_j0 - 1, P->rb_j1 - 1);
 }
 
 // An all-level y-pass of the step, bracketed by a pair of events on the
@@ -693,19 +712,24 @@ static int assemble_b(ibpm_prob* P)
             launch_coarsify(cr, s);
         }
         for (int l = L - 1; l >= 0; l--) {
-            const bool bc = l < L - 1;
             XArgs a1 = xargs(P);
             a1.src = P->bG + (long)l * K * nG; a1.dst = P->bt1; a1.nlev = kc; a1.lev0 = l; a1.fixlev = 1;
-            if (bc) {
-                a1.bcr = P->bpsi + (long)(l + 1) * K * nG;
-                a1.bcstride = nG;
-            }
-            launch_xpass(bc ? X_BC : X_PLAIN, a1, s);
+            launch_xpass(X_PLAIN, a1, s);
             YArgs y1 = yargs(P, Y_DINV, l, kc, P->bt1);
             y1.fixlev = 1;
+            if (l < L - 1) {  // apply_bc! from ψ of level l + 1, in the y-pass
+                y1.bcr = P->bpsi + (long)(l + 1) * K * nG;
+                y1.bcstride = nG;
+                y1.edge = P->bedge;
+                y1.edstride = 2L * g.n2;
+            }
             launch_ypass(y1, P->ytile, s);
             XArgs c1 = xargs(P);
             c1.src = P->bt1; c1.dst = P->bpsi + (long)l * K * nG; c1.nlev = kc; c1.lev0 = l; c1.fixlev = 1;
+            if (l > 0) {
+                c1.edge = P->bedge;
+                c1.edstride = 2L * g.n2;
+            }
             launch_xpass(X_PLAIN, c1, s);
         }
         CurlArgs ca{};
@@ -1047,6 +1071,7 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
     DA(P->q, nq * mg); DA(P->q0, nq * mg); DA(P->G, nG * mg); DA(P->Gs, nG * mg); DA(P->psi, nG * mg);
     DA(P->nl[0], nG * mg); DA(P->nl[1], nG * mg);
     DA(P->qs1, nq); DA(P->T1, nG * mg); DA(P->T2, nG * mg); DA(P->Gtmp, nG); DA(P->qreg, nq);
+    DA(P->edge, 2 * (long)g.n2);
     DA(P->Fb, P->nf); DA(P->Ftmp, P->nf); DA(P->fbv, P->nf); DA(P->CD, P->nbodies); DA(P->CL, P->nbodies);
     DA(P->scal, 16); DA(P->slot, 1);
     DA(P->BinvT, mf ? 1 : (size_t)P->nf * P->nf); DA(P->w, P->weight.size()); DA(P->Fw, P->nf);
@@ -1074,6 +1099,7 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
     // scratch fields whose untouched entries must stay zero
     launch_fill(P->qreg, nq, 0.0, P->stream);
     launch_fill(P->Gtmp, nG, 0.0, P->stream);
+    launch_fill(P->edge, 2L * g.n2, 0.0, P->stream);
     zero_state(P);
     if (mf && P->nf > 0) {
         // pinned staging of the force solve's right-hand side and F̃b
@@ -1088,6 +1114,7 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
         if ((e = dalloc(P, &(ptr), (size_t)(n))) != hipSuccess)                 \
             return bail(IBPM_E_HIP, std::string("hipMalloc failed for " #ptr));
         DB(P->bq, K * nq); DB(P->bgt, K * nG); DB(P->bt1, K * nG); DB(P->bG, K * mg * nG); DB(P->bpsi, K * mg * nG);
+        DB(P->bedge, K * 2 * (size_t)g.n2);
         DB(P->Bd, (size_t)P->nf * P->nf);
 #undef DB
         P->Bh.assign((size_t)P->nf * P->nf, 0.0);
@@ -1441,6 +1468,7 @@ struct ibpm_group {
     std::vector<CoarsifyRange> crange;
     double *G = nullptr, *Gs = nullptr, *psi = nullptr, *nl[2] = {nullptr, nullptr}, *q = nullptr, *q0 = nullptr;
     double *qs1 = nullptr, *T1 = nullptr, *T2 = nullptr, *Gt = nullptr, *qreg = nullptr;
+    double* edge = nullptr;  // per instance: the vort2flux! chain's get_bc! columns (see ibpm_prob::edge)
     int nl1 = 0, nl2 = 1;
     bool q0_uniform = true;
     // bodies: all instances' points and F̃b entries concatenated
@@ -1491,7 +1519,6 @@ XArgs gxargs(const ibpm_group* Gp)
     a.mg = Gp->L;
     a.pl = Gp->tx.plan;
     a.g = Gp->g;
-    a.fac = 1.0;
     a.zlo = 1;
     a.zhi = Gp->g.n2;
     a.waves = Gp->xwaves;
@@ -1507,6 +1534,8 @@ YArgs gyargs(const ibpm_group* Gp, int op, double* buf)
     a.cx = Gp->cx;
     a.cy = Gp->cy;
     a.scale = 1.0 / (double)(4L * Gp->g.nx * Gp->g.ny);
+    a.sinx = Gp->tx.sn;
+    a.bcfac = 1.0;
     a.ipar = Gp->ipar;
     a.pl = Gp->ty.plan;
     a.g = Gp->g;
@@ -1529,22 +1558,28 @@ void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, boo
         c.nb = n; c.bstride = ist; c.g = Gp->g; c.ipar = Gp->ipar; c.clev = lev;
         launch_coarsify(c, s);
     }
+    // forward x-transform of every level of every instance in one launch,
+    // then the chain (boundary terms in the y-pass, as in vort2flux_dev)
+    XArgs a = gxargs(Gp);
+    a.src = Gb; a.dst = Gp->T1; a.nlev = n * L; a.lev0 = 0; a.levmod = L;
+    launch_xpass(X_PLAIN, a, s);
     for (int l = L - 1; l >= 0; l--) {
-        const bool bc = l < L - 1;
-        XArgs a = gxargs(Gp);
-        a.src = Gb + (long)l * nG; a.dst = Gp->T1 + (long)l * nG; a.lstride = ist; a.nlev = n; a.lev0 = l;
-        a.fixlev = 1;
-        if (bc) {
-            a.bcr = psib + (long)(l + 1) * nG;
-            a.bcstride = ist;
-        }
-        launch_xpass(bc ? X_BC : X_PLAIN, a, s);
         YArgs y = gyargs(Gp, Y_DINV, Gp->T1 + (long)l * nG);
         y.lstride = ist; y.nlev = n; y.lev0 = l; y.fixlev = 1;
+        if (l < L - 1) {  // (an instance whose coarsest level is l has none: the kernel checks)
+            y.bcr = psib + (long)(l + 1) * nG;
+            y.bcstride = ist;
+            y.edge = Gp->edge;
+            y.edstride = 2L * Gp->g.n2;
+        }
         launch_ypass(y, Gp->ytile_single, s);
         XArgs c = gxargs(Gp);
         c.src = Gp->T1 + (long)l * nG; c.dst = psib + (long)l * nG; c.lstride = ist; c.nlev = n; c.lev0 = l;
         c.fixlev = 1;
+        if (l > 0) {
+            c.edge = Gp->edge;
+            c.edstride = 2L * Gp->g.n2;
+        }
         launch_xpass(X_PLAIN, c, s);
     }
     CurlArgs ca{};
@@ -1741,7 +1776,7 @@ int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* probs, ibpm_group*
         return bail(IBPM_E_HIP, std::string("hipMalloc failed for group " #ptr));
     GA(Gp->G, fG); GA(Gp->Gs, fG); GA(Gp->psi, fG); GA(Gp->nl[0], fG); GA(Gp->nl[1], fG);
     GA(Gp->q, fq); GA(Gp->q0, fq); GA(Gp->qs1, (size_t)n * nq); GA(Gp->T1, fG); GA(Gp->T2, fG); GA(Gp->Gt, fG);
-    GA(Gp->qreg, (size_t)n * nq);
+    GA(Gp->qreg, (size_t)n * nq); GA(Gp->edge, (size_t)n * 2 * Gp->g.n2);
     GA(Gp->Fb, Gp->NF); GA(Gp->Ftmp, Gp->NF); GA(Gp->BinvT, atot); GA(Gp->w, (size_t)2 * Gp->NBT * 169);
     GA(Gp->bidx, 2 * (size_t)Gp->NBT); GA(Gp->pinst, Gp->NBT); GA(Gp->plocal, Gp->NBT); GA(Gp->inbt, n);
     GA(Gp->ifoff, n); GA(Gp->inf, n); GA(Gp->rinst, Gp->NF); GA(Gp->rloc, Gp->NF); GA(Gp->slots, n);
@@ -1951,7 +1986,7 @@ int ibpm_op_lapinv(ibpm_prob* P, int kind, int lev, const double* b, double* x)
     double* db = io.put(b, n);
     double* dx = io.put(nullptr, n);
     if (!db || !dx) return fail(P->ctx, IBPM_E_HIP, "alloc");
-    lap_level(P, kind == 0 ? Y_DINV : (kind == 1 ? Y_A : Y_AINV), lev - 1, db, dx, nullptr, false);
+    lap_level(P, kind == 0 ? Y_DINV : (kind == 1 ? Y_A : Y_AINV), lev - 1, db, dx, false);
     return get_back(P, x, dx, n);
 }
 
@@ -2146,9 +2181,10 @@ int ibpm_time_kernel(ibpm_prob* P, int which, int reps, double* ms_avg)
         } else if (which == 7) {
             launch_copy(P->T2, P->T1, P->g.nG * L, P->stream);
         } else if (which == 8 && L > 1) {
-            XArgs a = xargs(P);
-            a.src = P->G; a.dst = P->T1; a.nlev = 1; a.lev0 = 0; a.bcr = P->psi + P->g.nG;
-            launch_xpass(X_BC, a, P->stream);
+            YArgs y = yargs(P, Y_DINV, 0, 1, P->T1);
+            y.bcr = P->psi + P->g.nG;
+            y.edge = P->edge;
+            launch_ypass(y, P->ytile_single, P->stream);
         }
     };
     if (which < 0 || which > 8 || which == 3 || (which == 8 && L < 2)) {

@@ -733,6 +733,67 @@ __device__ inline DstTab load_dst_tab_1024(double* lds, const DstPlan& pl, int t
     return t;
 }
 
+// The same tables in two phases, so that a thread has all its global loads
+// in flight at once (a copy loop that stores each value as it arrives waits
+// for every load in turn): tab1024_issue loads a thread's share into
+// registers, tab1024_store writes it to LDS.  NT threads take part; other
+// loads may be issued between the two calls.
+template <int NT>
+struct Tab1024Regs {
+    static constexpr int A = (f1024::P + NT - 1) / NT;          // tw, wm, mirrored-sine entries
+    static constexpr int B = (f1024::M / 2 + 1 + NT - 1) / NT;  // sine entries
+    double2 tw[A], wm[A];
+    double sm[A], sn[B];
+};
+
+template <int NT>
+__device__ __forceinline__ void tab1024_issue(Tab1024Regs<NT>& R, const DstPlan& pl, int tid)
+{
+    using namespace f1024;
+    // every register is loaded (indices past the end are clamped and their
+    // values never stored), which keeps the arrays out of scratch memory
+#pragma unroll
+    for (int q = 0; q < Tab1024Regs<NT>::A; q++) {
+        const int i = min(tid + NT * q, P - 1);
+        const int e = i < 64 ? 8 * (i & 7) * (i >> 3) : (i & 63) * (i >> 6);
+        const int m = P / 2 + (i >> 1);
+        R.tw[q] = pl.tw[e];
+        R.wm[q] = pl.wm[i];
+        R.sm[q] = pl.sn[M - 2 * m - (i & 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < Tab1024Regs<NT>::B; q++) R.sn[q] = pl.sn[min(tid + NT * q, M / 2)];
+}
+
+template <int NT>
+__device__ __forceinline__ DstTab tab1024_store(double* lds, const Tab1024Regs<NT>& R, int tid)
+{
+    using namespace f1024;
+    double2* tw = reinterpret_cast<double2*>(lds);
+    double2* wm = tw + P;
+    double* sn = reinterpret_cast<double*>(wm + TWP);
+    double* snm = sn + SNM;
+#pragma unroll
+    for (int q = 0; q < Tab1024Regs<NT>::A; q++) {
+        const int i = tid + NT * q;
+        if (i < P) {
+            tw[i] = R.tw[q];
+            wm[i + (i >> 3)] = R.wm[q];
+            snm[i] = R.sm[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Tab1024Regs<NT>::B; q++) {
+        const int i = tid + NT * q;
+        if (i <= M / 2) sn[i] = R.sn[q];
+    }
+    DstTab t;
+    t.tw = tw;
+    t.wm = wm;
+    t.sn = sn;
+    return t;
+}
+
 // exclusive prefix sum over the lanes (lane 0 gets 0): wave_shr:1, then scan
 __device__ __forceinline__ double wave_excl_scan(double v)
 {

@@ -19,7 +19,7 @@ struct InstPar {
 };
 
 // ---- DST passes -----------------------------------------------------------
-enum XMode { X_PLAIN = 0, X_BC = 1, X_SUB = 2, X_TRIAL = 3 };
+enum XMode { X_PLAIN = 0, X_SUB = 2, X_TRIAL = 3 };
 
 struct XArgs {
     const double* src;
@@ -28,10 +28,11 @@ struct XArgs {
     int nlev;             // batch size
     int lev0;             // 0-based grid level of batch entry 0
     int mg;
-    // X_BC: apply_bc!(line, get_bc!(bcr), fac) prologue (one level)
-    const double* bcr;
-    long bcstride;        // X_BC: element stride of bcr between batch entries (0: one bcr for all)
-    double fac;
+    // X_PLAIN: when edge != null, line j (1-based) also stores its values at
+    // i = nx/4 and 3nx/4 to edge[2 (j - 1)] and edge[2 (j - 1) + 1] (entry b at
+    // edge + b * edstride): the columns get_bc! of the next finer level samples
+    double* edge;
+    long edstride;
     // X_TRIAL: rhs = ((A Γ + c1 N1) + c2 N0) + rhsbc, then a second x-DST
     const double* n1;
     const double* n0;
@@ -69,6 +70,18 @@ struct YArgs {
     double dt;
     double Re;            // divisor (2 Re) hc^2 is formed per level in the kernel
     double scale;         // 1 / (4 nx ny)
+    // Y_DINV with boundary terms: the x-transform of apply_bc!(get_bc!(ψ of
+    // the next coarser level), bcfac), added to the x-transformed lines before
+    // the first y-transform (entries whose level is the instance's coarsest
+    // have none).  bcr: that coarser level (entry b at bcr + b * bcstride);
+    // edge: its columns nx/4, 3nx/4 as written by the X_PLAIN edge output
+    // (entry b at edge + b * edstride); sinx: sin(pi m / nx), m = 0..nx/2.
+    // bcr == null: no boundary terms.
+    const double* bcr;
+    const double* edge;
+    long bcstride, edstride;
+    double bcfac;
+    const double* sinx;
     // over a group: entry e = lev0 + b is level e % levmod of instance
     // e / levmod (fixlev: entry b is instance b at level lev0); dt and Re of
     // the instance from ipar (0 / null: one problem)

@@ -55,11 +55,16 @@ __host__ __device__ inline int tab_doubles_t(const DstPlan& pl)
     if constexpr (PT == 512) return f1024::TAB;
     else return dst_tab_doubles(pl);
 }
-template <int PT>
-__device__ __forceinline__ DstTab load_tab_t(double* lds, const DstPlan& pl, int tid, int nthreads)
+template <int PT, int NT>
+__device__ __forceinline__ DstTab load_tab_t(double* lds, const DstPlan& pl, int tid)
 {
-    if constexpr (PT == 512) return load_dst_tab_1024(lds, pl, tid, nthreads);
-    else return load_dst_tab(lds, pl, tid, nthreads);
+    if constexpr (PT == 512) {
+        Tab1024Regs<NT> R;
+        tab1024_issue(R, pl, tid);
+        return tab1024_store(lds, R, tid);
+    } else {
+        return load_dst_tab(lds, pl, tid, NT);
+    }
 }
 // line length: 1023 for PT = 512 (a compile-time constant), else the grid's
 template <int PT>
@@ -100,13 +105,14 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
     const Entry en = batch_entry(b, a.lev0, a.levmod, a.fixlev, a.mg, a.ipar);
     if (!en.active) return;  // the whole work-group: a level its instance does not have
     const int ntab = tab_doubles_t<PT>(a.pl);
-    const DstTab tb = load_tab_t<PT>(lds, a.pl, threadIdx.x, XW * WAVE);
+    const DstTab tb = load_tab_t<PT, XW * WAVE>(lds, a.pl, threadIdx.x);
     __syncthreads();
     if (line >= n2) return;
     if constexpr (MODE == X_PLAIN) {
         if (j < a.zlo || j > a.zhi) {  // zero line: its transform is zero
             double* dz = a.dst + b * a.lstride + (long)line * n1;
             for (int n = lane; n < n1; n += WAVE) dz[n] = 0.0;
+            if (a.edge && lane < 2) a.edge[b * a.edstride + 2 * line + lane] = 0.0;
             return;
         }
     }
@@ -118,55 +124,13 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
         int n = lane + WAVE * t;
         if (n < n1) v[t] = src[n];
     }
-    // apply_bc! (mg-utils.jl:43-60) terms: row terms (j = 1, j = ny-1), then
-    // column terms (i = 1, i = nx-1); fac * get_bc!(ψ[:, lev+1])
-    const bool rb = (j == 1), rt = (j == a.g.ny - 1);
-    double bl = 0.0, br = 0.0, wr[XPL];
-    const double* bcr = a.bcr + b * a.bcstride;
-    // (in a group an instance's coarsest level has no boundary terms)
-    const bool xbc = (MODE == X_BC) && en.lev < en.L - 1;
-    if (xbc) {
-        bl = a.fac * bc_col(bcr, a.g, a.g.nx / 4, 1 + j);
-        br = a.fac * bc_col(bcr, a.g, 3 * a.g.nx / 4, 1 + j);
-        if (rb || rt) {
-            // bc_row of every element: all loads issued before any is used
-            const int jr = rb ? a.g.ny / 4 : 3 * a.g.ny / 4;
-            const double* rowp = bcr + (a.g.nx / 4 - 1) + (long)a.g.n1 * (jr - 1);
-            double wa[XPL], wb[XPL];
-#pragma unroll
-            for (int t = 0; t < XPL; t++) {
-                const int n = lane + WAVE * t;
-                wa[t] = 0.0;
-                wb[t] = 0.0;
-                if (n < n1) {
-                    const double* p = rowp + (n + 2) / 2;  // bc_row(bcr, g, jr, n + 2)
-                    wa[t] = p[0];
-                    wb[t] = p[-1];
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < XPL; t++) {
-                const int n = lane + WAVE * t;
-                wr[t] = a.fac * (((n + 2) & 1) ? wa[t] : 0.5 * (wa[t] + wb[t]));
-            }
-        }
-    }
     const int ld = slot_doubles_t<PT>(a.pl);
     double* s = lds + ntab + w * ld;
     if (lane == 0) set_guards<PT>(s);
 #pragma unroll
     for (int t = 0; t < XPL; t++) {
         int n = lane + WAVE * t;
-        if (n < n1) {
-            double x = v[t];
-            if (xbc) {
-                if (rb || rt) x += wr[t];
-                if (rb && rt) x += a.fac * get_bc_val(bcr, a.g, (a.g.TOP + 1) + (n + 1));  // ny = 2 only
-                if (n == 0) x += bl;
-                if (n == n1 - 1) x += br;
-            }
-            s[lat<PT>(n)] = x;
-        }
+        if (n < n1) s[lat<PT>(n)] = v[t];
     }
     if constexpr (MODE == X_TRIAL) {
         // N1 and N0 of the rhs are in flight during the first transform
@@ -202,6 +166,7 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
             dt = ip.dt; Re = ip.Re; c1 = ip.c1; c2 = ip.c2;
         }
         const double fac = 0.25 * dt / (Re * (hc * hc));  // timestepping.jl:104
+        const bool rb = (j == 1), rt = (j == a.g.ny - 1);
         double cl = 0.0, cr = 0.0;
         if (hasbc) {
             cl = fac * bc_col(gc, a.g, a.g.nx / 4, 1 + j);
@@ -252,6 +217,11 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
         for (int t = 0; t < XPL; t++) {
             int n = lane + WAVE * t;
             if (n < n1) dst[n] = s[lat<PT>(n)];
+        }
+        if (MODE == X_PLAIN && a.edge && lane < 2) {
+            // the two columns get_bc! samples for the next finer level, beside each other
+            const int ie = (lane == 0 ? a.g.nx / 4 : 3 * a.g.nx / 4) - 1;
+            a.edge[b * a.edstride + 2 * line + lane] = s[lat<PT>(ie)];
         }
     }
 }
@@ -317,7 +287,6 @@ void launch_xpass(int mode, const XArgs& a, hipStream_t s)
 {
     switch (mode) {
     case X_PLAIN: launch_x_m<X_PLAIN>(a, s); break;
-    case X_BC: launch_x_m<X_BC>(a, s); break;
     case X_SUB: launch_x_m<X_SUB>(a, s); break;
     default: launch_x_m<X_TRIAL>(a, s); break;
     }
@@ -382,7 +351,9 @@ __host__ __device__ inline int cy_doubles_t(int n2)
     else return (n2 + 1) & ~1;
 }
 
-template <int TI, int PT, int YW>
+// BC: the launch carries boundary terms (a.bcr != null); a separate instance
+// so that launches without them keep their register budget
+template <int TI, int PT, int YW, bool BC>
 __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
 {
     extern __shared__ double lds[];
@@ -417,22 +388,153 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     const bool colok = ic < n1;
     // LDS: [DST tables][cos(pi j/ny) table][TI column slots]
     const int ntab = tab_doubles_t<PT>(a.pl);
-    const DstTab tb = load_tab_t<PT>(lds, a.pl, tid, YW * WAVE);
+    // the 1023-point tables and the cos table are loaded in two phases around
+    // the boundary and tile loads below (all in flight together)
+    constexpr int NT = YW * WAVE;
+    constexpr int CYN = (f1024::M - 1 + NT - 1) / NT;
+    Tab1024Regs<NT> TR;
+    double cyr[CYN];
+    DstTab tb{};
     double* cyl = lds + ntab;
     if constexpr (PT == 512) {
-        for (int k = tid; k < n2; k += YW * WAVE) cyl[f1024::gsl(k + 1)] = a.cy[k];
-        if (tid == 0) cyl[f1024::gsl(0)] = 1.0;
+        tab1024_issue(TR, a.pl, tid);
+#pragma unroll
+        for (int q = 0; q < CYN; q++) cyr[q] = a.cy[min(tid + NT * q, n2 - 1)];
     } else {
-        for (int k = tid; k < n2; k += YW * WAVE) cyl[k] = a.cy[k];
+        tb = load_tab_t<PT, NT>(lds, a.pl, tid);
+        for (int k = tid; k < n2; k += NT) cyl[k] = a.cy[k];
     }
     double* tile = cyl + cy_doubles_t<PT>(n2);
     if (tid < TI) set_guards<PT>(tile + tid * ld);
-    for (int rb = 0; rb < n2; rb += RPI * CH) {
-        double v[CH];
+    // apply_bc! (mg-utils.jl:43-60) with get_bc! (mg-utils.jl:7-32) of the next
+    // coarser level, carried through the x-transform.  With S(t) = sin(pi t/nx),
+    // C(t) = cos(pi t/nx), line k of the x-transformed field gets
+    //   2 fac S(k) (bcL(j) + (-1)^(k+1) bcR(j))          at every row j (columns i = 1, nx-1),
+    //   2 fac sum_i S(i k) bcB(i)  at j = 1,  2 fac sum_i S(i k) bcT(i)  at j = ny-1,
+    // where, for the coarse row c_0..c_{nx/2} that get_bc! interpolates,
+    //   sum_i S(i k) bc(i) = (1 + C(k)) sum_{m=1}^{nx/2-1} S(2 m k) c_m + S(k) (c_0 + (-1)^(k+1) c_{nx/2}) / 2
+    // (the odd samples are neighbour means and S(a - b) + S(a + b) = 2 S(a) C(b)).
+    const int nx = a.g.nx;
+    const bool bc = BC && en.lev < en.L - 1;
+    // LDS after the tile slots: the coarse (L, R) column pairs of lines
+    // ny/4 - 1 .. 3ny/4 - 1
+    double2* edl = reinterpret_cast<double2*>(tile + TI * ld);
+    const int nep = a.g.ny / 2 + 1;
+    constexpr int EPT = (MAXM / 2 + 1 + YW * WAVE - 1) / (YW * WAVE);  // pairs staged per thread
+    constexpr int NCM = (MAXM / 2 + WAVE - 1) / WAVE;                  // coarse-row samples per lane
+    double2 ep[EPT];
+    double addB[NCOL], addT[NCOL];  // row terms of this wave's columns
+    double cB[NCM], cT[NCM], sv[NCOL][NCM], cB0 = 0.0, cT0 = 0.0, cBn = 0.0, cTn = 0.0, Sk[NCOL];
+    unsigned neg[NCOL];  // bit q: S(2 m k) is minus the table entry
+    double ek = 0.0, esg = 0.0;
+    // (1) every load of the boundary data, issued before the tile's so that
+    // the row sums wait for these alone while the tile is in flight
+    if (bc) {
+        const double2* E2 = reinterpret_cast<const double2*>(a.edge + b * a.edstride) + (a.g.ny / 4 - 1);
 #pragma unroll
-        for (int t = 0; t < CH; t++) {
-            int r = rb + r0 + RPI * t;
-            v[t] = (colok && r < n2) ? base[ic + (long)n1 * r] : 0.0;
+        for (int q = 0; q < EPT; q++) {
+            const int e = tid + YW * WAVE * q;
+            ep[q] = e < nep ? E2[e] : make_double2(0.0, 0.0);
+        }
+        const double* rB = a.bcr + b * a.bcstride + (long)n1 * (a.g.ny / 4 - 1) + (nx / 4 - 1);
+        const double* rT = a.bcr + b * a.bcstride + (long)n1 * (3 * a.g.ny / 4 - 1) + (nx / 4 - 1);
+#pragma unroll
+        for (int q = 0; q < NCM; q++) {
+            const int m = lane + 1 + WAVE * q;
+            cB[q] = m < nx / 2 ? rB[m] : 0.0;
+            cT[q] = m < nx / 2 ? rT[m] : 0.0;
+        }
+        cB0 = rB[0]; cT0 = rT[0]; cBn = rB[nx / 2]; cTn = rT[nx / 2];
+        if (colok) {
+            const int k = ic + 1;
+            ek = a.sinx[min(k, nx - k)];  // scaled by 2 fac once loaded (below)
+            esg = (k & 1) ? 1.0 : -1.0;
+        }
+        // S(2 m k) for the row sums of line k = i0 + c + 1, c = w + YW kk:
+        // sin(2 pi u / nx) = sign * sinx[min(t, nx - t)], u = m k mod nx, t = 2u mod nx
+#pragma unroll
+        for (int kk = 0; kk < NCOL; kk++) {
+            const int k = min(i0 + w + YW * kk, n1 - 1) + 1;
+            Sk[kk] = a.sinx[min(k, nx - k)];
+            const int step = (WAVE * k) % nx;
+            int u = ((lane + 1) * k) % nx;
+            neg[kk] = 0u;
+#pragma unroll
+            for (int q = 0; q < NCM; q++) {
+                const int t2 = 2 * u;
+                const int tt = t2 < nx ? t2 : t2 - nx;
+                sv[kk][q] = a.sinx[min(tt, nx - tt)];  // the sign is applied once loaded
+                neg[kk] |= (t2 >= nx ? 1u : 0u) << q;
+                u += step;
+                if (u >= nx) u -= nx;
+            }
+        }
+    }
+    // (2) the first chunk of the tile
+    double v[CH];
+#pragma unroll
+    for (int t = 0; t < CH; t++) {
+        int r = r0 + RPI * t;
+        v[t] = (colok && r < n2) ? base[ic + (long)n1 * r] : 0.0;
+    }
+    if constexpr (PT == 512) {
+        tb = tab1024_store(lds, TR, tid);
+#pragma unroll
+        for (int q = 0; q < CYN; q++) {
+            const int k = tid + NT * q;
+            if (k < n2) cyl[f1024::gsl(k + 1)] = cyr[q];
+        }
+        if (tid == 0) cyl[f1024::gsl(0)] = 1.0;
+    }
+    // (3) row sums and the staged column pairs
+    if (bc) {
+        const double f2 = 2.0 * a.bcfac;
+#pragma unroll
+        for (int kk = 0; kk < NCOL; kk++) {
+            const int k = min(i0 + w + YW * kk, n1 - 1) + 1;
+            double sB = 0.0, sT = 0.0;
+#pragma unroll
+            for (int q = 0; q < NCM; q++) {
+                const double x = ((neg[kk] >> q) & 1u) ? -sv[kk][q] : sv[kk][q];
+                sB += x * cB[q];
+                sT += x * cT[q];
+            }
+            sB = wave_last(wave_incl_scan(sB, lane));
+            sT = wave_last(wave_incl_scan(sT, lane));
+            const double sgk = (k & 1) ? 1.0 : -1.0;
+            addB[kk] = f2 * ((1.0 + cxw[kk]) * sB + 0.5 * Sk[kk] * (cB0 + sgk * cBn));
+            addT[kk] = f2 * ((1.0 + cxw[kk]) * sT + 0.5 * Sk[kk] * (cT0 + sgk * cTn));
+        }
+        ek *= 2.0 * a.bcfac;
+#pragma unroll
+        for (int q = 0; q < EPT; q++) {
+            const int e = tid + YW * WAVE * q;
+            if (e < nep) edl[e] = ep[q];
+        }
+        __syncthreads();
+    }
+    // (4) the tile into LDS, with the column terms
+    for (int rb = 0; rb < n2; rb += RPI * CH) {
+        if (rb > 0) {
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                int r = rb + r0 + RPI * t;
+                v[t] = (colok && r < n2) ? base[ic + (long)n1 * r] : 0.0;
+            }
+        }
+        if (bc) {
+            // get_bc! sample idx = j + 1 = r + 2 of the left and right columns
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                const int r = rb + r0 + RPI * t;
+                if (r < n2) {
+                    const int idx = r + 2;
+                    const double2 ea = edl[idx / 2], eb = edl[idx / 2 - 1];
+                    const double bl = (idx & 1) ? ea.x : 0.5 * (ea.x + eb.x);
+                    const double br = (idx & 1) ? ea.y : 0.5 * (ea.y + eb.y);
+                    v[t] += ek * (bl + esg * br);
+                }
+            }
         }
 #pragma unroll
         for (int t = 0; t < CH; t++) {
@@ -460,6 +562,20 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
 #pragma unroll
         for (int k = 1; k < NCOL; k++)
             if ((it >> 1) == k) cxk = cxw[k];
+        if (bc && (it & 1) == 0) {
+            double ab = addB[0], at = addT[0];
+#pragma unroll
+            for (int k = 1; k < NCOL; k++)
+                if ((it >> 1) == k) {
+                    ab = addB[k];
+                    at = addT[k];
+                }
+            if (lane == 0) {
+                s[lat<PT>(0)] += ab;
+                s[lat<PT>(n2 - 1)] += at;
+            }
+            wave_sync();
+        }
         if constexpr (PT == 512) {
             dst1024_line(s, tb, lane, YOut1024{s, cyl, cxk, qc, a.scale, (it & 1) == 0 ? a.op : -1});
         } else {
@@ -503,11 +619,13 @@ size_t ypass_lds_bytes(const DstPlan& pl, int n2, int tile)
 template <int TI, int PT, int YW>
 static void launch_y_t(const YArgs& a, hipStream_t s)
 {
-    static bool once = (allow_big_lds(k_ypass<TI, PT, YW>), true);
+    static bool once = (allow_big_lds(k_ypass<TI, PT, YW, false>), allow_big_lds(k_ypass<TI, PT, YW, true>), true);
     (void)once;
     const int nblocks = ((a.g.n1 + TI - 1) / TI) * a.nlev;
     dim3 grid(NXCD * xcd_nper(nblocks));
-    hipLaunchKernelGGL((k_ypass<TI, PT, YW>), grid, dim3(YW * WAVE), ypass_lds_bytes_t<PT>(a.pl, a.g.n2, TI), s, a);
+    const size_t lds = ypass_lds_bytes_t<PT>(a.pl, a.g.n2, TI) + (a.bcr ? sizeof(double2) * (size_t)(a.g.ny / 2 + 1) : 0);
+    if (a.bcr) hipLaunchKernelGGL((k_ypass<TI, PT, YW, true>), grid, dim3(YW * WAVE), lds, s, a);
+    else hipLaunchKernelGGL((k_ypass<TI, PT, YW, false>), grid, dim3(YW * WAVE), lds, s, a);
 }
 
 template <int TI, int YW>

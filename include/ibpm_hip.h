@@ -260,7 +260,7 @@ int ibpm_profile_stats(ibpm_prob* prob, double* avg_ms, int64_t* count);
  * levels, 2 = nonlinear over all levels, 4 = x-pass on level 1 only,
  * 5 = fused y-pass (Δinv) on level 1 only, 6 = curl on level 1,
  * 7 = plain copy of nΓ*mg doubles (HBM-traffic counter calibration),
- * 8 = x-pass with the ψ boundary terms (vort2flux!'s apply_bc!) on level 1. */
+ * 8 = Δinv y-pass with the ψ boundary terms (vort2flux!'s apply_bc!) on level 1. */
 int ibpm_time_kernel(ibpm_prob* prob, int which, int reps, double* ms_avg);
 
 #ifdef __cplusplus
