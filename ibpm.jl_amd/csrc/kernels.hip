@@ -416,6 +416,7 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     // (the odd samples are neighbour means and S(a - b) + S(a + b) = 2 S(a) C(b)).
     const int nx = a.g.nx;
     const bool bc = BC && en.lev < en.L - 1;
+    const int ncm = (nx / 2 - 1 + WAVE - 1) / WAVE;  // iterations of the row sums this grid needs
     // LDS after the tile slots: the coarse (L, R) column pairs of lines
     // ny/4 - 1 .. 3ny/4 - 1
     double2* edl = reinterpret_cast<double2*>(tile + TI * ld);
@@ -441,8 +442,11 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
 #pragma unroll
         for (int q = 0; q < NCM; q++) {
             const int m = lane + 1 + WAVE * q;
-            cB[q] = m < nx / 2 ? rB[m] : 0.0;
-            cT[q] = m < nx / 2 ? rT[m] : 0.0;
+            cB[q] = cT[q] = 0.0;
+            if (q < ncm) {
+                cB[q] = m < nx / 2 ? rB[m] : 0.0;
+                cT[q] = m < nx / 2 ? rT[m] : 0.0;
+            }
         }
         cB0 = rB[0]; cT0 = rT[0]; cBn = rB[nx / 2]; cTn = rT[nx / 2];
         if (colok) {
@@ -463,7 +467,8 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
             for (int q = 0; q < NCM; q++) {
                 const int t2 = 2 * u;
                 const int tt = t2 < nx ? t2 : t2 - nx;
-                sv[kk][q] = a.sinx[min(tt, nx - tt)];  // the sign is applied once loaded
+                sv[kk][q] = 0.0;
+                if (q < ncm) sv[kk][q] = a.sinx[min(tt, nx - tt)];  // the sign is applied once loaded
                 neg[kk] |= (t2 >= nx ? 1u : 0u) << q;
                 u += step;
                 if (u >= nx) u -= nx;

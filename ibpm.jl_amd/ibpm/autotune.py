@@ -11,10 +11,12 @@ import ctypes
 
 from . import _capi as C
 
-# option -> (kernel id timed by ibpm_time_kernel, candidate values)
+# option -> (kernel id timed by ibpm_time_kernel, candidate values).  The
+# one-level y-passes are mostly those of the vort2flux! chain, which carry the
+# boundary terms (id 8); a one-level problem has none and times id 5.
 SWEEPS = {
     C.OPT_YTILE_BATCH: (1, (4, 8, 12)),
-    C.OPT_YTILE_SINGLE: (5, (4, 8, 12)),
+    C.OPT_YTILE_SINGLE: (8, (4, 8, 12)),
     C.OPT_XWAVES_BATCH: (0, (4, 8, 12, 14)),
 }
 
@@ -31,6 +33,8 @@ def autotune(prob, reps: int = 20) -> dict:
     L = C.lib()
     out = {}
     for opt, (which, values) in SWEEPS.items():
+        if which == 8 and prob.model.grid.mg < 2:
+            which = 5
         times = {}
         for v in values:
             if L.ibpm_set_option(prob.h, opt, v) != C.IBPM_OK:
