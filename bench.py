@@ -295,7 +295,7 @@ def main():
     dst_bytes = 2.0 * 8.0 * N * L
     ypass_ms = y_ms.value
     roof = {"bound": "hbm",
-            "kernel": f"k_ypass<{tuned['ytile_batch']}> (fused DST-I y-pass, all 5 levels; A and Ainv of every step)",
+            "kernel": f"k_ypass<{tuned['ytile_batch']}> (fused DST-I y-pass, all 5 levels; Ainv of every step)",
             "achieved": round(dst_bytes / (ypass_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(dst_bytes / (ypass_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic("k_ypass", tuned["ytile_batch"], L),

@@ -611,12 +611,9 @@ static void trial_dev(ibpm_prob* P)
     na.nlev = L; na.lev0 = 0; na.mg = L; na.g = g;
     set_nl_q0(P, na);
     launch_nonlinear(na, s);
-    XArgs a = xargs(P);
-    a.src = P->G; a.dst = P->T1; a.nlev = L; a.lev0 = 0;
-    launch_xpass(X_PLAIN, a, s);
-    timed_ypass(P, yargs(P, Y_A, 0, L, P->T1));
+    // rhs = A Γ + ... (A by its 5-point stencil) and Ainv's first x-transform in one pass
     XArgs t = xargs(P);
-    t.src = P->T1; t.dst = P->T2; t.nlev = L; t.lev0 = 0;
+    t.src = P->G; t.dst = P->T2; t.nlev = L; t.lev0 = 0;
     t.n1 = P->nl[out]; t.n0 = P->nl[P->nl2]; t.gam = P->G;
     t.c1 = P->beta[0] * P->dt;
     t.c2 = P->beta[1] * P->dt;
@@ -1606,14 +1603,9 @@ void group_step(ibpm_group* Gp)
     na.nlev = NE; na.lev0 = 0; na.mg = L; na.g = g; na.levmod = L; na.ipar = Gp->ipar;
     na.q0_uniform = Gp->q0_uniform ? 1 : 0;
     launch_nonlinear(na, s);
-    XArgs a = gxargs(Gp);
-    a.src = Gp->G; a.dst = Gp->T1; a.nlev = NE; a.lev0 = 0; a.levmod = L;
-    launch_xpass(X_PLAIN, a, s);
-    YArgs ya = gyargs(Gp, Y_A, Gp->T1);
-    ya.lstride = nG; ya.nlev = NE; ya.lev0 = 0; ya.levmod = L;
-    launch_ypass(ya, Gp->ytile, s);
+    // rhs = A Γ + ... (A by its 5-point stencil) and Ainv's first x-transform in one pass
     XArgs t = gxargs(Gp);
-    t.src = Gp->T1; t.dst = Gp->T2; t.nlev = NE; t.lev0 = 0;
+    t.src = Gp->G; t.dst = Gp->T2; t.nlev = NE; t.lev0 = 0;
     t.n1 = Gp->nl[out]; t.n0 = Gp->nl[Gp->nl2]; t.gam = Gp->G; t.levmod = L; t.ipar = Gp->ipar;
     launch_xpass(X_TRIAL, t, s);
     YArgs yi = gyargs(Gp, Y_AINV, Gp->T2);

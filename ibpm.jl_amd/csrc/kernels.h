@@ -33,7 +33,8 @@ struct XArgs {
     // edge + b * edstride): the columns get_bc! of the next finer level samples
     double* edge;
     long edstride;
-    // X_TRIAL: rhs = ((A Γ + c1 N1) + c2 N0) + rhsbc, then a second x-DST
+    // X_TRIAL: src = Γ (all levels); rhs = ((A Γ + c1 N1) + c2 N0) + rhsbc
+    // with A Γ from the 5-point stencil, then the x-DST of rhs
     const double* n1;
     const double* n0;
     const double* gam;    // state Γ (all levels) for rhsbc of level lev from lev+1

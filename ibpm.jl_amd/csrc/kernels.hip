@@ -124,6 +124,24 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
         int n = lane + WAVE * t;
         if (n < n1) v[t] = src[n];
     }
+    // X_TRIAL: the rows j - 1 and j + 1 of Γ (zero outside), N1 and N0 of the
+    // line, in flight with the row's own loads
+    double vm[XPL], vp[XPL], n1v[XPL], u[XPL];
+    if constexpr (MODE == X_TRIAL) {
+        const double* n1p = a.n1 + b * a.lstride + (long)line * n1;
+        const double* n0p = a.n0 + b * a.lstride + (long)line * n1;
+#pragma unroll
+        for (int t = 0; t < XPL; t++) {
+            const int n = lane + WAVE * t;
+            vm[t] = vp[t] = n1v[t] = u[t] = 0.0;
+            if (n < n1) {
+                if (line > 0) vm[t] = src[n - n1];
+                if (line < n2 - 1) vp[t] = src[n + n1];
+                n1v[t] = n1p[n];
+                u[t] = n0p[n];
+            }
+        }
+    }
     const int ld = slot_doubles_t<PT>(a.pl);
     double* s = lds + ntab + w * ld;
     if (lane == 0) set_guards<PT>(s);
@@ -133,28 +151,12 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
         if (n < n1) s[lat<PT>(n)] = v[t];
     }
     if constexpr (MODE == X_TRIAL) {
-        // N1 and N0 of the rhs are in flight during the first transform
-        const double* n1p = a.n1 + b * a.lstride + (long)line * n1;
-#pragma unroll
-        for (int t = 0; t < XPL; t++) {
-            int n = lane + WAVE * t;
-            if (n < n1) v[t] = n1p[n];
-        }
-    }
-    double u[XPL];
-    if constexpr (MODE == X_TRIAL) {
-        const double* n0p = a.n0 + b * a.lstride + (long)line * n1;
-#pragma unroll
-        for (int t = 0; t < XPL; t++) {
-            int n = lane + WAVE * t;
-            if (n < n1) u[t] = n0p[n];
-        }
-    }
-    wave_sync();
-    dst_line<PT>(s, a.pl, tb, lane);
-    if constexpr (MODE == X_TRIAL) {
         // get_trial_state! (timestepping.jl:111-118): rhs = A Γ; rhs += (β1 dt) N1;
-        // rhs += (β2 dt) N0; rhs += rhsbc, with rhsbc = apply_bc!(0, get_bc!(Γ[:, lev+1]), fac)
+        // rhs += (β2 dt) N0; rhs += rhsbc, with rhsbc = apply_bc!(0, get_bc!(Γ[:, lev+1]), fac),
+        // then the first x-transform of Ainv.  A = I - qc C'C with qc = dt/(2 Re hc^2)
+        // (laplacian.jl:73-81: the eigenvalues 1 - qc Λ of get_A are those of
+        // this 5-point stencil with zero Dirichlet values), so A Γ is formed
+        // from the rows j-1, j, j+1 of Γ itself.
         const int e = a.lev0 + b;
         const int lev = en.lev;
         const bool hasbc = lev < en.L - 1;
@@ -165,6 +167,7 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
             const InstPar& ip = a.ipar[en.inst];
             dt = ip.dt; Re = ip.Re; c1 = ip.c1; c2 = ip.c2;
         }
+        const double qc = dt / ((2.0 * Re) * (hc * hc));
         const double fac = 0.25 * dt / (Re * (hc * hc));  // timestepping.jl:104
         const bool rb = (j == 1), rt = (j == a.g.ny - 1);
         double cl = 0.0, cr = 0.0;
@@ -173,12 +176,16 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
             cr = fac * bc_col(gc, a.g, 3 * a.g.nx / 4, 1 + j);
         }
         const bool rowbc = hasbc && (rb || rt);
+        wave_sync();
 #pragma unroll
         for (int t = 0; t < XPL; t++) {
-            int n = lane + WAVE * t;
+            const int n = lane + WAVE * t;
             if (n < n1) {
-                double rhs = s[lat<PT>(n)];
-                rhs += c1 * v[t];
+                const double gw = n > 0 ? s[lat<PT>(n - 1)] : 0.0;
+                const double ge = n < n1 - 1 ? s[lat<PT>(n + 1)] : 0.0;
+                // A Γ = Γ - qc (4 Γ - neighbours)
+                double rhs = v[t] - qc * (4.0 * v[t] - ((gw + ge) + (vm[t] + vp[t])));
+                rhs += c1 * n1v[t];
                 rhs += c2 * u[t];
                 if (hasbc) {
                     // rhsbc = apply_bc!(0, bc, fac): row terms, then column terms
@@ -194,12 +201,18 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
                         rhs += rbc;
                     }
                 }
-                s[lat<PT>(n)] = rhs;
+                v[t] = rhs;
             }
         }
         wave_sync();
-        dst_line<PT>(s, a.pl, tb, lane);
+#pragma unroll
+        for (int t = 0; t < XPL; t++) {
+            const int n = lane + WAVE * t;
+            if (n < n1) s[lat<PT>(n)] = v[t];
+        }
     }
+    wave_sync();
+    dst_line<PT>(s, a.pl, tb, lane);
     double* dst = a.dst + b * a.lstride + (long)line * n1;
     if constexpr (MODE == X_SUB) {
 #pragma unroll
