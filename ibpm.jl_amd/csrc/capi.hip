@@ -124,9 +124,10 @@ struct ibpm_prob {
     };
     std::vector<StepGraph> gcache;
     std::vector<void*> allocs;
-    // in-step timing of the all-level y-pass launches (IBPM_OPT_PROFILE):
+    // in-step timing (IBPM_OPT_PROFILE): 1 the all-level y-pass launches, 2
+    // the one-level y-passes with boundary terms of the vort2flux! chain;
     // event pairs recorded on the problem's stream, summed after each call
-    bool prof = false;
+    int prof = 0;
     std::vector<hipEvent_t> pev;
     size_t pev_used = 0;
     double prof_ms = 0.0;
@@ -457,6 +458,8 @@ static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x,
     launch_xpass(sub ? X_SUB : X_PLAIN, c, s);
 }
 
+static void timed_ypass(ibpm_prob* P, const YArgs& y, int tile, int kind);
+
 // vort2flux! (curl.jl:133-165).  Gb is mutated on coarse levels.  When
 // `level1_only`, only q on level 1 is produced (into qout), which is all the
 // trial stage and B_times! read of it.
@@ -488,7 +491,7 @@ static void vort2flux_dev(ibpm_prob* P, double* Gb, double* psib, double* qout, 
             y.bcr = psib + (long)(l + 1) * nG;
             y.edge = P->edge;
         }
-        launch_ypass(y, P->ytile_single, s);
+        timed_ypass(P, y, P->ytile_single, l < L - 1 ? 2 : -1);
         XArgs c = xargs(P);
         c.src = P->T1 + (long)l * nG; c.dst = psib + (long)l * nG; c.nlev = 1; c.lev0 = l;
         if (l > 0) c.edge = P->edge;
@@ -520,12 +523,12 @@ static void project_forces(ibpm_prob* P, const double* f, double* Gout, bool sub
     lap_level(P, Y_AINV, 0, P->Gtmp, Gout, sub, P->rb_j0 - 1, P->rb_j1 - 1);
 }
 
-// An all-level y-pass of the step, bracketed by a pair of events on the
-// problem's stream when in-step profiling is on.
-static void timed_ypass(ibpm_prob* P, const YArgs& y)
+// A y-pass of the step with `tile` columns per work-group, bracketed by a pair
+// of events on the problem's stream when in-step profiling of `kind` is on.
+static void timed_ypass(ibpm_prob* P, const YArgs& y, int tile, int kind)
 {
     hipEvent_t e[2] = {nullptr, nullptr};
-    if (P->prof) {
+    if (P->prof == kind) {
         for (int k = 0; k < 2; k++) {
             if (P->pev_used == P->pev.size()) {
                 hipEvent_t ev;  // timing only: no system-scope fence (no cache writeback per record)
@@ -536,7 +539,7 @@ static void timed_ypass(ibpm_prob* P, const YArgs& y)
         }
     }
     if (e[1]) (void)hipEventRecord(e[0], P->stream);
-    launch_ypass(y, P->ytile, P->stream);
+    launch_ypass(y, tile, P->stream);
     if (e[1]) (void)hipEventRecord(e[1], P->stream);
 }
 
@@ -620,7 +623,7 @@ static void trial_dev(ibpm_prob* P)
     t.dt = P->dt;
     t.Re = P->Re;
     launch_xpass(X_TRIAL, t, s);
-    timed_ypass(P, yargs(P, Y_AINV, 0, L, P->T2));
+    timed_ypass(P, yargs(P, Y_AINV, 0, L, P->T2), P->ytile, 1);
     XArgs c = xargs(P);
     c.src = P->T2; c.dst = P->Gs; c.nlev = L; c.lev0 = 0;
     launch_xpass(X_PLAIN, c, s);
@@ -2108,10 +2111,10 @@ int ibpm_set_option(ibpm_prob* P, int option, int value)
         return IBPM_OK;
     }
     if (option == IBPM_OPT_PROFILE) {
-        if (value != 0 && value != 1) return fail(P->ctx, IBPM_E_ARG, "profile option is 0 or 1");
+        if (value < 0 || value > 2) return fail(P->ctx, IBPM_E_ARG, "profile option is 0, 1 or 2");
         HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
         collect_profile(P);
-        P->prof = value == 1;
+        P->prof = value;
         P->prof_ms = 0.0;
         P->prof_n = 0;
         return IBPM_OK;
@@ -2124,7 +2127,7 @@ int ibpm_get_option(ibpm_prob* P, int option, int* value)
     if (!P || !value) return IBPM_E_ARG;
     if (option == IBPM_OPT_YTILE_BATCH) *value = P->ytile;
     else if (option == IBPM_OPT_YTILE_SINGLE) *value = P->ytile_single;
-    else if (option == IBPM_OPT_PROFILE) *value = P->prof ? 1 : 0;
+    else if (option == IBPM_OPT_PROFILE) *value = P->prof;
     else if (option == IBPM_OPT_GRAPHS) *value = P->graphs ? 1 : 0;
     else if (option == IBPM_OPT_XWAVES_BATCH) *value = P->xwaves;
     else return fail(P->ctx, IBPM_E_ARG, "unknown option");

@@ -241,16 +241,18 @@ int ibpm_op_update_bodies(ibpm_prob* prob, const double* kin);
  * no kernels); used by the autotuner and the benchmark. */
 #define IBPM_OPT_YTILE_BATCH 0   /* columns per y-pass workgroup, all-level launches: 4, 8 or 12 */
 #define IBPM_OPT_YTILE_SINGLE 1  /* columns per y-pass workgroup, one-level launches: 4, 8 or 12 */
-#define IBPM_OPT_PROFILE 2       /* 1: time every all-level y-pass inside the steps (HIP events on the
-                                    problem's stream); setting it drains the stream and clears the sums */
+#define IBPM_OPT_PROFILE 2       /* 1: time every all-level y-pass inside the steps; 2: time every
+                                    one-level y-pass with the ψ boundary terms of the vort2flux! chain
+                                    (HIP events on the problem's stream); setting it drains the stream
+                                    and clears the sums */
 #define IBPM_OPT_GRAPHS 3        /* 1 (default): static-body steps are captured once per buffer role in a
                                     HIP graph and replayed; 0: every step launches its kernels */
 #define IBPM_OPT_XWAVES_BATCH 4  /* lines per work-group of the plain x-passes over all levels: 4 (default),
                                     8, 12 or 14 (one LDS copy of the transform tables for more lines) */
 int ibpm_set_option(ibpm_prob* prob, int option, int value);
 int ibpm_get_option(ibpm_prob* prob, int option, int* value);
-/* Average device ms and count of the all-level y-pass launches timed inside
- * the steps since IBPM_OPT_PROFILE was last set.  No counterpart in the
+/* Average device ms and count of the y-pass launches timed inside the steps
+ * (the kind IBPM_OPT_PROFILE selects) since that option was last set.  No counterpart in the
  * reference (it has no kernels); the benchmark's roofline reads it. */
 int ibpm_profile_stats(ibpm_prob* prob, double* avg_ms, int64_t* count);
 
