@@ -258,6 +258,28 @@ def test_run_sim_saves_and_cfl():
     assert st.cfl > 0  # computed after step 25 (display_freq)
 
 
+@pytest.mark.parametrize("name", ["small3", "tiny"])
+def test_autotune_applies_legal_shapes(name):
+    """The autotuner sweeps its options on a problem (one-level problems
+    time the y-pass without boundary terms) and leaves each at a value it
+    timed; stepping afterwards still works."""
+    import ctypes
+    from ibpm import _capi as C
+    from ibpm.autotune import SWEEPS, autotune
+    c = case(name)
+    gp = ibpm.IBProblem(c["grid"], c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
+    res = autotune(gp, reps=2)
+    assert set(res) == set(SWEEPS)
+    for opt, (best, times) in res.items():
+        assert best in times and all(t > 0.0 for t in times.values())
+        v = ctypes.c_int(-1)
+        gp.ctx.check(C.lib().ibpm_get_option(gp.h, opt, ctypes.byref(v)))
+        assert v.value == best
+    st = ibpm.IBState(gp)
+    _, _, done = ibpm.advance_n_(st, gp, np.arange(2) * c["dt"])
+    assert done == 2
+
+
 @pytest.mark.parametrize("name", ["small3", "spec", "config1"])
 def test_launch_shapes_give_identical_steps(name):
     """Every legal work-group shape of the x and y passes (the autotuner's
