@@ -438,8 +438,11 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     constexpr int NCM = (MAXM / 2 + WAVE - 1) / WAVE;                  // coarse-row samples per lane
     double2 ep[EPT];
     double addB[NCOL], addT[NCOL];  // row terms of this wave's columns
-    double cB[NCM], cT[NCM], sv[NCOL][NCM], cB0 = 0.0, cT0 = 0.0, cBn = 0.0, cTn = 0.0, Sk[NCOL];
-    unsigned neg[NCOL];  // bit q: S(2 m k) is minus the table entry
+    double cB[NCM], cT[NCM], cB0 = 0.0, cT0 = 0.0, cBn = 0.0, cTn = 0.0, Sk[NCOL];
+    // per column: sin and cos of the lane's first angle 2 pi m k / nx (m = lane + 1)
+    // and of the step 2 pi 64 k / nx, as loaded; bit r of sgn[kk]: entry r is negated
+    double rsv[NCOL][4];
+    unsigned sgn[NCOL];
     double ek = 0.0, esg = 0.0;
     // (1) every load of the boundary data, issued before the tile's so that
     // the row sums wait for these alone while the tile is in flight
@@ -467,24 +470,24 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
             ek = a.sinx[min(k, nx - k)];  // scaled by 2 fac once loaded (below)
             esg = (k & 1) ? 1.0 : -1.0;
         }
-        // S(2 m k) for the row sums of line k = i0 + c + 1, c = w + YW kk:
-        // sin(2 pi u / nx) = sign * sinx[min(t, nx - t)], u = m k mod nx, t = 2u mod nx
+        // S(2 m k) for the row sums of line k = i0 + c + 1, c = w + YW kk,
+        // m = lane + 1 + WAVE q: the angles advance by a fixed step, so two
+        // table values per lane and two per column start a rotation.  With
+        // sin(pi t / nx) = sign * sinx[min(t', nx - t')], t' = t mod nx, sign
+        // negative for t mod 2nx >= nx; cos(pi t / nx) = sin(pi (t + nx/2) / nx).
 #pragma unroll
         for (int kk = 0; kk < NCOL; kk++) {
             const int k = min(i0 + w + YW * kk, n1 - 1) + 1;
             Sk[kk] = a.sinx[min(k, nx - k)];
-            const int step = (WAVE * k) % nx;
-            int u = ((lane + 1) * k) % nx;
-            neg[kk] = 0u;
+            const int u0 = ((lane + 1) * k) % nx, ud = (WAVE * k) % nx;
+            const int tv[4] = {2 * u0, 2 * u0 + nx / 2, 2 * ud, 2 * ud + nx / 2};
+            sgn[kk] = 0u;
 #pragma unroll
-            for (int q = 0; q < NCM; q++) {
-                const int t2 = 2 * u;
-                const int tt = t2 < nx ? t2 : t2 - nx;
-                sv[kk][q] = 0.0;
-                if (q < ncm) sv[kk][q] = a.sinx[min(tt, nx - tt)];  // the sign is applied once loaded
-                neg[kk] |= (t2 >= nx ? 1u : 0u) << q;
-                u += step;
-                if (u >= nx) u -= nx;
+            for (int r = 0; r < 4; r++) {
+                const int t = tv[r] >= 2 * nx ? tv[r] - 2 * nx : tv[r];
+                const int tt = t >= nx ? t - nx : t;
+                rsv[kk][r] = a.sinx[min(tt, nx - tt)];
+                sgn[kk] |= (t >= nx ? 1u : 0u) << r;
             }
         }
     }
@@ -510,12 +513,19 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
 #pragma unroll
         for (int kk = 0; kk < NCOL; kk++) {
             const int k = min(i0 + w + YW * kk, n1 - 1) + 1;
+            double v4[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) v4[r] = ((sgn[kk] >> r) & 1u) ? -rsv[kk][r] : rsv[kk][r];
+            double sn = v4[0], cs = v4[1];
+            const double sd = v4[2], cd = v4[3];
             double sB = 0.0, sT = 0.0;
 #pragma unroll
             for (int q = 0; q < NCM; q++) {
-                const double x = ((neg[kk] >> q) & 1u) ? -sv[kk][q] : sv[kk][q];
-                sB += x * cB[q];
-                sT += x * cT[q];
+                sB += sn * cB[q];
+                sT += sn * cT[q];
+                const double s2 = sn * cd + cs * sd;
+                cs = cs * cd - sn * sd;
+                sn = s2;
             }
             sB = wave_last(wave_incl_scan(sB, lane));
             sT = wave_last(wave_incl_scan(sT, lane));
