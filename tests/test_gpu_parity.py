@@ -54,7 +54,9 @@ def initial_states(c, op, gp, seed=7):
     return S, st
 
 
-def compare_state(S, st, tol):
+def compare_state(S, st, tol, fb_tol=None):
+    """Norm-relative errors of the state fields; all must be <= tol (F̃b <=
+    fb_tol when given)."""
     errs = {
         "q": relerr(st.q, S.q), "q0": relerr(st.q0, S.q0), "Γ": relerr(st.Γ, S.G), "ψ": relerr(st.ψ, S.psi),
         "nonlin1": relerr(st.nonlin[0], S.nl1), "nonlin2": relerr(st.nonlin[1], S.nl2),
@@ -62,7 +64,7 @@ def compare_state(S, st, tol):
         # CL may vanish by symmetry: scale its error by the force magnitude
         "CL": np.max(np.abs(st.CL - S.CL)) / max(np.max(np.abs(S.CD)), np.max(np.abs(S.CL)), 1e-300),
     }
-    bad = {k: v for k, v in errs.items() if not v <= tol}
+    bad = {k: v for k, v in errs.items() if not v <= (fb_tol if k == "F̃b" and fb_tol else tol)}
     assert not bad, f"fields beyond {tol}: {bad}"
     return errs
 
@@ -460,3 +462,56 @@ def test_full_size_dinv_inverts_5pt(big, rng):
     b = np.ascontiguousarray(lap.ravel(order="F"))
     gp.ctx.check(C.lib().ibpm_op_lapinv(gp.h, 0, 1, C.ptr(b), C.ptr(x)))
     assert relerr(x, psi.ravel(order="F")) < 1e-11
+
+
+# ---------------------------------------------------------------------------
+# full-size steps against the oracle: the 1024 x 1024 and 800 x 400 grids,
+# 5 levels, and the 512 x 256, 4-level moving grid (the 1023-point transform
+# core, the compile-time cores and the vort2flux! boundary terms at these widths)
+# ---------------------------------------------------------------------------
+def _smooth_along_body(f):
+    """(1/4, 1/2, 1/4) filter along the body points of each force component
+    (F̃b holds all x components, then all y components)."""
+    out = []
+    for comp in np.split(np.asarray(f, dtype=np.float64).ravel(), 2):
+        out.append(0.25 * np.roll(comp, 1) + 0.5 * comp + 0.25 * np.roll(comp, -1))
+    return np.concatenate(out)
+
+
+@pytest.mark.parametrize("name,nsteps", [("config4", 10), ("config2", 10), ("config5", 10)])
+def test_full_size_steps_match_oracle(name, nsteps):
+    """vort2flux! and advance! steps at full size.  The oracle takes the
+    GPU's Binv: assembling B column by column would take the CPU minutes here,
+    and Binv itself is compared with the oracle's at the smaller sizes.
+
+    F̃b = Binv (E q - u_b) pointwise is ill-conditioned at these sizes
+    (cond(B) = 5.0e5 at 1024 x 1024): the 2e-12 rounding difference of q
+    between two FP64 transform orders reaches F̃b as up to 6e-10 relative
+    over 40 steps, all of it in modes that alternate along the body.  The
+    quantities built from the forces (CD, CL) and the forces smoothed along
+    the body are held to the 1e-10 bar; raw F̃b to 1e-8."""
+    import os
+    c = case(name)
+    g = c["grid"]
+    gp = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
+    op = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], c["dt"], oracle_bodies(c),
+                         motion=1 if "motion" in c else 0, binv=gp.Binv)
+    O.set_threads(min(8, os.cpu_count() or 1))
+    try:
+        S, st = initial_states(c, op, gp)
+        assert np.array_equal(st.Γ, S.G) and relerr(st.ψ, S.psi) < 1e-12
+        # q is differences of neighbouring ψ: at h = 1/128 they are ~1e-3 of ψ,
+        # so ψ's rounding shows in q about that much larger (8e-12 measured)
+        assert relerr(st.q, S.q) < STEP_TOL
+        ts = np.arange(nsteps) * c["dt"]
+        cd_o, cl_o = op.run(S, np.stack([sc7(c, t) for t in ts]))
+    finally:
+        O.set_threads(1)
+    cd, cl, done = ibpm.advance_n_(st, gp, ts)
+    assert done == nsteps
+    assert relerr(cd, cd_o) < STEP_TOL
+    assert np.max(np.abs(cl - cl_o)) < STEP_TOL * np.abs(cd_o).max()
+    errs = compare_state(S, st, STEP_TOL, fb_tol=1e-8)
+    errs["F̃b smoothed"] = relerr(_smooth_along_body(st.F̃b), _smooth_along_body(S.Fb))
+    print(name, {k: f"{v:.1e}" for k, v in errs.items()})
+    assert errs["F̃b smoothed"] < STEP_TOL
