@@ -6,6 +6,7 @@ differences in B by cond(B) ~ 1e4-1e5); whole steps 1e-10, the bar of the
 drop-in.  Index work (body_idx, E' accumulation order, rot!) is bit-exact.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -490,7 +491,6 @@ def test_full_size_steps_match_oracle(name, nsteps):
     over 40 steps, all of it in modes that alternate along the body.  The
     quantities built from the forces (CD, CL) and the forces smoothed along
     the body are held to the 1e-10 bar; raw F̃b to 1e-8."""
-    import os
     c = case(name)
     g = c["grid"]
     gp = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
@@ -515,3 +515,33 @@ def test_full_size_steps_match_oracle(name, nsteps):
     errs["F̃b smoothed"] = relerr(_smooth_along_body(st.F̃b), _smooth_along_body(S.Fb))
     print(name, {k: f"{v:.1e}" for k, v in errs.items()})
     assert errs["F̃b smoothed"] < STEP_TOL
+
+
+@pytest.mark.skipif(not os.environ.get("IBPM_LONG_STEPS"),
+                    reason="long full-size run: set IBPM_LONG_STEPS (e.g. 300) to run it")
+def test_full_size_long_run():
+    """IBPM_LONG_STEPS steps of the 1024 x 1024, 5-level cylinder against the
+    oracle (GPU's Binv), compared every 25 steps: CD, CL, Γ, ψ and the forces
+    smoothed along the body to 1e-10, raw F̃b to 1e-8."""
+    nsteps = int(os.environ["IBPM_LONG_STEPS"])
+    c = case("config4")
+    g = c["grid"]
+    gp = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
+    op = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], c["dt"], oracle_bodies(c), binv=gp.Binv)
+    O.set_threads(min(15, os.cpu_count() or 1))
+    try:
+        S, st = initial_states(c, op, gp)
+        k = 0
+        while k < nsteps:
+            ts = (k + np.arange(25)) * c["dt"]
+            cd_o, cl_o = op.run(S, np.stack([sc7(c, t) for t in ts]))
+            cd, cl, _ = ibpm.advance_n_(st, gp, ts)
+            k += 25
+            errs = compare_state(S, st, STEP_TOL, fb_tol=1e-8)
+            errs["CD run"] = relerr(cd, cd_o)
+            errs["CL run"] = np.max(np.abs(cl - cl_o)) / np.abs(cd_o).max()
+            errs["F̃b smoothed"] = relerr(_smooth_along_body(st.F̃b), _smooth_along_body(S.Fb))
+            print(k, {n: f"{v:.1e}" for n, v in errs.items()}, flush=True)
+            assert errs["CD run"] < STEP_TOL and errs["CL run"] < STEP_TOL and errs["F̃b smoothed"] < STEP_TOL
+    finally:
+        O.set_threads(1)
