@@ -168,10 +168,10 @@ def bench_configs(args):
     from ibpm.autotune import autotune
     from cases import boundary_of, case, gaussian_vortex_gamma, oracle_bodies, oracle_steps
 
-    runs = [("config1", "cylinder Re=40, 200x200, 3 levels", 600, False),
-            ("config2", "cylinder Re=100, 800x400, 5 levels, Gaussian vortex", 300, True),
-            ("config4", "cylinder Re=200, 1024x1024, 5 levels, Gaussian vortex", 300, True),
-            ("config5", "MovingGrid pitching plate Re=200, 512x256, 4 levels", 300, False),
+    runs = [("cyl_re40_200x200x3", "cylinder Re=40, 200x200, 3 levels", 600, False),
+            ("cyl_re100_800x400x5", "cylinder Re=100, 800x400, 5 levels, Gaussian vortex", 300, True),
+            ("cyl_re200_1024x1024x5", "cylinder Re=200, 1024x1024, 5 levels, Gaussian vortex", 300, True),
+            ("plate_pitch_512x256x4", "MovingGrid pitching plate Re=200, 512x256, 4 levels", 300, False),
             ("eldredge_lab", "MotionFunction pitching plate Re=100, 400x200, 5 levels (lab frame)", 200, False)]
     for name, desc, steps, vortex in runs:
         c = case(name)
@@ -242,7 +242,7 @@ def main():
     from ibpm import _capi as C
     from cases import case, gaussian_vortex_gamma
 
-    c = case("config4")
+    c = case("cyl_re200_1024x1024x5")
     g = c["grid"]
     nb = len(c["bodies"][0].ds)
     dt = c["dt"]

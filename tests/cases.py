@@ -72,7 +72,7 @@ def case(name: str):
                                    lambda t: 0.4])
         bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, motion=mot, n=40)]
         return dict(grid=grid, bodies=bodies, Re=100.0, dt=0.01, freestream=U1)
-    if name == "config5":  # pitching plate in the body-fixed frame: 512 x 256, 4 levels, nominal A = 15 deg, f = 0.2
+    if name == "plate_pitch_512x256x4":  # pitching plate in the body-fixed frame: 512 x 256, 4 levels, nominal A = 15 deg, f = 0.2
         grid = ibpm.make_grid(1.0 / 128, (-1.0, 3.0, -1.0, 1.0), mg=4)
         A, f = 15 * math.pi / 180, 0.2
         mot = ibpm.MovingGrid(lambda t: 1.0, lambda t: 0.0, lambda t: A * math.sin(2 * math.pi * f * t),
@@ -91,7 +91,7 @@ def case(name: str):
                                   [lambda t: 0.0, lambda t: 0.0, lambda t: -amax * Gd(t) / Gmax])
         bodies = [ibpm.make_plate(1.0, 0.0, grid.h, 0.0, 0.0, motion=mot, n=51)]
         return dict(grid=grid, bodies=bodies, Re=100.0, dt=1e-3, freestream=U1)
-    if name == "config1":  # Re 40 cylinder, 200 x 200, 3 levels
+    if name == "cyl_re40_200x200x3":  # Re 40 cylinder, impulsive start: 200 x 200, 3 levels
         grid = ibpm.make_grid(0.02, (-1.0, 3.0, -2.0, 2.0), mg=3)
         bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, n=157)]
         return dict(grid=grid, bodies=bodies, Re=40.0, dt=0.004, freestream=U1)
@@ -105,11 +105,11 @@ def case(name: str):
         bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0)]
         return dict(grid=grid, bodies=bodies, Re=40.0, dt=1e-2,
                     freestream={"Ux": lambda t: 1.0, "Uy": lambda t: 0.0, "inclination": lambda t: 0.0})
-    if name == "config2":  # Re 100 cylinder, 800 x 400, 5 levels
+    if name == "cyl_re100_800x400x5":  # Re 100 cylinder: 800 x 400, 5 levels
         grid = ibpm.make_grid(0.01, (-2.0, 6.0, -2.0, 2.0), mg=5)
         bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, n=314)]
         return dict(grid=grid, bodies=bodies, Re=100.0, dt=0.002, freestream=U1)
-    if name == "config4":  # Re 200 cylinder, 1024 x 1024, 5 levels, dx = 1/128
+    if name == "cyl_re200_1024x1024x5":  # Re 200 cylinder: 1024 x 1024, 5 levels, dx = 1/128
         grid = ibpm.make_grid(1.0 / 128, (-2.0, 6.0, -4.0, 4.0), mg=5)
         n = int(round(math.pi / grid.h))  # points spaced ~ dx on the unit-diameter circle
         bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, n=n)]

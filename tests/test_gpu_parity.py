@@ -283,7 +283,7 @@ def test_autotune_applies_legal_shapes(name):
     assert done == 2
 
 
-@pytest.mark.parametrize("name", ["small3", "spec", "config1"])
+@pytest.mark.parametrize("name", ["small3", "spec", "cyl_re40_200x200x3"])
 def test_launch_shapes_give_identical_steps(name):
     """Every legal work-group shape of the x and y passes (the autotuner's
     options) transforms each line with the same arithmetic: the steps are
@@ -415,9 +415,9 @@ def test_linearized_steps():
     compare_state(S, st, STEP_TOL)
 
 
-def test_config1_steps():
-    """BASELINE configs[0] (200 x 200, 3 levels, Re 40 cylinder), 20 steps."""
-    c, op, gp = build("config1")
+def test_cyl_re40_200x200_steps():
+    """The Re 40 cylinder from an impulsive start (200 x 200, 3 levels), 20 steps."""
+    c, op, gp = build("cyl_re40_200x200x3")
     assert relerr(gp.Binv, op.binv()) < 1e-9
     S = op.new_state(sc7(c, 0.0))
     st = ibpm.IBState(gp)
@@ -434,7 +434,7 @@ def test_config1_steps():
 # ---------------------------------------------------------------------------
 @pytest.fixture(scope="module")
 def big():
-    c = case("config4")
+    c = case("cyl_re200_1024x1024x5")
     g = c["grid"]
     nf = 2 * len(c["bodies"][0].ds)
     gp = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"], Binv=np.eye(nf))
@@ -479,7 +479,7 @@ def _smooth_along_body(f):
     return np.concatenate(out)
 
 
-@pytest.mark.parametrize("name,nsteps", [("config4", 10), ("config2", 10), ("config5", 10)])
+@pytest.mark.parametrize("name,nsteps", [("cyl_re200_1024x1024x5", 10), ("cyl_re100_800x400x5", 10), ("plate_pitch_512x256x4", 10)])
 def test_full_size_steps_match_oracle(name, nsteps):
     """vort2flux! and advance! steps at full size.  The oracle takes the
     GPU's Binv: assembling B column by column would take the CPU minutes here,
@@ -524,7 +524,7 @@ def test_full_size_long_run():
     oracle (GPU's Binv), compared every 25 steps: CD, CL, Γ, ψ and the forces
     smoothed along the body to 1e-10, raw F̃b to 1e-8."""
     nsteps = int(os.environ["IBPM_LONG_STEPS"])
-    c = case("config4")
+    c = case("cyl_re200_1024x1024x5")
     g = c["grid"]
     gp = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
     op = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], c["dt"], oracle_bodies(c), binv=gp.Binv)

@@ -34,7 +34,7 @@ def test_library_is_gfx950_code_object():
     assert b"gfx950" in blob
 
 
-@pytest.mark.parametrize("name", ["tiny", "small3", "mixed", "moving", "spec", "config1", "steady_bl", "config2", "config4"])
+@pytest.mark.parametrize("name", ["tiny", "small3", "mixed", "moving", "spec", "cyl_re40_200x200x3", "steady_bl", "cyl_re100_800x400x5", "cyl_re200_1024x1024x5"])
 def test_make_grid_matches_oracle(name):
     c = case(name)
     g = c["grid"]
