@@ -220,7 +220,8 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=12,
+                    help="oracle steps in the CPU baseline sample (about 1 s each on one core)")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--workload", choices=("cylinder1024", "naca64", "configs"), default="cylinder1024",
                     help="naca64: the batched 64-instance NACA workload (secondary line); configs: one line "
