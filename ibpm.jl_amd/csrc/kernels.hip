@@ -393,12 +393,21 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
         cxw[k] = (c < TI && i0 + c < n1) ? a.cx[i0 + c] : 0.0;
     }
     // tile load: thread (r0, c) moves rows r0, r0 + RPI, ... of column c; CH
-    // loads per thread are in flight before their LDS writes
+    // loads per thread are in flight before their LDS writes.  The BC
+    // instance gives each thread pairs of adjacent rows instead (2 r0 +
+    // 2 RPI u and the row after it), whose column terms share the same two
+    // staged boundary pairs; a wave still covers 16 rows per load either way.
     constexpr int RPI = YW * WAVE / TI;
     constexpr int CH = 16;
+    static_assert(CH % 2 == 0, "rows are taken in pairs");
     const int c0 = tid % TI, r0 = tid / TI;
     const int ic = i0 + c0;
     const bool colok = ic < n1;
+    // row of load t of the chunk that starts at row rb (a multiple of RPI CH)
+    auto trow = [&](int rb, int t) -> int {
+        if constexpr (BC) return rb + 2 * r0 + 2 * RPI * (t >> 1) + (t & 1);
+        else return rb + r0 + RPI * t;
+    };
     // LDS: [DST tables][cos(pi j/ny) table][TI column slots]
     const int ntab = tab_doubles_t<PT>(a.pl);
     // the 1023-point tables and the cos table are loaded in two phases around
@@ -495,7 +504,7 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     double v[CH];
 #pragma unroll
     for (int t = 0; t < CH; t++) {
-        int r = r0 + RPI * t;
+        const int r = trow(0, t);
         v[t] = (colok && r < n2) ? base[ic + (long)n1 * r] : 0.0;
     }
     if constexpr (PT == 512) {
@@ -546,27 +555,29 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
         if (rb > 0) {
 #pragma unroll
             for (int t = 0; t < CH; t++) {
-                int r = rb + r0 + RPI * t;
+                const int r = trow(rb, t);
                 v[t] = (colok && r < n2) ? base[ic + (long)n1 * r] : 0.0;
             }
         }
         if (bc) {
-            // get_bc! sample idx = j + 1 = r + 2 of the left and right columns
+            // get_bc! sample idx = j + 1 = r + 2 of the left and right columns:
+            // the even row re (idx = 2e, e = re/2 + 1) takes the mean of the
+            // staged pairs e and e - 1, the odd row re + 1 (idx = 2e + 1) pair e
 #pragma unroll
-            for (int t = 0; t < CH; t++) {
-                const int r = rb + r0 + RPI * t;
-                if (r < n2) {
-                    const int idx = r + 2;
-                    const double2 ea = edl[idx / 2], eb = edl[idx / 2 - 1];
-                    const double bl = (idx & 1) ? ea.x : 0.5 * (ea.x + eb.x);
-                    const double br = (idx & 1) ? ea.y : 0.5 * (ea.y + eb.y);
-                    v[t] += ek * (bl + esg * br);
+            for (int u = 0; u < CH / 2; u++) {
+                const int re = trow(rb, 2 * u);
+                if (re < n2) {
+                    const int e = re / 2 + 1;
+                    const double2 ea = edl[e], eb = edl[e - 1];
+                    const double bl = 0.5 * (ea.x + eb.x), br = 0.5 * (ea.y + eb.y);
+                    v[2 * u] += ek * (bl + esg * br);
+                    if (re + 1 < n2) v[2 * u + 1] += ek * (ea.x + esg * ea.y);
                 }
             }
         }
 #pragma unroll
         for (int t = 0; t < CH; t++) {
-            int r = rb + r0 + RPI * t;
+            const int r = trow(rb, t);
             if (r < n2) tile[c0 * ld + lat<PT>(r)] = v[t];
         }
     }
