@@ -364,6 +364,10 @@ __host__ __device__ inline int cy_doubles_t(int n2)
     else return (n2 + 1) & ~1;
 }
 
+// LDS slot (double2 units) of the staged boundary pair e: one pad per 8, so
+// that lanes reading pairs 8 apart (consecutive-row threads) hit distinct banks
+__host__ __device__ inline int ep_slot(int e) { return e + (e >> 3); }
+
 // BC: the launch carries boundary terms (a.bcr != null); a separate instance
 // so that launches without them keep their register budget
 template <int TI, int PT, int YW, bool BC>
@@ -394,18 +398,18 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     }
     // tile load: thread (r0, c) moves rows r0, r0 + RPI, ... of column c; CH
     // loads per thread are in flight before their LDS writes.  The BC
-    // instance gives each thread pairs of adjacent rows instead (2 r0 +
-    // 2 RPI u and the row after it), whose column terms share the same two
-    // staged boundary pairs; a wave still covers 16 rows per load either way.
+    // instance gives each thread CH consecutive rows instead (CH r0 ..
+    // CH r0 + CH - 1), whose column terms need CH/2 + 1 staged boundary pairs
+    // between them; a wave still covers 16 rows per load either way.
     constexpr int RPI = YW * WAVE / TI;
     constexpr int CH = 16;
-    static_assert(CH % 2 == 0, "rows are taken in pairs");
+    static_assert(CH % 2 == 0, "the column terms take rows in pairs");
     const int c0 = tid % TI, r0 = tid / TI;
     const int ic = i0 + c0;
     const bool colok = ic < n1;
     // row of load t of the chunk that starts at row rb (a multiple of RPI CH)
     auto trow = [&](int rb, int t) -> int {
-        if constexpr (BC) return rb + 2 * r0 + 2 * RPI * (t >> 1) + (t & 1);
+        if constexpr (BC) return rb + CH * r0 + t;
         else return rb + r0 + RPI * t;
     };
     // LDS: [DST tables][cos(pi j/ny) table][TI column slots]
@@ -440,7 +444,7 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     const bool bc = BC && en.lev < en.L - 1;
     const int ncm = (nx / 2 - 1 + WAVE - 1) / WAVE;  // iterations of the row sums this grid needs
     // LDS after the tile slots: the coarse (L, R) column pairs of lines
-    // ny/4 - 1 .. 3ny/4 - 1
+    // ny/4 - 1 .. 3ny/4 - 1, pair e at ep_slot(e)
     double2* edl = reinterpret_cast<double2*>(tile + TI * ld);
     const int nep = a.g.ny / 2 + 1;
     constexpr int EPT = (MAXM / 2 + 1 + YW * WAVE - 1) / (YW * WAVE);  // pairs staged per thread
@@ -546,7 +550,7 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
 #pragma unroll
         for (int q = 0; q < EPT; q++) {
             const int e = tid + YW * WAVE * q;
-            if (e < nep) edl[e] = ep[q];
+            if (e < nep) edl[ep_slot(e)] = ep[q];
         }
         __syncthreads();
     }
@@ -561,14 +565,20 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
         }
         if (bc) {
             // get_bc! sample idx = j + 1 = r + 2 of the left and right columns:
-            // the even row re (idx = 2e, e = re/2 + 1) takes the mean of the
-            // staged pairs e and e - 1, the odd row re + 1 (idx = 2e + 1) pair e
+            // the even row re = trow(rb, 2u) (idx = 2e, e = re/2 + 1) takes the
+            // mean of the staged pairs e and e - 1, the odd row re + 1
+            // (idx = 2e + 1) pair e.  The thread's rows need pairs e0 - 1 ..
+            // e0 - 1 + CH/2; those past the last pair (rows beyond the line)
+            // are read clamped and not used.
+            const int e0 = trow(rb, 0) / 2 + 1;
+            double2 pr[CH / 2 + 1];
+#pragma unroll
+            for (int u = 0; u <= CH / 2; u++) pr[u] = edl[ep_slot(min(e0 - 1 + u, nep - 1))];
 #pragma unroll
             for (int u = 0; u < CH / 2; u++) {
                 const int re = trow(rb, 2 * u);
                 if (re < n2) {
-                    const int e = re / 2 + 1;
-                    const double2 ea = edl[e], eb = edl[e - 1];
+                    const double2 ea = pr[u + 1], eb = pr[u];
                     const double bl = 0.5 * (ea.x + eb.x), br = 0.5 * (ea.y + eb.y);
                     v[2 * u] += ek * (bl + esg * br);
                     if (re + 1 < n2) v[2 * u + 1] += ek * (ea.x + esg * ea.y);
@@ -662,7 +672,8 @@ static void launch_y_t(const YArgs& a, hipStream_t s)
     (void)once;
     const int nblocks = ((a.g.n1 + TI - 1) / TI) * a.nlev;
     dim3 grid(NXCD * xcd_nper(nblocks));
-    const size_t lds = ypass_lds_bytes_t<PT>(a.pl, a.g.n2, TI) + (a.bcr ? sizeof(double2) * (size_t)(a.g.ny / 2 + 1) : 0);
+    const size_t lds = ypass_lds_bytes_t<PT>(a.pl, a.g.n2, TI) +
+                       (a.bcr ? sizeof(double2) * (size_t)(ep_slot(a.g.ny / 2) + 1) : 0);
     if (a.bcr) hipLaunchKernelGGL((k_ypass<TI, PT, YW, true>), grid, dim3(YW * WAVE), lds, s, a);
     else hipLaunchKernelGGL((k_ypass<TI, PT, YW, false>), grid, dim3(YW * WAVE), lds, s, a);
 }
