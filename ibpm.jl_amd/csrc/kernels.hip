@@ -398,18 +398,21 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     }
     // tile load: thread (r0, c) moves rows r0, r0 + RPI, ... of column c; CH
     // loads per thread are in flight before their LDS writes.  The BC
-    // instance gives each thread CH consecutive rows instead (CH r0 ..
-    // CH r0 + CH - 1), whose column terms need CH/2 + 1 staged boundary pairs
-    // between them; a wave still covers 16 rows per load either way.
+    // instance gives each thread groups of G consecutive rows instead
+    // (G r0 + G RPI g ..), whose column terms need G/2 + 1 staged boundary
+    // pairs per group; a wave still covers 16 rows per load either way.  G is
+    // CH for the guarded 1023-point layout, whose padding keeps lanes CH rows
+    // apart on distinct LDS banks, and 2 for the plain layouts.
     constexpr int RPI = YW * WAVE / TI;
     constexpr int CH = 16;
-    static_assert(CH % 2 == 0, "the column terms take rows in pairs");
+    constexpr int G = PT == 512 ? CH : 2;
+    static_assert(CH % G == 0 && G % 2 == 0, "the column terms take rows in pairs");
     const int c0 = tid % TI, r0 = tid / TI;
     const int ic = i0 + c0;
     const bool colok = ic < n1;
     // row of load t of the chunk that starts at row rb (a multiple of RPI CH)
     auto trow = [&](int rb, int t) -> int {
-        if constexpr (BC) return rb + CH * r0 + t;
+        if constexpr (BC) return rb + G * r0 + G * RPI * (t / G) + (t % G);
         else return rb + r0 + RPI * t;
     };
     // LDS: [DST tables][cos(pi j/ny) table][TI column slots]
@@ -567,21 +570,25 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
             // get_bc! sample idx = j + 1 = r + 2 of the left and right columns:
             // the even row re = trow(rb, 2u) (idx = 2e, e = re/2 + 1) takes the
             // mean of the staged pairs e and e - 1, the odd row re + 1
-            // (idx = 2e + 1) pair e.  The thread's rows need pairs e0 - 1 ..
-            // e0 - 1 + CH/2; those past the last pair (rows beyond the line)
+            // (idx = 2e + 1) pair e.  A group's rows need pairs e0 - 1 ..
+            // e0 - 1 + G/2; those past the last pair (rows beyond the line)
             // are read clamped and not used.
-            const int e0 = trow(rb, 0) / 2 + 1;
-            double2 pr[CH / 2 + 1];
 #pragma unroll
-            for (int u = 0; u <= CH / 2; u++) pr[u] = edl[ep_slot(min(e0 - 1 + u, nep - 1))];
+            for (int gi = 0; gi < CH / G; gi++) {
+                const int e0 = trow(rb, G * gi) / 2 + 1;
+                double2 pr[G / 2 + 1];
 #pragma unroll
-            for (int u = 0; u < CH / 2; u++) {
-                const int re = trow(rb, 2 * u);
-                if (re < n2) {
-                    const double2 ea = pr[u + 1], eb = pr[u];
-                    const double bl = 0.5 * (ea.x + eb.x), br = 0.5 * (ea.y + eb.y);
-                    v[2 * u] += ek * (bl + esg * br);
-                    if (re + 1 < n2) v[2 * u + 1] += ek * (ea.x + esg * ea.y);
+                for (int u = 0; u <= G / 2; u++) pr[u] = edl[ep_slot(min(e0 - 1 + u, nep - 1))];
+#pragma unroll
+                for (int u = 0; u < G / 2; u++) {
+                    const int t = G * gi + 2 * u;
+                    const int re = trow(rb, t);
+                    if (re < n2) {
+                        const double2 ea = pr[u + 1], eb = pr[u];
+                        const double bl = 0.5 * (ea.x + eb.x), br = 0.5 * (ea.y + eb.y);
+                        v[t] += ek * (bl + esg * br);
+                        if (re + 1 < n2) v[t + 1] += ek * (ea.x + esg * ea.y);
+                    }
                 }
             }
         }
