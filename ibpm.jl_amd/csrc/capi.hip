@@ -20,6 +20,7 @@ struct ibpm_ctx {
     std::string err;
 };
 
+
 struct ibpm_group;
 
 namespace {
@@ -33,6 +34,48 @@ struct DevTables {
 
 struct CoarsifyRange {
     int i0, ic0, ci, j0, jc0, cj;
+};
+
+// CD/CL history of queued steps: a device buffer written by k_forces and a
+// pinned host mirror it is copied to when the steps are waited for.  It
+// grows geometrically between waits.  The buffers it outgrows are released
+// only with their owner: hipFree and hipHostFree wait for the whole device,
+// which would serialise problems that step on their own streams.
+struct HistBuf {
+    double* d = nullptr;
+    double* h = nullptr;
+    size_t n = 0;  // doubles in d and in h
+    std::vector<void*> old_d, old_h;
+    hipError_t reserve(size_t want)
+    {
+        if (want <= n) return hipSuccess;
+        const size_t cap = std::max({want, 2 * n, (size_t)256});
+        double *nd = nullptr, *nh = nullptr;
+        hipError_t e = hipMalloc((void**)&nd, sizeof(double) * cap);
+        if (e != hipSuccess) return e;
+        e = hipHostMalloc((void**)&nh, sizeof(double) * cap, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            old_d.push_back(nd);
+            return e;
+        }
+        if (d) old_d.push_back(d);
+        if (h) old_h.push_back(h);
+        d = nd;
+        h = nh;
+        n = cap;
+        return hipSuccess;
+    }
+    void release()  // the owner's stream is drained
+    {
+        if (d) old_d.push_back(d);
+        if (h) old_h.push_back(h);
+        for (void* p : old_d) (void)hipFree(p);
+        for (void* p : old_h) (void)hipHostFree(p);
+        old_d.clear();
+        old_h.clear();
+        d = h = nullptr;
+        n = 0;
+    }
 };
 
 // E' of the current body points as a segmented gather: for every touched flux
@@ -106,7 +149,8 @@ struct ibpm_prob {
     std::vector<double> Bh;
     int cg_iters = 0;
     double cg_res = 0.0, cg_tol = 0.0;
-    double* hist = nullptr;   // CD history [hist_steps x nbodies], then CL history
+    HistBuf hb;               // CD/CL history storage (device and pinned host mirror)
+    double* hist = nullptr;   // = hb.d: CD history [hist_steps x nbodies], then CL history
     size_t hist_steps = 0;
     int pending = 0;          // steps enqueued by ibpm_advance_async, not yet waited for
     int* slot = nullptr;      // device: history entry of the next step (advanced by k_forces)
@@ -151,6 +195,15 @@ static int fail(ibpm_ctx* c, int code, const std::string& m)
         if (e_ != hipSuccess)                                                                       \
             return fail((ctx), IBPM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
     } while (0)
+
+// Entry points that replace buffers or captured step graphs refuse to run
+// while steps enqueued by ibpm_advance_async are still in flight.
+static int check_idle(ibpm_prob* P)
+{
+    if (P->pending > 0)
+        return fail(P->ctx, IBPM_E_ARG, "steps enqueued by ibpm_advance_async are not waited for");
+    return IBPM_OK;
+}
 
 template <typename T>
 static hipError_t dalloc(ibpm_prob* P, T** p, size_t n)
@@ -436,22 +489,24 @@ static hipError_t upload_reg(ibpm_prob* P)
 }
 
 // x = L b for one level (lev 0-based), L one of Δinv / A / Ainv (laplacian.jl:24-76),
-// with an optional subtract epilogue.
+// with an optional subtract epilogue.  tmp: the level's transform scratch
+// (nG doubles, every one written by the first pass), by default T1's level.
 static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x, bool sub, int zlo = 1,
-                      int zhi = 1 << 30)
+                      int zhi = 1 << 30, double* tmp = nullptr)
 {
     hipStream_t s = P->stream;
+    if (!tmp) tmp = P->T1 + (long)lev * P->g.nG;
     XArgs a = xargs(P);
     a.zlo = zlo;
     a.zhi = std::min(zhi, P->g.n2);
     a.src = b;
-    a.dst = P->T1 + (long)lev * P->g.nG;
+    a.dst = tmp;
     a.nlev = 1;
     a.lev0 = lev;
     launch_xpass(X_PLAIN, a, s);
-    launch_ypass(yargs(P, op, lev, 1, P->T1 + (long)lev * P->g.nG), P->ytile_single, s);
+    launch_ypass(yargs(P, op, lev, 1, tmp), P->ytile_single, s);
     XArgs c = xargs(P);
-    c.src = P->T1 + (long)lev * P->g.nG;
+    c.src = tmp;
     c.dst = x;
     c.nlev = 1;
     c.lev0 = lev;
@@ -1039,7 +1094,7 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
     };
     // ---- setup_reg (interface-coupling.jl:72-96)
     if (!setup_reg_host(P)) return bail(IBPM_E_ARG, "body point closer than 6 cells to the domain edge");
-    // ---- coarsify! ranges (mg-utils.jl:296-299, zip truncates to the shorter range)
+    // ---- coarsify! ranges (mg-utils.jl:77-80, zip truncates to the shorter range)
     {
         CoarsifyRange r{};
         int j0 = ny / 2 + ((-ny) / 2 + 2), j1 = ny / 2 + (ny / 2 - 2);
@@ -1142,9 +1197,9 @@ int ibpm_problem_destroy(ibpm_prob* P)
 {
     if (!P) return IBPM_OK;
     if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: destroy the group first");
-    if (P->hist && P->stream) (void)hipFreeAsync(P->hist, P->stream);
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     graphs_clear(P);
+    P->hb.release();
     for (void* p : P->allocs) (void)hipFree(p);
     if (P->pin) (void)hipHostFree(P->pin);
     for (hipEvent_t ev : P->pev) (void)hipEventDestroy(ev);
@@ -1157,6 +1212,7 @@ int ibpm_problem_linearize(ibpm_prob* P, const double* qB, const double* q0B, co
 {
     if (!P || !qB || !q0B || !GammaB) return IBPM_E_ARG;
     if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: step or change it through the group");
+    if (int rc = check_idle(P)) return rc;
     if (P->motion != IBPM_MOTION_STATIC)
         return fail(P->ctx, IBPM_E_UNSUPPORTED, "LinearizedIBProblem is defined for static bodies only");
     const GridDev& g = P->g;
@@ -1251,6 +1307,7 @@ int ibpm_state_upload(ibpm_prob* P, int field, const double* host, size_t n)
     double* d = field_ptr(P, field, &want, true);
     if (!d) return fail(P->ctx, IBPM_E_ARG, "unknown field");
     if (n != want) return fail(P->ctx, IBPM_E_ARG, "upload size mismatch");
+    if (int rc = check_idle(P)) return rc;
     graphs_clear(P);
     HIPTRY(P->ctx, hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, P->stream));
     HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
@@ -1273,6 +1330,7 @@ int ibpm_base_flux(ibpm_prob* P, const ibpm_step_scalars* s)
 {
     if (!P || !s) return IBPM_E_ARG;
     if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: step or change it through the group");
+    if (int rc = check_idle(P)) return rc;
     graphs_clear(P);  // captured nonlinear launches hold the base flux by value
     if (P->lin) {
         // static bodies of a linearized problem: q0 = 0 (timestepping.jl:238-250)
@@ -1310,6 +1368,7 @@ int ibpm_state_init(ibpm_prob* P, const ibpm_step_scalars* s)
 {
     if (!P || !s) return IBPM_E_ARG;
     if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: step or change it through the group");
+    if (int rc = check_idle(P)) return rc;
     zero_state(P);
     return ibpm_base_flux(P, s);
 }
@@ -1318,6 +1377,7 @@ int ibpm_vort2flux(ibpm_prob* P)
 {
     if (!P) return IBPM_E_ARG;
     if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: step or change it through the group");
+    if (int rc = check_idle(P)) return rc;
     vort2flux_dev(P, P->G, P->psi, P->q, false);
     HIPTRY(P->ctx, hipGetLastError());
     return IBPM_OK;
@@ -1341,17 +1401,13 @@ int ibpm_advance_async(ibpm_prob* P, int nsteps, const ibpm_step_scalars* s)
         // the history buffer grows only between waits (its halves hold CD and CL)
         if (P->pending > 0)
             return fail(P->ctx, IBPM_E_ARG, "history full: call ibpm_advance_wait before enqueuing more steps");
-        // Stream-ordered free and allocation: hipFree would wait for the whole
-        // device, i.e. for the steps other problems have queued on their own
-        // streams, and so serialise a batch of problems.  The capacity grows
-        // geometrically so that a run of growing calls reallocates rarely.
-        const size_t cap = std::max({want, 2 * P->hist_steps, (size_t)64});
-        if (P->hist) (void)hipFreeAsync(P->hist, P->stream);
-        P->hist = nullptr;
-        P->hist_steps = 0;
+        // No free here: hipFree would wait for the whole device, i.e. for the
+        // steps other problems have queued on their own streams (HistBuf keeps
+        // the outgrown buffers until the problem is destroyed).
         graphs_clear(P);  // captured steps hold the old history buffer
-        HIPTRY(P->ctx, hipMallocAsync((void**)&P->hist, sizeof(double) * 2 * cap * (size_t)nb, P->stream));
-        P->hist_steps = cap;
+        HIPTRY(P->ctx, P->hb.reserve(2 * want * (size_t)nb));
+        P->hist = P->hb.d;
+        P->hist_steps = P->hb.n / (2 * (size_t)nb);
     }
     if (nsteps > 0) launch_set_int(P->slot, P->pending, P->stream);
     if (P->motion == IBPM_MOTION_FUNCTION) {
@@ -1375,13 +1431,18 @@ int ibpm_advance_wait(ibpm_prob* P, double* cd_hist, double* cl_hist, int* steps
     if (steps_done) *steps_done = 0;
     const int nb = std::max(P->nbodies, 1);
     const size_t n = (size_t)P->pending;
-    std::vector<double> cd(n * nb), cl(n * nb);
+    // into the pinned mirror (a straight DMA), then out of it on the host
     if (n > 0) {
-        HIPTRY(P->ctx, hipMemcpyAsync(cd.data(), P->hist, sizeof(double) * cd.size(), hipMemcpyDeviceToHost, P->stream));
-        HIPTRY(P->ctx, hipMemcpyAsync(cl.data(), P->hist + P->hist_steps * nb, sizeof(double) * cl.size(),
-                                      hipMemcpyDeviceToHost, P->stream));
+        HIPTRY(P->ctx, hipMemcpyAsync(P->hb.h, P->hist, sizeof(double) * n * nb, hipMemcpyDeviceToHost, P->stream));
+        HIPTRY(P->ctx, hipMemcpyAsync(P->hb.h + P->hist_steps * nb, P->hist + P->hist_steps * nb,
+                                      sizeof(double) * n * nb, hipMemcpyDeviceToHost, P->stream));
     }
     HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    std::vector<double> cd(n * nb), cl(n * nb);
+    if (n > 0) {
+        std::copy(P->hb.h, P->hb.h + n * nb, cd.begin());
+        std::copy(P->hb.h + P->hist_steps * nb, P->hb.h + P->hist_steps * nb + n * nb, cl.begin());
+    }
     collect_profile(P);
     P->pending = 0;
     if (P->async_err) {
@@ -1406,7 +1467,14 @@ int ibpm_advance_n(ibpm_prob* P, int nsteps, const double* t, const ibpm_step_sc
     if (steps_done) *steps_done = 0;
     if (P->pending > 0) return fail(P->ctx, IBPM_E_ARG, "steps enqueued by ibpm_advance_async are not waited for");
     const int rc = ibpm_advance_async(P, nsteps, s);
-    if (rc != IBPM_OK) return rc;
+    if (rc != IBPM_OK) {
+        // a step that failed part way through (a MotionFunction body leaving
+        // the domain): collect the steps that finished, keep the failure
+        const std::string msg = P->ctx->err;
+        if (P->pending > 0) (void)ibpm_advance_wait(P, cd_hist, cl_hist, steps_done);
+        P->pending = 0;
+        return fail(P->ctx, rc, msg);
+    }
     return ibpm_advance_wait(P, cd_hist, cl_hist, steps_done);
 }
 
@@ -1442,6 +1510,7 @@ int ibpm_op_update_bodies(ibpm_prob* P, const double* kin)
     if (!P || !kin) return IBPM_E_ARG;
     if (P->motion != IBPM_MOTION_FUNCTION)
         return fail(P->ctx, IBPM_E_ARG, "update_bodies! moves MotionFunction bodies only");
+    if (int rc = check_idle(P)) return rc;
     graphs_clear(P);
     int rc = update_bodies_dev(P, kin);
     if (rc != IBPM_OK) return rc;
@@ -1485,7 +1554,8 @@ struct ibpm_group {
     int sc_ncell = 0;
     int rb_i0 = 2, rb_i1 = 1, rb_j0 = 2, rb_j1 = 1;
     std::vector<int> nbh, hoff;  // history: instance i's CD block at hist + hoff[i], CL block after it
-    double* hist = nullptr;
+    HistBuf hb;              // history storage (device and pinned host mirror)
+    double* hist = nullptr;  // = hb.d
     size_t hist_steps = 0;
     int pending = 0;
     int ytile = 4, ytile_single = 4, xwaves = 4;
@@ -1661,16 +1731,16 @@ int group_hist(ibpm_group* Gp, size_t steps)
 {
     if (steps <= Gp->hist_steps) return IBPM_OK;
     const size_t cap = std::max({steps, 2 * Gp->hist_steps, (size_t)64});
-    if (Gp->hist) (void)hipFreeAsync(Gp->hist, Gp->stream);
-    Gp->hist = nullptr;
-    Gp->hist_steps = 0;
     size_t tot = 0;
+    for (int i = 0; i < Gp->n; i++) tot += 2 * cap * (size_t)Gp->nbh[i];
+    HIPTRY(Gp->ctx, Gp->hb.reserve(tot));
+    Gp->hist = Gp->hb.d;
+    Gp->hist_steps = cap;
+    tot = 0;
     for (int i = 0; i < Gp->n; i++) {
         Gp->hoff[i] = (int)tot;
         tot += 2 * cap * (size_t)Gp->nbh[i];
     }
-    HIPTRY(Gp->ctx, hipMallocAsync((void**)&Gp->hist, sizeof(double) * tot, Gp->stream));
-    Gp->hist_steps = cap;
     for (int i = 0; i < Gp->n; i++) {
         Gp->fargs_h[i].cd_hist = Gp->hist + Gp->hoff[i];
         Gp->fargs_h[i].cl_hist = Gp->hist + Gp->hoff[i] + cap * (size_t)Gp->nbh[i];
@@ -1867,8 +1937,8 @@ int ibpm_group_destroy(ibpm_group* Gp)
         P->Fb = o.Fb;
         P->group = nullptr;
     }
-    if (Gp->hist) (void)hipFreeAsync(Gp->hist, Gp->stream);
     (void)hipStreamSynchronize(Gp->stream);
+    Gp->hb.release();
     for (void* p : Gp->allocs) (void)hipFree(p);
     if (Gp->stream) (void)hipStreamDestroy(Gp->stream);
     ibpm_ctx* ctx = Gp->ctx;
@@ -1911,13 +1981,13 @@ int ibpm_group_wait(ibpm_group* Gp, double* cd_hist, double* cl_hist, int32_t* s
     if (!Gp) return IBPM_E_ARG;
     const size_t m = (size_t)Gp->pending;
     std::vector<double> h;
+    size_t tot = 0;
     if (m > 0 && Gp->hist) {
-        size_t tot = 0;
         for (int i = 0; i < Gp->n; i++) tot = std::max(tot, Gp->hoff[i] + 2 * Gp->hist_steps * (size_t)Gp->nbh[i]);
-        h.resize(tot);
-        HIPTRY(Gp->ctx, hipMemcpyAsync(h.data(), Gp->hist, sizeof(double) * tot, hipMemcpyDeviceToHost, Gp->stream));
+        HIPTRY(Gp->ctx, hipMemcpyAsync(Gp->hb.h, Gp->hist, sizeof(double) * tot, hipMemcpyDeviceToHost, Gp->stream));
     }
     HIPTRY(Gp->ctx, hipStreamSynchronize(Gp->stream));
+    if (tot) h.assign(Gp->hb.h, Gp->hb.h + tot);
     Gp->pending = 0;
     bool nonfinite = false;
     size_t o = 0;
@@ -2089,6 +2159,7 @@ int ibpm_op_rot(ibpm_prob* P, const double* q, double* G)
 int ibpm_set_option(ibpm_prob* P, int option, int value)
 {
     if (!P) return IBPM_E_ARG;
+    if (int rc = check_idle(P)) return rc;
     graphs_clear(P);
     if (option == IBPM_OPT_XWAVES_BATCH) {
         if (value != 4 && value != 8 && value != 12 && value != 14)

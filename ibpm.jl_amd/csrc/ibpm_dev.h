@@ -4,7 +4,7 @@
 // comments): a circulation-like field of one grid level is an (nx-1) x (ny-1)
 // array; a flux field of one level is the x-fluxes as an (nx+1) x ny array
 // followed by the y-fluxes as an nx x (ny+1) array
-// (src/fluid-domain/fluid-domain.jl:192-202).  Levels are consecutive columns
+// (src/fluid-domain/fluid-domain.jl:55-65).  Levels are consecutive columns
 // of an (n x mg) matrix.
 #pragma once
 #include <hip/hip_runtime.h>

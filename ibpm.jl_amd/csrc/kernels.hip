@@ -1319,16 +1319,25 @@ void launch_set_unit(double* p, int n, int j, hipStream_t s)
     hipLaunchKernelGGL(k_set_unit, dim3(1), dim3(256), 0, s, p, n, j);
 }
 
+// max that keeps a NaN (fmax drops it): maximum(abs.(q)) of a state holding a
+// NaN is NaN in the reference (ibpm.jl:83-88)
+__device__ inline double nanmax(double m, double v)
+{
+    if (m != m) return m;
+    if (v != v) return v;
+    return v > m ? v : m;
+}
+
 __global__ void __launch_bounds__(256) k_absmax(const double* p, long n, double* out)
 {
     __shared__ double red[4];
     double m = 0.0;
-    for (long i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(p[i]));
+    for (long i = threadIdx.x; i < n; i += blockDim.x) m = nanmax(m, fabs(p[i]));
 #pragma unroll
-    for (int off = WAVE / 2; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, WAVE));
+    for (int off = WAVE / 2; off > 0; off >>= 1) m = nanmax(m, __shfl_xor(m, off, WAVE));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = m;
     __syncthreads();
-    if (threadIdx.x == 0) *out = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    if (threadIdx.x == 0) *out = nanmax(nanmax(red[0], red[1]), nanmax(red[2], red[3]));
 }
 void launch_absmax(const double* p, long n, double* out, hipStream_t s)
 {

@@ -1,6 +1,6 @@
 """Fluid and structure domains: the host-side constructors of the reference.
 
-MultiGrid / make_grid        -- src/fluid-domain/fluid-domain.jl:10-25,181-210
+MultiGrid / make_grid        -- src/fluid-domain/fluid-domain.jl:10-25,44-73
 Motion types                 -- src/structure-domain/motion-types.jl:9-50
 RigidBody                    -- src/structure-domain/body-types.jl:10-16
 make_plate / make_cylinder   -- src/structure-domain/sample-bodies.jl:17-65
@@ -65,7 +65,7 @@ def _julia_round(x: float) -> int:
 
 
 def make_grid(h: float, boundary, mg: int = 1) -> MultiGrid:
-    """make_grid(h, boundary; mg) -- fluid-domain.jl:181-210."""
+    """make_grid(h, boundary; mg) -- fluid-domain.jl:44-73."""
     x0, x1, y0, y1 = (float(b) for b in boundary)
     offx = -x0
     offy = -y0
@@ -126,7 +126,7 @@ class MovingGrid(Motion):
     """Body-fixed frame with prescribed U(t), V(t), θ(t), θ̇(t) (motion-types.jl:42-50).
 
     xc, yc are stored as in the reference, which never reads them: the
-    rotation centre is always (0, 0) (models.jl:128, problem-types.jl:195).
+    rotation centre is always (0, 0) (models.jl:128, problem-types.jl:36).
     """
 
     def __init__(self, U: Callable, V: Callable, θ: Callable, θ̇: Callable, xc: float = 0.0, yc: float = 0.0):

@@ -1,6 +1,6 @@
 """IBProblem / IBState / advance! backed by the HIP library.
 
-IBModel, IBProblem     -- src/pre-processing/models.jl:115-158, problem-types.jl:182-200
+IBModel, IBProblem     -- src/pre-processing/models.jl:115-158, problem-types.jl:23-41
 AdamsBashforth, AB2    -- src/pre-processing/models.jl:60-72
 IBState                -- src/pre-processing/state-types.jl:7-47
 advance!, advance      -- src/timestepping/timestepping.jl:9-67
@@ -40,7 +40,7 @@ def _zero(t):
 
 def _freestream(fs) -> dict:
     """Normalise the freestream closures; the reference default is all zero
-    (problem-types.jl:192)."""
+    (problem-types.jl:33)."""
     fs = dict(fs or {})
     out = {}
     for k in ("Ux", "Uy", "inclination"):
@@ -60,7 +60,7 @@ class IBModel:
 
 
 class IBProblem:
-    """IBProblem(grid, bodies, dt, Re; freestream) -- problem-types.jl:188-200.
+    """IBProblem(grid, bodies, dt, Re; freestream) -- problem-types.jl:23-41.
 
     The operators (Λ, DST plans, E, Ainv/A per level) and prob.Binv are built by
     the library on `device`; `Binv` may be given to skip the GPU assembly of B.
@@ -235,7 +235,7 @@ _FIELDS = {"q": C.FIELD_Q, "q0": C.FIELD_Q0, "Γ": C.FIELD_GAMMA, "ψ": C.FIELD_
 
 
 class IBState:
-    """IBState(prob) -- state-types.jl:270-293 (zeros, then base_flux! at t = 0)."""
+    """IBState(prob) -- state-types.jl:25-47 (zeros, then base_flux! at t = 0)."""
 
     def __init__(self, prob: IBProblem):
         self.prob = prob

@@ -5,8 +5,8 @@
  * The reference has no FFI for this path: its boundary is the Julia method
  *   advance!(state::IBState, prob::AbstractIBProblem, t::Float64)
  * (src/timestepping/timestepping.jl:35-67), with state and problem built by
- *   IBProblem(grid, bodies, dt, Re; freestream)  (src/pre-processing/problem-types.jl:188-200)
- *   IBState(prob)                                (src/pre-processing/state-types.jl:270-293)
+ *   IBProblem(grid, bodies, dt, Re; freestream)  (src/pre-processing/problem-types.jl:23-41)
+ *   IBState(prob)                                (src/pre-processing/state-types.jl:25-47)
  * The functions below are what a `ccall` layer binding that method would
  * call.  Every array crossing this ABI uses the reference's exact memory
  * layout (Julia column-major, one column per grid level):
@@ -16,7 +16,7 @@
  *   F̃b              : 2 nb_total  [fx(1..nb_total); fy(1..nb_total)]
  *   fb              : for each body j its (nb_j x 2) block, bodies in order
  *   CD, CL          : one value per body
- * (src/fluid-domain/fluid-domain.jl:181-210, src/pre-processing/state-types.jl:7-47).
+ * (src/fluid-domain/fluid-domain.jl:44-73, src/pre-processing/state-types.jl:7-47).
  *
  * Conventions: every function returns 0 on success or a negative IBPM_E_*
  * code; the message of the last failure on a context is returned by
@@ -67,8 +67,8 @@ typedef struct ibpm_group ibpm_group;
 #define IBPM_FIELD_CL 9
 
 /* Grid + scheme description: make_grid(h, boundary; mg)
- * (src/fluid-domain/fluid-domain.jl:181-210), Re and dt of IBProblem
- * (problem-types.jl:188-192), AB2 coefficients (models.jl:70-72). */
+ * (src/fluid-domain/fluid-domain.jl:44-73), Re and dt of IBProblem
+ * (problem-types.jl:29-33), AB2 coefficients (models.jl:70-72). */
 typedef struct {
     int32_t nx, ny, mg;
     double h;            /* grid.h = len/nx */
@@ -89,7 +89,7 @@ typedef struct {
 } ibpm_body_desc;
 
 /* Closures of the problem evaluated by the caller at the step time t
- * (freestream: src/pre-processing/problem-types.jl:192; MovingGrid U,V,θ,θ̇:
+ * (freestream: src/pre-processing/problem-types.jl:33; MovingGrid U,V,θ,θ̇:
  * src/structure-domain/motion-types.jl:42-50). */
 typedef struct {
     double Ux, Uy, inclination;      /* freestream.Ux(t), .Uy(t), .inclination(t) */
@@ -115,7 +115,7 @@ int ibpm_ctx_create(int device, ibpm_ctx** out);
 int ibpm_ctx_destroy(ibpm_ctx* ctx);
 const char* ibpm_last_error(const ibpm_ctx* ctx);
 
-/* IBProblem(grid, bodies, dt, Re; freestream) -- problem-types.jl:188-200,
+/* IBProblem(grid, bodies, dt, Re; freestream) -- problem-types.jl:23-41,
  * including IBModel/IBMatrices (models.jl:33-52,124-158: Λ, the DST plans,
  * E = setup_reg, interface-coupling.jl:72-96) and get_AB (laplacian.jl:83-89).
  * binv: if non-NULL, the (2 nb_total)^2 column-major prob.Binv to use; if
@@ -146,7 +146,7 @@ int ibpm_problem_get_body_idx(ibpm_prob* prob, int32_t* out, size_t n);
 int ibpm_state_upload(ibpm_prob* prob, int field, const double* host, size_t n);
 int ibpm_state_download(ibpm_prob* prob, int field, double* host, size_t n);
 /* IBState(prob): zero every field, then base_flux!(state, prob, 0)
- * (state-types.jl:270-293; timestepping.jl:252-270 / 273-318). */
+ * (state-types.jl:25-47; timestepping.jl:252-270 / 273-318). */
 int ibpm_state_init(ibpm_prob* prob, const ibpm_step_scalars* at_t0);
 /* base_flux!(state, prob, t) (timestepping.jl:230-318). */
 int ibpm_base_flux(ibpm_prob* prob, const ibpm_step_scalars* at_t);

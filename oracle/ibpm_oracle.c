@@ -225,7 +225,7 @@ static void o_dst2(double* y, const double* x, int n1, int n2, const odst_plan* 
 }
 
 /* ------------------------------------------------------------------------- */
-/* grid (src/fluid-domain/fluid-domain.jl:10-25,181-210)                      */
+/* grid (src/fluid-domain/fluid-domain.jl:10-25,44-73)                      */
 /* ------------------------------------------------------------------------- */
 typedef struct {
     int nx, ny, nG, nu, nv, nq, mg, nbc;
@@ -233,7 +233,7 @@ typedef struct {
     int LEFT, RIGHT, BOT, TOP;
 } og_grid;
 
-/* make_grid(h, boundary; mg) -- fluid-domain.jl:181-210.  Julia's round() is
+/* make_grid(h, boundary; mg) -- fluid-domain.jl:44-73.  Julia's round() is
  * round-half-to-even, which is nearbyint() under the default rounding mode. */
 static void og_make_grid(og_grid* g, double h, double x0, double x1, double y0, double y1, int mg)
 {
@@ -715,7 +715,7 @@ typedef struct {
 
 void oracle_problem_free(oprob* P);
 
-/* IBProblem(grid, bodies, dt, Re) -- problem-types.jl:188-200 with IBModel
+/* IBProblem(grid, bodies, dt, Re) -- problem-types.jl:23-41 with IBModel
  * (models.jl:124-158), IBMatrices (models.jl:33-52) and get_AB
  * (laplacian.jl:83-89).  When binv == NULL, Binv is formed by get_Binv
  * (laplacian.jl:98-119); otherwise it is copied from binv. */

@@ -137,7 +137,7 @@ class OState:
 
 
 class OracleProblem:
-    """IBProblem + IBModel restated on the CPU (problem-types.jl:188-200).
+    """IBProblem + IBModel restated on the CPU (problem-types.jl:23-41).
 
     bodies: list of (xb (n,2), ds (n,)) arrays.  motion: 0 Static, 1 MovingGrid,
     2 MotionFunction (lab-frame moving bodies; xb are the reference points x0).

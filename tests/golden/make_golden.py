@@ -6,7 +6,7 @@ Run in a container that has the reference mounted:
 Outputs (data only -- no reference source is copied):
   eldredge_force_jl2.npz  t, CL, CD of benchmarks/eldredge/results/force_jl2.mat,
                           the Julia body-fixed (MovingGrid) pitch-hold run
-                          (benchmarks/eldredge/pitch_maneuver.jl:204-275), lab-rotated.
+                          (benchmarks/eldredge/pitch_maneuver.jl:7-78), lab-rotated.
                           MAT v7.3 = HDF5; the three 10001-double datasets are
                           stored contiguously and uncompressed at the byte
                           offsets below (checked: t runs 0:0.001:10).

@@ -124,3 +124,38 @@ def test_leaving_the_grid_is_an_error_and_keeps_the_state():
     assert e.value.code == C.IBPM_E_ARG
     assert np.array_equal(st.Γ, G1)
     assert np.array_equal(body.xb, xb1)
+
+
+def test_body_leaving_domain_fails_step_and_keeps_problem_usable():
+    """A lab-frame body driven towards the domain edge: the step that would
+    bring a point within 6 cells of it fails with IBPM_E_ARG (the reference's
+    E window raises a BoundsError there), the steps before it are returned,
+    and the problem steps on afterwards."""
+    import ctypes
+    import math
+
+    grid = ibpm.make_grid(1.0 / 16, (-1.5, 2.5, -2.0, 2.0), mg=2)
+    mot = ibpm.MotionFunction([lambda t: 20.0 * t, lambda t: 0.0, lambda t: 0.0],
+                              [lambda t: 20.0, lambda t: 0.0, lambda t: 0.0])
+    bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, motion=mot, n=40)]
+    gp = ibpm.IBProblem(grid, bodies, 0.01, 100.0, freestream={"Ux": lambda t: 1.0})
+    st = ibpm.IBState(gp)
+    # x = 20 t: the right edge of the cylinder (x = 0.5 + 20 t) must stay below 2.1875
+    # (body_idx + 6 <= nx): steps up to t = 0.08 pass, the step at t = 0.09 fails
+    ts = np.arange(20) * 0.01
+    arr, keep = gp.step_scalars(ts)
+    cd = np.zeros((20, 1))
+    cl = np.zeros((20, 1))
+    done = ctypes.c_int(-1)
+    rc = C.lib().ibpm_advance_n(gp.h, 20, C.ptr(ts), arr, C.ptr(cd), C.ptr(cl), ctypes.byref(done))
+    assert rc == C.IBPM_E_ARG
+    k = done.value
+    assert k == 9 and np.all(np.isfinite(cd[:k]))
+    print(f"steps before the failing one: {k}")
+    # the failing step left the body where the last finished step put it
+    xb, _ = gp.body_state()
+    assert math.isclose(float(np.max(xb[:, 0])), 0.5 + 20.0 * ts[k - 1], rel_tol=0, abs_tol=1e-12)
+    # the problem is still usable: steps at times the body stays inside
+    ts2 = ts[:3]
+    cd2, cl2, done2 = ibpm.advance_n_(st, gp, ts2)
+    assert done2 == 3 and np.all(np.isfinite(cd2))

@@ -261,6 +261,57 @@ def test_run_sim_saves_and_cfl():
     assert st.cfl > 0  # computed after step 25 (display_freq)
 
 
+def test_cfl_matches_oracle_state():
+    """compute_cfl = max|q[:,1]| dt / h (ibpm.jl:83-88), against the same
+    maximum taken over the oracle's state after the same steps."""
+    c, op, gp = build("tiny", share_binv=True)
+    S, st = initial_states(c, op, gp)
+    ts = np.arange(12) * c["dt"]
+    op.run(S, np.stack([sc7(c, t) for t in ts]))
+    ibpm.advance_n_(st, gp, ts)
+    g = c["grid"]
+    cfl_o = float(np.max(np.abs(S.q[:, 0]))) * c["dt"] / g.h
+    cfl = ibpm.compute_cfl(st, gp)
+    print(f"cfl {cfl!r} oracle {cfl_o!r}")
+    assert cfl_o > 0.01 and abs(cfl - cfl_o) <= 1e-10 * cfl_o
+
+
+def test_nonfinite_cl_stops_run_sim():
+    """A NaN uploaded into Γ: the next step's CL is non-finite, ibpm_advance_n
+    returns IBPM_E_NONFINITE after that step, run_sim! stops there
+    (ibpm.jl:95-97) and compute_cfl reports NaN (the reference's maximum
+    propagates it)."""
+    import ctypes
+
+    from ibpm import _capi as C
+
+    c, op, gp = build("tiny", share_binv=True)
+    _, st = initial_states(c, op, gp)
+    G0 = np.array(st.Γ)
+    G = G0.copy()
+    G[G.shape[0] // 2, 0] = np.nan
+    st.Γ = G
+    t = np.arange(30) * c["dt"]
+    arr, keep = gp.step_scalars(t[:5])
+    cd = np.zeros((5, 1))
+    cl = np.zeros((5, 1))
+    done = ctypes.c_int(-1)
+    rc = C.lib().ibpm_advance_n(gp.h, 5, C.ptr(t[:5]), arr, C.ptr(cd), C.ptr(cl), ctypes.byref(done))
+    st.invalidate()
+    assert rc == C.IBPM_E_NONFINITE and done.value == 1 and not np.isfinite(cl[0, 0])
+    assert math.isnan(ibpm.compute_cfl(st, gp))
+    st.Γ = G
+    ibpm.vort2flux_(st, gp)
+    cd, cl = ibpm.run_sim_(t, st, gp, display_freq=25)
+    assert cl.shape == (1, 1) and not np.isfinite(cl[0, 0])
+    # the AB2 history now holds NaN too; a fresh state steps on the same problem
+    st = ibpm.IBState(gp)
+    st.Γ = G0
+    ibpm.vort2flux_(st, gp)
+    cd, cl = ibpm.run_sim_(t, st, gp, display_freq=25)
+    assert cl.shape == (30, 1) and np.all(np.isfinite(cl))
+
+
 @pytest.mark.parametrize("name", ["small3", "tiny"])
 def test_autotune_applies_legal_shapes(name):
     """The autotuner sweeps its options on a problem (one-level problems

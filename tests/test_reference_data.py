@@ -68,7 +68,7 @@ def test_oracle_steady_plate_rerun():
 # GPU
 # ---------------------------------------------------------------------------
 def eldredge_motion():
-    """Pitch-hold kinematics of benchmarks/eldredge/pitch_maneuver.jl:208-218."""
+    """Pitch-hold kinematics of benchmarks/eldredge/pitch_maneuver.jl:12-19."""
     a = 11.0
     t1, t2, t3, t4 = 1.0, 3.0, 4.0, 6.0
     amax = 45 * math.pi / 180
@@ -116,7 +116,7 @@ def test_gpu_steady_plate_cd(steady_plate):
 def test_gpu_eldredge_pitch_hold(steady_plate):
     """Pitch-hold from the steady state (pitch_maneuver.jl): MovingGrid with
     U = 1, θ(t) of the Eldredge maneuver, dt = 1e-3, t = 0:1e-3:10, forces
-    rotated into the lab frame (pitch_maneuver.jl:269-270).  The stored Julia
+    rotated into the lab frame (pitch_maneuver.jl:72-73).  The stored Julia
     run is reproduced to about 2.5e-12 of the peak lift over all 10001 steps
     (Julia and Fortran agree on it to about 1e-3)."""
     import ibpm
