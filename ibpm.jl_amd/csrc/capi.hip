@@ -18,8 +18,16 @@ using namespace ibpm;
 struct ibpm_ctx {
     int device = 0;
     std::string err;
+    // device address of this device's copy of `scratch` (ibpm_op_lapinv_noalloc)
+    double* scratch = nullptr;
 };
 
+// Scratch of ibpm_op_lapinv_noalloc: the x-transformed level between the
+// passes of one Laplacian-inverse application.  Each device holds its own copy
+// until the process ends.  In host code this name stands for a host-side
+// symbol, not a device address: kernels receive ibpm_ctx::scratch, which
+// ibpm_ctx_create takes from hipGetSymbolAddress with the device current.
+static __device__ __align__(256) double scratch[IBPM_NOALLOC_MAX_POINTS];
 
 struct ibpm_group;
 
@@ -1032,8 +1040,11 @@ int ibpm_ctx_create(int device, ibpm_ctx** out)
     if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return IBPM_E_HIP;
     if (hipSetDevice(device) != hipSuccess) return IBPM_E_HIP;
     kernels_init();
+    void* sp = nullptr;
+    if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) != hipSuccess || !sp) return IBPM_E_HIP;
     ibpm_ctx* c = new ibpm_ctx();
     c->device = device;
+    c->scratch = static_cast<double*>(sp);
     *out = c;
     return IBPM_OK;
 }
@@ -2053,6 +2064,21 @@ int ibpm_op_lapinv(ibpm_prob* P, int kind, int lev, const double* b, double* x)
     if (!db || !dx) return fail(P->ctx, IBPM_E_HIP, "alloc");
     lap_level(P, kind == 0 ? Y_DINV : (kind == 1 ? Y_A : Y_AINV), lev - 1, db, dx, false);
     return get_back(P, x, dx, n);
+}
+
+int ibpm_op_lapinv_noalloc(ibpm_prob* P, int kind, int lev, const double* b, double* x, size_t n)
+{
+    if (!P || !b || !x || kind < 0 || kind > 2 || lev < 1 || lev > P->mg) return IBPM_E_ARG;
+    if (n != (size_t)P->g.nG) return fail(P->ctx, IBPM_E_ARG, "n must be (nx-1)(ny-1) of the problem's grid");
+    if (n > (size_t)IBPM_NOALLOC_MAX_POINTS)
+        return fail(P->ctx, IBPM_E_ARG, "the level has more points than the fixed scratch holds (IBPM_NOALLOC_MAX_POINTS)");
+    if (!P->ctx->scratch) return fail(P->ctx, IBPM_E_HIP, "no scratch address for this device");
+    // the x-pass writes every entry of the scratch before the y-pass reads it
+    lap_level(P, kind == 0 ? Y_DINV : (kind == 1 ? Y_A : Y_AINV), lev - 1, b, x, false, 1, 1 << 30,
+              P->ctx->scratch);
+    HIPTRY(P->ctx, hipGetLastError());
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    return IBPM_OK;
 }
 
 int ibpm_op_dst2(ibpm_prob* P, const double* x, double* y)

@@ -19,6 +19,7 @@ IBPM_E_HIP = -2
 IBPM_E_UNSUPPORTED = -3
 IBPM_E_SINGULAR = -4
 IBPM_E_NONFINITE = -5
+IBPM_NOALLOC_MAX_POINTS = 1 << 20  # doubles in the fixed scratch of ibpm_op_lapinv_noalloc
 
 MOTION_STATIC = 0
 MOTION_MOVING_GRID = 1
@@ -38,7 +39,7 @@ EXPORTED = [
     "ibpm_op_nonlinear", "ibpm_op_vort2flux", "ibpm_op_regT", "ibpm_op_reg", "ibpm_op_rot", "ibpm_set_option",
     "ibpm_get_option", "ibpm_time_kernel", "ibpm_profile_stats", "ibpm_get_body_state", "ibpm_cg_stats",
     "ibpm_op_update_bodies", "ibpm_group_create", "ibpm_group_destroy", "ibpm_group_set_option",
-    "ibpm_group_advance_async", "ibpm_group_wait",
+    "ibpm_group_advance_async", "ibpm_group_wait", "ibpm_op_lapinv_noalloc",
 ]
 
 
@@ -108,6 +109,7 @@ def lib():
         L.ibpm_get_forces.argtypes = [_vp, _dp, _dp, _dp]
         L.ibpm_compute_cfl.argtypes = [_vp, _dp]
         L.ibpm_op_lapinv.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.ibpm_op_lapinv_noalloc.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_size_t]
         L.ibpm_op_dst2.argtypes = [_vp, _dp, _dp]
         L.ibpm_op_nonlinear.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.ibpm_op_vort2flux.argtypes = [_vp, _dp, _dp, _dp]

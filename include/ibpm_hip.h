@@ -221,6 +221,20 @@ int ibpm_compute_cfl(ibpm_prob* prob, double* cfl);
 /* --- operator-level entry points (unit parity tests; host arrays in/out) --- */
 /* kind: 0 = Δinv (laplacian.jl:24-41), 1 = A[lev] (:73-76), 2 = Ainv[lev] (:61-71) */
 int ibpm_op_lapinv(ibpm_prob* prob, int kind, int lev, const double* b, double* x);
+/* ibpm_op_lapinv for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  It
+ * has no counterpart in the reference.  b and x are DEVICE pointers of
+ * n = (nx-1)(ny-1) doubles (x may not alias b); the transform between the
+ * passes goes through a fixed scratch array of IBPM_NOALLOC_MAX_POINTS
+ * doubles (8 MiB) that each device holds for the life of the process, so the
+ * call makes no hipMalloc, hipFree or other allocating call.  The work runs on
+ * the problem's stream and is waited for before the call returns.  n above
+ * the maximum (or different from the problem's level size) gives IBPM_E_ARG.
+ * Calls that overlap on one GPU (from several host threads) race on the
+ * shared scratch; the scratch keeps the last call's contents, all of which
+ * the next call overwrites before reading. */
+#define IBPM_NOALLOC_MAX_POINTS 1048576
+int ibpm_op_lapinv_noalloc(ibpm_prob* prob, int kind, int lev, const double* b, double* x, size_t n);
 /* 2D RODFT00 of one (nx-1) x (ny-1) array (dst-inversion.jl:10-20) */
 int ibpm_op_dst2(ibpm_prob* prob, const double* x, double* y);
 /* nonlinear!(out, state, bc(lev+1), lev) on host copies of q, q0, Γ (nonlinear.jl:168-201) */
