@@ -12,6 +12,7 @@
 
 #include "../../include/ibpm_hip.h"
 #include "kernels.h"
+#include "mixed.h"
 
 using namespace ibpm;
 
@@ -176,6 +177,17 @@ struct ibpm_prob {
     };
     std::vector<StepGraph> gcache;
     std::vector<void*> allocs;
+    // Mixed precision (IBPM_OPT_PRECISION): -1 the FP64 transforms; MX_FP32 or
+    // MX_BF16X3 the MFMA sine-matrix transforms.  Padded sizes P1, P2 (n1, n2
+    // rounded up to 64), the sine matrices of both dimensions (FP32 and bf16
+    // high/low parts, P x P), two padded FP32 fields per level, the per-level
+    // operand scales
+    int prec = -1;
+    int mx_sweeps = 2;  // refinement sweeps of every mixed-precision solve (IBPM_OPT_MX_SWEEPS)
+    int P1 = 0, P2 = 0;
+    float *mx_sf1 = nullptr, *mx_sf2 = nullptr, *mx_buf[2] = {nullptr, nullptr};
+    __bf16 *mx_sh1 = nullptr, *mx_sl1 = nullptr, *mx_sh2 = nullptr, *mx_sl2 = nullptr;
+    double* mx_lscale = nullptr;
     // in-step timing (IBPM_OPT_PROFILE): 1 the all-level y-pass launches, 2
     // the one-level y-passes with boundary terms of the vort2flux! chain;
     // event pairs recorded on the problem's stream, summed after each call
@@ -621,6 +633,243 @@ static void collect_profile(ibpm_prob* P)
 
 static void trial_dev(ibpm_prob* P);
 
+// ---------------------------------------------------------------------------
+// Mixed precision: the Laplacian inverses of the step as dense sine-matrix
+// transforms on the MFMA units (mixed.hip), in increment form.  Ainv and
+// Δinv act on residuals computed in FP64 from the current state, and their
+// results are added to FP64 fields:
+//   trial:     Γs = Γ + Ainv(A Γ - Ainv^{-1} Γ + β1 dt N1 + β2 dt N0 + rhsbc),
+//              with A Γ - Ainv^{-1} Γ = -2 qc L Γ by the 5-point stencil;
+//   vort2flux: per level, coarse to fine, ψ += Δinv(Γ + apply_bc!(get_bc!(ψ of
+//              the coarser level)) - L ψ), ψ starting from the state's ψ;
+//   projection: Γ[:,1] -= Ainv rot(E' F̃b).
+// In exact arithmetic these are the reference's operators (A and Ainv are
+// diagonal in the same sine basis; vort2flux! is linear and the state's ψ
+// solves it for the state's Γ), so the low-precision transforms only see the
+// per-step change of each field, and their rounding is relative to it.  A
+// second sweep of each solve (the default) applies the transforms once more
+// to the FP64 residual the first one left, squaring the relative error.
+// ---------------------------------------------------------------------------
+static uint16_t bf16_bits(float x)
+{
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+static float bf16_value(uint16_t b)
+{
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// P x P sine matrix of RODFT00 for lines of n points, zero outside n x n, as
+// FP32 and as bf16 high and low parts
+static void sine_matrix(int n, int Pn, std::vector<float>& f, std::vector<uint16_t>& hi, std::vector<uint16_t>& lo)
+{
+    const int N2 = 2 * (n + 1);
+    std::vector<double> tab(N2);
+    for (int t = 0; t < N2; t++) tab[t] = (double)(2.0L * sinl(PI_L * (long double)t / (long double)(n + 1)));
+    f.assign((size_t)Pn * Pn, 0.f);
+    hi.assign((size_t)Pn * Pn, 0);
+    lo.assign((size_t)Pn * Pn, 0);
+    for (int k = 0; k < n; k++)
+        for (int i = 0; i < n; i++) {
+            const double v = tab[((long)(k + 1) * (i + 1)) % N2];
+            const size_t e = (size_t)k * Pn + i;
+            f[e] = (float)v;
+            const uint16_t h = bf16_bits((float)v);
+            hi[e] = h;
+            lo[e] = bf16_bits((float)(v - (double)bf16_value(h)));
+        }
+}
+
+// device tables and buffers of the mixed form (once per problem)
+static int mx_setup(ibpm_prob* P)
+{
+    if (P->mx_sf1) return IBPM_OK;
+    const GridDev& g = P->g;
+    P->P1 = (g.n1 + 63) / 64 * 64;
+    P->P2 = (g.n2 + 63) / 64 * 64;
+    const size_t a1 = (size_t)P->P1 * P->P1, a2 = (size_t)P->P2 * P->P2, fl = (size_t)P->P1 * P->P2 * P->mg;
+    HIPTRY(P->ctx, dalloc(P, &P->mx_sf1, a1));
+    HIPTRY(P->ctx, dalloc(P, &P->mx_sf2, a2));
+    HIPTRY(P->ctx, dalloc(P, (uint16_t**)&P->mx_sh1, a1));
+    HIPTRY(P->ctx, dalloc(P, (uint16_t**)&P->mx_sl1, a1));
+    HIPTRY(P->ctx, dalloc(P, (uint16_t**)&P->mx_sh2, a2));
+    HIPTRY(P->ctx, dalloc(P, (uint16_t**)&P->mx_sl2, a2));
+    HIPTRY(P->ctx, dalloc(P, &P->mx_buf[0], fl));
+    HIPTRY(P->ctx, dalloc(P, &P->mx_buf[1], fl));
+    HIPTRY(P->ctx, dalloc(P, &P->mx_lscale, (size_t)P->mg));
+    std::vector<float> f;
+    std::vector<uint16_t> hi, lo;
+    sine_matrix(g.n1, P->P1, f, hi, lo);
+    HIPTRY(P->ctx, hipMemcpy(P->mx_sf1, f.data(), sizeof(float) * a1, hipMemcpyHostToDevice));
+    HIPTRY(P->ctx, hipMemcpy(P->mx_sh1, hi.data(), 2 * a1, hipMemcpyHostToDevice));
+    HIPTRY(P->ctx, hipMemcpy(P->mx_sl1, lo.data(), 2 * a1, hipMemcpyHostToDevice));
+    sine_matrix(g.n2, P->P2, f, hi, lo);
+    HIPTRY(P->ctx, hipMemcpy(P->mx_sf2, f.data(), sizeof(float) * a2, hipMemcpyHostToDevice));
+    HIPTRY(P->ctx, hipMemcpy(P->mx_sh2, hi.data(), 2 * a2, hipMemcpyHostToDevice));
+    HIPTRY(P->ctx, hipMemcpy(P->mx_sl2, lo.data(), 2 * a2, hipMemcpyHostToDevice));
+    // per-level scale of the low-precision operands: Γ-like fields of level l
+    // carry hc^2 = 4^l h^2, so 4^-l keeps every level's operands alike (exact)
+    std::vector<double> ls(P->mg);
+    for (int l = 0; l < P->mg; l++) ls[l] = std::ldexp(1.0, -2 * l);
+    HIPTRY(P->ctx, hipMemcpy(P->mx_lscale, ls.data(), sizeof(double) * ls.size(), hipMemcpyHostToDevice));
+    return IBPM_OK;
+}
+
+static MxResidArgs mx_rargs(ibpm_prob* P, int mode, int lev0)
+{
+    MxResidArgs r{};
+    r.fstride = P->g.nG;
+    r.out = P->mx_buf[0];
+    r.ostride = (long)P->P1 * P->P2;
+    r.P1 = P->P1;
+    r.P2 = P->P2;
+    r.mode = mode;
+    r.lev0 = lev0;
+    r.mg = P->mg;
+    r.dt = P->dt;
+    r.Re = P->Re;
+    r.lscale = P->mx_lscale;
+    r.g = P->g;
+    return r;
+}
+
+// dst = base +/- L(residual) for nb levels from lev0, the residual already in
+// mx_buf[0]: x-transform, y-transform with the FP64 eigenvalue scaling,
+// y-transform, x-transform into the FP64 field
+static void mx_apply(ibpm_prob* P, int op, int lev0, int nb, int epi, double* dst, const double* base)
+{
+    hipStream_t s = P->stream;
+    MxGemmArgs a{};
+    a.istride = a.ostride = (long)P->P1 * P->P2;
+    a.fstride = P->g.nG;
+    a.P1 = P->P1;
+    a.P2 = P->P2;
+    a.lev0 = lev0;
+    a.op = op;
+    a.cx = P->cx;
+    a.cy = P->cy;
+    a.dt = P->dt;
+    a.Re = P->Re;
+    a.scale = 1.0 / (double)(4L * P->g.nx * P->g.ny);
+    a.lscale = P->mx_lscale;
+    a.g = P->g;
+    MxGemmArgs x = a, y = a;
+    x.sf = P->mx_sf1; x.sh = P->mx_sh1; x.sl = P->mx_sl1;
+    y.sf = P->mx_sf2; y.sh = P->mx_sh2; y.sl = P->mx_sl2;
+    x.in = P->mx_buf[0]; x.out = P->mx_buf[1];
+    launch_mx_gemm(P->prec, 0, MX_EPI_STORE, x, nb, s);
+    y.in = P->mx_buf[1]; y.out = P->mx_buf[0];
+    launch_mx_gemm(P->prec, 1, MX_EPI_SCALE, y, nb, s);
+    y.in = P->mx_buf[0]; y.out = P->mx_buf[1];
+    launch_mx_gemm(P->prec, 1, MX_EPI_STORE, y, nb, s);
+    x.in = P->mx_buf[1]; x.dst = dst; x.base = base;
+    launch_mx_gemm(P->prec, 0, epi, x, nb, s);
+}
+
+// vort2flux! in increment form on Γb (coarsified first): ψ updated in place,
+// coarse to fine; then q of level 1 (level1_only) or of every level
+static void mx_vort2flux(ibpm_prob* P, double* Gb, double* qout, bool level1_only)
+{
+    hipStream_t s = P->stream;
+    const long nG = P->g.nG;
+    const int L = P->mg;
+    for (int lev = 2; lev <= L; lev++) {
+        CoarsifyArgs c{};
+        const CoarsifyRange& r = P->crange[0];
+        c.Gc = Gb + (long)(lev - 1) * nG;
+        c.G = Gb + (long)(lev - 2) * nG;
+        c.i0 = r.i0; c.ic0 = r.ic0; c.ci = r.ci; c.j0 = r.j0; c.jc0 = r.jc0; c.cj = r.cj;
+        c.g = P->g;
+        launch_coarsify(c, s);
+    }
+    for (int l = L - 1; l >= 0; l--) {
+        double* pl = P->psi + (long)l * nG;
+        for (int sw = 0; sw < P->mx_sweeps; sw++) {
+            MxResidArgs r = mx_rargs(P, MX_RES_V2F, l);
+            r.G = Gb + (long)l * nG;
+            r.psi = pl;
+            launch_mx_resid(r, 1, s);
+            mx_apply(P, MX_OP_DINV, l, 1, MX_EPI_ADD, pl, pl);
+        }
+    }
+    CurlArgs ca{};
+    ca.psi = P->psi;
+    ca.q = qout;
+    ca.qstride = P->g.nq;
+    ca.nlev = level1_only ? 1 : L;
+    ca.lev0 = 0;
+    ca.mg = L;
+    ca.g = P->g;
+    launch_curl(ca, s);
+}
+
+// Γout[:,1] -/= Ainv[1] rot(reg(f)), mixed precision
+static void mx_project_forces(ibpm_prob* P, const double* f, double* Gout, bool sub)
+{
+    hipStream_t s = P->stream;
+    ScatterArgs sa{};
+    sa.cell = P->sc_cell; sa.off = P->sc_off; sa.ent_f = P->sc_f; sa.ent_w = P->sc_w;
+    sa.fb = f; sa.q = P->qreg; sa.ncell = P->sc_ncell;
+    launch_scatter(sa, s);
+    launch_rot(P->qreg, P->Gtmp, P->rb_i0, P->rb_i1, P->rb_j0, P->rb_j1, P->g, s);
+    MxResidArgs r = mx_rargs(P, MX_RES_COPY, 0);
+    r.G = P->Gtmp;
+    launch_mx_resid(r, 1, s);
+    if (P->mx_sweeps < 2) {
+        mx_apply(P, MX_OP_AINV, 0, 1, sub ? MX_EPI_SUB : MX_EPI_ADD, Gout, sub ? Gout : nullptr);
+        return;
+    }
+    // c = Ainv g in T1, refined by c += Ainv(g - Ainv^{-1} c), then applied
+    double* c = P->T1;
+    mx_apply(P, MX_OP_AINV, 0, 1, MX_EPI_ADD, c, nullptr);
+    for (int sw = 1; sw < P->mx_sweeps; sw++) {
+        MxResidArgs r2 = mx_rargs(P, MX_RES_PROJ2, 0);
+        r2.G = P->Gtmp;
+        r2.G2 = c;
+        launch_mx_resid(r2, 1, s);
+        mx_apply(P, MX_OP_AINV, 0, 1, MX_EPI_ADD, c, c);
+    }
+    if (sub) launch_mx_axpy(Gout, c, -1.0, P->g.nG, s);
+    else launch_copy(Gout, c, P->g.nG, s);
+}
+
+// advance! with the mixed-precision transforms (same stages as step_body)
+static void step_mixed(ibpm_prob* P)
+{
+    hipStream_t s = P->stream;
+    const GridDev& g = P->g;
+    const int L = P->mg;
+    const int out = (P->nl1 != P->nl2) ? P->nl1 : 1 - P->nl2;
+    NLArgs na{};
+    na.q = P->q; na.q0 = P->q0; na.G = P->G; na.out = P->nl[out];
+    na.nlev = L; na.lev0 = 0; na.mg = L; na.g = g;
+    set_nl_q0(P, na);
+    launch_nonlinear(na, s);
+    MxResidArgs r = mx_rargs(P, MX_RES_TRIAL, 0);
+    r.G = P->G;
+    r.N1 = P->nl[out];
+    r.N0 = P->nl[P->nl2];
+    r.c1 = P->beta[0] * P->dt;
+    r.c2 = P->beta[1] * P->dt;
+    launch_mx_resid(r, L, s);
+    mx_apply(P, MX_OP_AINV, 0, L, MX_EPI_ADD, P->Gs, P->G);
+    for (int sw = 1; sw < P->mx_sweeps; sw++) {
+        MxResidArgs r2 = r;
+        r2.mode = MX_RES_TRIAL2;
+        r2.G2 = P->Gs;
+        launch_mx_resid(r2, L, s);
+        mx_apply(P, MX_OP_AINV, 0, L, MX_EPI_ADD, P->Gs, P->Gs);
+    }
+    P->nl1 = P->nl2 = out;  // state.nonlin[2] .= state.nonlin[1]
+    mx_vort2flux(P, P->Gs, P->qs1, true);
+}
+
 // advance!(state, prob, t) -- timestepping.jl:35-67.  hist: optional CD/CL slot.
 static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, double* clh)
 {
@@ -641,7 +890,9 @@ static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, do
         launch_base_flux_moving(b, s);
         P->q0_uniform = false;
     }
-    trial_dev(P);
+    const bool mixed = P->prec >= 0;
+    if (mixed) step_mixed(P);
+    else trial_dev(P);
     // ---- boundary_forces! (timestepping.jl:154-163) and update_stress!
     if (P->nbt > 0) {
         GatherArgs ga{};
@@ -656,11 +907,13 @@ static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, do
         }
         launch_gemv_forces(P->BinvT, P->Ftmp, P->Fb, P->nf, fa, s);
         // ---- project_circ! (timestepping.jl:204-223): Γ = Γs; Γ[:,1] -= Ainv rot(reg(F̃b))
-        project_forces(P, P->Fb, P->Gs, true);
+        if (mixed) mx_project_forces(P, P->Fb, P->Gs, true);
+        else project_forces(P, P->Fb, P->Gs, true);
     }
     std::swap(P->G, P->Gs);
     // ---- vort2flux!(state.ψ, state.q, state.Γ) (timestepping.jl:64)
-    vort2flux_dev(P, P->G, P->psi, P->q, false);
+    if (mixed) mx_vort2flux(P, P->G, P->q, false);
+    else vort2flux_dev(P, P->G, P->psi, P->q, false);
     (void)nG;
 }
 
@@ -2062,6 +2315,14 @@ int ibpm_op_lapinv(ibpm_prob* P, int kind, int lev, const double* b, double* x)
     double* db = io.put(b, n);
     double* dx = io.put(nullptr, n);
     if (!db || !dx) return fail(P->ctx, IBPM_E_HIP, "alloc");
+    if (P->prec >= 0 && kind != 1) {
+        // the mixed-precision transforms (A is applied by its stencil in that form)
+        MxResidArgs r = mx_rargs(P, MX_RES_COPY, lev - 1);
+        r.G = db;
+        launch_mx_resid(r, 1, P->stream);
+        mx_apply(P, kind == 0 ? MX_OP_DINV : MX_OP_AINV, lev - 1, 1, MX_EPI_ADD, dx, nullptr);
+        return get_back(P, x, dx, n);
+    }
     lap_level(P, kind == 0 ? Y_DINV : (kind == 1 ? Y_A : Y_AINV), lev - 1, db, dx, false);
     return get_back(P, x, dx, n);
 }
@@ -2202,6 +2463,22 @@ int ibpm_set_option(ibpm_prob* P, int option, int value)
         (option == IBPM_OPT_YTILE_BATCH ? P->ytile : P->ytile_single) = value;
         return IBPM_OK;
     }
+    if (option == IBPM_OPT_PRECISION) {
+        if (value < 0 || value > 2) return fail(P->ctx, IBPM_E_ARG, "precision option is 0 (FP64), 1 (FP32-MFMA) or 2 (BF16x3-MFMA)");
+        if (value > 0 && P->motion == IBPM_MOTION_FUNCTION)
+            return fail(P->ctx, IBPM_E_UNSUPPORTED, "the mixed-precision form runs static and MovingGrid bodies");
+        if (value > 0) {
+            int rc = mx_setup(P);
+            if (rc != IBPM_OK) return rc;
+        }
+        P->prec = value == 0 ? -1 : (value == 1 ? MX_FP32 : MX_BF16X3);
+        return IBPM_OK;
+    }
+    if (option == IBPM_OPT_MX_SWEEPS) {
+        if (value != 1 && value != 2) return fail(P->ctx, IBPM_E_ARG, "mixed-precision sweeps are 1 or 2");
+        P->mx_sweeps = value;
+        return IBPM_OK;
+    }
     if (option == IBPM_OPT_GRAPHS) {
         if (value != 0 && value != 1) return fail(P->ctx, IBPM_E_ARG, "graphs option is 0 or 1");
         P->graphs = value == 1;
@@ -2227,6 +2504,8 @@ int ibpm_get_option(ibpm_prob* P, int option, int* value)
     else if (option == IBPM_OPT_PROFILE) *value = P->prof;
     else if (option == IBPM_OPT_GRAPHS) *value = P->graphs ? 1 : 0;
     else if (option == IBPM_OPT_XWAVES_BATCH) *value = P->xwaves;
+    else if (option == IBPM_OPT_PRECISION) *value = P->prec < 0 ? 0 : (P->prec == MX_FP32 ? 1 : 2);
+    else if (option == IBPM_OPT_MX_SWEEPS) *value = P->mx_sweeps;
     else return fail(P->ctx, IBPM_E_ARG, "unknown option");
     return IBPM_OK;
 }
@@ -2277,9 +2556,29 @@ int ibpm_time_kernel(ibpm_prob* P, int which, int reps, double* ms_avg)
             y.bcr = P->psi + P->g.nG;
             y.edge = P->edge;
             launch_ypass(y, P->ytile_single, P->stream);
+        } else if (which >= 9 && which <= 11) {
+            // mixed-precision sine-matrix transforms: 9 x over all levels, 10 y
+            // with the eigenvalue scaling over all levels, 11 x on level 1
+            MxGemmArgs a{};
+            a.istride = a.ostride = (long)P->P1 * P->P2;
+            a.fstride = P->g.nG;
+            a.P1 = P->P1; a.P2 = P->P2;
+            a.op = MX_OP_AINV; a.cx = P->cx; a.cy = P->cy; a.dt = P->dt; a.Re = P->Re;
+            a.scale = 1.0 / (double)(4L * P->g.nx * P->g.ny);
+            a.lscale = P->mx_lscale; a.g = P->g;
+            a.in = P->mx_buf[0]; a.out = P->mx_buf[1];
+            if (which == 10) { a.sf = P->mx_sf2; a.sh = P->mx_sh2; a.sl = P->mx_sl2; }
+            else { a.sf = P->mx_sf1; a.sh = P->mx_sh1; a.sl = P->mx_sl1; }
+            launch_mx_gemm(P->prec, which == 10 ? 1 : 0, which == 10 ? MX_EPI_SCALE : MX_EPI_STORE, a,
+                           which == 11 ? 1 : L, P->stream);
         }
     };
-    if (which < 0 || which > 8 || which == 3 || (which == 8 && L < 2)) {
+    if ((which >= 9 && which <= 11) && P->prec < 0) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return fail(P->ctx, IBPM_E_ARG, "the mixed-precision kernels need IBPM_OPT_PRECISION set");
+    }
+    if (which < 0 || which > 11 || which == 3 || (which == 8 && L < 2)) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         return fail(P->ctx, IBPM_E_ARG, "unknown kernel id (whole steps are timed through ibpm_advance_n)");

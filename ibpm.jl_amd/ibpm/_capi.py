@@ -28,7 +28,9 @@ MOTION_FUNCTION = 2
 FIELD_Q, FIELD_Q0, FIELD_GAMMA, FIELD_PSI, FIELD_NONLIN1, FIELD_NONLIN2 = 0, 1, 2, 3, 4, 5
 FIELD_FB_TILDE, FIELD_FB, FIELD_CD, FIELD_CL = 6, 7, 8, 9
 
-OPT_YTILE_BATCH, OPT_YTILE_SINGLE, OPT_PROFILE, OPT_GRAPHS, OPT_XWAVES_BATCH = 0, 1, 2, 3, 4
+OPT_YTILE_BATCH, OPT_YTILE_SINGLE, OPT_PROFILE, OPT_GRAPHS, OPT_XWAVES_BATCH, OPT_PRECISION = 0, 1, 2, 3, 4, 5
+OPT_MX_SWEEPS = 6
+PREC_FP64, PREC_FP32_MFMA, PREC_BF16X3_MFMA = 0, 1, 2
 
 EXPORTED = [
     "ibpm_ctx_create", "ibpm_ctx_destroy", "ibpm_last_error", "ibpm_problem_create", "ibpm_problem_destroy",

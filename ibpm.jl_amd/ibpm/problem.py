@@ -124,6 +124,26 @@ class IBProblem:
         self.ctx.check(C.lib().ibpm_problem_get_binv(self.h, C.ptr(out), out.size))
         return out
 
+    def set_option(self, option: int, value: int):
+        """ibpm_set_option: launch shapes, in-step profiling, graph replay and
+        the precision of the Laplacian inverses (include/ibpm_hip.h)."""
+        self.ctx.check(C.lib().ibpm_set_option(self.h, int(option), int(value)), "ibpm_set_option")
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int(0)
+        self.ctx.check(C.lib().ibpm_get_option(self.h, int(option), ctypes.byref(v)), "ibpm_get_option")
+        return v.value
+
+    def set_precision(self, precision: str = "fp64", sweeps: int = 2):
+        """Arithmetic of the Laplacian inverses in advance!: "fp64" (the
+        drop-in), or the mixed-precision form "fp32" / "bf16x3": every DST-I
+        direction a dense sine-matrix product on the MFMA units (FP32, or
+        split BF16 with FP32 accumulation), applied in increment form to FP64
+        residuals, `sweeps` (1 or 2) times per solve."""
+        code = {"fp64": C.PREC_FP64, "fp32": C.PREC_FP32_MFMA, "bf16x3": C.PREC_BF16X3_MFMA}[precision]
+        self.set_option(C.OPT_PRECISION, code)
+        self.set_option(C.OPT_MX_SWEEPS, sweeps)
+
     @property
     def body_idx(self) -> np.ndarray:
         out = np.zeros((self.nb_total, 2), dtype=np.int32, order="F")

@@ -263,6 +263,15 @@ int ibpm_op_update_bodies(ibpm_prob* prob, const double* kin);
                                     HIP graph and replayed; 0: every step launches its kernels */
 #define IBPM_OPT_XWAVES_BATCH 4  /* lines per work-group of the plain x-passes over all levels: 4 (default),
                                     8, 12 or 14 (one LDS copy of the transform tables for more lines) */
+#define IBPM_OPT_PRECISION 5     /* arithmetic of the Laplacian inverses in advance!: 0 FP64 (default, the
+                                    drop-in); 1 every DST-I direction a dense sine-matrix product on MFMA in
+                                    FP32; 2 the same in split BF16 (high and low parts, three products,
+                                    FP32 accumulation).  In 1 and 2 the transforms act on FP64 residuals of
+                                    the state (increment form) and the eigenvalue scaling, residuals,
+                                    updates, Schur solve and force sums stay FP64.  Static and MovingGrid
+                                    bodies; ibpm_op_lapinv (kinds 0 and 2) follows the setting. */
+#define IBPM_OPT_MX_SWEEPS 6     /* sweeps of each mixed-precision solve: 1, or 2 (default): a second
+                                    application of the transforms to the FP64 residual of the first */
 int ibpm_set_option(ibpm_prob* prob, int option, int value);
 int ibpm_get_option(ibpm_prob* prob, int option, int* value);
 /* Average device ms and count of the y-pass launches timed inside the steps
@@ -276,7 +285,10 @@ int ibpm_profile_stats(ibpm_prob* prob, double* avg_ms, int64_t* count);
  * levels, 2 = nonlinear over all levels, 4 = x-pass on level 1 only,
  * 5 = fused y-pass (Δinv) on level 1 only, 6 = curl on level 1,
  * 7 = plain copy of nΓ*mg doubles (HBM-traffic counter calibration),
- * 8 = Δinv y-pass with the ψ boundary terms (vort2flux!'s apply_bc!) on level 1. */
+ * 8 = Δinv y-pass with the ψ boundary terms (vort2flux!'s apply_bc!) on level 1;
+ * with IBPM_OPT_PRECISION set: 9 = sine-matrix x-transform (MFMA) over all
+ * levels, 10 = y-transform with the eigenvalue scaling over all levels,
+ * 11 = x-transform on level 1. */
 int ibpm_time_kernel(ibpm_prob* prob, int which, int reps, double* ms_avg);
 
 #ifdef __cplusplus

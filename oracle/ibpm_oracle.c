@@ -171,10 +171,128 @@ static int o_nthreads = 1;
 
 void oracle_set_threads(int n) { o_nthreads = n > 0 ? n : 1; }
 
+/* Variants of the 2D DST-I, for measuring how far FP64 codes that compute the
+ * transform differently drift apart (tests/golden/dst_order_spread.py).
+ * Variant 1: the same RODFT00 with one real line per complex FFT (instead of
+ * two) and dimension 2 transformed before dimension 1.  Variant 2: the
+ * half-length algorithm (below).  Variant 0 (the default) is the oracle's own
+ * order. */
+static int o_dst_variant = 0;
+
+void oracle_set_dst_variant(int v) { o_dst_variant = v; }
+
+static void o_dst2_variant1(double* y, const double* x, int n1, int n2, const odst_plan* p1, const odst_plan* p2)
+{
+    const int N1 = 2 * (n1 + 1), N2 = 2 * (n2 + 1);
+    const int Nmax = N1 > N2 ? N1 : N2;
+    if (y != x) memcpy(y, x, sizeof(double) * (size_t)n1 * (size_t)n2);
+    /* dimension 2 first: strided rows, one per transform */
+#pragma omp parallel num_threads(o_nthreads)
+    {
+        ocx* zin = (ocx*)malloc(sizeof(ocx) * (size_t)Nmax);
+        ocx* zout = (ocx*)malloc(sizeof(ocx) * (size_t)Nmax);
+        double* ta = (double*)malloc(sizeof(double) * (size_t)n2);
+        double* ua = (double*)malloc(sizeof(double) * (size_t)n2);
+#pragma omp for schedule(static)
+        for (int i = 0; i < n1; i++) {
+            for (int j = 0; j < n2; j++) ta[j] = y[i + (long)j * n1];
+            odst_pair(p2, ta, 1, NULL, 1, ua, 1, NULL, 1, zin, zout);
+            for (int j = 0; j < n2; j++) y[i + (long)j * n1] = ua[j];
+        }
+        free(zin); free(zout); free(ta); free(ua);
+    }
+    /* then dimension 1: contiguous columns, one per transform */
+#pragma omp parallel num_threads(o_nthreads)
+    {
+        ocx* zin = (ocx*)malloc(sizeof(ocx) * (size_t)Nmax);
+        ocx* zout = (ocx*)malloc(sizeof(ocx) * (size_t)Nmax);
+        double* ta = (double*)malloc(sizeof(double) * (size_t)n1);
+#pragma omp for schedule(static)
+        for (int j = 0; j < n2; j++) {
+            memcpy(ta, y + (long)j * n1, sizeof(double) * (size_t)n1);
+            odst_pair(p1, ta, 1, NULL, 1, y + (long)j * n1, 1, NULL, 1, zin, zout);
+        }
+        free(zin); free(zout); free(ta);
+    }
+}
+
 /* 2D RODFT00 over dims [1,2] of the n1 x n2 column-major array x -> y (may alias).
  * Same transform as FFTW.plan_r2r(b, RODFT00, [1,2]) (dst-inversion.jl:10-20). */
+/* Variant 2: the half-length DST-I algorithm (even M = n + 1 only): pre-
+ * processing y_j = sin(pi j/M)(x_j + x_{M-j}) + (x_j - x_{M-j})/2, a complex
+ * FFT of P = M/2 points of z_m = y_{2m} + i y_{2m+1}, real-DFT unpacking into
+ * F_k = R_k - i I_k, then Y(2k) = 2 I_k and the odd outputs by the recurrence
+ * Y(2k+1) = Y(2k-1) + 2 R_k, Y(1) = R_0.  This is the arithmetic structure of
+ * the HIP library's transform (restated here in plain loops); its recurrence
+ * sums the FFT's rounding errors along k. */
+static void o_dst_halflen_line(int n, const double* x, long sx, double* out, long so)
+{
+    const int M = n + 1, P = M / 2;
+    static __thread offt_plan pl;
+    static __thread int plP = 0;
+    static __thread ocx *zin = NULL, *zout = NULL, *tw = NULL;
+    static __thread double* yy = NULL;
+    if (plP != P) {
+        if (plP) { offt_plan_free(&pl); free(zin); free(zout); free(tw); free(yy); }
+        offt_plan_init(&pl, P);
+        zin = (ocx*)malloc(sizeof(ocx) * (size_t)P);
+        zout = (ocx*)malloc(sizeof(ocx) * (size_t)P);
+        tw = (ocx*)malloc(sizeof(ocx) * (size_t)(P + 1));
+        yy = (double*)malloc(sizeof(double) * (size_t)M);
+        for (int k = 0; k <= P; k++) {
+            long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)M;
+            tw[k].re = (double)cosl(a);
+            tw[k].im = (double)sinl(a);
+        }
+        plP = P;
+    }
+    for (int j = 0; j < M; j++) {
+        double xj = (j >= 1 && j <= n) ? x[(long)(j - 1) * sx] : 0.0;
+        double xm = (M - j >= 1 && M - j <= n) ? x[(long)(M - j - 1) * sx] : 0.0;
+        double sj = (double)sinl(3.14159265358979323846264338327950288L * (long double)j / (long double)M);
+        yy[j] = sj * (xj + xm) + 0.5 * (xj - xm);
+    }
+    for (int m = 0; m < P; m++) { zin[m].re = yy[2 * m]; zin[m].im = yy[2 * m + 1]; }
+    offt_rec(zout, zin, P, 1, &pl, 0, 1);
+    double s = 0.0;
+    for (int k = 0; k < P; k++) {
+        ocx a = zout[k], b = zout[(P - k) % P];
+        /* E = (a + conj b)/2, O = -i (a - conj b)/2, F = E + W^k O */
+        double er = 0.5 * (a.re + b.re), ei = 0.5 * (a.im - b.im);
+        double dr = 0.5 * (a.re - b.re), di = 0.5 * (a.im + b.im);
+        double orr = di, oi = -dr;
+        double fr = er + (tw[k].re * orr - tw[k].im * oi);
+        double fi = ei + (tw[k].re * oi + tw[k].im * orr);
+        double Rk = fr, Ik = -fi;
+        if (k == 0) s = Rk;
+        else {
+            out[(long)(2 * k - 1) * so] = 2.0 * Ik;
+            s = s + 2.0 * Rk;
+        }
+        out[(long)(2 * k) * so] = s;
+    }
+}
+
+static void o_dst2_variant2(double* y, const double* x, int n1, int n2)
+{
+    double* t = (double*)malloc(sizeof(double) * (size_t)n1 * (size_t)n2);
+#pragma omp parallel for num_threads(o_nthreads) schedule(static)
+    for (int j = 0; j < n2; j++) o_dst_halflen_line(n1, x + (long)j * n1, 1, t + (long)j * n1, 1);
+#pragma omp parallel for num_threads(o_nthreads) schedule(static)
+    for (int i = 0; i < n1; i++) o_dst_halflen_line(n2, t + i, n1, y + i, n1);
+    free(t);
+}
+
 static void o_dst2(double* y, const double* x, int n1, int n2, const odst_plan* p1, const odst_plan* p2)
 {
+    if (o_dst_variant == 1) {
+        o_dst2_variant1(y, x, n1, n2, p1, p2);
+        return;
+    }
+    if (o_dst_variant == 2 && (n1 + 1) % 2 == 0 && (n2 + 1) % 2 == 0) {
+        o_dst2_variant2(y, x, n1, n2);
+        return;
+    }
     const int N1 = 2 * (n1 + 1), N2 = 2 * (n2 + 1);
     const int Nmax = N1 > N2 ? N1 : N2;
     /* dimension 1: contiguous columns, paired */
@@ -434,9 +552,20 @@ typedef struct {
 
 /* dst_inv!(x, b, Λ, plan; scale) -- dst-inversion.jl:39-50, wrapped by
  * get_lap_inv -- laplacian.jl:24-41 (copy through b_temp/x_temp). */
+/* Optional replacement of dst_inv! for error studies of reduced-precision
+ * transforms (the hook receives x, b, Λ, scale, nx-1, ny-1); NULL restores
+ * the oracle's own FP64 transform. */
+typedef void (*o_lapinv_hook_t)(double* x, const double* b, const double* Lam, double scale, int n1, int n2);
+static o_lapinv_hook_t o_lapinv_hook = NULL;
+void oracle_set_lapinv_hook(o_lapinv_hook_t f) { o_lapinv_hook = f; }
+
 static void o_dst_inv(const oprob* P, double* x, const double* b, const double* Lam, double scale)
 {
     const og_grid* g = &P->g;
+    if (o_lapinv_hook) {
+        o_lapinv_hook(x, b, Lam, scale, g->nx - 1, g->ny - 1);
+        return;
+    }
     double* w = P->wdst;
     o_dst2(w, b, g->nx - 1, g->ny - 1, &P->p1, &P->p2);
     for (long k = 0; k < g->nG; k++) w[k] /= Lam[k];

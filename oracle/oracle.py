@@ -50,6 +50,7 @@ def lib():
         L.oracle_advance.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _dp]
         L.oracle_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_set_dst_variant.argtypes = [ctypes.c_int]
         L.oracle_delta_h.restype = ctypes.c_double
         L.oracle_delta_h.argtypes = [ctypes.c_double] * 3
         L.oracle_dst2_raw.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
@@ -81,6 +82,12 @@ def _p(a):
 
 def set_threads(n: int):
     lib().oracle_set_threads(int(n))
+
+
+def set_dst_variant(v: int):
+    """0: the oracle's own 2D DST-I; 1: the same transform in another
+    operation order (one line per FFT, dimension 2 first)."""
+    lib().oracle_set_dst_variant(int(v))
 
 
 def delta_h(rf, rb, dr):
