@@ -6,14 +6,17 @@ nested levels, dt = 1e-3, 402 body points, initial vorticity a Gaussian vortex
 of circulation 0.01 and width 0.1 at (1.5, 0.3) (the 5-level 1024^2 precision
 configuration run in FP64).  A step is one advance!(state, prob, t).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed] [--graphs 0|1]
     python bench.py --kernels-only [--kernel-reps R]   (per-kernel timing only)
     python bench.py --workload naca64 [--steps K]       (batched 64-instance NACA workload)
     python bench.py --workload configs                  (every configuration, CPU oracle beside it)
 
 Prints one JSON line.  Timed region: K consecutive steps enqueued by
 ibpm_advance_n with the state resident in HBM, bracketed by device
-synchronisation (the call returns after its stream has drained).
+synchronisation (the call returns after its stream has drained).  `value` is
+the FP64 drop-in's rate; the `mixed` object gives the mixed-precision form
+(MFMA sine-matrix transforms) on the same steps from the same start, with its
+CD/CL difference from the FP64 run and the MFMA rate of its transforms.
 """
 import argparse
 import ctypes
@@ -220,6 +223,9 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="0: every step launches its kernels (no HIP-graph replay), e.g. under a profiler")
     ap.add_argument("--cpu-steps", type=int, default=12,
                     help="oracle steps in the CPU baseline sample (about 1 s each on one core)")
     ap.add_argument("--kernel-reps", type=int, default=50)
@@ -263,6 +269,7 @@ def main():
     t0 = time.time()
     prob = ibpm.IBProblem(g, c["bodies"], dt, c["Re"], freestream=c["freestream"])
     setup_s = time.time() - t0
+    prob.set_option(C.OPT_GRAPHS, args.graphs)
     state = ibpm.IBState(prob)
     state.Γ = gaussian_vortex_gamma(g)
     ibpm.vort2flux_(state, prob)
@@ -326,22 +333,32 @@ def main():
                 "bytes_per_launch": dst_bytes, "avg_launch_ms": round(a_ms, 5), "launches_timed": a_n}
     step_bytes = algorithmic_bytes_per_step(g.nx, g.ny, g.mg, nb)
 
+    mixed = None if args.no_mixed else bench_mixed(prob, c, args, cd, cl)
+
     cpu = None
     if not args.no_cpu:
         import oracle as O
         from cases import boundary_of, oracle_bodies, sc7
 
-        O.set_threads(1)
-        op = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], dt, oracle_bodies(c), binv=prob.Binv)
-        S = op.new_state(sc7(c, 0.0))
-        S.G[:] = gaussian_vortex_gamma(g)
-        op.run(S, sc7(c, 0.0)[None, :])  # first step outside the timing
-        tc = time.perf_counter()
-        op.run(S, np.tile(sc7(c, 0.0), (args.cpu_steps, 1)))
-        ec = time.perf_counter() - tc
-        cpu = {"value": round(args.cpu_steps / ec, 4), "unit": "timesteps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} advance! steps of the same 1024x1024 5-level case (after 1 untimed), "
-                         f"C oracle, 1 thread, {ec:.1f}s"}
+        def oracle_rate(threads, nsteps):
+            O.set_threads(threads)
+            op = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], dt, oracle_bodies(c), binv=prob.Binv)
+            S = op.new_state(sc7(c, 0.0))
+            S.G[:] = gaussian_vortex_gamma(g)
+            op.run(S, sc7(c, 0.0)[None, :])  # first step outside the timing
+            tc = time.perf_counter()
+            op.run(S, np.tile(sc7(c, 0.0), (nsteps, 1)))
+            return nsteps, time.perf_counter() - tc
+
+        # all the host cores this job may use (the box's share, OMP_NUM_THREADS), then one core
+        nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        n_all, e_all = oracle_rate(nth, 4 * args.cpu_steps)
+        n_one, e_one = oracle_rate(1, args.cpu_steps)
+        cpu = {"value": round(n_all / e_all, 4), "unit": "timesteps/s", "cores": nth, "kind": "port",
+               "sample": f"{n_all} advance! steps of the same 1024x1024 5-level case (after 1 untimed), C oracle, "
+                         f"{nth} OpenMP threads, {e_all:.1f}s",
+               "one_core": {"value": round(n_one / e_one, 4), "cores": 1,
+                            "sample": f"{n_one} steps on 1 thread, {e_one:.1f}s"}}
 
     out = {
         "metric": METRIC, "value": round(sps, 2), "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
@@ -358,9 +375,54 @@ def main():
                      "frac_of_peak": round(step_bytes * sps / 1e9 / HBM_PEAK_GBS, 4)},
         # isolated repeated launches (operands stay in the 256 MB Infinity Cache): faster than in-step
         "kernel_ms_isolated": {k: round(v, 5) for k, v in kb.items()},
+        "mixed": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_mixed(prob, c, args, cd64, cl64):
+    """The mixed-precision form on the same problem: from the same start as
+    the FP64 run, the same warmup and timed steps in split BF16 and in FP32
+    (MFMA sine-matrix transforms, two sweeps per solve); rate, ratio to the
+    FP64 rate and max |CD - CD_fp64|, |CL - CL_fp64| over the timed steps
+    relative to max |CD_fp64|; and the MFMA rate of the all-level x-transform
+    (the transforms' hardware operations per launch / its average duration,
+    HIP events on the problem's stream)."""
+    import ibpm
+    from ibpm import _capi as C
+    from cases import gaussian_vortex_gamma
+
+    g = c["grid"]
+    dt = c["dt"]
+    P1 = (g.nx - 1 + 63) // 64 * 64
+    P2 = (g.ny - 1 + 63) // 64 * 64
+    res = {}
+    fp64_rate = None
+    for prec, name, peak in (("bf16x3", "split BF16 (hi*hi + hi*lo + lo*hi), FP32 accumulation", 2500.0),
+                             ("fp32", "FP32", 157.3)):
+        prob.set_precision(prec, 2)
+        st = ibpm.IBState(prob)
+        st.Γ = gaussian_vortex_gamma(g)
+        ibpm.vort2flux_(st, prob)
+        ibpm.advance_n_(st, prob, np.arange(args.warmup) * dt)
+        ts = (args.warmup + np.arange(args.steps)) * dt
+        t1 = time.perf_counter()
+        cd, cl, done = ibpm.advance_n_(st, prob, ts)
+        el = time.perf_counter() - t1
+        scale = float(np.max(np.abs(cd64)))
+        v = ctypes.c_double(0.0)
+        prob.ctx.check(C.lib().ibpm_time_kernel(prob.h, 9, args.kernel_reps, ctypes.byref(v)))
+        ops = 2.0 * P1 * P1 * P2 * g.mg * (3 if prec == "bf16x3" else 1)
+        tf = ops / (v.value * 1e-3) / 1e12
+        res[prec] = {"value": round(done / el, 2), "unit": "timesteps/s", "arithmetic": name, "sweeps": 2,
+                     "cd_err_vs_fp64": float(np.max(np.abs(cd - cd64)) / scale),
+                     "cl_err_vs_fp64": float(np.max(np.abs(cl - cl64)) / scale),
+                     "mfma": {"kernel": "k_mx_gemm x-transform, all levels", "achieved": round(tf, 1),
+                              "peak": peak, "unit": "TFLOP/s", "frac": round(tf / peak, 4),
+                              "avg_launch_ms": round(v.value, 5)}}
+    prob.set_precision("fp64")
+    return res
 
 
 if __name__ == "__main__":
