@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/ibpm_hip.h"
+#include "adjoint.h"
 #include "kernels.h"
 #include "mixed.h"
 
@@ -188,6 +189,8 @@ struct ibpm_prob {
     float *mx_sf1 = nullptr, *mx_sf2 = nullptr, *mx_buf[2] = {nullptr, nullptr};
     __bf16 *mx_sh1 = nullptr, *mx_sl1 = nullptr, *mx_sh2 = nullptr, *mx_sl2 = nullptr;
     double* mx_lscale = nullptr;
+    // tangent-linear / adjoint work space (allocated by their first call)
+    struct AdjWork* adj = nullptr;
     // in-step timing (IBPM_OPT_PROFILE): 1 the all-level y-pass launches, 2
     // the one-level y-passes with boundary terms of the vort2flux! chain;
     // event pairs recorded on the problem's stream, summed after each call
@@ -199,6 +202,21 @@ struct ibpm_prob {
 };
 
 static void graphs_clear(ibpm_prob* P);
+
+// Device buffers of the tangent-linear and adjoint steps (MovingGrid bodies).
+// Per level: nG for circulation-like fields, nq for fluxes, nbc for boundary
+// vectors.  snap: the binomial checkpoints, (Γ, nonlin[2]) each.
+struct AdjWork {
+    int nsteps_cap = 0, nsnap = 0;
+    double *dG = nullptr, *dGs = nullptr, *dN[2] = {nullptr, nullptr}, *tG = nullptr, *dpsi = nullptr;
+    double *dq = nullptr, *dq0 = nullptr, *dqs1 = nullptr, *Q = nullptr, *dQ = nullptr, *fq = nullptr;
+    double *F1 = nullptr, *F2 = nullptr, *BinvC = nullptr, *bc = nullptr, *bcb = nullptr, *tG1 = nullptr;
+    double *Gb = nullptr, *Gbo = nullptr, *Nb1 = nullptr, *Nb0 = nullptr, *Rb = nullptr, *psib = nullptr;
+    double *qb = nullptr, *q0b = nullptr, *qs1b = nullptr, *part = nullptr;
+    double* sums = nullptr;  // per step: TLM Σ δF̃b_x, Σ δF̃b_y; adjoint Ūx0, Ūy0, Ω̄
+    std::vector<double*> snap;
+    std::vector<void*> allocs;
+};
 
 // ---------------------------------------------------------------------------
 // error plumbing
@@ -1210,6 +1228,402 @@ static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, dou
     step_body(P, sc, cdh, clh);
 }
 
+// ---------------------------------------------------------------------------
+// Tangent-linear and adjoint advance! (MovingGrid bodies, FP64).
+//
+// A step maps the state (Γ, N0 = nonlin[2]) and the step's scalars s_k to
+// (Γ', N1), with q = vort2flux!(Γ) taken inside it (the state's q is always
+// that function of the state's Γ).  Everything but the product of the
+// nonlinear term (bilinear in q + q0 and Γ with its boundary ring) and the
+// base flux q0(s_k) (timestepping.jl:284-317) is linear, so the tangent and
+// the adjoint of a step need only its input state and scalars:
+//   δq = V_q(δΓ), δq0 = (∂q0/∂s) δs,
+//   δN1 = rot(D(avg(δq + δq0), Γ) + D(avg(q + q0), δΓ)) / (4 hc^2),
+//   δΓs = Ainv(A δΓ + β1 dt δN1 + β2 dt δN0 + rhsbc(δΓ)),
+//   δF̃b = Binv E(V_q1(δΓs) + δq0), δCD = 2 (h/dt) Σ δF̃b_x, δCL likewise,
+//   δΓ' = C(C(δΓs) - Ainv rot E' δF̃b)    (C: coarsify!, V: vort2flux!),
+// and the adjoint step applies the transposes in reverse order.  The
+// objective is J = Σ_k (wcd_k CD_k + wcl_k CL_k) over the steps.
+// ---------------------------------------------------------------------------
+template <typename T>
+static hipError_t walloc(AdjWork* W, T** p, size_t n)
+{
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, sizeof(T) * (n > 0 ? n : 1));
+    if (e == hipSuccess) {
+        W->allocs.push_back(v);
+        *p = (T*)v;
+    }
+    return e;
+}
+
+static void adj_free(ibpm_prob* P)
+{
+    if (!P->adj) return;
+    for (void* v : P->adj->allocs) (void)hipFree(v);
+    delete P->adj;
+    P->adj = nullptr;
+}
+
+static int adj_setup(ibpm_prob* P, int nsteps, int nsnap)
+{
+    const GridDev& g = P->g;
+    const long nG = g.nG, nq = g.nq, L = P->mg;
+    if (!P->adj) {
+        AdjWork* W = new AdjWork();
+        P->adj = W;
+#define WA(ptr, n) HIPTRY(P->ctx, walloc(W, &(ptr), (size_t)(n)))
+        WA(W->dG, nG * L); WA(W->dGs, nG * L); WA(W->dN[0], nG * L); WA(W->dN[1], nG * L); WA(W->tG, nG * L);
+        WA(W->dpsi, nG * L); WA(W->dq, nq * L); WA(W->dq0, nq * L); WA(W->dqs1, nq); WA(W->Q, nq); WA(W->dQ, nq);
+        WA(W->fq, nq); WA(W->F1, P->nf); WA(W->F2, P->nf); WA(W->BinvC, (size_t)P->nf * P->nf);
+        WA(W->bc, g.nbc); WA(W->bcb, g.nbc); WA(W->tG1, nG);
+        WA(W->Gb, nG * L); WA(W->Gbo, nG * L); WA(W->Nb1, nG * L); WA(W->Nb0, nG * L); WA(W->Rb, nG * L);
+        WA(W->psib, nG * L); WA(W->qb, nq * L); WA(W->q0b, nq * L); WA(W->qs1b, nq);
+        WA(W->part, 3L * adj_q0_reduce_parts(g, P->mg));
+#undef WA
+        // Binv' as row-major rows: the column-major Binv itself
+        HIPTRY(P->ctx, hipMemcpy(W->BinvC, P->binv.data(), sizeof(double) * P->binv.size(), hipMemcpyHostToDevice));
+        launch_fill(W->qs1b, nq, 0.0, P->stream);  // E' writes only its support cells
+    }
+    AdjWork* W = P->adj;
+    if (nsteps > W->nsteps_cap) {
+        HIPTRY(P->ctx, walloc(W, &W->sums, 3 * (size_t)nsteps));
+        W->nsteps_cap = nsteps;
+    }
+    while ((int)W->snap.size() < nsnap) {
+        double* sp = nullptr;
+        HIPTRY(P->ctx, walloc(W, &sp, 2 * (size_t)(nG * L)));
+        W->snap.push_back(sp);
+    }
+    return IBPM_OK;
+}
+
+// base-flux parameters of a MovingGrid step (timestepping.jl:284-298)
+struct MGFlux {
+    double Ux0, Uy0, Om;
+};
+static MGFlux mg_flux(const ibpm_step_scalars& sc)
+{
+    const double Om = -sc.theta_dot, al = -sc.theta;
+    double Ux0 = sc.U * std::cos(al) - sc.V * std::sin(al);
+    double Uy0 = sc.U * std::sin(al) + sc.V * std::cos(al);
+    Ux0 += sc.Ux * std::cos(sc.inclination) - sc.Uy * std::sin(sc.inclination);
+    Uy0 += sc.Ux * std::sin(sc.inclination) + sc.Uy * std::cos(sc.inclination);
+    return {Ux0, Uy0, Om};
+}
+// d(Ux0, Uy0, Ω) for perturbations d[0..6] = {Ux, Uy, inclination, U, V, θ, θ̇}
+static MGFlux mg_flux_tangent(const ibpm_step_scalars& sc, const double* d)
+{
+    const double al = -sc.theta, ca = std::cos(al), sa = std::sin(al);
+    const double cf = std::cos(sc.inclination), sf = std::sin(sc.inclination);
+    const double dal = -d[5];
+    MGFlux r;
+    r.Ux0 = d[3] * ca - d[4] * sa + (-sc.U * sa - sc.V * ca) * dal + d[0] * cf - d[1] * sf +
+            (-sc.Ux * sf - sc.Uy * cf) * d[2];
+    r.Uy0 = d[3] * sa + d[4] * ca + (sc.U * ca - sc.V * sa) * dal + d[0] * sf + d[1] * cf +
+            (sc.Ux * cf - sc.Uy * sf) * d[2];
+    r.Om = -d[6];
+    return r;
+}
+// the transpose: cotangents of the 7 scalars from (Ūx0, Ūy0, Ω̄)
+static void mg_flux_adjoint(const ibpm_step_scalars& sc, const double* b, double* out)
+{
+    const double al = -sc.theta, ca = std::cos(al), sa = std::sin(al);
+    const double cf = std::cos(sc.inclination), sf = std::sin(sc.inclination);
+    const double ux = b[0], uy = b[1], om = b[2];
+    out[0] = ux * cf + uy * sf;
+    out[1] = -ux * sf + uy * cf;
+    out[2] = ux * (-sc.Ux * sf - sc.Uy * cf) + uy * (sc.Ux * cf - sc.Uy * sf);
+    out[3] = ux * ca + uy * sa;
+    out[4] = -ux * sa + uy * ca;
+    out[5] = -(ux * (-sc.U * sa - sc.V * ca) + uy * (sc.U * ca - sc.V * sa));
+    out[6] = -om;
+}
+
+static void set_q0(ibpm_prob* P, double* q0, const MGFlux& f)
+{
+    BaseFluxArgs b{};
+    b.q0 = q0; b.mg = P->mg; b.Ux0 = f.Ux0; b.Uy0 = f.Uy0; b.Om = f.Om; b.offx = P->offx; b.offy = P->offy; b.g = P->g;
+    launch_base_flux_moving(b, P->stream);
+}
+
+// x = L b over every level (batched passes), L one of Δinv / A / Ainv
+static void lap_all(ibpm_prob* P, int op, const double* b, double* x)
+{
+    hipStream_t s = P->stream;
+    XArgs a = xargs(P);
+    a.src = b; a.dst = P->T1; a.nlev = P->mg; a.lev0 = 0;
+    launch_xpass(X_PLAIN, a, s);
+    launch_ypass(yargs(P, op, 0, P->mg, P->T1), P->ytile, s);
+    XArgs c = xargs(P);
+    c.src = P->T1; c.dst = x; c.nlev = P->mg; c.lev0 = 0;
+    launch_xpass(X_PLAIN, c, s);
+}
+
+static CoarsifyArgs coarsify_args(ibpm_prob* P, double* Gb, int lev)  // lev: the coarse level, 2..L (1-based)
+{
+    CoarsifyArgs c{};
+    const CoarsifyRange& r = P->crange[0];
+    c.Gc = Gb + (long)(lev - 1) * P->g.nG;
+    c.G = Gb + (long)(lev - 2) * P->g.nG;
+    c.i0 = r.i0; c.ic0 = r.ic0; c.ci = r.ci; c.j0 = r.j0; c.jc0 = r.jc0; c.cj = r.cj;
+    c.g = P->g;
+    return c;
+}
+
+static void coarsify_all(ibpm_prob* P, double* Gb)
+{
+    for (int lev = 2; lev <= P->mg; lev++) launch_coarsify(coarsify_args(P, Gb, lev), P->stream);
+}
+
+static void coarsify_all_T(ibpm_prob* P, double* Gb)
+{
+    for (int lev = P->mg; lev >= 2; lev--) {
+        CoarsifyArgs c = coarsify_args(P, Gb, lev);
+        launch_adj_coarsify_T(Gb + (long)(lev - 2) * P->g.nG, c.Gc, c, P->stream);
+    }
+}
+
+// chainᵀ of vort2flux! (curl.jl:141-162): ḡ = Δinv ψ̄ level by level, fine
+// to coarse, ψ̄ of the coarser level += get_bc!ᵀ apply_bc!ᵀ ḡ.  psib is consumed.
+static void chain_T(ibpm_prob* P, double* psib, double* gout)
+{
+    const long nG = P->g.nG;
+    const int L = P->mg;
+    AdjWork* W = P->adj;
+    for (int l = 0; l < L; l++) {
+        lap_level(P, Y_DINV, l, psib + l * nG, gout + l * nG, false);
+        if (l < L - 1) {
+            launch_adj_apply_bc_T(gout + l * nG, 1.0, W->bcb, P->g, P->stream);
+            launch_adj_get_bc_T(W->bcb, 1.0, psib + (l + 1) * nG, P->g, P->stream);
+        }
+    }
+}
+
+// ψ̄ from q̄ through the multigrid curl! of levels 0 .. nl-1 (q̄ level l at qb + l nq)
+static void curl_T(ibpm_prob* P, const double* qb, int nl, double* psib)
+{
+    const long nG = P->g.nG, nq = P->g.nq;
+    AdjWork* W = P->adj;
+    launch_fill(psib, nG * P->mg, 0.0, P->stream);
+    for (int l = 0; l < nl; l++) {
+        launch_adj_rot(qb + l * nq, psib + l * nG, 1.0, true, P->g, P->stream);
+        if (l < P->mg - 1) {
+            launch_adj_curl_bc_T(qb + l * nq, W->bcb, P->g, P->stream);
+            launch_adj_get_bc_T(W->bcb, 1.0, psib + (l + 1) * nG, P->g, P->stream);
+        }
+    }
+}
+
+// the nonlinear term of every level, linearised about the state (q, q0, Γ):
+// dN = rot(D(avg(dq + dq0), Γ) + D(avg(q + q0), dG)) / (4 hc^2)
+static void nonlinear_tangent(ibpm_prob* P, const double* dG, const double* dq, const double* dq0, double* dN)
+{
+    const GridDev& g = P->g;
+    const long nG = g.nG, nq = g.nq;
+    const int L = P->mg;
+    AdjWork* W = P->adj;
+    hipStream_t s = P->stream;
+    for (int l = 0; l < L; l++) {
+        const double* Gl = P->G + l * nG;
+        const double* Gc = l < L - 1 ? Gl + nG : nullptr;
+        const double* dGl = dG + l * nG;
+        const double* dGc = l < L - 1 ? dGl + nG : nullptr;
+        launch_adj_avg_flux(P->q + l * nq, P->q0 + l * nq, W->Q, g, s);
+        launch_adj_avg_flux(dq + l * nq, dq0 + l * nq, W->dQ, g, s);
+        launch_adj_dprod(W->dQ, Gl, Gc, W->fq, false, g, s);
+        launch_adj_dprod(W->Q, dGl, dGc, W->fq, true, g, s);
+        const double hc = level_hc(g.h, l);
+        launch_adj_rot(W->fq, dN + l * nG, 1.0 / (4.0 * (hc * hc)), false, g, s);
+    }
+}
+
+// One tangent-linear step about the working state (the input of step k).
+// W->dG / W->dN[n0] hold (δΓ, δN0); afterwards W->dG holds δΓ' and the new
+// δN0 is W->dN[1 - n0].  Σ δF̃b_x, Σ δF̃b_y go to sums[2k], sums[2k+1].
+static void tlm_step(ibpm_prob* P, const ibpm_step_scalars& sc, const double* ds, int k, int n0)
+{
+    const GridDev& g = P->g;
+    const long nG = g.nG;
+    const int L = P->mg;
+    AdjWork* W = P->adj;
+    hipStream_t s = P->stream;
+    set_q0(P, P->q0, mg_flux(sc));
+    set_q0(P, W->dq0, mg_flux_tangent(sc, ds));
+    // δq = V_q(δΓ) on a copy (vort2flux! coarsifies its input)
+    launch_copy(W->tG, W->dG, nG * L, s);
+    vort2flux_dev(P, W->tG, W->dpsi, W->dq, false);
+    double* dN1 = W->dN[1 - n0];
+    nonlinear_tangent(P, W->dG, W->dq, W->dq0, dN1);
+    // δΓs = Ainv(A δΓ + β1 dt δN1 + β2 dt δN0 + rhsbc(δΓ)): the trial passes
+    XArgs t = xargs(P);
+    t.src = W->dG; t.dst = P->T2; t.nlev = L; t.lev0 = 0;
+    t.n1 = dN1; t.n0 = W->dN[n0]; t.gam = W->dG;
+    t.c1 = P->beta[0] * P->dt; t.c2 = P->beta[1] * P->dt; t.dt = P->dt; t.Re = P->Re;
+    launch_xpass(X_TRIAL, t, s);
+    launch_ypass(yargs(P, Y_AINV, 0, L, P->T2), P->ytile, s);
+    XArgs c = xargs(P);
+    c.src = P->T2; c.dst = W->dGs; c.nlev = L; c.lev0 = 0;
+    launch_xpass(X_PLAIN, c, s);
+    vort2flux_dev(P, W->dGs, W->dpsi, W->dqs1, true);
+    // δF̃b = Binv E(δqs1 + δq0)
+    GatherArgs ga{};
+    ga.q = W->dqs1; ga.q0 = W->dq0; ga.w = P->w; ga.bidx = P->bidx; ga.out = W->F1; ga.nbt = P->nbt; ga.g = g;
+    launch_gather(ga, s);
+    launch_adj_gemv(P->BinvT, W->F1, W->F2, P->nf, s);
+    launch_adj_sum_halves(W->F2, P->nf, W->sums + 2L * k, s);
+    project_forces(P, W->F2, W->dGs, true);
+    std::swap(W->dG, W->dGs);
+    coarsify_all(P, W->dG);
+}
+
+// One adjoint step about the working state (the input of step k): from
+// (Γ̄ of its output Γ' in W->Gb, N̄ of its N1 in W->Nb1) to those of its
+// input (W->Gb, W->Nb1 on return); Ūx0, Ūy0, Ω̄ go to sums[3k..3k+2].
+static void adj_step(ibpm_prob* P, const ibpm_step_scalars& sc, double wcd, double wcl, int k)
+{
+    const GridDev& g = P->g;
+    const long nG = g.nG, nq = g.nq;
+    const int L = P->mg;
+    AdjWork* W = P->adj;
+    hipStream_t s = P->stream;
+    set_q0(P, P->q0, mg_flux(sc));
+    // Γ' = C(Γs) - Ainv rot E' F (level 1), then the state's coarsify!
+    coarsify_all_T(P, W->Gb);
+    // F̄ = -E curl(Ainv Γ̄'_1) + the objective's weights
+    lap_level(P, Y_AINV, 0, W->Gb, W->tG1, false);
+    launch_adj_curl_sg(W->tG1, W->dqs1, -1.0, g, s);
+    GatherArgs ga{};
+    ga.q = W->dqs1; ga.q0 = nullptr; ga.w = P->w; ga.bidx = P->bidx; ga.out = W->F1; ga.nbt = P->nbt; ga.g = g;
+    launch_gather(ga, s);
+    const double cD = 2.0 * g.h / P->dt;
+    launch_adj_add_halves(W->F1, P->nf, cD * wcd, cD * wcl, s);
+    // ū = Binv' F̄; q̄s1 = E' ū (also q̄0 of level 1)
+    launch_adj_gemv(W->BinvC, W->F1, W->F2, P->nf, s);
+    ScatterArgs sa{};
+    sa.cell = P->sc_cell; sa.off = P->sc_off; sa.ent_f = P->sc_f; sa.ent_w = P->sc_w;
+    sa.fb = W->F2; sa.q = W->qs1b; sa.ncell = P->sc_ncell;
+    launch_scatter(sa, s);
+    launch_fill(W->q0b, nq * L, 0.0, s);
+    launch_copy(W->q0b, W->qs1b, nq, s);
+    // Γ̄s = C'(Γ̄' + chain' curl_1' q̄s1)
+    curl_T(P, W->qs1b, 1, W->psib);
+    chain_T(P, W->psib, W->tG);
+    launch_adj_axpy(W->tG, W->Gb, 1.0, nG * L, s);
+    coarsify_all_T(P, W->tG);
+    // R̄ = Ainv Γ̄s; Γ̄ = A R̄ + rhsbc'; N̄1 += β1 dt R̄; N̄0 = β2 dt R̄
+    lap_all(P, Y_AINV, W->tG, W->Rb);
+    lap_all(P, Y_A, W->Rb, W->Gbo);
+    launch_adj_axpy(W->Nb1, W->Rb, P->beta[0] * P->dt, nG * L, s);
+    launch_fill(W->Nb0, nG * L, 0.0, s);
+    launch_adj_axpy(W->Nb0, W->Rb, P->beta[1] * P->dt, nG * L, s);
+    for (int l = 0; l < L - 1; l++) {
+        const double hc = level_hc(g.h, l);
+        launch_adj_apply_bc_T(W->Rb + l * nG, 1.0, W->bcb, g, s);
+        launch_adj_get_bc_T(W->bcb, 0.25 * P->dt / (P->Re * hc * hc), W->Gbo + (l + 1) * nG, g, s);
+    }
+    // the nonlinear term of every level
+    for (int l = 0; l < L; l++) {
+        const double hc = level_hc(g.h, l);
+        const double* Gl = P->G + l * nG;
+        const double* Gc = l < L - 1 ? Gl + nG : nullptr;
+        launch_adj_curl_sg(W->Nb1 + l * nG, W->fq, 1.0 / (4.0 * (hc * hc)), g, s);
+        launch_adj_avg_flux(P->q + l * nq, P->q0 + l * nq, W->Q, g, s);
+        launch_adj_dprod_TQ(W->fq, Gl, Gc, W->dQ, g, s);
+        launch_adj_dprod_TG(W->fq, W->Q, W->Gbo + l * nG, W->bc, g, s);
+        if (l < L - 1) launch_adj_get_bc_T(W->bc, 0.25, W->Gbo + (l + 1) * nG, g, s);
+        launch_adj_avg_flux_T(W->dQ, W->qb + l * nq, false, g, s);
+        launch_adj_avg_flux_T(W->dQ, W->q0b + l * nq, true, g, s);
+    }
+    // q = V_q(Γ): Γ̄ += C' chain' curl' q̄
+    curl_T(P, W->qb, L, W->psib);
+    chain_T(P, W->psib, W->tG);
+    coarsify_all_T(P, W->tG);
+    launch_adj_axpy(W->Gbo, W->tG, 1.0, nG * L, s);
+    launch_adj_q0_reduce(W->q0b, L, P->offx, P->offy, W->part, W->sums + 3L * k, g, s);
+    std::swap(W->Gb, W->Gbo);
+    std::swap(W->Nb1, W->Nb0);
+}
+
+// checkpoints of the working state: (Γ, nonlin[2]) copied out and back; a
+// restore recomputes ψ and q by vort2flux! (coarsify! of a coarsified Γ
+// rewrites the same values), which is how the forward step left them
+static void snap_store(ibpm_prob* P, int slot)
+{
+    const long n = P->g.nG * P->mg;
+    double* sp = P->adj->snap[slot];
+    launch_copy(sp, P->G, n, P->stream);
+    launch_copy(sp + n, P->nl[P->nl2], n, P->stream);
+}
+static void snap_restore(ibpm_prob* P, int slot)
+{
+    const long n = P->g.nG * P->mg;
+    const double* sp = P->adj->snap[slot];
+    P->nl1 = P->nl2;
+    launch_copy(P->G, sp, n, P->stream);
+    launch_copy(P->nl[P->nl2], sp + n, n, P->stream);
+    vort2flux_dev(P, P->G, P->psi, P->q, false);
+}
+
+static long binom(int n, int k)
+{
+    if (k < 0 || k > n) return 0;
+    long r = 1;
+    for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+    return r;
+}
+
+// Reverse steps l .. l+t-1 with the state of step l in snapshot `base` and
+// `free_` further snapshots after it (binomial checkpointing: with s free
+// snapshots and r the least recomputation count with C(s+r, s) >= t, the
+// next snapshot goes t - C(s-1+r, s-1) steps ahead, so that the part after
+// it is reversible with s-1 snapshots and the part before with r-1 sweeps)
+struct AdjRun {
+    ibpm_prob* P;
+    const ibpm_step_scalars* s;
+    const double *wcd, *wcl;
+    long fwd_steps = 0;
+};
+static void rev(AdjRun& R, int l, int t, int base, int free_)
+{
+    ibpm_prob* P = R.P;
+    if (t <= 0) return;
+    if (t == 1) {
+        snap_restore(P, base);
+        adj_step(P, R.s[l], R.wcd[l], R.wcl[l], l);
+        return;
+    }
+    if (free_ == 0) {
+        for (int k = t - 1; k >= 0; k--) {
+            snap_restore(P, base);
+            for (int j = 0; j < k; j++) step_body(P, R.s[l + j], nullptr, nullptr);
+            R.fwd_steps += k;
+            adj_step(P, R.s[l + k], R.wcd[l + k], R.wcl[l + k], l + k);
+        }
+        return;
+    }
+    int r = 0;
+    while (binom(free_ + r, free_) < t) r++;
+    const int m = std::max(1, t - (int)binom(free_ - 1 + r, free_ - 1));
+    snap_restore(P, base);
+    for (int j = 0; j < m; j++) step_body(P, R.s[l + j], nullptr, nullptr);
+    R.fwd_steps += m;
+    snap_store(P, base + 1);
+    rev(R, l + m, t - m, base + 1, free_ - 1);
+    rev(R, l, m, base, free_);
+}
+
+static int adj_check(ibpm_prob* P)
+{
+    if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group");
+    if (int rc = check_idle(P)) return rc;
+    if (P->motion != IBPM_MOTION_MOVING_GRID || P->nbodies != 1)
+        return fail(P->ctx, IBPM_E_UNSUPPORTED, "the tangent and adjoint steps are built for one MovingGrid body");
+    if (P->prec >= 0 || P->lin)
+        return fail(P->ctx, IBPM_E_UNSUPPORTED, "the tangent and adjoint steps run the FP64 nonlinear problem");
+    return IBPM_OK;
+}
+
 // get_Binv (laplacian.jl:98-119): B[:, j] = B_times!(e_j) on the device,
 // then inv(B) on the host.
 static int build_binv(ibpm_prob* P)
@@ -1464,6 +1878,7 @@ int ibpm_problem_destroy(ibpm_prob* P)
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     graphs_clear(P);
     P->hb.release();
+    adj_free(P);
     for (void* p : P->allocs) (void)hipFree(p);
     if (P->pin) (void)hipHostFree(P->pin);
     for (hipEvent_t ev : P->pev) (void)hipEventDestroy(ev);
@@ -1740,6 +2155,95 @@ int ibpm_advance_n(ibpm_prob* P, int nsteps, const double* t, const ibpm_step_sc
         return fail(P->ctx, rc, msg);
     }
     return ibpm_advance_wait(P, cd_hist, cl_hist, steps_done);
+}
+
+// ---- tangent-linear and adjoint runs ------------------------------------------
+static int hist_reserve(ibpm_prob* P, int nsteps)
+{
+    const int nb = std::max(P->nbodies, 1);
+    if ((size_t)nsteps > P->hist_steps) {
+        graphs_clear(P);
+        HIPTRY(P->ctx, P->hb.reserve(2 * (size_t)nsteps * nb));
+        P->hist = P->hb.d;
+        P->hist_steps = P->hb.n / (2 * (size_t)nb);
+    }
+    launch_set_int(P->slot, 0, P->stream);
+    return IBPM_OK;
+}
+
+// J = Σ_k wcd_k CD_k + wcl_k CL_k of the steps just run (one body)
+static int hist_objective(ibpm_prob* P, int nsteps, const double* wcd, const double* wcl, double* J)
+{
+    HIPTRY(P->ctx, hipMemcpyAsync(P->hb.h, P->hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, P->stream));
+    HIPTRY(P->ctx, hipMemcpyAsync(P->hb.h + P->hist_steps, P->hist + P->hist_steps, sizeof(double) * nsteps,
+                                  hipMemcpyDeviceToHost, P->stream));
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    double j = 0.0;
+    for (int k = 0; k < nsteps; k++) j += wcd[k] * P->hb.h[k] + wcl[k] * P->hb.h[P->hist_steps + k];
+    *J = j;
+    return IBPM_OK;
+}
+
+int ibpm_tangent_n(ibpm_prob* P, int nsteps, const ibpm_step_scalars* s, const double* ds, const double* dGamma0,
+                   const double* wcd, const double* wcl, double* J, double* dJ)
+{
+    if (!P || !s || !ds || !wcd || !wcl || !J || !dJ || nsteps < 1) return IBPM_E_ARG;
+    if (int rc = adj_check(P)) return rc;
+    if (int rc = adj_setup(P, nsteps, 1)) return rc;
+    if (int rc = hist_reserve(P, nsteps)) return rc;
+    AdjWork* W = P->adj;
+    const long n = P->g.nG * P->mg;
+    snap_store(P, 0);
+    if (dGamma0) HIPTRY(P->ctx, hipMemcpyAsync(W->dG, dGamma0, sizeof(double) * n, hipMemcpyHostToDevice, P->stream));
+    else launch_fill(W->dG, n, 0.0, P->stream);
+    launch_fill(W->dN[0], n, 0.0, P->stream);
+    int n0 = 0;
+    for (int k = 0; k < nsteps; k++) {
+        tlm_step(P, s[k], ds + 7L * k, k, n0);
+        n0 = 1 - n0;
+        step_body(P, s[k], P->hist, P->hist + P->hist_steps);
+    }
+    HIPTRY(P->ctx, hipGetLastError());
+    std::vector<double> sums(2 * (size_t)nsteps);
+    HIPTRY(P->ctx, hipMemcpyAsync(sums.data(), W->sums, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, P->stream));
+    if (int rc = hist_objective(P, nsteps, wcd, wcl, J)) return rc;
+    const double cD = 2.0 * P->g.h / P->dt;
+    double d = 0.0;
+    for (int k = 0; k < nsteps; k++) d += cD * (wcd[k] * sums[2 * k] + wcl[k] * sums[2 * k + 1]);
+    *dJ = d;
+    snap_restore(P, 0);
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    return IBPM_OK;
+}
+
+int ibpm_adjoint_n(ibpm_prob* P, int nsteps, const ibpm_step_scalars* s, const double* wcd, const double* wcl,
+                   int nsnap, double* J, double* sbar, double* Gamma0bar, int64_t* forward_steps)
+{
+    if (!P || !s || !wcd || !wcl || !J || !sbar || nsteps < 1 || nsnap < 1) return IBPM_E_ARG;
+    if (int rc = adj_check(P)) return rc;
+    if (int rc = adj_setup(P, nsteps, nsnap)) return rc;
+    if (int rc = hist_reserve(P, nsteps)) return rc;
+    AdjWork* W = P->adj;
+    const long n = P->g.nG * P->mg;
+    snap_store(P, 0);
+    for (int k = 0; k < nsteps; k++) step_body(P, s[k], P->hist, P->hist + P->hist_steps);
+    HIPTRY(P->ctx, hipGetLastError());
+    if (int rc = hist_objective(P, nsteps, wcd, wcl, J)) return rc;
+    launch_fill(W->Gb, n, 0.0, P->stream);
+    launch_fill(W->Nb1, n, 0.0, P->stream);
+    AdjRun R{P, s, wcd, wcl};
+    R.fwd_steps = nsteps;
+    rev(R, 0, nsteps, 0, nsnap - 1);
+    HIPTRY(P->ctx, hipGetLastError());
+    std::vector<double> sums(3 * (size_t)nsteps);
+    HIPTRY(P->ctx, hipMemcpyAsync(sums.data(), W->sums, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, P->stream));
+    if (Gamma0bar) HIPTRY(P->ctx, hipMemcpyAsync(Gamma0bar, W->Gb, sizeof(double) * n, hipMemcpyDeviceToHost, P->stream));
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    for (int k = 0; k < nsteps; k++) mg_flux_adjoint(s[k], sums.data() + 3 * (size_t)k, sbar + 7 * (size_t)k);
+    if (forward_steps) *forward_steps = R.fwd_steps;
+    snap_restore(P, 0);
+    HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
+    return IBPM_OK;
 }
 
 int ibpm_get_forces(ibpm_prob* P, double* CD, double* CL, double* Fb)

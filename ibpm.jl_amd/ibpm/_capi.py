@@ -41,7 +41,7 @@ EXPORTED = [
     "ibpm_op_nonlinear", "ibpm_op_vort2flux", "ibpm_op_regT", "ibpm_op_reg", "ibpm_op_rot", "ibpm_set_option",
     "ibpm_get_option", "ibpm_time_kernel", "ibpm_profile_stats", "ibpm_get_body_state", "ibpm_cg_stats",
     "ibpm_op_update_bodies", "ibpm_group_create", "ibpm_group_destroy", "ibpm_group_set_option",
-    "ibpm_group_advance_async", "ibpm_group_wait", "ibpm_op_lapinv_noalloc",
+    "ibpm_group_advance_async", "ibpm_group_wait", "ibpm_op_lapinv_noalloc", "ibpm_tangent_n", "ibpm_adjoint_n",
 ]
 
 
@@ -128,6 +128,9 @@ def lib():
         L.ibpm_group_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.ibpm_group_advance_async.argtypes = [_vp, ctypes.c_int]
         L.ibpm_group_wait.argtypes = [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int32)]
+        L.ibpm_tangent_n.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(StepScalars), _dp, _dp, _dp, _dp, _dp, _dp]
+        L.ibpm_adjoint_n.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(StepScalars), _dp, _dp, ctypes.c_int, _dp, _dp,
+                                     _dp, ctypes.POINTER(ctypes.c_int64)]
         L.ibpm_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.ibpm_get_option.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         _LIB = L

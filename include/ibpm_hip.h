@@ -213,6 +213,29 @@ int ibpm_group_advance_async(ibpm_group* group, int nsteps);
  * IBPM_E_NONFINITE if any member's CL became non-finite. */
 int ibpm_group_wait(ibpm_group* group, double* cd_hist, double* cl_hist, int32_t* steps_done);
 
+/* Tangent-linear and adjoint of advance! -- the discrete derivative of
+ *   J = sum_k (wcd[k] CD_k + wcl[k] CL_k)
+ * over nsteps steps from the current state (timestepping.jl:35-67 with the
+ * MovingGrid base flux :273-318), for one MovingGrid body and the FP64 path.
+ * A step's scalars are its 7 doubles {Ux, Uy, inclination, U, V, θ, θ̇} in the
+ * order of ibpm_step_scalars; ds / sbar hold nsteps rows of them.  No
+ * counterpart in the reference: these are the pieces a ChainRulesCore rrule
+ * of the run would call (INTEGRATION.md gives it).  Both leave the problem's
+ * state as they found it.
+ *
+ * ibpm_tangent_n: J and dJ = the directional derivative for perturbations ds
+ * of the scalars and dGamma0 (nG x mg, may be NULL) of the initial Γ,
+ * computed by tangent-linear steps alongside the forward ones.
+ * ibpm_adjoint_n: J, sbar = dJ/ds for every step, and (if non-NULL)
+ * Gamma0bar = dJ/dΓ of the initial state, by adjoint steps in reverse with
+ * binomial checkpointing of (Γ, nonlin[2]) in nsnap snapshots (nsnap >= 1;
+ * every reverse step recomputes its input state from the nearest snapshot).
+ * forward_steps (may be NULL): forward steps taken, the first run included. */
+int ibpm_tangent_n(ibpm_prob* prob, int nsteps, const ibpm_step_scalars* s, const double* ds, const double* dGamma0,
+                   const double* wcd, const double* wcl, double* J, double* dJ);
+int ibpm_adjoint_n(ibpm_prob* prob, int nsteps, const ibpm_step_scalars* s, const double* wcd, const double* wcl,
+                   int nsnap, double* J, double* sbar, double* Gamma0bar, int64_t* forward_steps);
+
 /* state.CD, state.CL (nbodies each) and state.F̃b (2 nb_total); any may be NULL. */
 int ibpm_get_forces(ibpm_prob* prob, double* CD, double* CL, double* Fb);
 /* compute_cfl(state, prob) = max|q[:,1]| dt / h (ibpm.jl:83-88). */
