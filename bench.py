@@ -125,7 +125,8 @@ def bench_naca64(args):
     # groups overlapping on their streams measured faster than 3 mixed groups
     for k, c in enumerate(cs):
         shapes[(c["grid"].nx, c["grid"].ny, c["grid"].mg)].append(k)
-    groups = [ibpm.Group([probs[k] for k in ks], [states[k] for k in ks]) for ks in shapes.values()]
+    members = list(shapes.values())
+    groups = [ibpm.Group([probs[k] for k in ks], [states[k] for k in ks]) for ks in members]
     for grp in groups:
         grp.advance_async(W)
     for grp in groups:
@@ -134,9 +135,20 @@ def bench_naca64(args):
     for grp in groups:
         grp.advance_async(K)
     gres = [grp.wait() for grp in groups]
-    el = time.perf_counter() - t3
+    el_groups = time.perf_counter() - t3
     if any(r[2] != K for g in gres for r in g):
         raise RuntimeError("non-finite forces in a group")
+    # the 9 groups in one graph per step (a branch per group), replayed K times
+    ibpm.advance_groups_(groups, W)
+    t4 = time.perf_counter()
+    jres = ibpm.advance_groups_(groups, K)
+    el = time.perf_counter() - t4
+    if any(r[2] != K for g in jres for r in g):
+        raise RuntimeError("non-finite forces in the joint step")
+    final = {}
+    for ks, gr in zip(members, jres):
+        for k, (cd, cl, _) in zip(ks, gr):
+            final[k] = (float(cd[-1, 0]), float(cl[-1, 0]))
     for grp in groups:
         grp.close()
     total = len(cs) * K
@@ -149,13 +161,20 @@ def bench_naca64(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "64 independent NACA 0012 simulations, Re 100-1000, alpha 0-20 deg, dt = 0.25 dx, "
                                "impulsive start; the instances of each grid shape in one group (every pass one "
-                               "launch over the group), the groups on their own streams",
+                               "launch over the group), one step of all groups one HIP graph with a branch per "
+                               "group (ibpm_groups_advance_async)",
                    "instances": len(cs), "groups": len(groups), "setup_s": round(setup_s, 1)},
+        "groups_on_streams": {"value": round(total / el_groups, 1), "unit": "timesteps/s",
+                              "how": "the same groups, each on its own stream, its steps replayed from its own graphs"},
         "streams": {"value": round(total / el_str, 1), "unit": "timesteps/s",
                     "how": "every problem on its own stream (ibpm_advance_async/_wait), graph-replayed steps"},
         "one_by_one": {"value": round(total / el_seq, 1), "unit": "timesteps/s"},
         "step_hbm": {"algorithmic_bytes": byts, "achieved_GBs": round(byts / el / 1e9, 1),
                      "frac_of_peak": round(byts / el / 1e9 / HBM_PEAK_GBS, 4)},
+        "per_instance": [{"grid": [c["grid"].nx, c["grid"].ny, c["grid"].mg], "Re": round(c["Re"], 3),
+                          "alpha_deg": round(math.degrees(c["freestream"]["inclination"](0.0)), 4),
+                          "CD": final[k][0], "CL": final[k][1], "steps_from_start": 3 * W + 4 * K}
+                         for k, c in enumerate(cs)],
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

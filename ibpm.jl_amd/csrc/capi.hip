@@ -2327,6 +2327,16 @@ struct ibpm_group {
     size_t hist_steps = 0;
     int pending = 0;
     int ytile = 4, ytile_single = 4, xwaves = 4;
+    // HIP graphs of one group step, one per buffer role (as ibpm_prob's);
+    // every launch argument is fixed by the roles (the force arguments and
+    // history pointers are read from device memory when the kernels run)
+    bool graphs = true;
+    struct StepGraph {
+        double* G;
+        int out, nl2;
+        hipGraphExec_t exec;
+    };
+    std::vector<StepGraph> gcache;
     std::vector<void*> allocs;
     // the members' own buffers, restored when the group is destroyed
     struct Own {
@@ -2520,6 +2530,99 @@ int group_hist(ibpm_group* Gp, size_t steps)
     return IBPM_OK;
 }
 
+void group_graphs_clear(ibpm_group* Gp)
+{
+    for (auto& e : Gp->gcache) (void)hipGraphExecDestroy(e.exec);
+    Gp->gcache.clear();
+}
+
+// group_step replayed from a graph captured for the current buffer roles
+// (captured, instantiated and launched on first use of a role)
+void group_step_dev(ibpm_group* Gp)
+{
+    if (!Gp->graphs) {
+        group_step(Gp);
+        return;
+    }
+    const int out = (Gp->nl1 != Gp->nl2) ? Gp->nl1 : 1 - Gp->nl2;
+    for (auto& e : Gp->gcache) {
+        if (e.G == Gp->G && e.out == out && e.nl2 == Gp->nl2) {
+            (void)hipGraphLaunch(e.exec, Gp->stream);
+            // the host side of group_step: roles of the buffers
+            Gp->nl1 = Gp->nl2 = out;
+            std::swap(Gp->G, Gp->Gs);
+            for (int i = 0; i < Gp->n; i++) {
+                ibpm_prob* P = Gp->probs[i];
+                const long ist = (long)Gp->L * Gp->g.nG;
+                P->G = Gp->G + (long)i * ist;
+                P->Gs = Gp->Gs + (long)i * ist;
+                P->nl1 = Gp->nl1;
+                P->nl2 = Gp->nl2;
+            }
+            return;
+        }
+    }
+    ibpm_group::StepGraph e{Gp->G, out, Gp->nl2, nullptr};
+    double *G0 = Gp->G, *Gs0 = Gp->Gs;
+    const int nl1 = Gp->nl1, nl2 = Gp->nl2;
+    hipGraph_t graph = nullptr;
+    bool ok = hipStreamBeginCapture(Gp->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        group_step(Gp);
+        ok = hipStreamEndCapture(Gp->stream, &graph) == hipSuccess && graph;
+    }
+    if (ok) ok = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ok) ok = hipGraphLaunch(e.exec, Gp->stream) == hipSuccess;
+    if (ok) {
+        Gp->gcache.push_back(e);
+        return;
+    }
+    // capture unavailable: undo the host side of the captured step, run it directly
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    (void)hipGetLastError();
+    Gp->graphs = false;
+    Gp->G = G0; Gp->Gs = Gs0; Gp->nl1 = nl1; Gp->nl2 = nl2;
+    group_step(Gp);
+}
+
+// Several groups stepped by one graph: one step of every group captured with
+// a branch per group (each group's launches on its own stream, forked from and
+// joined into the first group's stream), kept per combination of the groups'
+// buffer roles.  The registry lets ibpm_group_destroy drop the graphs that
+// name a destroyed group.
+struct MultiGraph {
+    std::vector<ibpm_group*> groups;
+    std::vector<double*> G;
+    std::vector<int> out, nl2;
+    hipGraphExec_t exec;
+};
+std::vector<MultiGraph> g_multi;
+std::vector<hipEvent_t> g_multi_ev;
+
+void multi_drop(ibpm_group* Gp)
+{
+    for (size_t k = 0; k < g_multi.size();) {
+        auto& m = g_multi[k];
+        if (std::find(m.groups.begin(), m.groups.end(), Gp) != m.groups.end()) {
+            (void)hipGraphExecDestroy(m.exec);
+            g_multi.erase(g_multi.begin() + k);
+        } else {
+            k++;
+        }
+    }
+}
+
+int multi_events(size_t n)
+{
+    while (g_multi_ev.size() < n) {
+        hipEvent_t ev;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return IBPM_E_HIP;
+        g_multi_ev.push_back(ev);
+    }
+    return IBPM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2684,6 +2787,9 @@ int ibpm_group_destroy(ibpm_group* Gp)
 {
     if (!Gp) return IBPM_OK;
     (void)hipStreamSynchronize(Gp->stream);
+    (void)hipDeviceSynchronize();  // a joint graph of several groups may still run on another group's stream
+    group_graphs_clear(Gp);
+    multi_drop(Gp);
     const long nG = Gp->g.nG, nq = Gp->g.nq;
     int rc = IBPM_OK;
     for (int i = 0; i < Gp->n; i++) {
@@ -2720,9 +2826,13 @@ int ibpm_group_set_option(ibpm_group* Gp, int option, int value)
     // the members' checks hold for the shared grid
     int rc = ibpm_set_option(Gp->probs[0], option, value);
     if (rc != IBPM_OK) return rc;
+    if (Gp->pending > 0) return fail(Gp->ctx, IBPM_E_ARG, "wait for the group's queued steps first");
+    group_graphs_clear(Gp);
+    multi_drop(Gp);
     if (option == IBPM_OPT_YTILE_BATCH) Gp->ytile = value;
     else if (option == IBPM_OPT_YTILE_SINGLE) Gp->ytile_single = value;
     else if (option == IBPM_OPT_XWAVES_BATCH) Gp->xwaves = value;
+    else if (option == IBPM_OPT_GRAPHS) Gp->graphs = value == 1;
     return IBPM_OK;
 }
 
@@ -2738,9 +2848,111 @@ int ibpm_group_advance_async(ibpm_group* Gp, int nsteps)
     }
     if (nsteps > 0)
         for (int i = 0; i < Gp->n; i++) launch_set_int(Gp->slots + i, Gp->pending, Gp->stream);
-    for (int k = 0; k < nsteps; k++) group_step(Gp);
+    for (int k = 0; k < nsteps; k++) group_step_dev(Gp);
     Gp->pending += nsteps;
     HIPTRY(Gp->ctx, hipGetLastError());
+    return IBPM_OK;
+}
+
+int ibpm_groups_advance_async(int ngroups, ibpm_group* const* groups, int nsteps)
+{
+    if (ngroups < 1 || !groups || nsteps < 0) return IBPM_E_ARG;
+    for (int a = 0; a < ngroups; a++) {
+        if (!groups[a]) return IBPM_E_ARG;
+        for (int b = 0; b < a; b++)
+            if (groups[b] == groups[a]) return fail(groups[0]->ctx, IBPM_E_ARG, "a group is listed twice");
+        if (groups[a]->ctx->device != groups[0]->ctx->device)
+            return fail(groups[0]->ctx, IBPM_E_ARG, "the groups must be on one device");
+    }
+    ibpm_ctx* C = groups[0]->ctx;
+    hipStream_t s0 = groups[0]->stream;
+    // histories and the per-instance history slots, on each group's stream
+    for (int a = 0; a < ngroups; a++) {
+        ibpm_group* Gp = groups[a];
+        const size_t want = (size_t)Gp->pending + (size_t)nsteps;
+        if (want > Gp->hist_steps) {
+            if (Gp->pending > 0)
+                return fail(C, IBPM_E_ARG, "history full: call ibpm_group_wait before enqueuing more steps");
+            int rc = group_hist(Gp, want);
+            if (rc != IBPM_OK) return rc;
+        }
+    }
+    if (int rc = multi_events(2 * (size_t)ngroups + 1)) return fail(C, rc, "hipEventCreate failed");
+    if (nsteps == 0) return IBPM_OK;
+    for (int a = 0; a < ngroups; a++) {
+        ibpm_group* Gp = groups[a];
+        for (int i = 0; i < Gp->n; i++) launch_set_int(Gp->slots + i, Gp->pending, Gp->stream);
+        if (a > 0) {
+            HIPTRY(C, hipEventRecord(g_multi_ev[a], Gp->stream));
+            HIPTRY(C, hipStreamWaitEvent(s0, g_multi_ev[a], 0));
+        }
+    }
+    for (int k = 0; k < nsteps; k++) {
+        // the graph for the groups' current roles, else capture one
+        MultiGraph* mg = nullptr;
+        for (auto& m : g_multi) {
+            if ((int)m.groups.size() != ngroups) continue;
+            bool same = true;
+            for (int a = 0; a < ngroups && same; a++) {
+                ibpm_group* Gp = groups[a];
+                const int out = (Gp->nl1 != Gp->nl2) ? Gp->nl1 : 1 - Gp->nl2;
+                same = m.groups[a] == Gp && m.G[a] == Gp->G && m.out[a] == out && m.nl2[a] == Gp->nl2;
+            }
+            if (same) { mg = &m; break; }
+        }
+        if (mg) {
+            HIPTRY(C, hipGraphLaunch(mg->exec, s0));
+            for (int a = 0; a < ngroups; a++) {  // the host side of group_step
+                ibpm_group* Gp = groups[a];
+                const int out = (Gp->nl1 != Gp->nl2) ? Gp->nl1 : 1 - Gp->nl2;
+                Gp->nl1 = Gp->nl2 = out;
+                std::swap(Gp->G, Gp->Gs);
+                const long ist = (long)Gp->L * Gp->g.nG;
+                for (int i = 0; i < Gp->n; i++) {
+                    ibpm_prob* P = Gp->probs[i];
+                    P->G = Gp->G + (long)i * ist;
+                    P->Gs = Gp->Gs + (long)i * ist;
+                    P->nl1 = Gp->nl1;
+                    P->nl2 = Gp->nl2;
+                }
+            }
+            continue;
+        }
+        MultiGraph m;
+        for (int a = 0; a < ngroups; a++) {
+            ibpm_group* Gp = groups[a];
+            m.groups.push_back(Gp);
+            m.G.push_back(Gp->G);
+            m.out.push_back((Gp->nl1 != Gp->nl2) ? Gp->nl1 : 1 - Gp->nl2);
+            m.nl2.push_back(Gp->nl2);
+        }
+        hipGraph_t graph = nullptr;
+        HIPTRY(C, hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+        bool ok = hipEventRecord(g_multi_ev[0], s0) == hipSuccess;
+        for (int a = 1; a < ngroups && ok; a++) ok = hipStreamWaitEvent(groups[a]->stream, g_multi_ev[0], 0) == hipSuccess;
+        if (ok)
+            for (int a = 0; a < ngroups; a++) group_step(groups[a]);
+        for (int a = 1; a < ngroups && ok; a++) {
+            ok = hipEventRecord(g_multi_ev[ngroups + a], groups[a]->stream) == hipSuccess &&
+                 hipStreamWaitEvent(s0, g_multi_ev[ngroups + a], 0) == hipSuccess;
+        }
+        const bool ended = hipStreamEndCapture(s0, &graph) == hipSuccess && graph;
+        ok = ok && ended && hipGraphInstantiate(&m.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!ok) {
+            (void)hipGetLastError();
+            return fail(C, IBPM_E_HIP, "capturing the joint step of the groups failed");
+        }
+        HIPTRY(C, hipGraphLaunch(m.exec, s0));
+        g_multi.push_back(m);
+    }
+    // every group's own stream waits for the joint steps (ibpm_group_wait drains it)
+    HIPTRY(C, hipEventRecord(g_multi_ev[2 * ngroups], s0));
+    for (int a = 0; a < ngroups; a++) {
+        if (a > 0) HIPTRY(C, hipStreamWaitEvent(groups[a]->stream, g_multi_ev[2 * ngroups], 0));
+        groups[a]->pending += nsteps;
+    }
+    HIPTRY(C, hipGetLastError());
     return IBPM_OK;
 }
 

@@ -18,16 +18,16 @@ from .domain import (InertialMotion, Motion, MotionFunction, MotionType, MovingG
                      get_body_info, make_body, make_cylinder, make_grid, make_naca, make_plate, naca)
 from .problem import (AB2, AdamsBashforth, Group, IBModel, IBProblem, IBState, LinearizedIBProblem, StateSnapshot,
                       UserVar,
-                      advance, advance_, advance_batch_,
+                      advance, advance_, advance_batch_, advance_groups_,
                       advance_n_, base_flux_, compute_cfl, copy_, init_save_data, load_state, run_sim_, save_data_,
                       save_state, snapshot, vort2flux_)
-from . import _capi
+from . import _capi, adjoint
 
 __all__ = [
     "Group",
     "MultiGrid", "make_grid", "Motion", "InertialMotion", "Static", "MovingGrid", "MotionFunction", "RigidBody",
     "make_plate", "make_cylinder", "make_naca", "naca", "make_body", "MotionType", "get_body_info", "AB2",
-    "AdamsBashforth", "IBModel", "IBProblem", "LinearizedIBProblem", "IBState", "advance_", "advance", "advance_n_", "advance_batch_", "base_flux_",
+    "AdamsBashforth", "IBModel", "IBProblem", "LinearizedIBProblem", "IBState", "advance_", "advance", "advance_n_", "advance_batch_", "advance_groups_", "base_flux_",
     "vort2flux_", "compute_cfl", "run_sim_", "UserVar", "init_save_data", "save_data_", "StateSnapshot", "snapshot",
-    "copy_", "save_state", "load_state",
+    "copy_", "save_state", "load_state", "adjoint",
 ]

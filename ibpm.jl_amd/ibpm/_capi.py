@@ -42,6 +42,7 @@ EXPORTED = [
     "ibpm_get_option", "ibpm_time_kernel", "ibpm_profile_stats", "ibpm_get_body_state", "ibpm_cg_stats",
     "ibpm_op_update_bodies", "ibpm_group_create", "ibpm_group_destroy", "ibpm_group_set_option",
     "ibpm_group_advance_async", "ibpm_group_wait", "ibpm_op_lapinv_noalloc", "ibpm_tangent_n", "ibpm_adjoint_n",
+    "ibpm_groups_advance_async",
 ]
 
 
@@ -127,6 +128,7 @@ def lib():
         L.ibpm_group_destroy.argtypes = [_vp]
         L.ibpm_group_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.ibpm_group_advance_async.argtypes = [_vp, ctypes.c_int]
+        L.ibpm_groups_advance_async.argtypes = [ctypes.c_int, ctypes.POINTER(_vp), ctypes.c_int]
         L.ibpm_group_wait.argtypes = [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int32)]
         L.ibpm_tangent_n.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(StepScalars), _dp, _dp, _dp, _dp, _dp, _dp]
         L.ibpm_adjoint_n.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(StepScalars), _dp, _dp, ctypes.c_int, _dp, _dp,

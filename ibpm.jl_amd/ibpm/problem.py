@@ -460,6 +460,18 @@ class Group:
             pass
 
 
+def advance_groups_(groups, nsteps: int):
+    """Step several groups together (ibpm_groups_advance_async): one step of
+    every group is a single HIP graph with a branch per group, replayed
+    nsteps times; returns each group's wait() result."""
+    arr = (ctypes.c_void_p * len(groups))(*[g.h.value for g in groups])
+    groups[0].ctx.check(C.lib().ibpm_groups_advance_async(len(groups), arr, int(nsteps)),
+                        "ibpm_groups_advance_async")
+    for g in groups:
+        g._pending = getattr(g, "_pending", 0) + int(nsteps)
+    return [g.wait() for g in groups]
+
+
 def advance(x: IBState, prob: IBProblem, t: float):
     """advance(x, prob, t) -- timestepping.jl:22-28.  The device state is
     single-copy per problem, so this advances `x` in place and returns it."""

@@ -203,10 +203,18 @@ int ibpm_cg_stats(ibpm_prob* prob, int32_t* iterations, double* residual, double
  * member alone. */
 int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* members, ibpm_group** out);
 int ibpm_group_destroy(ibpm_group* group);
-/* IBPM_OPT_YTILE_* and IBPM_OPT_XWAVES_BATCH for the group's launches */
+/* IBPM_OPT_YTILE_*, IBPM_OPT_XWAVES_BATCH and IBPM_OPT_GRAPHS (capture of the
+ * group's steps in HIP graphs, on by default) for the group's launches */
 int ibpm_group_set_option(ibpm_group* group, int option, int value);
 /* Enqueue nsteps steps of every member (static bodies: no per-step scalars). */
 int ibpm_group_advance_async(ibpm_group* group, int nsteps);
+/* Several groups (for instance one per grid shape) stepped together: one step
+ * of every group is captured in a single HIP graph, a branch per group (each
+ * group's launches on its own stream, forked from and joined into the first
+ * group's stream), and the graph is replayed nsteps times.  Collect each
+ * group's histories with ibpm_group_wait as after ibpm_group_advance_async.
+ * No counterpart in the reference (many run_sim! loops, ibpm.jl:93-105). */
+int ibpm_groups_advance_async(int ngroups, ibpm_group* const* groups, int nsteps);
 /* Drain; cd_hist / cl_hist (may be NULL) receive, member after member, each
  * member's (steps x nbodies) history of the steps since the last wait;
  * steps_done (n entries, may be NULL) as in ibpm_advance_wait per member;

@@ -185,3 +185,28 @@ def test_plate_gradient_full_run():
     assert abs(J - Jo) <= 1e-8 * abs(Jo)
     assert np.allclose(ga, gt, rtol=1e-8, atol=1e-10 * np.max(np.abs(gt)))
     assert np.allclose(ga, fd, rtol=1e-4, atol=1e-5 * np.max(np.abs(fd)))
+
+
+def test_torch_autograd_function():
+    """ibpm.adjoint.pitching_mean_drag as a torch.autograd.Function: its
+    backward (the adjoint) against torch.autograd.gradcheck's central
+    differences of the forward, on the 20-step pitching plate."""
+    import torch
+
+    from ibpm.adjoint import pitching_mean_drag
+
+    spec = json.load(open(os.path.join(GOLDEN, "adjoint_fd_plate_20.json")))
+    c = case("plate_pitch_512x256x4")
+    g = c["grid"]
+    gp = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream={})
+    ibpm.IBState(gp)
+    ts = np.arange(20) * c["dt"]
+    p = torch.tensor([spec["A"], spec["f"], spec["beta"]], dtype=torch.float64, requires_grad=True)
+    J = pitching_mean_drag(p, gp, ts, 10)
+    J.backward()
+    fd = np.array([spec["grad"][k] for k in ("A", "f", "beta")])
+    print("autograd", p.grad.numpy(), "oracle FD", fd)
+    assert abs(J.item() - spec["J"]) <= 1e-10 * abs(spec["J"])
+    assert np.allclose(p.grad.numpy(), fd, rtol=1e-5, atol=1e-6 * np.max(np.abs(fd)))
+    assert torch.autograd.gradcheck(lambda q: pitching_mean_drag(q, gp, ts, 10), (p,), eps=1e-6, atol=1e-7,
+                                    rtol=1e-5)

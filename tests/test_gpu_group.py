@@ -125,3 +125,65 @@ def test_group_rejects_mixed_grids():
     other = ibpm.IBProblem(g2, [ibpm.make_cylinder(0.3, g2.h, 0.0, 0.0, n=24)], 0.01, 100.0)
     with pytest.raises(C.IBPMError):
         ibpm.Group([pa[0], other])
+
+
+def test_group_graph_replay_equals_launches():
+    """A group's steps replayed from captured graphs (the default) are the
+    same launches as stepping it without graphs: bit-identical."""
+    pa, sa = members()
+    pg, sg = members([p.Binv for p in pa])
+    ga = ibpm.Group(pa, sa)
+    ga.set_option(C.OPT_GRAPHS, 0)
+    gb = ibpm.Group(pg, sg)
+    ra = ga.advance_n_(9)
+    rb = gb.advance_n_(9)
+    rb2 = gb.advance_n_(4)  # graphs replayed with the roles of later steps
+    ra2 = ga.advance_n_(4)
+    for (cd, cl, d), (cdg, clg, dg) in zip(ra + ra2, rb + rb2):
+        assert d == dg and np.array_equal(cd, cdg) and np.array_equal(cl, clg)
+    ga.close()
+    gb.close()
+    for st, stg in zip(sa, sg):
+        for a, b in zip(fields(st), fields(stg)):
+            assert np.array_equal(a, b)
+
+
+def test_joint_graph_of_three_naca_shapes_matches_oracle():
+    """One NACA 0012 instance of each grid shape of the batched workload
+    (400 x 200 with 3 levels, 600 x 300 with 4, 800 x 400 with 5), each in
+    its own group, the three groups stepped by one graph per step for 20
+    steps from the impulsive start: every instance against the oracle (which
+    takes the GPU's Binv) to the 1e-10 bar."""
+    import oracle as O
+    from cases import boundary_of, oracle_bodies, sc7
+    from conftest import relerr
+
+    want = [(400, 200, 3), (600, 300, 4), (800, 400, 5)]
+    cs = naca64_cases()
+    picked = [next(c for c in cs if (c["grid"].nx, c["grid"].ny, c["grid"].mg) == w) for w in want]
+    O.set_threads(16)
+    n = 20
+    probs, states, oracles = [], [], []
+    for c in picked:
+        g = c["grid"]
+        p = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
+        st = ibpm.IBState(p)
+        op = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], c["dt"], oracle_bodies(c), binv=p.Binv)
+        S = op.new_state(sc7(c, 0.0))
+        cd_o, cl_o = op.run(S, np.stack([sc7(c, k * c["dt"]) for k in range(n)]))
+        probs.append(p)
+        states.append(st)
+        oracles.append((S, cd_o, cl_o))
+    groups = [ibpm.Group([p], [st]) for p, st in zip(probs, states)]
+    res = ibpm.advance_groups_(groups, n)
+    for grp in groups:
+        grp.close()
+    for c, (r,), st, (S, cd_o, cl_o) in zip(picked, res, states, oracles):
+        cd, cl, d = r
+        assert d == n
+        scale = max(np.max(np.abs(cd_o)), np.max(np.abs(cl_o)))
+        errs = {"CD": float(np.max(np.abs(cd - cd_o)) / scale), "CL": float(np.max(np.abs(cl - cl_o)) / scale),
+                "Γ": relerr(st.Γ, S.G), "ψ": relerr(st.ψ, S.psi), "q": relerr(st.q, S.q)}
+        g = c["grid"]
+        print(f"{g.nx}x{g.ny}x{g.mg}: final CD {cd[-1, 0]:.8f} CL {cl[-1, 0]:.3e};", errs)
+        assert all(v <= 1e-10 for v in errs.values()), errs
