@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(CSRC, "libibpm_hip.so")
-SOURCES = ["kernels.hip", "mixed.hip", "adjoint.hip", "capi.hip"]
+SOURCES = ["kernels.hip", "mixed.hip", "adjoint.hip", "setup.hip", "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -349,56 +349,6 @@ static double delta_h(double rf, double rb, double dr)
     return del_h;
 }
 
-// In-place inverse of a column-major n x n matrix: LU with partial pivoting
-// (getrf), then solves against the identity (getri), as inv() does.
-static bool invert_lu(std::vector<double>& A, int n)
-{
-    std::vector<double> LU(A);
-    std::vector<int> piv(n);
-    auto at = [&](int i, int j) -> double& { return LU[(size_t)i + (size_t)n * j]; };
-    for (int k = 0; k < n; k++) {
-        int p = k;
-        double best = std::fabs(at(k, k));
-        for (int i = k + 1; i < n; i++)
-            if (std::fabs(at(i, k)) > best) { best = std::fabs(at(i, k)); p = i; }
-        piv[k] = p;
-        if (best == 0.0) return false;
-        if (p != k)
-            for (int j = 0; j < n; j++) std::swap(at(k, j), at(p, j));
-        double inv = 1.0 / at(k, k);
-        for (int i = k + 1; i < n; i++) at(i, k) *= inv;
-        for (int j = k + 1; j < n; j++) {
-            double akj = at(k, j);
-            if (akj != 0.0) {
-                double* cj = &LU[(size_t)n * j];
-                const double* ck = &LU[(size_t)n * k];
-                for (int i = k + 1; i < n; i++) cj[i] -= ck[i] * akj;
-            }
-        }
-    }
-    for (int c = 0; c < n; c++) {
-        double* x = &A[(size_t)n * c];
-        for (int i = 0; i < n; i++) x[i] = 0.0;
-        x[c] = 1.0;
-        for (int k = 0; k < n; k++)
-            if (piv[k] != k) std::swap(x[k], x[piv[k]]);
-        for (int j = 0; j < n; j++) {
-            double xj = x[j];
-            if (xj != 0.0) {
-                const double* cj = &LU[(size_t)n * j];
-                for (int i = j + 1; i < n; i++) x[i] -= cj[i] * xj;
-            }
-        }
-        for (int j = n - 1; j >= 0; j--) {
-            const double* cj = &LU[(size_t)n * j];
-            x[j] /= cj[j];
-            double xj = x[j];
-            for (int i = 0; i < j; i++) x[i] -= cj[i] * xj;
-        }
-    }
-    return true;
-}
-
 // ---------------------------------------------------------------------------
 // operator schedules on the device
 // ---------------------------------------------------------------------------
@@ -1629,27 +1579,66 @@ static int adj_check(ibpm_prob* P)
 static int build_binv(ibpm_prob* P)
 {
     ibpm_ctx* C = P->ctx;
-    hipStream_t s = P->stream;
+    const GridDev& g = P->g;
     const int nf = P->nf;
-    const long nG = P->g.nG;
-    double* Bd = nullptr;
-    HIPTRY(C, hipMalloc(&Bd, sizeof(double) * (size_t)nf * nf));
-    // scratch: Gs (Γ), psi (ψ), qs1 (q[:,1]); the coarse-level ring of Γ stays 0
-    launch_fill(P->Gs, nG * P->mg, 0.0, s);
-    for (int j = 0; j < nf; j++) {
-        launch_set_unit(P->Ftmp, nf, j, s);
-        project_forces(P, P->Ftmp, P->Gs, false);
-        vort2flux_dev(P, P->Gs, P->psi, P->qs1, true);
-        GatherArgs ga{};
-        ga.q = P->qs1; ga.q0 = nullptr; ga.w = P->w; ga.bidx = P->bidx; ga.out = Bd + (long)nf * j;
-        ga.nbt = P->nbt; ga.g = P->g;
-        launch_gather(ga, s);
+    const long nG = g.nG, nq = g.nq, L = P->mg;
+    // All 2 nb unit forces through B_times! in batches (assemble_b): as many
+    // fields per batch as fit in 2 GiB of scratch, allocated for the setup only
+    const size_t per = sizeof(double) * ((size_t)nq + 2 * (size_t)nG + 2 * (size_t)L * nG + 2 * (size_t)g.n2);
+    const int K = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nf, ((size_t)2 << 30) / per, (size_t)32768}));
+    double *bq = nullptr, *bgt = nullptr, *bt1 = nullptr, *bG = nullptr, *bpsi = nullptr, *bedge = nullptr,
+           *Bd = nullptr;
+    auto release = [&]() {
+        for (double* p : {bq, bgt, bt1, bG, bpsi, bedge, Bd}) if (p) (void)hipFree(p);
+        P->bq = P->bgt = P->bt1 = P->bG = P->bpsi = P->bedge = P->Bd = nullptr;
+        P->bk = 0;
+    };
+    hipError_t e = hipMalloc(&bq, sizeof(double) * K * nq);
+    if (e == hipSuccess) e = hipMalloc(&bgt, sizeof(double) * K * nG);
+    if (e == hipSuccess) e = hipMalloc(&bt1, sizeof(double) * K * nG);
+    if (e == hipSuccess) e = hipMalloc(&bG, sizeof(double) * K * L * nG);
+    if (e == hipSuccess) e = hipMalloc(&bpsi, sizeof(double) * K * L * nG);
+    if (e == hipSuccess) e = hipMalloc(&bedge, sizeof(double) * K * 2 * (size_t)g.n2);
+    if (e == hipSuccess) e = hipMalloc(&Bd, sizeof(double) * (size_t)nf * nf);
+    if (e != hipSuccess) {
+        release();
+        return fail(C, IBPM_E_HIP, "hipMalloc failed for the B assembly batches");
     }
-    std::vector<double> B((size_t)nf * nf);
-    hipError_t e = hipMemcpyAsync(B.data(), Bd, sizeof(double) * B.size(), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(Bd);
-    if (e != hipSuccess) return fail(C, IBPM_E_HIP, std::string("B assembly: ") + hipGetErrorString(e));
+    P->bq = bq; P->bgt = bgt; P->bt1 = bt1; P->bG = bG; P->bpsi = bpsi; P->bedge = bedge; P->Bd = Bd; P->bk = K;
+    launch_fill(bG, (long)K * L * nG, 0.0, P->stream);  // the coarse-level rings stay zero
+    P->Bh.assign((size_t)nf * nf, 0.0);
+    int rc = assemble_b(P);
+    if (rc != IBPM_OK) {
+        release();
+        return rc;
+    }
+    // inv(B) on the device: Gauss-Jordan with row pivoting on [B | I]
+    double *M = nullptr, *pv = nullptr, *f = nullptr;
+    int* iv = nullptr;
+    e = hipMalloc(&M, sizeof(double) * 2 * (size_t)nf * nf);
+    if (e == hipSuccess) e = hipMalloc(&pv, sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&f, sizeof(double) * nf);
+    if (e == hipSuccess) e = hipMalloc(&iv, 2 * sizeof(int));
+    std::vector<double> eye((size_t)nf * nf, 0.0);
+    for (int i = 0; i < nf; i++) eye[(size_t)i + (size_t)nf * i] = 1.0;
+    int hv[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(M, Bd, sizeof(double) * nf * nf, hipMemcpyDeviceToDevice, P->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(M + (size_t)nf * nf, eye.data(), sizeof(double) * nf * nf, hipMemcpyHostToDevice, P->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(iv, hv, 2 * sizeof(int), hipMemcpyHostToDevice, P->stream);
+    if (e == hipSuccess) {
+        launch_gauss_jordan(M, nf, iv, pv, iv + 1, f, P->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(eye.data(), M + (size_t)nf * nf, sizeof(double) * nf * nf,
+                                            hipMemcpyDeviceToHost, P->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hv, iv, 2 * sizeof(int), hipMemcpyDeviceToHost, P->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+    for (void* p : {(void*)M, (void*)pv, (void*)f, (void*)iv}) if (p) (void)hipFree(p);
+    release();
+    if (e != hipSuccess) return fail(C, IBPM_E_HIP, std::string("inverting B: ") + hipGetErrorString(e));
+    if (hv[1]) return fail(C, IBPM_E_SINGULAR, "B is singular");
+    std::vector<double>& B = P->Bh;
     double num = 0.0, den = 0.0;
     for (int i = 0; i < nf; i++)
         for (int k = 0; k < nf; k++) {
@@ -1658,8 +1647,9 @@ static int build_binv(ibpm_prob* P)
             den += B[(size_t)i + (size_t)nf * k] * B[(size_t)i + (size_t)nf * k];
         }
     P->b_asym = den > 0 ? std::sqrt(num / den) : 0.0;
-    P->binv = B;
-    if (!invert_lu(P->binv, nf)) return fail(C, IBPM_E_SINGULAR, "B is singular");
+    P->binv.swap(eye);
+    B.clear();
+    B.shrink_to_fit();
     return IBPM_OK;
 }
 

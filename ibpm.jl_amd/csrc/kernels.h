@@ -241,6 +241,12 @@ void launch_copy(double* dst, const double* src, long n, hipStream_t s);
 void launch_set_unit(double* p, int n, int j, hipStream_t s);
 void launch_absmax(const double* p, long n, double* out, hipStream_t s);  // out: single double (>= 0)
 
+// inv(M) of an n x n matrix by Gauss-Jordan elimination with row pivoting
+// (setup.hip): M is the n x 2n column-major augmented [B | I], overwritten,
+// the inverse left in its right half; piv, pval, singular: one device int,
+// double, int (singular set to 1 on a zero pivot column); f: n doubles
+void launch_gauss_jordan(double* M, int n, int* piv, double* pval, int* singular, double* f, hipStream_t s);
+
 void kernels_init();  // sets large-LDS attributes
 
 }  // namespace ibpm

@@ -516,6 +516,49 @@ def test_full_size_dinv_inverts_5pt(big, rng):
     assert relerr(x, psi.ravel(order="F")) < 1e-11
 
 
+def test_full_size_b_columns_and_binv():
+    """Columns of the Schur matrix B at 1024 x 1024 x 5 (804 unknowns):
+    B_times! (laplacian.jl:143-169) of 8 unit forces through the GPU's
+    operators (E', rot!, Ainv, vort2flux!, E) against the oracle's B_times!,
+    to 1e-12; and the GPU's Binv (B assembled on the GPU, inverted on the
+    host) applied to the oracle's columns gives the unit vectors back, to the
+    level cond(B) ~ 5e5 allows."""
+    import ctypes
+
+    from ibpm import _capi as C
+
+    c = case("cyl_re200_1024x1024x5")
+    g = c["grid"]
+    gp = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
+    nf = gp.nf
+    op = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], c["dt"], oracle_bodies(c), binv=np.eye(nf))
+    O.set_threads(16)
+    Binv = gp.Binv
+    L = C.lib()
+    worst_b, worst_r = 0.0, 0.0
+    for j in (0, 1, nf // 4, nf // 2 - 1, nf // 2, nf // 2 + 7, 3 * nf // 4, nf - 1):
+        e = np.zeros(nf)
+        e[j] = 1.0
+        b_o = op.B_times(e)
+        (q,) = _op(gp, "ibpm_op_reg", e, out_sizes=[g.nq])
+        (G1,) = _op(gp, "ibpm_op_rot", q, out_sizes=[g.nΓ])
+        x = np.zeros(g.nΓ)
+        gp.ctx.check(L.ibpm_op_lapinv(gp.h, 2, 1, C.ptr(G1), C.ptr(x)))
+        G = np.zeros((g.nΓ, g.mg), order="F")
+        G[:, 0] = x
+        psi = np.zeros((g.nΓ, g.mg), order="F")
+        qa = np.zeros((g.nq, g.mg), order="F")
+        gp.ctx.check(L.ibpm_op_vort2flux(gp.h, C.ptr(G), C.ptr(psi), C.ptr(qa)))
+        (b,) = _op(gp, "ibpm_op_regT", qa[:, 0].copy(), out_sizes=[nf])
+        r = Binv @ b_o
+        r[j] -= 1.0
+        worst_b = max(worst_b, relerr(b, b_o))
+        worst_r = max(worst_r, float(np.linalg.norm(r)))
+    print(f"B columns: worst relative difference {worst_b:.2e}; |Binv B e_j - e_j| worst {worst_r:.2e}")
+    assert worst_b < 1e-12
+    assert worst_r < 1e-7
+
+
 # ---------------------------------------------------------------------------
 # full-size steps against the oracle: the 1024 x 1024 and 800 x 400 grids,
 # 5 levels, and the 512 x 256, 4-level moving grid (the 1023-point transform

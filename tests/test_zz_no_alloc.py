@@ -19,7 +19,17 @@ class _Hip:
     """The few HIP runtime calls the test needs for device buffers."""
 
     def __init__(self):
-        self.h = ctypes.CDLL("libamdhip64.so")
+        # the HIP runtime libibpm_hip.so itself uses (another copy, such as
+        # one bundled with a Python package, would be a separate runtime)
+        C.lib()
+        path = "libamdhip64.so"
+        with open("/proc/self/maps") as fh:
+            for line in fh:
+                f = line.split()
+                if f and "libamdhip64" in f[-1] and f[-1].startswith("/opt/rocm"):
+                    path = f[-1]
+                    break
+        self.h = ctypes.CDLL(path)
         self.h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         self.h.hipFree.argtypes = [ctypes.c_void_p]
         self.h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
