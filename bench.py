@@ -282,6 +282,15 @@ def main():
         ibpm.vort2flux_(state, prob)
         tuned = {TUNED_NAMES[k]: v[0] for k, v in autotune(prob).items()}
         kb, _ = time_kernels(prob, tuned, args.kernel_reps)
+        # the mixed-precision sine-matrix transforms (MFMA), both arithmetics
+        for prec in ("bf16x3", "fp32"):
+            prob.set_precision(prec)
+            for which, name in ((9, "x-transform (all levels)"), (10, "y-transform + eigenvalue scaling (all levels)"),
+                                (11, "x-transform (level 1)")):
+                v = ctypes.c_double(0.0)
+                prob.ctx.check(C.lib().ibpm_time_kernel(prob.h, which, args.kernel_reps, ctypes.byref(v)))
+                kb[f"k_mx_gemm {prec} {name}"] = v.value
+        prob.set_precision("fp64")
         print(json.dumps({"kernels_only": True, "tiles": tuned, "kernel_ms": {k: round(v, 5) for k, v in kb.items()}}),
               flush=True)
         return
