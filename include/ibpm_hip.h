@@ -228,8 +228,7 @@ int ibpm_group_wait(ibpm_group* group, double* cd_hist, double* cl_hist, int32_t
  * A step's scalars are its 7 doubles {Ux, Uy, inclination, U, V, θ, θ̇} in the
  * order of ibpm_step_scalars; ds / sbar hold nsteps rows of them.  No
  * counterpart in the reference: these are the pieces a ChainRulesCore rrule
- * of the run would call (INTEGRATION.md gives it).  Both leave the problem's
- * state as they found it.
+ * of the run would call.  Both leave the problem's state as they found it.
  *
  * ibpm_tangent_n: J and dJ = the directional derivative for perturbations ds
  * of the scalars and dGamma0 (nG x mg, may be NULL) of the initial Γ,
