@@ -224,9 +224,12 @@ static void o_dst2_variant1(double* y, const double* x, int n1, int n2, const od
  * F_k = R_k - i I_k, then Y(2k) = 2 I_k and the odd outputs by the recurrence
  * Y(2k+1) = Y(2k-1) + 2 R_k, Y(1) = R_0.  This is the arithmetic structure of
  * the HIP library's transform (restated here in plain loops); its recurrence
- * sums the FFT's rounding errors along k. */
+ * sums the FFT's rounding errors along k.  Variant 3: the same with the
+ * recurrence summed with Neumaier's compensation (the running sum's rounding
+ * error carried in a second term). */
 static void o_dst_halflen_line(int n, const double* x, long sx, double* out, long so)
 {
+    const int comp = o_dst_variant == 3;
     const int M = n + 1, P = M / 2;
     static __thread offt_plan pl;
     static __thread int plP = 0;
@@ -254,7 +257,7 @@ static void o_dst_halflen_line(int n, const double* x, long sx, double* out, lon
     }
     for (int m = 0; m < P; m++) { zin[m].re = yy[2 * m]; zin[m].im = yy[2 * m + 1]; }
     offt_rec(zout, zin, P, 1, &pl, 0, 1);
-    double s = 0.0;
+    double s = 0.0, c = 0.0;
     for (int k = 0; k < P; k++) {
         ocx a = zout[k], b = zout[(P - k) % P];
         /* E = (a + conj b)/2, O = -i (a - conj b)/2, F = E + W^k O */
@@ -267,9 +270,11 @@ static void o_dst_halflen_line(int n, const double* x, long sx, double* out, lon
         if (k == 0) s = Rk;
         else {
             out[(long)(2 * k - 1) * so] = 2.0 * Ik;
-            s = s + 2.0 * Rk;
+            const double y2 = 2.0 * Rk, t = s + y2;
+            if (comp) c += fabs(s) >= fabs(y2) ? (s - t) + y2 : (y2 - t) + s;
+            s = t;
         }
-        out[(long)(2 * k) * so] = s;
+        out[(long)(2 * k) * so] = s + c;
     }
 }
 
@@ -289,7 +294,7 @@ static void o_dst2(double* y, const double* x, int n1, int n2, const odst_plan* 
         o_dst2_variant1(y, x, n1, n2, p1, p2);
         return;
     }
-    if (o_dst_variant == 2 && (n1 + 1) % 2 == 0 && (n2 + 1) % 2 == 0) {
+    if ((o_dst_variant == 2 || o_dst_variant == 3) && (n1 + 1) % 2 == 0 && (n2 + 1) % 2 == 0) {
         o_dst2_variant2(y, x, n1, n2);
         return;
     }

@@ -86,7 +86,9 @@ def set_threads(n: int):
 
 def set_dst_variant(v: int):
     """0: the oracle's own 2D DST-I; 1: the same transform in another
-    operation order (one line per FFT, dimension 2 first)."""
+    operation order (one line per FFT, dimension 2 first); 2: the HIP
+    library's half-length algorithm; 3: that algorithm with a compensated
+    sum of its odd outputs."""
     lib().oracle_set_dst_variant(int(v))
 
 

@@ -2,7 +2,8 @@
 differently.  The oracle with its own transform (variant 0) is compared with
   variant 1: the same RODFT00 computed one line per FFT, dimension 2 first;
   variant 2: the half-length algorithm of the HIP library (a P = M/2 complex
-             FFT and a recurrence over k for the odd outputs).
+             FFT and a recurrence over k for the odd outputs);
+  variant 3: variant 2 with the recurrence summed with compensation.
 All runs share one Binv and one initial state, so the spread comes from the
 transform's rounding alone.
 
@@ -54,7 +55,7 @@ def main(threads=8, checkpoints=(10, 40)):
     P0 = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], c["dt"], oracle_bodies(c))
     binv = P0.binv()
     print(f"Binv assembled in {time.time() - t0:.0f}s", flush=True)
-    variants = (0, 1, 2)
+    variants = (0, 1, 2, 3)
     probs = [P0] + [O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], c["dt"], oracle_bodies(c), binv=binv)
                     for _ in variants[1:]]
     G = gaussian_vortex_gamma(g)

@@ -14,10 +14,11 @@ is held to the oracle over the whole history:
 
 Tolerances: CD to 1e-10 relative (every step), CL to 1e-10 of max |CD|, the
 Γ and ψ norms of every level to 1e-10 relative.  Measured on MI355X: CD and
-CL within 3e-14 over the 1000 and 5000 steps, the norms within 3e-13
-(200 x 200) and 5.5e-12 (800 x 400 after 5000 steps): two FP64 solvers
-whose transforms round differently drift apart slowly, and a wrong operator
-moves CD by far more than 1e-10 within a few steps.
+CL within 4e-14 on all three runs, the norms within 3e-13 (200 x 200),
+5.5e-12 (800 x 400 after 5000 steps) and 2.8e-11 (ψ at 1024 x 1024 after
+2000 steps): two FP64 solvers whose transforms round differently drift apart
+slowly, and a wrong operator moves CD by far more than 1e-10 within a few
+steps.
 """
 import os
 
