@@ -119,7 +119,7 @@ struct ibpm_prob {
     // the base state) and base circulation ΓB on the device; q0 stays zero
     bool lin = false;
     double *QB = nullptr, *GB = nullptr;
-    int ytile = 8;         // y-pass columns per workgroup, all-level launches
+    int ytile = 4;         // y-pass columns per workgroup, all-level launches (and batches of fields)
     int ytile_single = 4;  // same, one-level launches (vort2flux chain)
     int xwaves = 4;        // lines per work-group of the plain x-passes over all levels (or a batch)
 
@@ -1583,9 +1583,9 @@ static int build_binv(ibpm_prob* P)
     const int nf = P->nf;
     const long nG = g.nG, nq = g.nq, L = P->mg;
     // All 2 nb unit forces through B_times! in batches (assemble_b): as many
-    // fields per batch as fit in 2 GiB of scratch, allocated for the setup only
+    // fields per batch as fit in 8 GiB of scratch, allocated for the setup only
     const size_t per = sizeof(double) * ((size_t)nq + 2 * (size_t)nG + 2 * (size_t)L * nG + 2 * (size_t)g.n2);
-    const int K = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nf, ((size_t)2 << 30) / per, (size_t)32768}));
+    const int K = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nf, ((size_t)8 << 30) / per, (size_t)32768}));
     double *bq = nullptr, *bgt = nullptr, *bt1 = nullptr, *bG = nullptr, *bpsi = nullptr, *bedge = nullptr,
            *Bd = nullptr;
     auto release = [&]() {
