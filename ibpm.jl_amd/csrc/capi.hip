@@ -214,6 +214,10 @@ struct AdjWork {
     double *Gb = nullptr, *Gbo = nullptr, *Nb1 = nullptr, *Nb0 = nullptr, *Rb = nullptr, *psib = nullptr;
     double *qb = nullptr, *q0b = nullptr, *qs1b = nullptr, *part = nullptr;
     double* sums = nullptr;  // per step: TLM Σ δF̃b_x, Σ δF̃b_y; adjoint Ūx0, Ūy0, Ω̄
+    // the state's q0, F̃b, CD and CL at the call, put back at its end
+    double* aux = nullptr;
+    bool q0_uniform = true;
+    double q0vx = 0.0, q0vy = 0.0;
     std::vector<double*> snap;
     std::vector<void*> allocs;
 };
@@ -1230,6 +1234,7 @@ static int adj_setup(ibpm_prob* P, int nsteps, int nsnap)
         WA(W->Gb, nG * L); WA(W->Gbo, nG * L); WA(W->Nb1, nG * L); WA(W->Nb0, nG * L); WA(W->Rb, nG * L);
         WA(W->psib, nG * L); WA(W->qb, nq * L); WA(W->q0b, nq * L); WA(W->qs1b, nq);
         WA(W->part, 3L * adj_q0_reduce_parts(g, P->mg));
+        WA(W->aux, nq * L + P->nf + 2L * P->nbodies);
 #undef WA
         // Binv' as row-major rows: the column-major Binv itself
         HIPTRY(P->ctx, hipMemcpy(W->BinvC, P->binv.data(), sizeof(double) * P->binv.size(), hipMemcpyHostToDevice));
@@ -1498,6 +1503,28 @@ static void adj_step(ibpm_prob* P, const ibpm_step_scalars& sc, double wcd, doub
 // checkpoints of the working state: (Γ, nonlin[2]) copied out and back; a
 // restore recomputes ψ and q by vort2flux! (coarsify! of a coarsified Γ
 // rewrites the same values), which is how the forward step left them
+// save (or put back) the parts of the state a run overwrites besides Γ,
+// nonlin[2], ψ and q: the base flux q0 and the forces F̃b, CD, CL
+static void aux_state(ibpm_prob* P, bool save)
+{
+    AdjWork* W = P->adj;
+    double* a = W->aux;
+    auto cp = [&](double* x, long n) {
+        if (save) launch_copy(a, x, n, P->stream);
+        else launch_copy(x, a, n, P->stream);
+        a += n;
+    };
+    cp(P->q0, P->g.nq * P->mg);
+    cp(P->Fb, P->nf);
+    cp(P->CD, P->nbodies);
+    cp(P->CL, P->nbodies);
+    if (save) {
+        W->q0_uniform = P->q0_uniform; W->q0vx = P->q0vx; W->q0vy = P->q0vy;
+    } else {
+        P->q0_uniform = W->q0_uniform; P->q0vx = W->q0vx; P->q0vy = W->q0vy;
+    }
+}
+
 static void snap_store(ibpm_prob* P, int slot)
 {
     const long n = P->g.nG * P->mg;
@@ -2184,6 +2211,7 @@ int ibpm_tangent_n(ibpm_prob* P, int nsteps, const ibpm_step_scalars* s, const d
     AdjWork* W = P->adj;
     const long n = P->g.nG * P->mg;
     snap_store(P, 0);
+    aux_state(P, true);
     if (dGamma0) HIPTRY(P->ctx, hipMemcpyAsync(W->dG, dGamma0, sizeof(double) * n, hipMemcpyHostToDevice, P->stream));
     else launch_fill(W->dG, n, 0.0, P->stream);
     launch_fill(W->dN[0], n, 0.0, P->stream);
@@ -2202,6 +2230,7 @@ int ibpm_tangent_n(ibpm_prob* P, int nsteps, const ibpm_step_scalars* s, const d
     for (int k = 0; k < nsteps; k++) d += cD * (wcd[k] * sums[2 * k] + wcl[k] * sums[2 * k + 1]);
     *dJ = d;
     snap_restore(P, 0);
+    aux_state(P, false);
     HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
     return IBPM_OK;
 }
@@ -2216,6 +2245,7 @@ int ibpm_adjoint_n(ibpm_prob* P, int nsteps, const ibpm_step_scalars* s, const d
     AdjWork* W = P->adj;
     const long n = P->g.nG * P->mg;
     snap_store(P, 0);
+    aux_state(P, true);
     for (int k = 0; k < nsteps; k++) step_body(P, s[k], P->hist, P->hist + P->hist_steps);
     HIPTRY(P->ctx, hipGetLastError());
     if (int rc = hist_objective(P, nsteps, wcd, wcl, J)) return rc;
@@ -2232,6 +2262,7 @@ int ibpm_adjoint_n(ibpm_prob* P, int nsteps, const ibpm_step_scalars* s, const d
     for (int k = 0; k < nsteps; k++) mg_flux_adjoint(s[k], sums.data() + 3 * (size_t)k, sbar + 7 * (size_t)k);
     if (forward_steps) *forward_steps = R.fwd_steps;
     snap_restore(P, 0);
+    aux_state(P, false);
     HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
     return IBPM_OK;
 }

@@ -145,6 +145,27 @@ def test_snapshot_count_does_not_change_the_gradient():
     assert counts[0] > counts[1] > counts[2] == 2 * n - 1
 
 
+def test_runs_leave_the_state_as_they_found_it():
+    """ibpm_tangent_n and ibpm_adjoint_n run steps on the problem's own state
+    and put it back: Γ, ψ, q, q0, nonlin, F̃b, CD and CL read after each call
+    equal those before it, bit for bit."""
+    c, g, gp, st = moving_problem()
+    rows = pitch_rows(np.arange(6) * c["dt"], 15 * math.pi / 180, 0.2, 0.0)
+    ibpm.advance_n_(st, gp, np.arange(3) * c["dt"])  # a state with forces and AB2 history
+    names = ("Γ", "ψ", "q", "q0", "F̃b", "CD", "CL")
+    st.invalidate()
+    before = {k: np.array(getattr(st, k), copy=True) for k in names}
+    nl_before = [np.array(a, copy=True) for a in st.nonlin]
+    wcd, wcl = np.full(6, 1.0), np.zeros(6)
+    tangent(gp, rows, np.zeros((6, 7)), None, wcd, wcl)
+    adjoint(gp, rows, wcd, wcl, 2)
+    st.invalidate()
+    for k in names:
+        assert np.array_equal(np.asarray(getattr(st, k)), before[k]), k
+    for a, b in zip(st.nonlin, nl_before):
+        assert np.array_equal(a, b)
+
+
 def plate_gradient(nsteps, nsnap=32):
     """dJ/d(A, f, beta) by the adjoint and by the tangent on the pitching plate
     (512 x 256 x 4) of tests/golden/make_adjoint_fd.py."""
