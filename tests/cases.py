@@ -56,6 +56,14 @@ def case(name: str):
         grid = ibpm.make_grid(1.0 / 32, (-4.0, 4.0, -3.125, 3.125), mg=2)
         bodies = [ibpm.make_cylinder(0.5, grid.h, 0.3, -0.2, n=100)]
         return dict(grid=grid, bodies=bodies, Re=150.0, dt=0.005, freestream=U1)
+    if name == "direct":  # 56 x 44, 2 levels: P = 28 and 22 have factors 7 and 11, so both directions take the direct O(n^2) DST
+        grid = ibpm.make_grid(0.125, (-2.0, 5.0, -2.75, 2.75), mg=2)
+        bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, n=26)]
+        return dict(grid=grid, bodies=bodies, Re=50.0, dt=0.02, freestream=U1)
+    if name == "single":  # 64 x 48, one level: no boundary terms anywhere (P = 32 and 24 = 8 x 3)
+        grid = ibpm.make_grid(1.0 / 16, (-1.5, 2.5, -1.5, 1.5), mg=1)
+        bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, n=40)]
+        return dict(grid=grid, bodies=bodies, Re=100.0, dt=0.01, freestream=U1)
     if name == "moving":  # MovingGrid pitching plate about its mid-chord at the origin
         grid = ibpm.make_grid(1.0 / 32, (-1.0, 3.0, -1.0, 1.0), mg=3)
         A, f = 15 * math.pi / 180, 0.2
