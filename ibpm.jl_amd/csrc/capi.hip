@@ -187,6 +187,7 @@ struct ibpm_prob {
     int mx_sweeps = 2;  // refinement sweeps of every mixed-precision solve (IBPM_OPT_MX_SWEEPS)
     int P1 = 0, P2 = 0;
     float *mx_sf1 = nullptr, *mx_sf2 = nullptr, *mx_buf[2] = {nullptr, nullptr};
+    float* mx_part = nullptr;  // split-K partial products (MX_SPLIT_MAX x mg fields)
     __bf16 *mx_sh1 = nullptr, *mx_sl1 = nullptr, *mx_sh2 = nullptr, *mx_sl2 = nullptr;
     double* mx_lscale = nullptr;
     // tangent-linear / adjoint work space (allocated by their first call)
@@ -674,6 +675,7 @@ static int mx_setup(ibpm_prob* P)
     HIPTRY(P->ctx, dalloc(P, (uint16_t**)&P->mx_sl2, a2));
     HIPTRY(P->ctx, dalloc(P, &P->mx_buf[0], fl));
     HIPTRY(P->ctx, dalloc(P, &P->mx_buf[1], fl));
+    HIPTRY(P->ctx, dalloc(P, &P->mx_part, (size_t)MX_SPLIT_MAX * fl));
     HIPTRY(P->ctx, dalloc(P, &P->mx_lscale, (size_t)P->mg));
     std::vector<float> f;
     std::vector<uint16_t> hi, lo;
@@ -731,6 +733,8 @@ static void mx_apply(ibpm_prob* P, int op, int lev0, int nb, int epi, double* ds
     a.scale = 1.0 / (double)(4L * P->g.nx * P->g.ny);
     a.lscale = P->mx_lscale;
     a.g = P->g;
+    a.part = P->mx_part;
+    a.partcap = (long)MX_SPLIT_MAX * P->P1 * P->P2 * P->mg;
     MxGemmArgs x = a, y = a;
     x.sf = P->mx_sf1; x.sh = P->mx_sh1; x.sl = P->mx_sl1;
     y.sf = P->mx_sf2; y.sh = P->mx_sh2; y.sl = P->mx_sl2;
@@ -3303,6 +3307,7 @@ int ibpm_time_kernel(ibpm_prob* P, int which, int reps, double* ms_avg)
             a.op = MX_OP_AINV; a.cx = P->cx; a.cy = P->cy; a.dt = P->dt; a.Re = P->Re;
             a.scale = 1.0 / (double)(4L * P->g.nx * P->g.ny);
             a.lscale = P->mx_lscale; a.g = P->g;
+            a.part = P->mx_part; a.partcap = (long)MX_SPLIT_MAX * P->P1 * P->P2 * P->mg;
             a.in = P->mx_buf[0]; a.out = P->mx_buf[1];
             if (which == 10) { a.sf = P->mx_sf2; a.sh = P->mx_sh2; a.sl = P->mx_sl2; }
             else { a.sf = P->mx_sf1; a.sh = P->mx_sh1; a.sl = P->mx_sl1; }

@@ -67,7 +67,15 @@ struct MxGemmArgs {
     double dt, Re, scale;
     const double* lscale;
     GridDev g;
+    // split-K scratch: MX_SPLIT_MAX x (entries) x P1 P2 FP32 partial products
+    // (launches with few output tiles split K over MX_SPLIT_MAX or fewer
+    // work-groups per tile and add the partials in a fixed order); null:
+    // never split.  splitk is set by the launcher.
+    float* part;
+    long partcap;  // floats in part
+    int splitk;
 };
+constexpr int MX_SPLIT_MAX = 4;
 // prec: MxPrec; dir 0: along x (i), 1: along y (j); epi: MxEpi
 void launch_mx_gemm(int prec, int dir, int epi, const MxGemmArgs& a, int nbatch, hipStream_t s);
 

@@ -185,6 +185,55 @@ __device__ __forceinline__ void put8(MxLds<PREC>& L, bool isA, int r, int c8, co
     }
 }
 
+// The epilogue of one 4-row piece (rows m..m+3 of column n, entry b at level
+// lev) of a product: padded FP32 store, with the FP64 eigenvalue scaling, or
+// the FP64 update of the unpadded field.
+template <int EPI>
+__device__ __forceinline__ void mx_epilogue(const MxGemmArgs& a, int b, int lev, int m, int n, float v0, float v1,
+                                            float v2, float v3)
+{
+    const GridDev& g = a.g;
+    const float v[4] = {v0, v1, v2, v3};
+    if constexpr (EPI == MX_EPI_STORE || EPI == MX_EPI_SCALE) {
+        float o[4] = {v[0], v[1], v[2], v[3]};
+        if constexpr (EPI == MX_EPI_SCALE) {
+            // FP64 eigenvalue scaling: Δinv scale/Λ, Ainv scale/(1 + Λ dt/(2 Re hc^2))
+            const double hc = level_hc(g.h, lev);
+            const double qc = a.dt / (2.0 * a.Re * hc * hc);
+            const double cyv = n < g.n2 ? a.cy[n] : 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = m + r;
+                double f = 0.0;
+                if (i < g.n1 && n < g.n2) {
+                    const double lam = (4.0 - 2.0 * a.cx[i]) - 2.0 * cyv;
+                    const double d = a.op == MX_OP_DINV ? lam : 1.0 + lam * qc;
+                    f = a.scale / d;
+                }
+                o[r] = (float)((double)v[r] * f);
+            }
+        }
+        float* op = a.out + (long)b * a.ostride + m + (long)a.P1 * n;
+        *reinterpret_cast<float4*>(op) = make_float4(o[0], o[1], o[2], o[3]);
+    } else {  // FP64 update of the unpadded field: dst = base +/- acc / lscale
+        if (n < g.n2) {
+            const double us = 1.0 / a.lscale[lev];
+            double* dst = a.dst + (long)b * a.fstride;
+            const double* base = a.base ? a.base + (long)b * a.fstride : nullptr;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = m + r;
+                if (i < g.n1) {
+                    const long p = i + (long)g.n1 * n;
+                    const double d = (double)v[r] * us;
+                    const double b0 = base ? base[p] : 0.0;
+                    dst[p] = EPI == MX_EPI_ADD ? b0 + d : b0 - d;
+                }
+            }
+        }
+    }
+}
+
 template <int PREC, int DIR, int EPI>
 __global__ void __launch_bounds__(256) k_mx_gemm(MxGemmArgs a)
 {
@@ -194,10 +243,12 @@ __global__ void __launch_bounds__(256) k_mx_gemm(MxGemmArgs a)
     const int mt = a.P1 / MT, nt = a.P2 / NT;
     const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
     if (lin >= mt * nt) return;
-    const int b = blockIdx.y;
+    const int S = a.splitk;              // 1: the whole K; else this work-group's share ks of it
+    const int b = blockIdx.y / S, ks = blockIdx.y % S;
     const int m0 = (lin % mt) * MT, n0 = (lin / mt) * NT;
     const float* in = a.in + (long)b * a.istride;
     const int K = DIR == 0 ? a.P1 : a.P2;
+    const int kb = ks * (K / S), ke = kb + K / S;  // K / S is a multiple of KC
 
     f32x4 acc[2][2];
 #pragma unroll
@@ -211,7 +262,7 @@ __global__ void __launch_bounds__(256) k_mx_gemm(MxGemmArgs a)
     const int sr = tid >> 2, sc = (tid & 3) * 8;
     const int tk = tid >> 3, tm = (tid & 7) * 8;
 
-    for (int k0 = 0; k0 < K; k0 += KC) {
+    for (int k0 = kb; k0 < ke; k0 += KC) {
         float va[8], vb[8];
         // ---- global loads of the chunk (all in flight before any LDS store)
         // sine operand rows (k-contiguous): A rows m (DIR 0) or B rows n (DIR 1)
@@ -323,7 +374,6 @@ __global__ void __launch_bounds__(256) k_mx_gemm(MxGemmArgs a)
 
     // ---- epilogue: lane holds rows m = mb + 4 (lane / 16) + r of column n = nb + lane % 16
     const int lev = a.lev0 + (a.fixlev ? 0 : b);
-    const GridDev& g = a.g;
 #pragma unroll
     for (int x = 0; x < 2; x++)
 #pragma unroll
@@ -331,53 +381,64 @@ __global__ void __launch_bounds__(256) k_mx_gemm(MxGemmArgs a)
             const int m = m0 + wm * 32 + x * 16 + 4 * (lane >> 4);
             const int n = n0 + wn * 32 + y * 16 + (lane & 15);
             const f32x4 v = acc[x][y];
-            if constexpr (EPI == MX_EPI_STORE || EPI == MX_EPI_SCALE) {
-                float o[4] = {v[0], v[1], v[2], v[3]};
-                if constexpr (EPI == MX_EPI_SCALE) {
-                    // FP64 eigenvalue scaling: Δinv scale/Λ, Ainv scale/(1 + Λ dt/(2 Re hc^2))
-                    const double hc = level_hc(g.h, lev);
-                    const double qc = a.dt / (2.0 * a.Re * hc * hc);
-                    const double cyv = n < g.n2 ? a.cy[n] : 0.0;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int i = m + r;
-                        double f = 0.0;
-                        if (i < g.n1 && n < g.n2) {
-                            const double lam = (4.0 - 2.0 * a.cx[i]) - 2.0 * cyv;
-                            const double d = a.op == MX_OP_DINV ? lam : 1.0 + lam * qc;
-                            f = a.scale / d;
-                        }
-                        o[r] = (float)((double)v[r] * f);
-                    }
-                }
-                float* op = a.out + (long)b * a.ostride + m + (long)a.P1 * n;
-                *reinterpret_cast<float4*>(op) = make_float4(o[0], o[1], o[2], o[3]);
-            } else {  // FP64 update of the unpadded field: dst = base +/- acc / lscale
-                if (n < g.n2) {
-                    const double us = 1.0 / a.lscale[lev];
-                    double* dst = a.dst + (long)b * a.fstride;
-                    const double* base = a.base ? a.base + (long)b * a.fstride : nullptr;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int i = m + r;
-                        if (i < g.n1) {
-                            const long p = i + (long)g.n1 * n;
-                            const double d = (double)v[r] * us;
-                            const double b0 = base ? base[p] : 0.0;
-                            dst[p] = EPI == MX_EPI_ADD ? b0 + d : b0 - d;
-                        }
-                    }
-                }
+            if (S > 1) {  // partial product: the reduction applies the epilogue
+                float* pp = a.part + ((long)ks * (gridDim.y / S) + b) * a.ostride + m + (long)a.P1 * n;
+                *reinterpret_cast<float4*>(pp) = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
+                mx_epilogue<EPI>(a, b, lev, m, n, v[0], v[1], v[2], v[3]);
             }
         }
 }
 
-template <int PREC, int DIR, int EPI>
-static void launch_g(const MxGemmArgs& a, int nbatch, hipStream_t s)
+// Split-K reduction: the S partial products of every output entry added in
+// the order of their K ranges, then the epilogue.  One thread per 4 rows m of
+// a column n of entry b.
+template <int EPI>
+__global__ void __launch_bounds__(256) k_mx_reduce(MxGemmArgs a, int nbatch)
 {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.y;
+    const long nq = (long)(a.P1 / 4) * a.P2;
+    if (e >= nq) return;
+    const int m = 4 * (int)(e % (a.P1 / 4)), n = (int)(e / (a.P1 / 4));
+    const long off = m + (long)a.P1 * n;
+    float4 v = *reinterpret_cast<const float4*>(a.part + (long)b * a.ostride + off);
+    for (int s = 1; s < a.splitk; s++) {
+        const float4 u = *reinterpret_cast<const float4*>(a.part + ((long)s * nbatch + b) * a.ostride + off);
+        v.x += u.x; v.y += u.y; v.z += u.z; v.w += u.w;
+    }
+    const int lev = a.lev0 + (a.fixlev ? 0 : b);
+    mx_epilogue<EPI>(a, b, lev, m, n, v.x, v.y, v.z, v.w);
+}
+
+// Launches with fewer than 4 work-groups per CU (one level of a large grid,
+// or small grids) split K over S <= MX_SPLIT_MAX work-groups per output tile
+// when K / S stays a multiple of the chunk KC.
+constexpr int MX_CUS = 256;
+
+template <int PREC, int DIR, int EPI>
+static void launch_g(const MxGemmArgs& a0, int nbatch, hipStream_t s)
+{
+    MxGemmArgs a = a0;
     const int tiles = (a.P1 / MT) * (a.P2 / NT);
-    dim3 grid((unsigned)(NXCD * xcd_nper(tiles)), (unsigned)nbatch);
+    const int K = DIR == 0 ? a.P1 : a.P2;
+    int S = 1;
+    if (a.part && tiles * nbatch < 4 * MX_CUS) {
+        for (int t = MX_SPLIT_MAX; t > 1; t--) {
+            if ((K / KC) % t == 0 && (long)t * nbatch * a.ostride <= a.partcap) {
+                S = t;
+                break;
+            }
+        }
+    }
+    a.splitk = S;
+    dim3 grid((unsigned)(NXCD * xcd_nper(tiles)), (unsigned)(nbatch * S));
     hipLaunchKernelGGL((k_mx_gemm<PREC, DIR, EPI>), grid, dim3(256), 0, s, a);
+    if (S > 1) {
+        const long nq = (long)(a.P1 / 4) * a.P2;
+        hipLaunchKernelGGL((k_mx_reduce<EPI>), dim3((unsigned)((nq + 255) / 256), (unsigned)nbatch), dim3(256), 0, s,
+                           a, nbatch);
+    }
 }
 
 template <int PREC, int DIR>
