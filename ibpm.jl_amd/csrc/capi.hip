@@ -166,6 +166,14 @@ struct ibpm_prob {
     int* slot = nullptr;      // device: history entry of the next step (advanced by k_forces)
     bool async_err = false;   // a graph launch failed (reported by the next wait)
     ibpm_group* group = nullptr;  // set while the problem's state lives in a group's arrays
+    // q_stale: P->q does not hold the state's q; curl! of P->psi (all levels)
+    // gives it.  The FP64 step leaves q so, since the only reader of q inside a
+    // step, the next step's nonlinear!, forms the fluxes from psi itself;
+    // every other reader forms q first (ensure_q).
+    bool q_stale = false;
+    // 0-based lines of the level-1 ψ that E of the trial flux reads (curl! of
+    // its window fluxes), or -1 when a window reaches a flux on the boundary
+    int tlo = -1, thi = -1;
     // HIP graphs of one step (static bodies): replayed instead of ~54 launches.
     // A step's launches depend on the buffer roles only (Γ/Γs and the nonlin
     // pair alternate), so a graph is kept per role; anything else a captured
@@ -174,6 +182,7 @@ struct ibpm_prob {
     struct StepGraph {
         double* G;
         int out, nl2;
+        bool qs;  // q_stale at the start of the step: its nonlinear! reads psi
         hipGraphExec_t exec;
     };
     std::vector<StepGraph> gcache;
@@ -458,6 +467,14 @@ static bool setup_reg_host(ibpm_prob* P)
     }
     R.off.push_back((int)ents.size());
     P->sc_ncell = (int)R.cell.size();
+    // the windows' fluxes (i, j), i = ib0..ib1-1, j = jb0..jb1-1, read ψ rows
+    // j-1 and j (1-based); all interior: curl! of them needs no boundary value
+    if (ib0 >= 2 && ib1 - 1 <= nx - 1 && jb0 >= 2 && jb1 - 1 <= ny - 1) {
+        P->tlo = jb0 - 2;
+        P->thi = jb1 - 2;
+    } else {
+        P->tlo = P->thi = -1;
+    }
     P->rb_i0 = std::max(ib0, 2); P->rb_i1 = std::min(ib1, nx);
     P->rb_j0 = std::max(jb0, 2); P->rb_j1 = std::min(jb1, ny);
     return true;
@@ -511,7 +528,15 @@ static void timed_ypass(ibpm_prob* P, const YArgs& y, int tile, int kind);
 // vort2flux! (curl.jl:133-165).  Gb is mutated on coarse levels.  When
 // `level1_only`, only q on level 1 is produced (into qout), which is all the
 // trial stage and B_times! read of it.
-static void vort2flux_dev(ibpm_prob* P, double* Gb, double* psib, double* qout, bool level1_only)
+// what vort2flux_dev stores of q = curl!(ψ): every level, level 1 only, or
+// nothing (the caller forms the fluxes it needs from ψ).  CURL_TRIAL: nothing,
+// and of ψ only what the trial's E and the chain itself read: the level-1
+// lines tlo..thi, and of each coarser level the lines ny/4 .. 3ny/4 (1-based)
+// the next finer level's boundary values come from, whose edge columns are
+// kept and whose rows ny/4 and 3ny/4 are stored (every line when tlo < 0).
+enum { CURL_ALL = 0, CURL_LEVEL1 = 1, CURL_NONE = 2, CURL_TRIAL = 3 };
+
+static void vort2flux_dev(ibpm_prob* P, double* Gb, double* psib, double* qout, int curl)
 {
     hipStream_t s = P->stream;
     const long nG = P->g.nG;
@@ -543,17 +568,55 @@ static void vort2flux_dev(ibpm_prob* P, double* Gb, double* psib, double* qout, 
         XArgs c = xargs(P);
         c.src = P->T1 + (long)l * nG; c.dst = psib + (long)l * nG; c.nlev = 1; c.lev0 = l;
         if (l > 0) c.edge = P->edge;
+        if (curl == CURL_TRIAL && P->tlo >= 0) {
+            c.lsel = 1;
+            if (l > 0) {
+                c.llo = c.keep0 = P->g.ny / 4 - 1;
+                c.lhi = c.keep1 = 3 * P->g.ny / 4 - 1;
+                c.keep = 1;
+            } else {
+                c.llo = P->tlo;
+                c.lhi = P->thi;
+            }
+        }
         launch_xpass(X_PLAIN, c, s);
     }
+    if (qout == P->q) P->q_stale = curl != CURL_ALL;
+    if (curl == CURL_NONE || curl == CURL_TRIAL) return;
     CurlArgs ca{};
     ca.psi = psib;
     ca.q = qout;
     ca.qstride = P->g.nq;
-    ca.nlev = level1_only ? 1 : L;
+    ca.nlev = curl == CURL_LEVEL1 ? 1 : L;
     ca.lev0 = 0;
     ca.mg = L;
     ca.g = P->g;
     launch_curl(ca, s);
+}
+
+// the state's q formed from its ψ when the last step left it unformed
+static void ensure_q(ibpm_prob* P)
+{
+    if (!P->q_stale) return;
+    CurlArgs ca{};
+    ca.psi = P->psi;
+    ca.q = P->q;
+    ca.qstride = P->g.nq;
+    ca.nlev = P->mg;
+    ca.lev0 = 0;
+    ca.mg = P->mg;
+    ca.g = P->g;
+    launch_curl(ca, P->stream);
+    P->q_stale = false;
+}
+
+// where a step's nonlinear! reads the state's fluxes: q, or ψ when the last
+// step left q unformed (the linearized product reads q, which is formed first)
+static void nl_flux_source(ibpm_prob* P, NLArgs& na)
+{
+    if (!P->q_stale) return;
+    if (P->lin) ensure_q(P);
+    else na.psi = P->psi;
 }
 
 // Γout[:,1] = Ainv[1] * rot(reg(f)) -- the body-force projection of
@@ -605,6 +668,7 @@ static void collect_profile(ibpm_prob* P)
 }
 
 static void trial_dev(ibpm_prob* P);
+static void trial_flux_from_psi(ibpm_prob* P, GatherArgs& ga);
 
 // ---------------------------------------------------------------------------
 // Mixed precision: the Laplacian inverses of the step as dense sine-matrix
@@ -824,6 +888,7 @@ static void step_mixed(ibpm_prob* P)
     const int out = (P->nl1 != P->nl2) ? P->nl1 : 1 - P->nl2;
     NLArgs na{};
     na.q = P->q; na.q0 = P->q0; na.G = P->G; na.out = P->nl[out];
+    nl_flux_source(P, na);
     na.nlev = L; na.lev0 = 0; na.mg = L; na.g = g;
     set_nl_q0(P, na);
     launch_nonlinear(na, s);
@@ -873,6 +938,7 @@ static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, do
     if (P->nbt > 0) {
         GatherArgs ga{};
         ga.q = P->qs1; ga.q0 = P->q0; ga.w = P->w; ga.bidx = P->bidx; ga.out = P->Ftmp; ga.nbt = P->nbt; ga.g = g;
+        if (!mixed) trial_flux_from_psi(P, ga);
         launch_gather(ga, s);
         ForceArgs fa{};
         fa.Fb = P->Fb; fa.ds = P->ds_d; fa.boff = P->boff_d; fa.fb = P->fbv; fa.CD = P->CD; fa.CL = P->CL;
@@ -887,14 +953,22 @@ static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, do
         else project_forces(P, P->Fb, P->Gs, true);
     }
     std::swap(P->G, P->Gs);
-    // ---- vort2flux!(state.ψ, state.q, state.Γ) (timestepping.jl:64)
-    if (mixed) mx_vort2flux(P, P->G, P->q, false);
-    else vort2flux_dev(P, P->G, P->psi, P->q, false);
+    // ---- vort2flux!(state.ψ, state.q, state.Γ) (timestepping.jl:64); the
+    // FP64 path leaves q to be formed from ψ by its readers
+    if (mixed) {
+        mx_vort2flux(P, P->G, P->q, false);
+        P->q_stale = false;
+    } else {
+        vort2flux_dev(P, P->G, P->psi, P->q, CURL_NONE);
+    }
     (void)nG;
 }
 
 // get_trial_state! (timestepping.jl:86-129), all levels at once: Γs, the
-// trial flux qs[:,1], the AB2 history shift and the trial ψ in state.ψ
+// AB2 history shift and the trial ψ in state.ψ, of which only the lines E and
+// the vort2flux! chain read are stored (the step's own vort2flux! rewrites
+// every line).  The trial flux qs[:,1] is not stored: E forms it from that ψ
+// (trial_flux_from_psi).
 static void trial_dev(ibpm_prob* P)
 {
     hipStream_t s = P->stream;
@@ -903,6 +977,7 @@ static void trial_dev(ibpm_prob* P)
     const int out = (P->nl1 != P->nl2) ? P->nl1 : 1 - P->nl2;
     NLArgs na{};
     na.q = P->q; na.q0 = P->q0; na.G = P->G; na.out = P->nl[out];
+    nl_flux_source(P, na);
     na.nlev = L; na.lev0 = 0; na.mg = L; na.g = g;
     set_nl_q0(P, na);
     launch_nonlinear(na, s);
@@ -920,7 +995,16 @@ static void trial_dev(ibpm_prob* P)
     c.src = P->T2; c.dst = P->Gs; c.nlev = L; c.lev0 = 0;
     launch_xpass(X_PLAIN, c, s);
     P->nl1 = P->nl2 = out;  // state.nonlin[2] .= state.nonlin[1]
-    vort2flux_dev(P, P->Gs, P->psi, P->qs1, true);
+    vort2flux_dev(P, P->Gs, P->psi, P->qs1, CURL_TRIAL);
+}
+
+// E of the trial flux qs[:,1] = curl!(ψ of the trial) read from ψ itself
+// (levels 1 and 2), as trial_dev leaves it
+static void trial_flux_from_psi(ibpm_prob* P, GatherArgs& ga)
+{
+    ga.q = nullptr;
+    ga.psi = P->psi;
+    ga.psic = P->mg > 1 ? P->psi + P->g.nG : nullptr;
 }
 
 // move_body!(::Type{MotionFunction}, body, t) for every body (move-body-utils.jl:
@@ -1076,11 +1160,14 @@ static int step_mf(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, doubl
     if (!sc.body_kin) return fail(C, IBPM_E_ARG, "MotionFunction steps need body_kin (6 values per body)");
     int rc = update_bodies_dev(P, sc.body_kin);
     if (rc != IBPM_OK) return rc;
+    // q formed first, so that a step that fails below leaves the state's q
+    ensure_q(P);
     trial_dev(P);
     if (nf > 0) {
         // b = E(qs + q0) - ub h, and the initial guess: the state's F̃b
         GatherArgs ga{};
         ga.q = P->qs1; ga.q0 = P->q0; ga.w = P->w; ga.bidx = P->bidx; ga.out = P->Fw; ga.nbt = P->nbt; ga.g = g;
+        trial_flux_from_psi(P, ga);
         launch_gather(ga, s);
         HIPTRY(C, hipMemcpyAsync(P->pin + nf, P->Fw, sizeof(double) * nf, hipMemcpyDeviceToHost, s));
         HIPTRY(C, hipMemcpyAsync(P->pin, P->Fb, sizeof(double) * nf, hipMemcpyDeviceToHost, s));
@@ -1127,7 +1214,7 @@ static int step_mf(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, doubl
         project_forces(P, P->Fb, P->Gs, true);
     }
     std::swap(P->G, P->Gs);
-    vort2flux_dev(P, P->G, P->psi, P->q, false);
+    vort2flux_dev(P, P->G, P->psi, P->q, CURL_NONE);
     HIPTRY(C, hipGetLastError());
     // the pinned staging is reused by the next step: drain before returning
     HIPTRY(C, hipStreamSynchronize(s));
@@ -1153,18 +1240,20 @@ static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, dou
     }
     const int out = (P->nl1 != P->nl2) ? P->nl1 : 1 - P->nl2;
     for (auto& e : P->gcache) {
-        if (e.G == P->G && e.out == out && e.nl2 == P->nl2) {
+        if (e.G == P->G && e.out == out && e.nl2 == P->nl2 && e.qs == P->q_stale) {
             if (hipGraphLaunch(e.exec, P->stream) != hipSuccess) P->async_err = true;
-            // the host side of step_body: nonlin roles, then Γ <-> Γs
+            // the host side of step_body: nonlin roles, Γ <-> Γs, q left to ψ (FP64)
             P->nl1 = P->nl2 = out;
             std::swap(P->G, P->Gs);
+            P->q_stale = P->prec < 0;
             return;
         }
     }
-    ibpm_prob::StepGraph e{P->G, out, P->nl2, nullptr};
+    ibpm_prob::StepGraph e{P->G, out, P->nl2, P->q_stale, nullptr};
     double* G0 = P->G;
     double* Gs0 = P->Gs;
     const int nl1 = P->nl1, nl2 = P->nl2;
+    const bool qs0 = P->q_stale;
     hipGraph_t graph = nullptr;
     bool ok = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
@@ -1182,7 +1271,7 @@ static void step_dev(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, dou
     if (e.exec) (void)hipGraphExecDestroy(e.exec);
     (void)hipGetLastError();
     P->graphs = false;
-    P->G = G0; P->Gs = Gs0; P->nl1 = nl1; P->nl2 = nl2;
+    P->G = G0; P->Gs = Gs0; P->nl1 = nl1; P->nl2 = nl2; P->q_stale = qs0;
     step_body(P, sc, cdh, clh);
 }
 
@@ -1407,11 +1496,12 @@ static void tlm_step(ibpm_prob* P, const ibpm_step_scalars& sc, const double* ds
     const int L = P->mg;
     AdjWork* W = P->adj;
     hipStream_t s = P->stream;
+    ensure_q(P);  // the working state's q (a forward step leaves it to ψ)
     set_q0(P, P->q0, mg_flux(sc));
     set_q0(P, W->dq0, mg_flux_tangent(sc, ds));
     // δq = V_q(δΓ) on a copy (vort2flux! coarsifies its input)
     launch_copy(W->tG, W->dG, nG * L, s);
-    vort2flux_dev(P, W->tG, W->dpsi, W->dq, false);
+    vort2flux_dev(P, W->tG, W->dpsi, W->dq, CURL_ALL);
     double* dN1 = W->dN[1 - n0];
     nonlinear_tangent(P, W->dG, W->dq, W->dq0, dN1);
     // δΓs = Ainv(A δΓ + β1 dt δN1 + β2 dt δN0 + rhsbc(δΓ)): the trial passes
@@ -1424,7 +1514,7 @@ static void tlm_step(ibpm_prob* P, const ibpm_step_scalars& sc, const double* ds
     XArgs c = xargs(P);
     c.src = P->T2; c.dst = W->dGs; c.nlev = L; c.lev0 = 0;
     launch_xpass(X_PLAIN, c, s);
-    vort2flux_dev(P, W->dGs, W->dpsi, W->dqs1, true);
+    vort2flux_dev(P, W->dGs, W->dpsi, W->dqs1, CURL_LEVEL1);
     // δF̃b = Binv E(δqs1 + δq0)
     GatherArgs ga{};
     ga.q = W->dqs1; ga.q0 = W->dq0; ga.w = P->w; ga.bidx = P->bidx; ga.out = W->F1; ga.nbt = P->nbt; ga.g = g;
@@ -1446,6 +1536,7 @@ static void adj_step(ibpm_prob* P, const ibpm_step_scalars& sc, double wcd, doub
     const int L = P->mg;
     AdjWork* W = P->adj;
     hipStream_t s = P->stream;
+    ensure_q(P);  // the working state's q (a forward step leaves it to ψ)
     set_q0(P, P->q0, mg_flux(sc));
     // Γ' = C(Γs) - Ainv rot E' F (level 1), then the state's coarsify!
     coarsify_all_T(P, W->Gb);
@@ -1543,7 +1634,7 @@ static void snap_restore(ibpm_prob* P, int slot)
     P->nl1 = P->nl2;
     launch_copy(P->G, sp, n, P->stream);
     launch_copy(P->nl[P->nl2], sp + n, n, P->stream);
-    vort2flux_dev(P, P->G, P->psi, P->q, false);
+    vort2flux_dev(P, P->G, P->psi, P->q, CURL_ALL);
 }
 
 static long binom(int n, int k)
@@ -1699,6 +1790,7 @@ static int zero_state(ibpm_prob* P)
     hipStream_t s = P->stream;
     const long nG = P->g.nG, nq = P->g.nq;
     launch_fill(P->q, nq * P->mg, 0.0, s);
+    P->q_stale = false;
     launch_fill(P->q0, nq * P->mg, 0.0, s);
     launch_fill(P->G, nG * P->mg, 0.0, s);
     launch_fill(P->Gs, nG * P->mg, 0.0, s);
@@ -2009,6 +2101,7 @@ int ibpm_state_upload(ibpm_prob* P, int field, const double* host, size_t n)
     if (n != want) return fail(P->ctx, IBPM_E_ARG, "upload size mismatch");
     if (int rc = check_idle(P)) return rc;
     graphs_clear(P);
+    ensure_q(P);  // the state's q before any of its fields changes
     HIPTRY(P->ctx, hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, P->stream));
     HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
     return IBPM_OK;
@@ -2021,6 +2114,7 @@ int ibpm_state_download(ibpm_prob* P, int field, double* host, size_t n)
     double* d = field_ptr(P, field, &want, false);
     if (!d) return fail(P->ctx, IBPM_E_ARG, "unknown field");
     if (n != want) return fail(P->ctx, IBPM_E_ARG, "download size mismatch");
+    if (field == IBPM_FIELD_Q) ensure_q(P);
     HIPTRY(P->ctx, hipMemcpyAsync(host, d, sizeof(double) * n, hipMemcpyDeviceToHost, P->stream));
     HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
     return IBPM_OK;
@@ -2078,7 +2172,7 @@ int ibpm_vort2flux(ibpm_prob* P)
     if (!P) return IBPM_E_ARG;
     if (P->group) return fail(P->ctx, IBPM_E_ARG, "the problem belongs to a group: step or change it through the group");
     if (int rc = check_idle(P)) return rc;
-    vort2flux_dev(P, P->G, P->psi, P->q, false);
+    vort2flux_dev(P, P->G, P->psi, P->q, CURL_ALL);
     HIPTRY(P->ctx, hipGetLastError());
     return IBPM_OK;
 }
@@ -2685,8 +2779,10 @@ int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* probs, ibpm_group*
         delete Gp;
         return fail(ctx, code, m);
     };
-    for (int i = 0; i < n; i++)
+    for (int i = 0; i < n; i++) {
+        ensure_q(probs[i]);  // the group's steps read q
         if (hipStreamSynchronize(probs[i]->stream) != hipSuccess) return bail(IBPM_E_HIP, "draining a member failed");
+    }
     if (hipStreamCreate(&Gp->stream) != hipSuccess) return bail(IBPM_E_HIP, "hipStreamCreate failed");
     const long nG = Gp->g.nG, nq = Gp->g.nq, L = Gp->L;
     const size_t fG = (size_t)n * L * nG, fq = (size_t)n * L * nq;
@@ -2834,6 +2930,7 @@ int ibpm_group_destroy(ibpm_group* Gp)
         if (!ok) rc = IBPM_E_HIP;
         P->G = o.G; P->Gs = o.Gs; P->psi = o.psi; P->nl[0] = o.nl0; P->nl[1] = o.nl1b; P->q = o.q; P->q0 = o.q0;
         P->Fb = o.Fb;
+        P->q_stale = false;  // a group step stores q
         P->group = nullptr;
     }
     (void)hipStreamSynchronize(Gp->stream);
@@ -3017,6 +3114,7 @@ int ibpm_group_wait(ibpm_group* Gp, double* cd_hist, double* cl_hist, int32_t* s
 int ibpm_compute_cfl(ibpm_prob* P, double* cfl)
 {
     if (!P || !cfl) return IBPM_E_ARG;
+    ensure_q(P);
     launch_absmax(P->q, P->g.nq, P->scal, P->stream);
     double m = 0.0;
     HIPTRY(P->ctx, hipMemcpyAsync(&m, P->scal, sizeof(double), hipMemcpyDeviceToHost, P->stream));
@@ -3142,7 +3240,7 @@ int ibpm_op_vort2flux(ibpm_prob* P, double* G, double* psi, double* q)
     const size_t nq = P->g.nq * P->mg, nG = P->g.nG * P->mg;
     double *dG = io.put(G, nG), *dpsi = io.put(nullptr, nG), *dq = io.put(nullptr, nq);
     if (!dG || !dpsi || !dq) return fail(P->ctx, IBPM_E_HIP, "alloc");
-    vort2flux_dev(P, dG, dpsi, dq, false);
+    vort2flux_dev(P, dG, dpsi, dq, CURL_ALL);
     int rc = get_back(P, G, dG, nG);
     if (!rc) rc = get_back(P, psi, dpsi, nG);
     if (!rc) rc = get_back(P, q, dq, nq);

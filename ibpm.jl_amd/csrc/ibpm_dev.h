@@ -940,4 +940,37 @@ __device__ __forceinline__ double get_bc_val(const double* __restrict__ r, const
     return bc_row(r, g, 3 * g.ny / 4, k - g.TOP);
 }
 
+// ---------------------------------------------------------------------------
+// curl! on the multigrid (src/fluid-operators/curl.jl:81-116) at one flux of
+// one level: psi is the level's streamfunction, its boundary values
+// get_bc!(pc) of the next coarser level's psi (zero when !hasbc, the
+// coarsest level).  Every kernel that needs a flux from psi calls these, so
+// the value is the same bit for bit wherever it is formed.
+// ---------------------------------------------------------------------------
+// x-flux (i, j), i = 1..nx+1, j = 1..ny
+__device__ __forceinline__ double curl_qx(const double* __restrict__ psi, const double* __restrict__ pc, bool hasbc,
+                                          const GridDev& g, int i, int j)
+{
+    auto PS = [&](int ii, int jj) -> double { return psi[(ii - 1) + (long)g.n1 * (jj - 1)]; };
+    auto BC = [&](int k) -> double { return hasbc ? get_bc_val(pc, g, k) : 0.0; };
+    if (i >= 2 && i <= g.nx && j >= 2 && j <= g.ny - 1) return PS(i - 1, j) - PS(i - 1, j - 1);
+    if (i == 1) return BC((g.LEFT + 1) + j) - BC(g.LEFT + j);
+    if (i == g.nx + 1) return BC((g.RIGHT + 1) + j) - BC(g.RIGHT + j);
+    if (j == 1) return PS(i - 1, 1) - BC((g.BOT - 1) + i);
+    return BC(i + g.TOP) - PS(i - 1, g.ny - 1);  // j == ny
+}
+
+// y-flux (i, j), i = 1..nx, j = 1..ny+1
+__device__ __forceinline__ double curl_qy(const double* __restrict__ psi, const double* __restrict__ pc, bool hasbc,
+                                          const GridDev& g, int i, int j)
+{
+    auto PS = [&](int ii, int jj) -> double { return psi[(ii - 1) + (long)g.n1 * (jj - 1)]; };
+    auto BC = [&](int k) -> double { return hasbc ? get_bc_val(pc, g, k) : 0.0; };
+    if (j >= 2 && j <= g.ny && i >= 2 && i <= g.nx - 1) return PS(i - 1, j - 1) - PS(i, j - 1);
+    if (j == 1) return BC(g.BOT + i) - BC((g.BOT + 1) + i);
+    if (j == g.ny + 1) return BC(g.TOP + i) - BC((g.TOP + 1) + i);
+    if (i == 1) return BC(g.LEFT + j) - PS(1, j - 1);
+    return PS(g.nx - 1, j - 1) - BC(g.RIGHT + j);  // i == nx
+}
+
 }  // namespace ibpm

@@ -43,6 +43,11 @@ struct XArgs {
     // X_PLAIN: input lines j (1-based) outside [zlo, zhi] are known to be zero;
     // their output lines are written as zeros without a transform
     int zlo, zhi;
+    // X_PLAIN, lsel != 0: only the lines llo..lhi (0-based) are transformed
+    // (the launch covers no others); keep != 0: of those, only lines keep0 and
+    // keep1 store their output (the edge columns are stored for every line)
+    int lsel, llo, lhi;
+    int keep, keep0, keep1;
     // X_PLAIN over a batch (nlev > 1): waves (lines) per work-group, 8, 12 or
     // 14; anything else (0) gives the default 4
     int waves;
@@ -98,6 +103,9 @@ size_t ypass_lds_bytes(const DstPlan& pl, int n2, int tile);
 // ---- field kernels ---------------------------------------------------------
 struct NLArgs {
     const double* q;
+    // when non-null, the fluxes are formed from psi (all levels: entry e at
+    // e nG, its boundary values from entry e + 1) and q is not read
+    const double* psi;
     const double* q0;
     const double* G;
     double* out;
@@ -156,6 +164,11 @@ void launch_rot(const double* q, double* G, int ib0, int ib1, int jb0, int jb1, 
 
 struct GatherArgs {
     const double* q;
+    // when psi is non-null (one problem, level 1), the fluxes are formed from
+    // psi by curl! with the boundary values of psic (the level-2 psi; null:
+    // zero) instead of being read from q
+    const double* psi;
+    const double* psic;
     const double* q0;  // may be null
     const double* w;   // [(2k + c) * 169 + l + 13 m]
     const int* bidx;   // [k] = bx, [nbt + k] = by (1-based)

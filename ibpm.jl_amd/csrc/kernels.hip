@@ -99,7 +99,8 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
     const int lane = threadIdx.x & (WAVE - 1);
     const int w = threadIdx.x / WAVE;
     const int n1 = line_len<PT>(a.g.n1), n2 = a.g.n2;
-    const int line = blockIdx.x * XW + w;  // 0-based j - 1
+    const int lend = a.lsel ? a.lhi + 1 : n2;  // one past the last line of the launch
+    const int line = (a.lsel ? a.llo : 0) + blockIdx.x * XW + w;  // 0-based j - 1
     const int b = blockIdx.y;
     const int j = line + 1;
     const Entry en = batch_entry(b, a.lev0, a.levmod, a.fixlev, a.mg, a.ipar);
@@ -107,7 +108,7 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
     const int ntab = tab_doubles_t<PT>(a.pl);
     const DstTab tb = load_tab_t<PT, XW * WAVE>(lds, a.pl, threadIdx.x);
     __syncthreads();
-    if (line >= n2) return;
+    if (line >= lend) return;
     if constexpr (MODE == X_PLAIN) {
         if (j < a.zlo || j > a.zhi) {  // zero line: its transform is zero
             double* dz = a.dst + b * a.lstride + (long)line * n1;
@@ -226,10 +227,12 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
             if (n < n1) dst[n] = v[t] - s[lat<PT>(n)];
         }
     } else {
+        if (!a.keep || line == a.keep0 || line == a.keep1) {
 #pragma unroll
-        for (int t = 0; t < XPL; t++) {
-            int n = lane + WAVE * t;
-            if (n < n1) dst[n] = s[lat<PT>(n)];
+            for (int t = 0; t < XPL; t++) {
+                int n = lane + WAVE * t;
+                if (n < n1) dst[n] = s[lat<PT>(n)];
+            }
         }
         if (MODE == X_PLAIN && a.edge && lane < 2) {
             // the two columns get_bc! samples for the next finer level, beside each other
@@ -251,7 +254,8 @@ static void launch_x_t(const XArgs& a, hipStream_t s)
 {
     static bool once = (allow_big_lds(k_xpass<MODE, PT, XW>), true);
     (void)once;
-    dim3 grid((a.g.n2 + XW - 1) / XW, a.nlev);
+    const int nlines = a.lsel ? a.lhi - a.llo + 1 : a.g.n2;
+    dim3 grid((nlines + XW - 1) / XW, a.nlev);
     size_t lds = sizeof(double) * ((size_t)tab_doubles_t<PT>(a.pl) + (size_t)XW * slot_doubles_t<PT>(a.pl));
     hipLaunchKernelGGL((k_xpass<MODE, PT, XW>), grid, dim3(XW * WAVE), lds, s, a);
 }
@@ -714,68 +718,68 @@ void launch_ypass(const YArgs& a, int tile, hipStream_t s)
 // nonlinear! (nonlinear.jl:168-201), fused: avg_flux (:17-38),
 // direct_product_loops! (:73-114), rot! (curl.jl:22-34) and the 1/(4 hc^2)
 // scaling.  Boundary circulation is get_bc!(Γ[:, lev+1]) .* 0.25.
-// One thread per output Γ point.
+// A thread takes a vertical strip of NL_JS output Γ points.
+//
+// The fluxes come from the state's q, or are formed from its ψ by curl!
+// (the values curl_qx / curl_qy give, which are what curl! stores in q): a
+// strip whose stencil stays off the edges reads ψ directly, an edge strip
+// goes through curl_qx / curl_qy with the boundary values.  The products
+// are the same expression in every form.
 // ===========================================================================
 constexpr int NL_JS = 8;  // output rows per thread (vertical strip)
 
-template <bool LIN>
-__global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
-{
-    const GridDev& g = a.g;
-    const int nx = g.nx, ny = g.ny, n1 = g.n1;
-    // logical block (64 i, 4 strips of NL_JS rows, level), XCD-contiguous
-    const int nbx = (nx - 1 + 63) / 64;
-    const int nbs = ((ny - 1 + NL_JS - 1) / NL_JS + 3) / 4;
-    const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
-    const int b = lin / (nbx * nbs);
-    if (b >= a.nlev) return;
-    const int rem = lin - b * (nbx * nbs);
-    const int i = (rem % nbx) * 64 + threadIdx.x + 2;                    // 2..nx
-    const int j0 = ((rem / nbx) * 4 + threadIdx.y) * NL_JS + 2;          // first output row, 2..ny
-    if (i > nx || j0 > ny) return;
-    const int j1 = min(j0 + NL_JS - 1, ny);
-    const int e = a.lev0 + b;  // batch entry: its fields at e * nq, e * nG
-    const Entry en = batch_entry(b, a.lev0, a.levmod, 0, a.mg, a.ipar);
-    if (!en.active) return;
-    const int lev = en.lev;    // its grid level
-    const long nq = g.nq, nu = g.nu;
-    const double* __restrict__ q = a.q + (long)e * nq;
-    const double* __restrict__ q0 = a.q0 + (long)e * nq;
-    const double* __restrict__ G = a.G + (long)e * g.nG;
-    const bool hasbc = lev < en.L - 1;
-    const double* __restrict__ Gc = a.G + (long)(e + 1) * g.nG;
-    const bool u0 = a.q0_uniform != 0;
-    const double hcl = level_hc(g.h, lev);
-    double q0vx = a.q0vx, q0vy = a.q0vy;
-    if (a.ipar) {
-        q0vx = a.ipar[en.inst].q0vx;
-        q0vy = a.ipar[en.inst].q0vy;
+// Flux and circulation sources of one level.  qx / qy: the flux without the
+// base flux; gam: Γ with its boundary ring; gb: ΓB with its ring (linearized).
+struct NLLevel {
+    const GridDev* g;
+    const double* __restrict__ G;
+    const double* __restrict__ Gc;  // Γ of the next coarser level (hasbc)
+    const double* __restrict__ GBl;
+    bool hasbc;
+    __device__ __forceinline__ double bcv(int k) const { return hasbc ? get_bc_val(Gc, *g, k) * 0.25 : 0.0; }
+    // a circulation field extended by the boundary ring get_bc!(Γ[:, lev+1]) .* 0.25:
+    // X(ii, jj), ii = 0..nx, jj = 0..ny (corners are never read)
+    __device__ __forceinline__ double ext(const double* __restrict__ F, int ii, int jj) const
+    {
+        if (jj == 0) return bcv(g->BOT + ii + 1);
+        if (jj == g->ny) return bcv(g->TOP + ii + 1);
+        if (ii == 0) return bcv(g->LEFT + jj + 1);
+        if (ii == g->nx) return bcv(g->RIGHT + jj + 1);
+        return F[(ii - 1) + (long)g->n1 * (jj - 1)];
     }
-    const double cx0 = q0vx * hcl, cy0 = q0vy * hcl;  // as k_base_flux_inertial writes them
-
-    // boundary circulation get_bc!(Γ[:, lev+1]) .* 0.25 (nonlinear.jl:186-188)
-    auto BCV = [&](int k) -> double { return hasbc ? get_bc_val(Gc, g, k) * 0.25 : 0.0; };
-    // a circulation field extended by that boundary ring: X(ii, jj), ii = 0..nx,
-    // jj = 0..ny; the ring carries the values direct_product_loops!
-    // (nonlinear.jl:73-114) uses at the edges (corners are never read)
-    auto EXT = [&](const double* __restrict__ F, int ii, int jj) -> double {
-        if (jj == 0) return BCV(g.BOT + ii + 1);
-        if (jj == ny) return BCV(g.TOP + ii + 1);
-        if (ii == 0) return BCV(g.LEFT + jj + 1);
-        if (ii == nx) return BCV(g.RIGHT + jj + 1);
-        return F[(ii - 1) + (long)n1 * (jj - 1)];
-    };
-    auto GAX = [&](int ii, int jj) -> double { return EXT(G, ii, jj); };
+    __device__ __forceinline__ double gam(int ii, int jj) const { return ext(G, ii, jj); }
+    __device__ __forceinline__ double gb(int ii, int jj) const { return ext(GBl, ii, jj); }
+};
+struct NLFromQ : NLLevel {
+    const double* __restrict__ q;
+    __device__ __forceinline__ double qx(int ii, int jj) const { return q[(ii - 1) + (long)(g->nx + 1) * (jj - 1)]; }
+    __device__ __forceinline__ double qy(int ii, int jj) const { return q[g->nu + (ii - 1) + (long)g->nx * (jj - 1)]; }
+};
+struct NLFromPsi : NLLevel {
+    const double* __restrict__ ps;   // ψ of the level
+    const double* __restrict__ psc;  // ψ of the next coarser level (hasbc)
+    __device__ __forceinline__ double qx(int ii, int jj) const { return curl_qx(ps, psc, hasbc, *g, ii, jj); }
+    __device__ __forceinline__ double qy(int ii, int jj) const { return curl_qy(ps, psc, hasbc, *g, ii, jj); }
+};
+// One strip: output points (i-1, j-1) for j = j0..j1.  UNROLL: the row loop
+// unrolled (the common strips) or rolled (edge strips, whose sources carry
+// the boundary branches).
+template <bool LIN, int UNROLL, class SRC>
+__device__ __forceinline__ void nl_strip(const NLArgs& a, const SRC& S, const double* __restrict__ q0, bool u0,
+                                         double cx0, double cy0, double hcl, int i, int j0, int j1,
+                                         const double* __restrict__ QBl, double* __restrict__ out)
+{
+    const GridDev& g = *S.g;
+    const long nu = g.nu;
+    const int nx = g.nx, ny = g.ny, n1 = g.n1;
     // flux components with the base flux: x-flux (ii, jj) and y-flux (ii, jj)
     auto QXV = [&](int ii, int jj, double& f, double& f0) {
-        const long p = (ii - 1) + (long)(nx + 1) * (jj - 1);
-        f = q[p];
-        f0 = (LIN || u0) ? cx0 : q0[p];
+        f = S.qx(ii, jj);
+        f0 = (LIN || u0) ? cx0 : q0[(ii - 1) + (long)(nx + 1) * (jj - 1)];
     };
     auto QYV = [&](int ii, int jj, double& f, double& f0) {
-        const long p = nu + (ii - 1) + (long)nx * (jj - 1);
-        f = q[p];
-        f0 = (LIN || u0) ? cy0 : q0[p];
+        f = S.qy(ii, jj);
+        f0 = (LIN || u0) ? cy0 : q0[nu + (ii - 1) + (long)nx * (jj - 1)];
     };
     // avg_flux (nonlinear.jl:17-38; linearized :45-65 without the base flux):
     // Qy(ii, jj) over (ii, jj), (ii-1, jj); Qx over (ii, jj), (ii, jj-1)
@@ -787,32 +791,28 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
         else return ((f + f0) + fm) + fm0;
     };
     // linearized problem: base flux QB (already averaged) and ΓB with the ring
-    const double* __restrict__ QBl = LIN ? a.QB + (long)lev * nq : nullptr;
-    const double* __restrict__ GBl = LIN ? a.GB + (long)lev * g.nG : nullptr;
     auto QBX = [&](int ii, int jj) -> double { return QBl[(ii - 1) + (long)(nx + 1) * (jj - 1)]; };
     auto QBY = [&](int ii, int jj) -> double { return QBl[nu + (ii - 1) + (long)nx * (jj - 1)]; };
-    auto GBX = [&](int ii, int jj) -> double { return EXT(GBl, ii, jj); };
 
     // state carried from row j-1 to row j
     double qxr[3], q0r[3];  // x-flux (i-1..i+1, j-1)
-    double gr[3], gbr[3];   // GAX(i-2..i, j-1), GBX(i-2..i, j-1)
+    double gr[3], gbr[3];   // Γ(i-2..i, j-1), ΓB(i-2..i, j-1) with the ring
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         QXV(i - 1 + d, j0 - 1, qxr[d], q0r[d]);
-        gr[d] = GAX(i - 2 + d, j0 - 1);
-        gbr[d] = LIN ? GBX(i - 2 + d, j0 - 1) : 0.0;
+        gr[d] = S.gam(i - 2 + d, j0 - 1);
+        gbr[d] = LIN ? S.gb(i - 2 + d, j0 - 1) : 0.0;
     }
     double qyj = QY(i, j0);  // Qy(i, j)
     double qbyj = LIN ? QBY(i, j0) : 0.0;
     // fq_x(i, j-1): Q x Γ, or Q x ΓB + QB x Γ
     double fqxa;
     if constexpr (LIN)
-        fqxa = (qyj * gbr[1] + QY(i, j0 - 1) * GBX(i - 1, j0 - 2)) + (qbyj * gr[1] + QBY(i, j0 - 1) * GAX(i - 1, j0 - 2));
+        fqxa = (qyj * gbr[1] + QY(i, j0 - 1) * S.gb(i - 1, j0 - 2)) + (qbyj * gr[1] + QBY(i, j0 - 1) * S.gam(i - 1, j0 - 2));
     else
-        fqxa = qyj * gr[1] + QY(i, j0 - 1) * GAX(i - 1, j0 - 2);
+        fqxa = qyj * gr[1] + QY(i, j0 - 1) * S.gam(i - 1, j0 - 2);
     const double sl = 1.0 / (4.0 * (hcl * hcl));  // nonlinear.jl:197-198
-    double* __restrict__ out = a.out + (long)b * g.nG;
-#pragma unroll
+#pragma unroll UNROLL
     for (int t = 0; t < NL_JS; t++) {
         const int j = j0 + t;
         if (j > j1) break;
@@ -827,14 +827,14 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
                 QX[d] = ((qxn[d] + q0n[d]) + qxr[d]) + q0r[d];
             }
         }
-        gn[1] = GAX(i - 1, j);
-        if constexpr (LIN) gbn[1] = GBX(i - 1, j);
+        gn[1] = S.gam(i - 1, j);
+        if constexpr (LIN) gbn[1] = S.gb(i - 1, j);
         if (j < ny) {
-            gn[0] = GAX(i - 2, j);
-            gn[2] = GAX(i, j);
+            gn[0] = S.gam(i - 2, j);
+            gn[2] = S.gam(i, j);
             if constexpr (LIN) {
-                gbn[0] = GBX(i - 2, j);
-                gbn[2] = GBX(i, j);
+                gbn[0] = S.gb(i - 2, j);
+                gbn[2] = S.gb(i, j);
             }
         } else {
             gn[0] = gn[2] = 0.0;  // row ny is the top ring: only X(i-1, ny) is read
@@ -871,12 +871,216 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
     }
 }
 
+// A strip whose stencil is interior, fluxes from ψ (LIN = false): each flux
+// is the interior branch of curl_qx / curl_qy, the ψ rows i-2..i are
+// carried from row to row, and the products are nl_strip's.
+__device__ __forceinline__ void nl_strip_psi_inner(const double* __restrict__ ps, const double* __restrict__ G,
+                                                   const double* __restrict__ q0, bool u0, double cx0, double cy0,
+                                                   double hcl, int i, int j0, int j1, const GridDev& g,
+                                                   double* __restrict__ out)
+{
+    const int n1 = g.n1, nx = g.nx;
+    const long nu = g.nu;
+    auto PS = [&](int ii, int jj) -> double { return ps[(ii - 1) + (long)n1 * (jj - 1)]; };
+    auto GA = [&](int ii, int jj) -> double { return G[(ii - 1) + (long)n1 * (jj - 1)]; };
+    auto Q0X = [&](int ii, int jj) -> double { return u0 ? cx0 : q0[(ii - 1) + (long)(nx + 1) * (jj - 1)]; };
+    auto Q0Y = [&](int ii, int jj) -> double { return u0 ? cy0 : q0[nu + (ii - 1) + (long)nx * (jj - 1)]; };
+    // avg_flux's Qy(i, jj) from the ψ row jj - 1 (columns i-2..i): the y-fluxes
+    // (i, jj) = ψ(i-1, jj-1) - ψ(i, jj-1) and (i-1, jj) = ψ(i-2, jj-1) - ψ(i-1, jj-1)
+    auto QY = [&](const double* pr, int jj) -> double {
+        const double f = pr[1] - pr[2], fm = pr[0] - pr[1];
+        return ((f + Q0Y(i, jj)) + fm) + Q0Y(i - 1, jj);
+    };
+    double pm2[3], pm1[3];  // ψ rows j0-2 and j0-1, columns i-2..i
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        pm2[d] = PS(i - 2 + d, j0 - 2);
+        pm1[d] = PS(i - 2 + d, j0 - 1);
+    }
+    double qxr[3], q0r[3], gr[3];  // x-flux (i-1..i+1, j-1), Γ(i-2..i, j-1)
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        qxr[d] = pm1[d] - pm2[d];
+        q0r[d] = Q0X(i - 1 + d, j0 - 1);
+        gr[d] = GA(i - 2 + d, j0 - 1);
+    }
+    double qyj = QY(pm1, j0);
+    double fqxa = qyj * gr[1] + QY(pm2, j0 - 1) * GA(i - 1, j0 - 2);
+    const double sl = 1.0 / (4.0 * (hcl * hcl));
+#pragma unroll
+    for (int t = 0; t < NL_JS; t++) {
+        const int j = j0 + t;
+        if (j > j1) break;
+        double pj[3], QX[3], q0n[3], gn[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            pj[d] = PS(i - 2 + d, j);
+            q0n[d] = Q0X(i - 1 + d, j);
+            gn[d] = GA(i - 2 + d, j);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double qxn = pj[d] - pm1[d];
+            QX[d] = ((qxn + q0n[d]) + qxr[d]) + q0r[d];
+            qxr[d] = qxn;
+        }
+        const double qyn = QY(pj, j + 1);
+        const double fqxb = qyn * gn[1] + qyj * gr[1];
+        const double fqya = (QX[2] * gr[2] + QX[1] * gr[1]) * -1.0;
+        const double fqyb = (QX[1] * gr[1] + QX[0] * gr[0]) * -1.0;
+        const double r = (fqxa - fqxb) + (fqya - fqyb);
+        out[(i - 2) + (long)n1 * (j - 2)] = r * sl;
+        fqxa = fqxb;
+        qyj = qyn;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            pm1[d] = pj[d];
+            q0r[d] = q0n[d];
+            gr[d] = gn[d];
+        }
+    }
+}
+
+// Every flux and Γ value a strip reads is interior (x-fluxes (i-1..i+1,
+// j0-1..j1), y-fluxes (i-1..i, j0-1..j1+1), Γ (i-2..i, j0-2..j1)): true but
+// for the strips of the first and last strip rows and of the columns i = 2, nx.
+__device__ __forceinline__ bool nl_strip_interior(const GridDev& g, int i, int j0, int j1)
+{
+    return i >= 3 && i <= g.nx - 1 && j0 >= 3 && j1 <= g.ny - 1;
+}
+
+// The output points of the strips nl_strip_interior rejects, fluxes from ψ
+// through curl_qx / curl_qy with the boundary values.  One thread per point
+// (a strip of one row: the values a strip carries from row to row are formed
+// again, by the same expressions), so that the edge work does not run as a
+// few long serial strips: thread k of entry b = blockIdx.y takes the points
+// of the first strip row, then of the last, then the columns i = 2 and nx of
+// the rows between.
+struct NLFrame {
+    int ni, r0, rl0, rl, nm;  // i count; rows of the first strip row; first row and count of the last; rows between
+};
+__host__ __device__ inline NLFrame nl_frame(int nx, int ny)
+{
+    NLFrame f;
+    const int ns = (ny - 1 + NL_JS - 1) / NL_JS;  // strip rows: j0 = 2 + NL_JS s
+    f.ni = nx - 1;
+    f.r0 = ns > 1 ? NL_JS : ny - 1;
+    f.rl0 = 2 + NL_JS * (ns - 1);
+    f.rl = ns > 1 ? ny - f.rl0 + 1 : 0;
+    f.nm = ns > 2 ? NL_JS * (ns - 2) : 0;
+    return f;
+}
+
+__global__ void __launch_bounds__(256) k_nonlinear_frame(NLArgs a)
+{
+    const GridDev& g = a.g;
+    const int nx = g.nx, ny = g.ny;
+    const NLFrame f = nl_frame(nx, ny);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    int i, j;
+    if (k < f.r0 * f.ni) {
+        i = 2 + k % f.ni;
+        j = 2 + k / f.ni;
+    } else if (k < (f.r0 + f.rl) * f.ni) {
+        const int c = k - f.r0 * f.ni;
+        i = 2 + c % f.ni;
+        j = f.rl0 + c / f.ni;
+    } else {
+        const int c = k - (f.r0 + f.rl) * f.ni;
+        if (c >= 2 * f.nm) return;
+        i = (c & 1) ? nx : 2;
+        j = 2 + NL_JS + (c >> 1);
+    }
+    const int s = (j - 2) / NL_JS, j0 = 2 + NL_JS * s;
+    if (nl_strip_interior(g, i, j0, min(j0 + NL_JS - 1, ny))) return;
+    const int e = a.lev0 + b;
+    const Entry en = batch_entry(b, a.lev0, a.levmod, 0, a.mg, a.ipar);
+    if (!en.active) return;
+    const int lev = en.lev;
+    const double hcl = level_hc(g.h, lev);
+    double q0vx = a.q0vx, q0vy = a.q0vy;
+    if (a.ipar) {
+        q0vx = a.ipar[en.inst].q0vx;
+        q0vy = a.ipar[en.inst].q0vy;
+    }
+    NLFromPsi S;
+    S.g = &g;
+    S.G = a.G + (long)e * g.nG;
+    S.Gc = a.G + (long)(e + 1) * g.nG;
+    S.GBl = nullptr;
+    S.hasbc = lev < en.L - 1;
+    S.ps = a.psi + (long)e * g.nG;
+    S.psc = a.psi + (long)(e + 1) * g.nG;
+    nl_strip<false, 1>(a, S, a.q0 + (long)e * g.nq, a.q0_uniform != 0, q0vx * hcl, q0vy * hcl, hcl, i, j, j, nullptr,
+                       a.out + (long)b * g.nG);
+}
+
+// PSI: the fluxes from the state's ψ (NLArgs::psi) instead of q
+template <bool LIN, bool PSI>
+__global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
+{
+    const GridDev& g = a.g;
+    const int nx = g.nx, ny = g.ny;
+    // logical block (64 i, 4 strips of NL_JS rows, level), XCD-contiguous
+    const int nbx = (nx - 1 + 63) / 64;
+    const int nbs = ((ny - 1 + NL_JS - 1) / NL_JS + 3) / 4;
+    const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
+    const int b = lin / (nbx * nbs);
+    if (b >= a.nlev) return;
+    const int rem = lin - b * (nbx * nbs);
+    const int i = (rem % nbx) * 64 + threadIdx.x + 2;                    // 2..nx
+    const int j0 = ((rem / nbx) * 4 + threadIdx.y) * NL_JS + 2;          // first output row, 2..ny
+    if (i > nx || j0 > ny) return;
+    const int j1 = min(j0 + NL_JS - 1, ny);
+    const int e = a.lev0 + b;  // batch entry: its fields at e * nq, e * nG
+    const Entry en = batch_entry(b, a.lev0, a.levmod, 0, a.mg, a.ipar);
+    if (!en.active) return;
+    const int lev = en.lev;    // its grid level
+    const double* __restrict__ q0 = a.q0 + (long)e * g.nq;
+    const bool u0 = a.q0_uniform != 0;
+    const double hcl = level_hc(g.h, lev);
+    double q0vx = a.q0vx, q0vy = a.q0vy;
+    if (a.ipar) {
+        q0vx = a.ipar[en.inst].q0vx;
+        q0vy = a.ipar[en.inst].q0vy;
+    }
+    const double cx0 = q0vx * hcl, cy0 = q0vy * hcl;  // as k_base_flux_inertial writes them
+    NLLevel lv;
+    lv.g = &g;
+    lv.G = a.G + (long)e * g.nG;
+    lv.Gc = a.G + (long)(e + 1) * g.nG;
+    lv.GBl = LIN ? a.GB + (long)lev * g.nG : nullptr;
+    lv.hasbc = lev < en.L - 1;
+    const double* __restrict__ QBl = LIN ? a.QB + (long)lev * g.nq : nullptr;
+    double* __restrict__ out = a.out + (long)b * g.nG;
+    if constexpr (PSI) {
+        static_assert(!LIN, "the linearized product reads q");
+        // the interior strips; k_nonlinear_frame takes the others
+        if (nl_strip_interior(g, i, j0, j1))
+            nl_strip_psi_inner(a.psi + (long)e * g.nG, lv.G, q0, u0, cx0, cy0, hcl, i, j0, j1, g, out);
+    } else {
+        NLFromQ S;
+        static_cast<NLLevel&>(S) = lv;
+        S.q = a.q + (long)e * g.nq;
+        nl_strip<LIN, NL_JS>(a, S, q0, u0, cx0, cy0, hcl, i, j0, j1, QBl, out);
+    }
+}
+
 void launch_nonlinear(const NLArgs& a, hipStream_t s)
 {
     dim3 blk(64, 4);  // k_nonlinear derives its tile from this shape
     const int nblocks = ((a.g.nx - 1 + 63) / 64) * (((a.g.ny - 1 + NL_JS - 1) / NL_JS + 3) / 4) * a.nlev;
-    if (a.QB) hipLaunchKernelGGL(k_nonlinear<true>, dim3(NXCD * xcd_nper(nblocks)), blk, 0, s, a);
-    else hipLaunchKernelGGL(k_nonlinear<false>, dim3(NXCD * xcd_nper(nblocks)), blk, 0, s, a);
+    const dim3 grid(NXCD * xcd_nper(nblocks));
+    if (a.psi && !a.QB) {
+        hipLaunchKernelGGL((k_nonlinear<false, true>), grid, blk, 0, s, a);
+        const NLFrame f = nl_frame(a.g.nx, a.g.ny);
+        const int nframe = (f.r0 + f.rl) * f.ni + 2 * f.nm;
+        hipLaunchKernelGGL(k_nonlinear_frame, dim3((nframe + 255) / 256, a.nlev), dim3(256), 0, s, a);
+    } else {
+        if (a.QB) hipLaunchKernelGGL((k_nonlinear<true, false>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((k_nonlinear<false, false>), grid, blk, 0, s, a);
+    }
 }
 
 // ===========================================================================
@@ -908,28 +1112,10 @@ __global__ void __launch_bounds__(256) k_curl(CurlArgs a)
     const double* __restrict__ psi = a.psi + pe;
     const bool hasbc = lev < en.L - 1;
     const double* __restrict__ pc = a.psi + pe + pl1;
-    auto PS = [&](int ii, int jj) -> double { return psi[(ii - 1) + (long)n1 * (jj - 1)]; };
-    auto BC = [&](int k) -> double { return hasbc ? get_bc_val(pc, g, k) : 0.0; };
-    const int T = g.TOP, B = g.BOT, L = g.LEFT, R = g.RIGHT;
+    (void)n1;
     double* __restrict__ q = a.q + (long)b * a.qstride;
-    if (j <= ny) {  // x-flux (i, j)
-        double v;
-        if (i == 1) v = BC((L + 1) + j) - BC(L + j);
-        else if (i == nx + 1) v = BC((R + 1) + j) - BC(R + j);
-        else if (j == 1) v = PS(i - 1, 1) - BC((B - 1) + i);
-        else if (j == ny) v = BC(i + T) - PS(i - 1, ny - 1);
-        else v = PS(i - 1, j) - PS(i - 1, j - 1);
-        q[(i - 1) + (long)(nx + 1) * (j - 1)] = v;
-    }
-    if (i <= nx) {  // y-flux (i, j)
-        double v;
-        if (j == 1) v = BC(B + i) - BC((B + 1) + i);
-        else if (j == ny + 1) v = BC(T + i) - BC((T + 1) + i);
-        else if (i == 1) v = BC(L + j) - PS(1, j - 1);
-        else if (i == nx) v = PS(nx - 1, j - 1) - BC(R + j);
-        else v = PS(i - 1, j - 1) - PS(i, j - 1);
-        q[g.nu + (i - 1) + (long)nx * (j - 1)] = v;
-    }
+    if (j <= ny) q[(i - 1) + (long)(nx + 1) * (j - 1)] = curl_qx(psi, pc, hasbc, g, i, j);
+    if (i <= nx) q[g.nu + (i - 1) + (long)nx * (j - 1)] = curl_qy(psi, pc, hasbc, g, i, j);
 }
 
 void launch_curl(const CurlArgs& a, hipStream_t s)
@@ -1038,7 +1224,11 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs a)
         const int l = e % 13, m = e / 13;
         const int i = bx + l - 6, j = by + m - 6;
         long p = (c == 0) ? (i - 1) + (long)(nx + 1) * (j - 1) : a.g.nu + (i - 1) + (long)nx * (j - 1);
-        double v = q0f ? (qf[p] + q0f[p]) : qf[p];
+        double f;
+        if (a.psi) f = (c == 0) ? curl_qx(a.psi, a.psic, a.psic != nullptr, a.g, i, j)
+                                : curl_qy(a.psi, a.psic, a.psic != nullptr, a.g, i, j);
+        else f = qf[p];
+        double v = q0f ? (f + q0f[p]) : f;
         acc += w[e] * v;
     }
     acc = wave_sum(acc);
