@@ -507,14 +507,21 @@ static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x,
     hipStream_t s = P->stream;
     if (!tmp) tmp = P->T1 + (long)lev * P->g.nG;
     XArgs a = xargs(P);
-    a.zlo = zlo;
-    a.zhi = std::min(zhi, P->g.n2);
     a.src = b;
     a.dst = tmp;
     a.nlev = 1;
     a.lev0 = lev;
+    YArgs y = yargs(P, op, lev, 1, tmp);
+    zhi = std::min(zhi, P->g.n2);
+    if (zlo > 1 || zhi < P->g.n2) {
+        // input lines outside zlo..zhi are zero: only those lines are
+        // transformed, and the y-pass reads no others
+        a.lsel = y.rsel = 1;
+        a.llo = y.rlo = zlo - 1;
+        a.lhi = y.rhi = zhi - 1;
+    }
     launch_xpass(X_PLAIN, a, s);
-    launch_ypass(yargs(P, op, lev, 1, tmp), P->ytile_single, s);
+    launch_ypass(y, P->ytile_single, s);
     XArgs c = xargs(P);
     c.src = tmp;
     c.dst = x;

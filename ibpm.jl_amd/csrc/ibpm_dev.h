@@ -940,6 +940,25 @@ __device__ __forceinline__ double get_bc_val(const double* __restrict__ r, const
     return bc_row(r, g, 3 * g.ny / 4, k - g.TOP);
 }
 
+// get_bc_val without branches, for code that evaluates many boundary values
+// at once: the coarse sample address of boundary index k is selected, both
+// samples are loaded, the value is get_bc_val's expression.  With !valid the
+// loads go to r0 (any field of the grid's size) and the value is zero.
+__device__ __forceinline__ double get_bc_val_sel(const double* __restrict__ r, const double* __restrict__ r0,
+                                                 bool valid, const GridDev& g, int k)
+{
+    const bool col = k <= g.BOT;                            // LEFT / RIGHT: a column of the coarse level
+    const bool first = col ? (k <= g.RIGHT) : (k <= g.TOP);  // LEFT / BOT
+    const int idx = k - (col ? (first ? g.LEFT : g.RIGHT) : (first ? g.BOT : g.TOP));
+    const int ir = first ? g.nx / 4 : 3 * g.nx / 4;
+    const int jr = first ? g.ny / 4 : 3 * g.ny / 4;
+    const long off = col ? (ir - 1) + (long)g.n1 * (g.ny / 4 + idx / 2 - 1) : (g.nx / 4 + idx / 2 - 1) + (long)g.n1 * (jr - 1);
+    const long back = col ? (long)g.n1 : 1;
+    const double* p = (valid ? r : r0) + off;
+    const double a = p[0], b = p[-back];
+    return valid ? ((idx & 1) ? a : 0.5 * (a + b)) : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // curl! on the multigrid (src/fluid-operators/curl.jl:81-116) at one flux of
 // one level: psi is the level's streamfunction, its boundary values
@@ -958,6 +977,57 @@ __device__ __forceinline__ double curl_qx(const double* __restrict__ psi, const 
     if (i == g.nx + 1) return BC((g.RIGHT + 1) + j) - BC(g.RIGHT + j);
     if (j == 1) return PS(i - 1, 1) - BC((g.BOT - 1) + i);
     return BC(i + g.TOP) - PS(i - 1, g.ny - 1);  // j == ny
+}
+
+// curl_qx and curl_qy without branches: the two operands of the difference
+// (a ψ value or a boundary value each) are both loaded and selected, so that
+// a thread evaluating many fluxes has all their loads in flight at once.
+// The value is the branching form's, bit for bit.
+__device__ __forceinline__ double curl_qx_sel(const double* __restrict__ psi, const double* __restrict__ pc, bool hasbc,
+                                              const GridDev& g, int i, int j)
+{
+    const int nx = g.nx, ny = g.ny;
+    // operand a: ψ(i-1, j) [interior, j = 1 -> ψ(i-1, 1)] or bc; operand b: ψ(i-1, j-1) [interior,
+    // j = ny -> ψ(i-1, ny-1) as a] or bc
+    const bool in = i >= 2 && i <= nx;
+    const bool bl = i == 1, br = i == nx + 1;
+    // boundary indices of the operands that are boundary values
+    const int ka = bl ? g.LEFT + 1 + j : br ? g.RIGHT + 1 + j : i + g.TOP;            // a is bc: i edge, or j == ny
+    const int kb = bl ? g.LEFT + j : br ? g.RIGHT + j : (g.BOT - 1) + i;             // b is bc: i edge, or j == 1
+    const bool abc = !in || j == ny;
+    const bool bbc = !in || j == 1;
+    // ψ operands (clamped into the field; used only when interior)
+    const int ic = min(max(i - 1, 1), nx - 1);
+    const int ja = j == ny ? ny - 1 : min(max(j, 1), ny - 1);   // j == ny: the ψ operand is b's, ψ(i-1, ny-1)
+    const int jb = min(max(j - 1, 1), ny - 1);
+    const double pa = psi[(ic - 1) + (long)g.n1 * (ja - 1)];
+    const double pb = psi[(ic - 1) + (long)g.n1 * (jb - 1)];
+    const double ca = get_bc_val_sel(pc, psi, hasbc, g, abc ? ka : g.LEFT + 1);
+    const double cb = get_bc_val_sel(pc, psi, hasbc, g, bbc ? kb : g.LEFT + 1);
+    // j == ny (interior i): a = bc(i + TOP), b = ψ(i-1, ny-1) (loaded as pa)
+    if (in && j == ny) return ca - pa;
+    return (abc ? ca : pa) - (bbc ? cb : pb);
+}
+
+__device__ __forceinline__ double curl_qy_sel(const double* __restrict__ psi, const double* __restrict__ pc, bool hasbc,
+                                              const GridDev& g, int i, int j)
+{
+    const int nx = g.nx, ny = g.ny;
+    const bool jn = j >= 2 && j <= ny;
+    const bool bb = j == 1, bt = j == ny + 1;
+    // j edge: bc - bc; i == 1: bc(LEFT + j) - ψ(1, j-1); i == nx: ψ(nx-1, j-1) - bc(RIGHT + j)
+    const int ka = bb ? g.BOT + i : bt ? g.TOP + i : g.LEFT + j;
+    const int kb = bb ? (g.BOT + 1) + i : bt ? (g.TOP + 1) + i : g.RIGHT + j;
+    const bool abc = !jn || i == 1;
+    const bool bbc = !jn || i == nx;
+    const int jc = min(max(j - 1, 1), ny - 1);
+    const int ia = i == nx ? nx - 1 : min(max(i - 1, 1), nx - 1);  // i == nx: ψ(nx-1, j-1) is operand a
+    const int ib = min(max(i, 1), nx - 1);                          // i == 1: ψ(1, j-1) is operand b
+    const double pa = psi[(ia - 1) + (long)g.n1 * (jc - 1)];
+    const double pb = psi[(ib - 1) + (long)g.n1 * (jc - 1)];
+    const double ca = get_bc_val_sel(pc, psi, hasbc, g, abc ? ka : g.LEFT + 1);
+    const double cb = get_bc_val_sel(pc, psi, hasbc, g, bbc ? kb : g.LEFT + 1);
+    return (abc ? ca : pa) - (bbc ? cb : pb);
 }
 
 // y-flux (i, j), i = 1..nx, j = 1..ny+1

@@ -93,6 +93,9 @@ struct YArgs {
     // the instance from ipar (0 / null: one problem)
     int levmod;
     const InstPar* ipar;
+    // rsel != 0: input rows (0-based lines) outside rlo..rhi are known to be
+    // zero; they are not read
+    int rsel, rlo, rhi;
     DstPlan pl;
     GridDev g;
 };

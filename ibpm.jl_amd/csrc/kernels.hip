@@ -511,12 +511,13 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
             }
         }
     }
-    // (2) the first chunk of the tile
+    // (2) the first chunk of the tile (rows outside rlo..rhi are zero when rsel)
+    const int rlo = a.rsel ? a.rlo : 0, rhi = a.rsel ? a.rhi : n2 - 1;
     double v[CH];
 #pragma unroll
     for (int t = 0; t < CH; t++) {
         const int r = trow(0, t);
-        v[t] = (colok && r < n2) ? base[ic + (long)n1 * r] : 0.0;
+        v[t] = (colok && r >= rlo && r <= rhi) ? base[ic + (long)n1 * r] : 0.0;
     }
     if constexpr (PT == 512) {
         tb = tab1024_store(lds, TR, tid);
@@ -567,7 +568,7 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
 #pragma unroll
             for (int t = 0; t < CH; t++) {
                 const int r = trow(rb, t);
-                v[t] = (colok && r < n2) ? base[ic + (long)n1 * r] : 0.0;
+                v[t] = (colok && r >= rlo && r <= rhi) ? base[ic + (long)n1 * r] : 0.0;
             }
         }
         if (bc) {
@@ -755,11 +756,27 @@ struct NLFromQ : NLLevel {
     __device__ __forceinline__ double qx(int ii, int jj) const { return q[(ii - 1) + (long)(g->nx + 1) * (jj - 1)]; }
     __device__ __forceinline__ double qy(int ii, int jj) const { return q[g->nu + (ii - 1) + (long)g->nx * (jj - 1)]; }
 };
-struct NLFromPsi : NLLevel {
+// fluxes from ψ and Γ with its ring, every value by selection rather than
+// branches (curl_qx_sel, curl_qy_sel, get_bc_val_sel), so that all loads of
+// an edge strip are in flight together; the values are the branching forms'
+struct NLFromPsiSel : NLLevel {
     const double* __restrict__ ps;   // ψ of the level
     const double* __restrict__ psc;  // ψ of the next coarser level (hasbc)
-    __device__ __forceinline__ double qx(int ii, int jj) const { return curl_qx(ps, psc, hasbc, *g, ii, jj); }
-    __device__ __forceinline__ double qy(int ii, int jj) const { return curl_qy(ps, psc, hasbc, *g, ii, jj); }
+    __device__ __forceinline__ double qx(int ii, int jj) const { return curl_qx_sel(ps, psc, hasbc, *g, ii, jj); }
+    __device__ __forceinline__ double qy(int ii, int jj) const { return curl_qy_sel(ps, psc, hasbc, *g, ii, jj); }
+    __device__ __forceinline__ double gam(int ii, int jj) const
+    {
+        const GridDev& gg = *g;
+        const bool ring = jj == 0 || jj == gg.ny || ii == 0 || ii == gg.nx;
+        const int k = jj == 0 ? gg.BOT + ii + 1
+                    : jj == gg.ny ? gg.TOP + ii + 1
+                    : ii == 0 ? gg.LEFT + jj + 1
+                    : ii == gg.nx ? gg.RIGHT + jj + 1 : gg.LEFT + 1;
+        const int ic = min(max(ii, 1), gg.nx - 1), jc = min(max(jj, 1), gg.ny - 1);
+        const double vin = G[(ic - 1) + (long)gg.n1 * (jc - 1)];
+        const double vr = get_bc_val_sel(Gc, G, hasbc, gg, k) * 0.25;
+        return ring ? vr : vin;
+    }
 };
 // One strip: output points (i-1, j-1) for j = j0..j1.  UNROLL: the row loop
 // unrolled (the common strips) or rolled (edge strips, whose sources carry
@@ -829,16 +846,18 @@ __device__ __forceinline__ void nl_strip(const NLArgs& a, const SRC& S, const do
         }
         gn[1] = S.gam(i - 1, j);
         if constexpr (LIN) gbn[1] = S.gb(i - 1, j);
-        if (j < ny) {
-            gn[0] = S.gam(i - 2, j);
-            gn[2] = S.gam(i, j);
+        // row ny is the top ring: only X(i-1, ny) is read (the others are
+        // evaluated, valid ring entries, and replaced by zero)
+        {
+            const bool top = j >= ny;
+            const double a0 = S.gam(i - 2, j), a2 = S.gam(i, j);
+            gn[0] = top ? 0.0 : a0;
+            gn[2] = top ? 0.0 : a2;
             if constexpr (LIN) {
-                gbn[0] = S.gb(i - 2, j);
-                gbn[2] = S.gb(i, j);
+                const double b0 = S.gb(i - 2, j), b2 = S.gb(i, j);
+                gbn[0] = top ? 0.0 : b0;
+                gbn[2] = top ? 0.0 : b2;
             }
-        } else {
-            gn[0] = gn[2] = 0.0;  // row ny is the top ring: only X(i-1, ny) is read
-            if constexpr (LIN) gbn[0] = gbn[2] = 0.0;
         }
         const double qyn = QY(i, j + 1);
         // direct_product_loops!: fqx(i, j) = Qy(i, j+1) Γ(i-1, j) + Qy(i, j) Γ(i-1, j-1);
@@ -941,83 +960,99 @@ __device__ __forceinline__ void nl_strip_psi_inner(const double* __restrict__ ps
     }
 }
 
-// Every flux and Γ value a strip reads is interior (x-fluxes (i-1..i+1,
-// j0-1..j1), y-fluxes (i-1..i, j0-1..j1+1), Γ (i-2..i, j0-2..j1)): true but
-// for the strips of the first and last strip rows and of the columns i = 2, nx.
-__device__ __forceinline__ bool nl_strip_interior(const GridDev& g, int i, int j0, int j1)
-{
-    return i >= 3 && i <= g.nx - 1 && j0 >= 3 && j1 <= g.ny - 1;
-}
+// The fluxes from ψ: k_nonlinear_psi takes the output points (i-1, j-1) with
+// 3 <= i <= nx-1 and 3 <= j <= ny-1 in strips (every flux and Γ value they
+// read is interior), k_nonlinear_frame the ring around them: the rows j = 2
+// and ny (i = 2..nx), then the columns i = 2 and nx (j = 3..ny-1).
+__host__ __device__ inline int nl_frame_points(int nx, int ny) { return 2 * (nx - 1) + 2 * (ny - 3); }
 
-// The output points of the strips nl_strip_interior rejects, fluxes from ψ
-// through curl_qx / curl_qy with the boundary values.  One thread per point
-// (a strip of one row: the values a strip carries from row to row are formed
-// again, by the same expressions), so that the edge work does not run as a
-// few long serial strips: thread k of entry b = blockIdx.y takes the points
-// of the first strip row, then of the last, then the columns i = 2 and nx of
-// the rows between.
-struct NLFrame {
-    int ni, r0, rl0, rl, nm;  // i count; rows of the first strip row; first row and count of the last; rows between
+// The values one output point (i-1, j-1) of nonlinear! reads, as a strip of
+// one row (nl_strip with j0 = j1 = j): x-fluxes (i-1+d, j-1+e) at d*2+e,
+// y-fluxes (i-1+d, j-1+e) at 6+d*3+e, Γ with its ring (i-2+d, j-2+e) at 12+d*3+e.
+constexpr int NLF_V = 21;
+struct NLFromLds {
+    const GridDev* g;
+    const double* v;  // the point's NLF_V values
+    int i, j;
+    __device__ __forceinline__ double qx(int ii, int jj) const { return v[(ii - i + 1) * 2 + (jj - j + 1)]; }
+    __device__ __forceinline__ double qy(int ii, int jj) const { return v[6 + (ii - i + 1) * 3 + (jj - j + 1)]; }
+    __device__ __forceinline__ double gam(int ii, int jj) const { return v[12 + (ii - i + 2) * 3 + (jj - j + 2)]; }
+    __device__ __forceinline__ double gb(int, int) const { return 0.0; }
 };
-__host__ __device__ inline NLFrame nl_frame(int nx, int ny)
-{
-    NLFrame f;
-    const int ns = (ny - 1 + NL_JS - 1) / NL_JS;  // strip rows: j0 = 2 + NL_JS s
-    f.ni = nx - 1;
-    f.r0 = ns > 1 ? NL_JS : ny - 1;
-    f.rl0 = 2 + NL_JS * (ns - 1);
-    f.rl = ns > 1 ? ny - f.rl0 + 1 : 0;
-    f.nm = ns > 2 ? NL_JS * (ns - 2) : 0;
-    return f;
-}
 
+// One work-group per 64 consecutive ring points of one entry (blockIdx.y).
+// Phase 1: wavefront w forms value kinds w, w+4, .. of all 64 points (each
+// kind the same curl_qx / curl_qy / ring expression across the wavefront)
+// into LDS; phase 2: thread p forms point p from them with nl_strip's
+// arithmetic.  Every value is the one the strip kernels form, so the
+// result is theirs bit for bit.
 __global__ void __launch_bounds__(256) k_nonlinear_frame(NLArgs a)
 {
+    __shared__ double sv[64][NLF_V + 1];
     const GridDev& g = a.g;
     const int nx = g.nx, ny = g.ny;
-    const NLFrame f = nl_frame(nx, ny);
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
-    int i, j;
-    if (k < f.r0 * f.ni) {
-        i = 2 + k % f.ni;
-        j = 2 + k / f.ni;
-    } else if (k < (f.r0 + f.rl) * f.ni) {
-        const int c = k - f.r0 * f.ni;
-        i = 2 + c % f.ni;
-        j = f.rl0 + c / f.ni;
-    } else {
-        const int c = k - (f.r0 + f.rl) * f.ni;
-        if (c >= 2 * f.nm) return;
-        i = (c & 1) ? nx : 2;
-        j = 2 + NL_JS + (c >> 1);
-    }
-    const int s = (j - 2) / NL_JS, j0 = 2 + NL_JS * s;
-    if (nl_strip_interior(g, i, j0, min(j0 + NL_JS - 1, ny))) return;
     const int e = a.lev0 + b;
     const Entry en = batch_entry(b, a.lev0, a.levmod, 0, a.mg, a.ipar);
     if (!en.active) return;
-    const int lev = en.lev;
-    const double hcl = level_hc(g.h, lev);
+    const int nf = nl_frame_points(nx, ny);
+    auto point = [&](int k, int& i, int& j) -> bool {
+        if (k >= nf) return false;
+        if (k < 2 * (nx - 1)) {
+            i = 2 + k % (nx - 1);
+            j = k < nx - 1 ? 2 : ny;
+        } else {
+            const int c = k - 2 * (nx - 1);
+            i = (c & 1) ? nx : 2;
+            j = 3 + (c >> 1);
+        }
+        return true;
+    };
+    NLLevel lv;
+    lv.g = &g;
+    lv.G = a.G + (long)e * g.nG;
+    lv.Gc = a.G + (long)(e + 1) * g.nG;
+    lv.GBl = nullptr;
+    lv.hasbc = en.lev < en.L - 1;
+    const double* __restrict__ ps = a.psi + (long)e * g.nG;
+    const double* __restrict__ psc = a.psi + (long)(e + 1) * g.nG;
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    int i = 2, j = 2;
+    const bool ok = point(blockIdx.x * WAVE + lane, i, j);
+    // wave 0: the six x-fluxes, wave 1: the six y-fluxes, waves 2 and 3: the
+    // nine Γ values; each wave's values without branches, so that their loads
+    // are in flight together (points past the ring evaluate the point (2, 2))
+    if (w == 0) {
+#pragma unroll
+        for (int kind = 0; kind < 6; kind++)
+            sv[lane][kind] = curl_qx_sel(ps, psc, lv.hasbc, g, i - 1 + kind / 2, j - 1 + kind % 2);
+    } else if (w == 1) {
+#pragma unroll
+        for (int kind = 6; kind < 12; kind++)
+            sv[lane][kind] = curl_qy_sel(ps, psc, lv.hasbc, g, i - 1 + (kind - 6) / 3, j - 1 + (kind - 6) % 3);
+    } else {
+        NLFromPsiSel S;
+        static_cast<NLLevel&>(S) = lv;
+#pragma unroll
+        for (int t = 0; t < 5; t++) {
+            const int kind = 12 + (w - 2) + 2 * t;  // wave 2: 12, 14, .., 20; wave 3: 13, .., 19
+            if (kind < NLF_V) sv[lane][kind] = S.gam(i - 2 + (kind - 12) / 3, j - 2 + (kind - 12) % 3);
+        }
+    }
+    __syncthreads();
+    if (w != 0 || !ok) return;
+    const double hcl = level_hc(g.h, en.lev);
     double q0vx = a.q0vx, q0vy = a.q0vy;
     if (a.ipar) {
         q0vx = a.ipar[en.inst].q0vx;
         q0vy = a.ipar[en.inst].q0vy;
     }
-    NLFromPsi S;
-    S.g = &g;
-    S.G = a.G + (long)e * g.nG;
-    S.Gc = a.G + (long)(e + 1) * g.nG;
-    S.GBl = nullptr;
-    S.hasbc = lev < en.L - 1;
-    S.ps = a.psi + (long)e * g.nG;
-    S.psc = a.psi + (long)(e + 1) * g.nG;
+    NLFromLds S{&g, sv[lane], i, j};
     nl_strip<false, 1>(a, S, a.q0 + (long)e * g.nq, a.q0_uniform != 0, q0vx * hcl, q0vy * hcl, hcl, i, j, j, nullptr,
                        a.out + (long)b * g.nG);
 }
 
-// PSI: the fluxes from the state's ψ (NLArgs::psi) instead of q
-template <bool LIN, bool PSI>
+template <bool LIN>
 __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
 {
     const GridDev& g = a.g;
@@ -1054,32 +1089,54 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
     lv.hasbc = lev < en.L - 1;
     const double* __restrict__ QBl = LIN ? a.QB + (long)lev * g.nq : nullptr;
     double* __restrict__ out = a.out + (long)b * g.nG;
-    if constexpr (PSI) {
-        static_assert(!LIN, "the linearized product reads q");
-        // the interior strips; k_nonlinear_frame takes the others
-        if (nl_strip_interior(g, i, j0, j1))
-            nl_strip_psi_inner(a.psi + (long)e * g.nG, lv.G, q0, u0, cx0, cy0, hcl, i, j0, j1, g, out);
-    } else {
-        NLFromQ S;
-        static_cast<NLLevel&>(S) = lv;
-        S.q = a.q + (long)e * g.nq;
-        nl_strip<LIN, NL_JS>(a, S, q0, u0, cx0, cy0, hcl, i, j0, j1, QBl, out);
+    NLFromQ S;
+    static_cast<NLLevel&>(S) = lv;
+    S.q = a.q + (long)e * g.nq;
+    nl_strip<LIN, NL_JS>(a, S, q0, u0, cx0, cy0, hcl, i, j0, j1, QBl, out);
+}
+
+// nonlinear! with the fluxes from ψ over the interior points: strips of
+// NL_JS rows from j = 3, i = 3..nx-1 (k_nonlinear_frame takes the ring)
+__global__ void __launch_bounds__(256) k_nonlinear_psi(NLArgs a)
+{
+    const GridDev& g = a.g;
+    const int nx = g.nx, ny = g.ny;
+    const int nbx = (nx - 3 + 63) / 64;
+    const int nbs = ((ny - 3 + NL_JS - 1) / NL_JS + 3) / 4;
+    const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
+    const int b = lin / (nbx * nbs);
+    if (b >= a.nlev) return;
+    const int rem = lin - b * (nbx * nbs);
+    const int i = (rem % nbx) * 64 + threadIdx.x + 3;                    // 3..nx-1
+    const int j0 = ((rem / nbx) * 4 + threadIdx.y) * NL_JS + 3;          // first output row, 3..ny-1
+    if (i > nx - 1 || j0 > ny - 1) return;
+    const int j1 = min(j0 + NL_JS - 1, ny - 1);
+    const int e = a.lev0 + b;
+    const Entry en = batch_entry(b, a.lev0, a.levmod, 0, a.mg, a.ipar);
+    if (!en.active) return;
+    const double hcl = level_hc(g.h, en.lev);
+    double q0vx = a.q0vx, q0vy = a.q0vy;
+    if (a.ipar) {
+        q0vx = a.ipar[en.inst].q0vx;
+        q0vy = a.ipar[en.inst].q0vy;
     }
+    nl_strip_psi_inner(a.psi + (long)e * g.nG, a.G + (long)e * g.nG, a.q0 + (long)e * g.nq, a.q0_uniform != 0,
+                       q0vx * hcl, q0vy * hcl, hcl, i, j0, j1, g, a.out + (long)b * g.nG);
 }
 
 void launch_nonlinear(const NLArgs& a, hipStream_t s)
 {
     dim3 blk(64, 4);  // k_nonlinear derives its tile from this shape
     const int nblocks = ((a.g.nx - 1 + 63) / 64) * (((a.g.ny - 1 + NL_JS - 1) / NL_JS + 3) / 4) * a.nlev;
-    const dim3 grid(NXCD * xcd_nper(nblocks));
     if (a.psi && !a.QB) {
-        hipLaunchKernelGGL((k_nonlinear<false, true>), grid, blk, 0, s, a);
-        const NLFrame f = nl_frame(a.g.nx, a.g.ny);
-        const int nframe = (f.r0 + f.rl) * f.ni + 2 * f.nm;
-        hipLaunchKernelGGL(k_nonlinear_frame, dim3((nframe + 255) / 256, a.nlev), dim3(256), 0, s, a);
+        const int nb = ((a.g.nx - 3 + 63) / 64) * (((a.g.ny - 3 + NL_JS - 1) / NL_JS + 3) / 4) * a.nlev;
+        hipLaunchKernelGGL(k_nonlinear_psi, dim3(NXCD * xcd_nper(nb)), blk, 0, s, a);
+        const int nframe = nl_frame_points(a.g.nx, a.g.ny);
+        hipLaunchKernelGGL(k_nonlinear_frame, dim3((nframe + WAVE - 1) / WAVE, a.nlev), dim3(256), 0, s, a);
     } else {
-        if (a.QB) hipLaunchKernelGGL((k_nonlinear<true, false>), grid, blk, 0, s, a);
-        else hipLaunchKernelGGL((k_nonlinear<false, false>), grid, blk, 0, s, a);
+        const dim3 grid(NXCD * xcd_nper(nblocks));
+        if (a.QB) hipLaunchKernelGGL(k_nonlinear<true>, grid, blk, 0, s, a);
+        else hipLaunchKernelGGL(k_nonlinear<false>, grid, blk, 0, s, a);
     }
 }
 
