@@ -315,9 +315,9 @@ def main():
     if done != args.steps:
         raise RuntimeError(f"non-finite forces after {done} steps")
     # the roofline kernels are timed inside further, untimed steps (their event
-    # pairs would slow the timed ones): every one-level y-pass with boundary
-    # terms of the vort2flux! chains (profile kind 2), then every all-level
-    # y-pass (kind 1), bracketed by HIP events on the problem's stream
+    # pairs would slow the timed ones): the one-level y-passes with boundary
+    # terms of the closing vort2flux! chain (profile kind 2), then every
+    # all-level y-pass (kind 1), bracketed by HIP events on the problem's stream
     n_extra = min(args.steps, 100)
     t_next = args.warmup + args.steps
     prof = {}
@@ -346,8 +346,9 @@ def main():
     ypass_ms = y_ms
     roof = {"bound": "hbm",
             "kernel": f"k_ypass<{tuned['ytile_single']}, BC> (one-level Δinv y-pass with the ψ boundary terms; "
-                      f"{2 * (L - 1)} per step, the vort2flux! chains' levels below the coarsest; "
-                      f"the largest share of a step's kernel time)",
+                      f"{2 * (L - 1)} per step, the vort2flux! chains' levels below the coarsest, the largest "
+                      f"share of a step's kernel time; timed: the {L - 1} of the step's closing vort2flux!, which "
+                      f"read and write the whole level (the trial chain's store only the rows its next pass reads))",
             "achieved": round(bc_bytes / (ypass_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(bc_bytes / (ypass_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic("k_ypass_bc", tuned["ytile_single"], 1),

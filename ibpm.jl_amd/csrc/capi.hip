@@ -565,17 +565,24 @@ static void vort2flux_dev(ibpm_prob* P, double* Gb, double* psib, double* qout, 
     XArgs a = xargs(P);
     a.src = Gb; a.dst = P->T1; a.nlev = L; a.lev0 = 0;
     launch_xpass(X_PLAIN, a, s);
+    const bool trial = curl == CURL_TRIAL && P->tlo >= 0;
     for (int l = L - 1; l >= 0; l--) {
         YArgs y = yargs(P, Y_DINV, l, 1, P->T1 + (long)l * nG);
         if (l < L - 1) {
             y.bcr = psib + (long)(l + 1) * nG;
             y.edge = P->edge;
         }
-        timed_ypass(P, y, P->ytile_single, l < L - 1 ? 2 : -1);
+        if (trial) {  // the rows the trial's x-pass of this level reads (below)
+            y.wsel = 1;
+            y.wlo = l > 0 ? P->g.ny / 4 - 1 : P->tlo;
+            y.whi = l > 0 ? 3 * P->g.ny / 4 - 1 : P->thi;
+        }
+        // in-step timing (kind 2) of the launches that read and write the whole level
+        timed_ypass(P, y, P->ytile_single, (l < L - 1 && !trial) ? 2 : -1);
         XArgs c = xargs(P);
         c.src = P->T1 + (long)l * nG; c.dst = psib + (long)l * nG; c.nlev = 1; c.lev0 = l;
         if (l > 0) c.edge = P->edge;
-        if (curl == CURL_TRIAL && P->tlo >= 0) {
+        if (trial) {
             c.lsel = 1;
             if (l > 0) {
                 c.llo = c.keep0 = P->g.ny / 4 - 1;

@@ -94,8 +94,10 @@ struct YArgs {
     int levmod;
     const InstPar* ipar;
     // rsel != 0: input rows (0-based lines) outside rlo..rhi are known to be
-    // zero; they are not read
+    // zero; they are not read.  wsel != 0: only output rows wlo..whi are
+    // stored (the others are not read after the pass)
     int rsel, rlo, rhi;
+    int wsel, wlo, whi;
     DstPlan pl;
     GridDev g;
 };

@@ -649,6 +649,7 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     }
     __syncthreads();
     if (colok) {
+        const int wlo = a.wsel ? a.wlo : 0, whi = a.wsel ? a.whi : n2 - 1;
         for (int rb = 0; rb < n2; rb += RPI * CH) {
             double v[CH];
 #pragma unroll
@@ -659,7 +660,7 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
 #pragma unroll
             for (int t = 0; t < CH; t++) {
                 int r = rb + r0 + RPI * t;
-                if (r < n2) base[ic + (long)n1 * r] = v[t];
+                if (r >= wlo && r <= whi) base[ic + (long)n1 * r] = v[t];
             }
         }
     }
