@@ -35,7 +35,7 @@ import numpy as np  # noqa: E402
 METRIC = "timesteps/s, 5-level 1024² grid: FP64 vs mixed @CD err≤1e-6; % HBM peak"
 HBM_PEAK_GBS = 8000.0
 # launch-shape options of the autotuner (ibpm_set_option ids) by name
-TUNED_NAMES = {0: "ytile_batch", 1: "ytile_single", 4: "xwaves_batch"}
+TUNED_NAMES = {0: "ytile_batch", 1: "ytile_single", 4: "xwaves_batch", 7: "pipe"}
 
 
 def algorithmic_bytes_per_step(nx, ny, mg, nb):

@@ -121,6 +121,7 @@ struct ibpm_prob {
     double *QB = nullptr, *GB = nullptr;
     int ytile = 4;         // y-pass columns per workgroup, all-level launches (and batches of fields)
     int ytile_single = 4;  // same, one-level launches (vort2flux chain)
+    int pipe = 1;          // IBPM_OPT_PIPE
     int xwaves = 4;        // lines per work-group of the plain x-passes over all levels (or a batch)
 
     // device state (IBState) and work arrays
@@ -405,6 +406,7 @@ static YArgs yargs(ibpm_prob* P, int op, int lev0, int nlev, double* buf)
     a.bcfac = 1.0;
     a.pl = P->ty.plan;
     a.g = P->g;
+    a.pipe = P->pipe;
     return a;
 }
 
@@ -501,8 +503,10 @@ static hipError_t upload_reg(ibpm_prob* P)
 // x = L b for one level (lev 0-based), L one of Δinv / A / Ainv (laplacian.jl:24-76),
 // with an optional subtract epilogue.  tmp: the level's transform scratch
 // (nG doubles, every one written by the first pass), by default T1's level.
+// xmode X_ROT: b is a level-1 flux field and the operand is rot!(b), formed in
+// the first x-pass
 static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x, bool sub, int zlo = 1,
-                      int zhi = 1 << 30, double* tmp = nullptr)
+                      int zhi = 1 << 30, double* tmp = nullptr, int xmode = X_PLAIN)
 {
     hipStream_t s = P->stream;
     if (!tmp) tmp = P->T1 + (long)lev * P->g.nG;
@@ -520,7 +524,7 @@ static void lap_level(ibpm_prob* P, int op, int lev, const double* b, double* x,
         a.llo = y.rlo = zlo - 1;
         a.lhi = y.rhi = zhi - 1;
     }
-    launch_xpass(X_PLAIN, a, s);
+    launch_xpass(xmode, a, s);
     launch_ypass(y, P->ytile_single, s);
     XArgs c = xargs(P);
     c.src = tmp;
@@ -548,14 +552,21 @@ static void vort2flux_dev(ibpm_prob* P, double* Gb, double* psib, double* qout, 
     hipStream_t s = P->stream;
     const long nG = P->g.nG;
     const int L = P->mg;
-    for (int lev = 2; lev <= L; lev++) {
+    // coarsify!, two level transitions per launch
+    for (int lev = 2; lev <= L;) {
         CoarsifyArgs c{};
         const CoarsifyRange& r = P->crange[0];
         c.Gc = Gb + (long)(lev - 1) * nG;
         c.G = Gb + (long)(lev - 2) * nG;
         c.i0 = r.i0; c.ic0 = r.ic0; c.ci = r.ci; c.j0 = r.j0; c.jc0 = r.jc0; c.cj = r.cj;
         c.g = P->g;
-        launch_coarsify(c, s);
+        if (lev + 1 <= L) {
+            launch_coarsify2(c, Gb + (long)lev * nG, s);
+            lev += 2;
+        } else {
+            launch_coarsify(c, s);
+            lev += 1;
+        }
     }
     // Δinv of every level, coarse to fine.  The x-transform of Γ does not
     // depend on the chain, so all levels take it in one launch; the boundary
@@ -643,9 +654,9 @@ static void project_forces(ibpm_prob* P, const double* f, double* Gout, bool sub
     sa.cell = P->sc_cell; sa.off = P->sc_off; sa.ent_f = P->sc_f; sa.ent_w = P->sc_w;
     sa.fb = f; sa.q = P->qreg; sa.ncell = P->sc_ncell;
     launch_scatter(sa, s);
-    launch_rot(P->qreg, P->Gtmp, P->rb_i0, P->rb_i1, P->rb_j0, P->rb_j1, P->g, s);
-    // Gtmp is zero outside the rows rb_j0 - 1 .. rb_j1 - 1 of the rot! box
-    lap_level(P, Y_AINV, 0, P->Gtmp, Gout, sub, P->rb_j0 - 1, P->rb_j1 - 1);
+    // rot!(qreg) is formed in Ainv's first x-pass; it is zero outside the rows
+    // rb_j0 - 1 .. rb_j1 - 1 of the rot! box (qreg is zero outside the E' support)
+    lap_level(P, Y_AINV, 0, P->qreg, Gout, sub, P->rb_j0 - 1, P->rb_j1 - 1, nullptr, X_ROT);
 }
 
 // A y-pass of the step with `tile` columns per work-group, bracketed by a pair
@@ -2459,7 +2470,7 @@ struct ibpm_group {
     double* hist = nullptr;  // = hb.d
     size_t hist_steps = 0;
     int pending = 0;
-    int ytile = 4, ytile_single = 4, xwaves = 4;
+    int ytile = 4, ytile_single = 4, xwaves = 4, pipe = 1;
     // HIP graphs of one group step, one per buffer role (as ibpm_prob's);
     // every launch argument is fixed by the roles (the force arguments and
     // history pointers are read from device memory when the kernels run)
@@ -2520,6 +2531,7 @@ YArgs gyargs(const ibpm_group* Gp, int op, double* buf)
     a.ipar = Gp->ipar;
     a.pl = Gp->ty.plan;
     a.g = Gp->g;
+    a.pipe = Gp->pipe;
     return a;
 }
 
@@ -2785,7 +2797,7 @@ int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* probs, ibpm_group*
     for (int i = 0; i < n; i++) Gp->L = std::max(Gp->L, probs[i]->mg);
     Gp->g = P0->g;
     Gp->tx = P0->tx; Gp->ty = P0->ty; Gp->cx = P0->cx; Gp->cy = P0->cy; Gp->crange = P0->crange;
-    Gp->ytile = P0->ytile; Gp->ytile_single = P0->ytile_single; Gp->xwaves = P0->xwaves;
+    Gp->ytile = P0->ytile; Gp->ytile_single = P0->ytile_single; Gp->xwaves = P0->xwaves; Gp->pipe = P0->pipe;
     Gp->probs.assign(probs, probs + n);
     auto bail = [&](int code, const std::string& m) {
         for (void* p : Gp->allocs) (void)hipFree(p);
@@ -2969,6 +2981,7 @@ int ibpm_group_set_option(ibpm_group* Gp, int option, int value)
     else if (option == IBPM_OPT_YTILE_SINGLE) Gp->ytile_single = value;
     else if (option == IBPM_OPT_XWAVES_BATCH) Gp->xwaves = value;
     else if (option == IBPM_OPT_GRAPHS) Gp->graphs = value == 1;
+    else if (option == IBPM_OPT_PIPE) Gp->pipe = value;
     return IBPM_OK;
 }
 
@@ -3337,6 +3350,11 @@ int ibpm_set_option(ibpm_prob* P, int option, int value)
         P->graphs = value == 1;
         return IBPM_OK;
     }
+    if (option == IBPM_OPT_PIPE) {
+        if (value != 0 && value != 1) return fail(P->ctx, IBPM_E_ARG, "pipe option is 0 or 1");
+        P->pipe = value;
+        return IBPM_OK;
+    }
     if (option == IBPM_OPT_PROFILE) {
         if (value < 0 || value > 2) return fail(P->ctx, IBPM_E_ARG, "profile option is 0, 1 or 2");
         HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
@@ -3357,6 +3375,7 @@ int ibpm_get_option(ibpm_prob* P, int option, int* value)
     else if (option == IBPM_OPT_PROFILE) *value = P->prof;
     else if (option == IBPM_OPT_GRAPHS) *value = P->graphs ? 1 : 0;
     else if (option == IBPM_OPT_XWAVES_BATCH) *value = P->xwaves;
+    else if (option == IBPM_OPT_PIPE) *value = P->pipe;
     else if (option == IBPM_OPT_PRECISION) *value = P->prec < 0 ? 0 : (P->prec == MX_FP32 ? 1 : 2);
     else if (option == IBPM_OPT_MX_SWEEPS) *value = P->mx_sweeps;
     else return fail(P->ctx, IBPM_E_ARG, "unknown option");

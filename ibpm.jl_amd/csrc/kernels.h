@@ -19,7 +19,9 @@ struct InstPar {
 };
 
 // ---- DST passes -----------------------------------------------------------
-enum XMode { X_PLAIN = 0, X_SUB = 2, X_TRIAL = 3 };
+// X_ROT: the input line is rot! (curl.jl:22-34) of the level-1 flux field src
+// (the E' output of project_circ!), formed in the pass
+enum XMode { X_PLAIN = 0, X_ROT = 1, X_SUB = 2, X_TRIAL = 3 };
 
 struct XArgs {
     const double* src;
@@ -98,6 +100,9 @@ struct YArgs {
     // stored (the others are not read after the pass)
     int rsel, rlo, rhi;
     int wsel, wlo, whi;
+    // pipe != 0: a launch of many tiles (no boundary terms, 4-column tiles)
+    // runs as persistent work-groups that prefetch their next tile
+    int pipe;
     DstPlan pl;
     GridDev g;
 };
@@ -162,6 +167,8 @@ struct CoarsifyArgs {
     GridDev g;
 };
 void launch_coarsify(const CoarsifyArgs& a, hipStream_t s);
+// one problem (nb <= 1): a.Gc <- coarsify!(a.G) and Gc2 <- coarsify!(a.Gc) in one launch
+void launch_coarsify2(const CoarsifyArgs& a, double* Gc2, hipStream_t s);
 
 // rot! over the rectangle [ib0, ib1] x [jb0, jb1] of 1-based Γ indices (i, j
 // here are the flux-grid indices 2..nx, 2..ny of rot!, curl.jl:26)

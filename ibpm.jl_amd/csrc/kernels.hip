@@ -120,10 +120,29 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
     // every global load of the line is in flight before the first wait
     const double* src = a.src + b * a.lstride + (long)line * n1;
     double v[XPL];
+    if constexpr (MODE == X_ROT) {
+        // Γ(m, j), m = n + 1: rot! at flux-grid (i, jj) = (m + 1, j + 1) of the
+        // x-fluxes (i, jj-1), (i, jj) and y-fluxes (i, jj), (i-1, jj), as k_rot forms it
+        const double* q = a.src + b * a.lstride;
+        const int nx = a.g.nx;
+        const double* qx = q + (long)(nx + 1) * (j - 1);  // x-fluxes of row jj - 1 = j
+        const double* qy = q + a.g.nu + (long)nx * j;     // y-fluxes of row jj = j + 1
 #pragma unroll
-    for (int t = 0; t < XPL; t++) {
-        int n = lane + WAVE * t;
-        if (n < n1) v[t] = src[n];
+        for (int t = 0; t < XPL; t++) {
+            const int n = lane + WAVE * t;
+            if (n < n1) {
+                const int i = n + 2;
+                const double qa = qx[i - 1], qb = qx[(i - 1) + (nx + 1)];
+                const double ya = qy[i - 1], yb = qy[i - 2];
+                v[t] = (qa - qb) + (ya - yb);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < XPL; t++) {
+            int n = lane + WAVE * t;
+            if (n < n1) v[t] = src[n];
+        }
     }
     // X_TRIAL: the rows j - 1 and j + 1 of Γ (zero outside), N1 and N0 of the
     // line, in flight with the row's own loads
@@ -304,6 +323,7 @@ void launch_xpass(int mode, const XArgs& a, hipStream_t s)
 {
     switch (mode) {
     case X_PLAIN: launch_x_m<X_PLAIN>(a, s); break;
+    case X_ROT: launch_x_m<X_ROT>(a, s); break;
     case X_SUB: launch_x_m<X_SUB>(a, s); break;
     default: launch_x_m<X_TRIAL>(a, s); break;
     }
@@ -706,11 +726,177 @@ static void launch_y_m(const YArgs& a, hipStream_t s)
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// The fused y-pass (no boundary terms) over many column tiles, as persistent
+// work-groups: a work-group loads the tables once and walks a list of tiles,
+// the global loads of its next tile in flight while the current tile is
+// transformed, and the stores of a tile draining while the next one is.
+// The arithmetic of a tile is k_ypass's (4 columns, a wavefront each).
+// Work-group bid runs on XCD bid % NXCD; the XCDs take contiguous ranges of
+// tiles and a work-group every wpx-th tile of its XCD's range, so that the
+// work-groups of one XCD work on neighbouring tiles at the same time.
+// ---------------------------------------------------------------------------
+template <int PT>
+__global__ void __launch_bounds__(4 * WAVE) k_ypass_pipe(YArgs a)
+{
+    constexpr int TI = 4, YW = 4, NT = YW * WAVE, RPI = NT / TI, CH = 16;
+    static_assert(RPI * CH >= MAXM - 1, "one chunk of loads covers a column");
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int w = tid / WAVE;
+    const int n1 = a.g.n1, n2 = line_len<PT>(a.g.n2);
+    const int ntiles = (n1 + TI - 1) / TI;
+    const int total = ntiles * a.nlev;
+    const int wpx = gridDim.x / NXCD;
+    const int xcd = blockIdx.x % NXCD, li = blockIdx.x / NXCD;
+    const int tpx = (total + NXCD - 1) / NXCD;
+    const int t_end = min(total, (xcd + 1) * tpx);
+    int t = xcd * tpx + li;
+    if (t >= t_end) return;  // the whole work-group, before any barrier
+    const int ld = slot_doubles_t<PT>(a.pl);
+    const int ntab = tab_doubles_t<PT>(a.pl);
+    const int c0 = tid % TI, r0 = tid / TI;
+    // tables (once per work-group)
+    constexpr int CYN = (f1024::M - 1 + NT - 1) / NT;
+    DstTab tb{};
+    double* cyl = lds + ntab;
+    if constexpr (PT == 512) {
+        Tab1024Regs<NT> TR;
+        double cyr[CYN];
+        tab1024_issue(TR, a.pl, tid);
+#pragma unroll
+        for (int q = 0; q < CYN; q++) cyr[q] = a.cy[min(tid + NT * q, n2 - 1)];
+        tb = tab1024_store(lds, TR, tid);
+#pragma unroll
+        for (int q = 0; q < CYN; q++) {
+            const int k = tid + NT * q;
+            if (k < n2) cyl[f1024::gsl(k + 1)] = cyr[q];
+        }
+        if (tid == 0) cyl[f1024::gsl(0)] = 1.0;
+    } else {
+        tb = load_tab_t<PT, NT>(lds, a.pl, tid);
+        for (int k = tid; k < n2; k += NT) cyl[k] = a.cy[k];
+    }
+    double* tile = cyl + cy_doubles_t<PT>(n2);
+    if (tid < TI) set_guards<PT>(tile + tid * ld);
+    // tile t: entry b = t / ntiles, columns i0 .. i0 + TI - 1
+    auto active = [&](int tt) -> bool {
+        const int b = tt / ntiles;
+        return batch_entry(b, a.lev0, a.levmod, a.fixlev, 1 << 30, a.ipar).active;
+    };
+    auto load = [&](int tt, double* v) {
+        const int b = tt / ntiles, ic = (tt - b * ntiles) * TI + c0;
+        const double* base = a.buf + b * a.lstride;
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            const int r = r0 + RPI * k;
+            v[k] = (ic < n1 && r < n2) ? base[ic + (long)n1 * r] : 0.0;
+        }
+    };
+    double v[CH];
+    bool act = active(t);
+    if (act) load(t, v);
+    for (;;) {
+        // the current tile into LDS
+        if (act) {
+#pragma unroll
+            for (int k = 0; k < CH; k++) {
+                const int r = r0 + RPI * k;
+                if (r < n2) tile[c0 * ld + lat<PT>(r)] = v[k];
+            }
+        }
+        __syncthreads();
+        // the next tile's loads in flight during this tile's transforms
+        const int tn = t + wpx;
+        const bool has_next = tn < t_end;
+        const bool actn = has_next && active(tn);
+        if (actn) load(tn, v);
+        if (act) {
+            const int b = t / ntiles, i0 = (t - b * ntiles) * TI;
+            const Entry en = batch_entry(b, a.lev0, a.levmod, a.fixlev, 1 << 30, a.ipar);
+            const double hcl = level_hc(a.g.h, en.lev);
+            double dt = a.dt, Re = a.Re;
+            if (a.ipar) {
+                dt = a.ipar[en.inst].dt;
+                Re = a.ipar[en.inst].Re;
+            }
+            const double qc = dt / ((2.0 * Re) * (hcl * hcl));  // dt / (2 Re h^2), laplacian.jl:69,74
+            const int c = w;  // TI == YW: one column per wavefront
+            if (i0 + c < n1) {
+                const double cxk = a.cx[i0 + c];
+                double* s = tile + c * ld;
+#pragma unroll 1
+                for (int it = 0; it < 2; it++) {
+                    if constexpr (PT == 512) {
+                        dst1024_line(s, tb, lane, YOut1024{s, cyl, cxk, qc, a.scale, it == 0 ? a.op : -1});
+                    } else {
+                        dst_line<PT>(s, a.pl, tb, lane);
+                        if (it == 0) {
+                            for (int kj = lane; kj < n2; kj += WAVE) s[kj] *= eig_factor(cxk, cyl[kj], a.op, qc, a.scale);
+                            wave_sync();
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // this tile's stores (they drain during the next tile's transforms)
+        if (act) {
+            const int b = t / ntiles, ic = (t - b * ntiles) * TI + c0;
+            double* base = a.buf + b * a.lstride;
+            if (ic < n1) {
+                double o[CH];
+#pragma unroll
+                for (int k = 0; k < CH; k++) {
+                    const int r = r0 + RPI * k;
+                    if (r < n2) o[k] = tile[c0 * ld + lat<PT>(r)];
+                }
+#pragma unroll
+                for (int k = 0; k < CH; k++) {
+                    const int r = r0 + RPI * k;
+                    if (r < n2) base[ic + (long)n1 * r] = o[k];
+                }
+            }
+        }
+        if (!has_next) break;
+        __syncthreads();  // the slots are read before the next tile overwrites them
+        t = tn;
+        act = actn;
+    }
+}
+
+template <int PT>
+static void launch_y_pipe_t(const YArgs& a, int nwg, hipStream_t s)
+{
+    static bool once = (allow_big_lds(k_ypass_pipe<PT>), true);
+    (void)once;
+    const size_t lds = ypass_lds_bytes_t<PT>(a.pl, a.g.n2, 4);
+    hipLaunchKernelGGL((k_ypass_pipe<PT>), dim3(nwg), dim3(4 * WAVE), lds, s, a);
+}
+
 // tile 4: four waves, a column each; 8: four waves, two columns each; 12:
 // twelve waves, a column each (one copy of the tables for three times the
 // columns of tile 4)
 void launch_ypass(const YArgs& a, int tile, hipStream_t s)
 {
+    // pipelined persistent work-groups (tile option 4 with pipe): launches of
+    // more 4-column tiles than two work-groups per CU hold at once
+    const int ntiles4 = ((a.g.n1 + 3) / 4) * a.nlev;
+    if (tile == 4 && a.pipe && !a.bcr && ntiles4 > 2 * 256) {
+        const int nwg = NXCD * 64;  // two per CU of the 256
+        const int P = a.pl.kind == 0 ? a.pl.P : 0;
+        switch (P) {
+        case 512: launch_y_pipe_t<512>(a, nwg, s); return;
+        case 400: launch_y_pipe_t<400>(a, nwg, s); return;
+        case 256: launch_y_pipe_t<256>(a, nwg, s); return;
+        case 200: launch_y_pipe_t<200>(a, nwg, s); return;
+        case 128: launch_y_pipe_t<128>(a, nwg, s); return;
+        case 100: launch_y_pipe_t<100>(a, nwg, s); return;
+        default: launch_y_pipe_t<0>(a, nwg, s); return;
+        }
+    }
     if (tile == 4) launch_y_m<4, YWAVES>(a, s);
     else if (tile == 12) launch_y_m<12, 12>(a, s);
     else launch_y_m<8, YWAVES>(a, s);
@@ -1214,6 +1400,53 @@ void launch_coarsify(const CoarsifyArgs& a, hipStream_t s)
     if (a.ci <= 0 || a.cj <= 0) return;
     const int nblocks = ((a.ci + 63) / 64) * ((a.cj + 3) / 4) * (a.nb > 1 ? a.nb : 1);
     hipLaunchKernelGGL(k_coarsify, dim3(NXCD * xcd_nper(nblocks)), dim3(64, 4), 0, s, a);
+}
+
+// coarsify! of two level transitions in one launch: lev -> lev+1 (Gc1 from
+// G) and lev+1 -> lev+2 (Gc2 from Gc1).  Threads 0 .. ci cj - 1 write the
+// central block of Gc1; threads ci cj .. 2 ci cj - 1 the central block of
+// Gc2, forming each value of Gc1 they read inside its central block by the
+// same expression from G (so none reads what the first half writes).  Both
+// levels' values are k_coarsify's bit for bit.
+__device__ __forceinline__ double coarsify_pt(const double* __restrict__ G, int n1, int i, int j)
+{
+    auto F = [&](int ii, int jj) -> double { return G[(ii - 1) + (long)n1 * (jj - 1)]; };
+    double s1 = ((F(i + 1, j) + F(i, j + 1)) + F(i - 1, j)) + F(i, j - 1);
+    double s2 = ((F(i + 1, j + 1) + F(i + 1, j - 1)) + F(i - 1, j - 1)) + F(i - 1, j + 1);
+    return (F(i, j) + 0.5 * s1) + 0.25 * s2;
+}
+
+__global__ void __launch_bounds__(256) k_coarsify2(CoarsifyArgs a, double* Gc2)
+{
+    const int n1 = a.g.n1;
+    const long m = (long)a.ci * a.cj;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * m) return;
+    const bool second = t >= m;
+    const long u = second ? t - m : t;
+    const int ai = (int)(u % a.ci), bj = (int)(u / a.ci);
+    const int i = a.i0 + 2 * ai, ic = a.ic0 + ai;
+    const int j = a.j0 + 2 * bj, jc = a.jc0 + bj;
+    if (!second) {
+        a.Gc[(ic - 1) + (long)n1 * (jc - 1)] = coarsify_pt(a.G, n1, i, j);
+        return;
+    }
+    // Gc1 at (x, y): inside the central block the value this launch writes, else as stored
+    auto F1 = [&](int x, int y) -> double {
+        const int dx = x - a.ic0, dy = y - a.jc0;
+        if (dx >= 0 && dx < a.ci && dy >= 0 && dy < a.cj) return coarsify_pt(a.G, n1, a.i0 + 2 * dx, a.j0 + 2 * dy);
+        return a.Gc[(x - 1) + (long)n1 * (y - 1)];
+    };
+    double s1 = ((F1(i + 1, j) + F1(i, j + 1)) + F1(i - 1, j)) + F1(i, j - 1);
+    double s2 = ((F1(i + 1, j + 1) + F1(i + 1, j - 1)) + F1(i - 1, j - 1)) + F1(i - 1, j + 1);
+    Gc2[(ic - 1) + (long)n1 * (jc - 1)] = (F1(i, j) + 0.5 * s1) + 0.25 * s2;
+}
+
+void launch_coarsify2(const CoarsifyArgs& a, double* Gc2, hipStream_t s)
+{
+    if (a.ci <= 0 || a.cj <= 0) return;
+    const long nt = 2L * a.ci * a.cj;
+    hipLaunchKernelGGL(k_coarsify2, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a, Gc2);
 }
 
 // ===========================================================================

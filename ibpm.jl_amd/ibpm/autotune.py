@@ -18,6 +18,9 @@ SWEEPS = {
     C.OPT_YTILE_BATCH: (1, (4, 8, 12)),
     C.OPT_YTILE_SINGLE: (8, (4, 8, 12)),
     C.OPT_XWAVES_BATCH: (0, (4, 8, 12, 14)),
+    # persistent prefetching work-groups for the passes over many tiles
+    # (timed on the all-level y-pass, after its tile is chosen)
+    C.OPT_PIPE: (1, (1, 0)),
 }
 
 

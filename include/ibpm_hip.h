@@ -302,6 +302,9 @@ int ibpm_op_update_bodies(ibpm_prob* prob, const double* kin);
                                     bodies; ibpm_op_lapinv (kinds 0 and 2) follows the setting. */
 #define IBPM_OPT_MX_SWEEPS 6     /* sweeps of each mixed-precision solve: 1, or 2 (default): a second
                                     application of the transforms to the FP64 residual of the first */
+#define IBPM_OPT_PIPE 7          /* 1 (default): passes over more tiles than the chip holds at once run as
+                                    persistent work-groups that load their next tile while transforming the
+                                    current one; 0: one work-group per tile */
 int ibpm_set_option(ibpm_prob* prob, int option, int value);
 int ibpm_get_option(ibpm_prob* prob, int option, int* value);
 /* Average device ms and count of the y-pass launches timed inside the steps
