@@ -552,21 +552,14 @@ static void vort2flux_dev(ibpm_prob* P, double* Gb, double* psib, double* qout, 
     hipStream_t s = P->stream;
     const long nG = P->g.nG;
     const int L = P->mg;
-    // coarsify!, two level transitions per launch
-    for (int lev = 2; lev <= L;) {
+    for (int lev = 2; lev <= L; lev++) {
         CoarsifyArgs c{};
         const CoarsifyRange& r = P->crange[0];
         c.Gc = Gb + (long)(lev - 1) * nG;
         c.G = Gb + (long)(lev - 2) * nG;
         c.i0 = r.i0; c.ic0 = r.ic0; c.ci = r.ci; c.j0 = r.j0; c.jc0 = r.jc0; c.cj = r.cj;
         c.g = P->g;
-        if (lev + 1 <= L) {
-            launch_coarsify2(c, Gb + (long)lev * nG, s);
-            lev += 2;
-        } else {
-            launch_coarsify(c, s);
-            lev += 1;
-        }
+        launch_coarsify(c, s);
     }
     // Δinv of every level, coarse to fine.  The x-transform of Γ does not
     // depend on the chain, so all levels take it in one launch; the boundary

@@ -167,8 +167,6 @@ struct CoarsifyArgs {
     GridDev g;
 };
 void launch_coarsify(const CoarsifyArgs& a, hipStream_t s);
-// one problem (nb <= 1): a.Gc <- coarsify!(a.G) and Gc2 <- coarsify!(a.Gc) in one launch
-void launch_coarsify2(const CoarsifyArgs& a, double* Gc2, hipStream_t s);
 
 // rot! over the rectangle [ib0, ib1] x [jb0, jb1] of 1-based Γ indices (i, j
 // here are the flux-grid indices 2..nx, 2..ny of rot!, curl.jl:26)

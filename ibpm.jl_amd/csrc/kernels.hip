@@ -1402,53 +1402,6 @@ void launch_coarsify(const CoarsifyArgs& a, hipStream_t s)
     hipLaunchKernelGGL(k_coarsify, dim3(NXCD * xcd_nper(nblocks)), dim3(64, 4), 0, s, a);
 }
 
-// coarsify! of two level transitions in one launch: lev -> lev+1 (Gc1 from
-// G) and lev+1 -> lev+2 (Gc2 from Gc1).  Threads 0 .. ci cj - 1 write the
-// central block of Gc1; threads ci cj .. 2 ci cj - 1 the central block of
-// Gc2, forming each value of Gc1 they read inside its central block by the
-// same expression from G (so none reads what the first half writes).  Both
-// levels' values are k_coarsify's bit for bit.
-__device__ __forceinline__ double coarsify_pt(const double* __restrict__ G, int n1, int i, int j)
-{
-    auto F = [&](int ii, int jj) -> double { return G[(ii - 1) + (long)n1 * (jj - 1)]; };
-    double s1 = ((F(i + 1, j) + F(i, j + 1)) + F(i - 1, j)) + F(i, j - 1);
-    double s2 = ((F(i + 1, j + 1) + F(i + 1, j - 1)) + F(i - 1, j - 1)) + F(i - 1, j + 1);
-    return (F(i, j) + 0.5 * s1) + 0.25 * s2;
-}
-
-__global__ void __launch_bounds__(256) k_coarsify2(CoarsifyArgs a, double* Gc2)
-{
-    const int n1 = a.g.n1;
-    const long m = (long)a.ci * a.cj;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * m) return;
-    const bool second = t >= m;
-    const long u = second ? t - m : t;
-    const int ai = (int)(u % a.ci), bj = (int)(u / a.ci);
-    const int i = a.i0 + 2 * ai, ic = a.ic0 + ai;
-    const int j = a.j0 + 2 * bj, jc = a.jc0 + bj;
-    if (!second) {
-        a.Gc[(ic - 1) + (long)n1 * (jc - 1)] = coarsify_pt(a.G, n1, i, j);
-        return;
-    }
-    // Gc1 at (x, y): inside the central block the value this launch writes, else as stored
-    auto F1 = [&](int x, int y) -> double {
-        const int dx = x - a.ic0, dy = y - a.jc0;
-        if (dx >= 0 && dx < a.ci && dy >= 0 && dy < a.cj) return coarsify_pt(a.G, n1, a.i0 + 2 * dx, a.j0 + 2 * dy);
-        return a.Gc[(x - 1) + (long)n1 * (y - 1)];
-    };
-    double s1 = ((F1(i + 1, j) + F1(i, j + 1)) + F1(i - 1, j)) + F1(i, j - 1);
-    double s2 = ((F1(i + 1, j + 1) + F1(i + 1, j - 1)) + F1(i - 1, j - 1)) + F1(i - 1, j + 1);
-    Gc2[(ic - 1) + (long)n1 * (jc - 1)] = (F1(i, j) + 0.5 * s1) + 0.25 * s2;
-}
-
-void launch_coarsify2(const CoarsifyArgs& a, double* Gc2, hipStream_t s)
-{
-    if (a.ci <= 0 || a.cj <= 0) return;
-    const long nt = 2L * a.ci * a.cj;
-    hipLaunchKernelGGL(k_coarsify2, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a, Gc2);
-}
-
 // ===========================================================================
 // rot! (curl.jl:22-34) over a rectangle of output points
 // ===========================================================================
