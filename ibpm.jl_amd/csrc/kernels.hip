@@ -1149,7 +1149,7 @@ __device__ __forceinline__ void nl_strip_psi_inner(const double* __restrict__ ps
 
 // The fluxes from ψ: k_nonlinear_psi takes the output points (i-1, j-1) with
 // 3 <= i <= nx-1 and 3 <= j <= ny-1 in strips (every flux and Γ value they
-// read is interior), k_nonlinear_frame the ring around them: the rows j = 2
+// read is interior), nl_frame_block the ring around them: the rows j = 2
 // and ny (i = 2..nx), then the columns i = 2 and nx (j = 3..ny-1).
 __host__ __device__ inline int nl_frame_points(int nx, int ny) { return 2 * (nx - 1) + 2 * (ny - 3); }
 
@@ -1173,12 +1173,12 @@ struct NLFromLds {
 // into LDS; phase 2: thread p forms point p from them with nl_strip's
 // arithmetic.  Every value is the one the strip kernels form, so the
 // result is theirs bit for bit.
-__global__ void __launch_bounds__(256) k_nonlinear_frame(NLArgs a)
+// frame block fb of entry b, 256 threads (tid = lane + 64 wave)
+__device__ __forceinline__ void nl_frame_block(const NLArgs& a, int fb, int b, int tid)
 {
     __shared__ double sv[64][NLF_V + 1];
     const GridDev& g = a.g;
     const int nx = g.nx, ny = g.ny;
-    const int b = blockIdx.y;
     const int e = a.lev0 + b;
     const Entry en = batch_entry(b, a.lev0, a.levmod, 0, a.mg, a.ipar);
     if (!en.active) return;
@@ -1203,9 +1203,9 @@ __global__ void __launch_bounds__(256) k_nonlinear_frame(NLArgs a)
     lv.hasbc = en.lev < en.L - 1;
     const double* __restrict__ ps = a.psi + (long)e * g.nG;
     const double* __restrict__ psc = a.psi + (long)(e + 1) * g.nG;
-    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    const int lane = tid & (WAVE - 1), w = tid / WAVE;
     int i = 2, j = 2;
-    const bool ok = point(blockIdx.x * WAVE + lane, i, j);
+    const bool ok = point(fb * WAVE + lane, i, j);
     // wave 0: the six x-fluxes, wave 1: the six y-fluxes, waves 2 and 3: the
     // nine Γ values; each wave's values without branches, so that their loads
     // are in flight together (points past the ring evaluate the point (2, 2))
@@ -1282,15 +1282,21 @@ __global__ void __launch_bounds__(256) k_nonlinear(NLArgs a)
     nl_strip<LIN, NL_JS>(a, S, q0, u0, cx0, cy0, hcl, i, j0, j1, QBl, out);
 }
 
-// nonlinear! with the fluxes from ψ over the interior points: strips of
-// NL_JS rows from j = 3, i = 3..nx-1 (k_nonlinear_frame takes the ring)
-__global__ void __launch_bounds__(256) k_nonlinear_psi(NLArgs a)
+// nonlinear! with the fluxes from ψ, one launch: the first nint work-groups
+// take the interior points in strips of NL_JS rows from j = 3, i = 3..nx-1,
+// the rest the ring (nl_frame_block, 64 points each, nfb per entry).
+__global__ void __launch_bounds__(256) k_nonlinear_psi(NLArgs a, int nint, int nfb)
 {
     const GridDev& g = a.g;
     const int nx = g.nx, ny = g.ny;
+    if ((int)blockIdx.x >= nint) {
+        const int k = blockIdx.x - nint;
+        nl_frame_block(a, k % nfb, k / nfb, threadIdx.x + WAVE * threadIdx.y);
+        return;
+    }
     const int nbx = (nx - 3 + 63) / 64;
     const int nbs = ((ny - 3 + NL_JS - 1) / NL_JS + 3) / 4;
-    const int lin = xcd_block(blockIdx.x, gridDim.x / NXCD);
+    const int lin = xcd_block(blockIdx.x, nint / NXCD);
     const int b = lin / (nbx * nbs);
     if (b >= a.nlev) return;
     const int rem = lin - b * (nbx * nbs);
@@ -1317,9 +1323,9 @@ void launch_nonlinear(const NLArgs& a, hipStream_t s)
     const int nblocks = ((a.g.nx - 1 + 63) / 64) * (((a.g.ny - 1 + NL_JS - 1) / NL_JS + 3) / 4) * a.nlev;
     if (a.psi && !a.QB) {
         const int nb = ((a.g.nx - 3 + 63) / 64) * (((a.g.ny - 3 + NL_JS - 1) / NL_JS + 3) / 4) * a.nlev;
-        hipLaunchKernelGGL(k_nonlinear_psi, dim3(NXCD * xcd_nper(nb)), blk, 0, s, a);
-        const int nframe = nl_frame_points(a.g.nx, a.g.ny);
-        hipLaunchKernelGGL(k_nonlinear_frame, dim3((nframe + WAVE - 1) / WAVE, a.nlev), dim3(256), 0, s, a);
+        const int nint = NXCD * xcd_nper(nb);
+        const int nfb = (nl_frame_points(a.g.nx, a.g.ny) + WAVE - 1) / WAVE;
+        hipLaunchKernelGGL(k_nonlinear_psi, dim3(nint + nfb * a.nlev), blk, 0, s, a, nint, nfb);
     } else {
         const dim3 grid(NXCD * xcd_nper(nblocks));
         if (a.QB) hipLaunchKernelGGL(k_nonlinear<true>, grid, blk, 0, s, a);
