@@ -640,13 +640,15 @@ static void nl_flux_source(ibpm_prob* P, NLArgs& na)
 // Γout[:,1] = Ainv[1] * rot(reg(f)) -- the body-force projection of
 // project_circ! (timestepping.jl:217-218) and of B_times! (laplacian.jl:162).
 // sub: Γout[:,1] -= result instead of storing it.
-static void project_forces(ibpm_prob* P, const double* f, double* Gout, bool sub)
+// fa: update_stress! of F̃b = f, run in the scatter's launch (null: none)
+static void project_forces(ibpm_prob* P, const double* f, double* Gout, bool sub, const ForceArgs* fa = nullptr)
 {
     hipStream_t s = P->stream;
     ScatterArgs sa{};
     sa.cell = P->sc_cell; sa.off = P->sc_off; sa.ent_f = P->sc_f; sa.ent_w = P->sc_w;
     sa.fb = f; sa.q = P->qreg; sa.ncell = P->sc_ncell;
-    launch_scatter(sa, s);
+    if (fa) launch_scatter_forces(sa, *fa, s);
+    else launch_scatter(sa, s);
     // rot!(qreg) is formed in Ainv's first x-pass; it is zero outside the rows
     // rb_j0 - 1 .. rb_j1 - 1 of the rot! box (qreg is zero outside the E' support)
     lap_level(P, Y_AINV, 0, P->qreg, Gout, sub, P->rb_j0 - 1, P->rb_j1 - 1, nullptr, X_ROT);
@@ -965,10 +967,15 @@ static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, do
             fa.slot = P->slot;
             fa.nbh = std::max(P->nbodies, 1);
         }
-        launch_gemv_forces(P->BinvT, P->Ftmp, P->Fb, P->nf, fa, s);
-        // ---- project_circ! (timestepping.jl:204-223): Γ = Γs; Γ[:,1] -= Ainv rot(reg(F̃b))
-        if (mixed) mx_project_forces(P, P->Fb, P->Gs, true);
-        else project_forces(P, P->Fb, P->Gs, true);
+        // ---- project_circ! (timestepping.jl:204-223): Γ = Γs; Γ[:,1] -= Ainv rot(reg(F̃b));
+        // the FP64 path forms update_stress! in the launch of reg(F̃b)
+        if (mixed) {
+            launch_gemv_forces(P->BinvT, P->Ftmp, P->Fb, P->nf, fa, s);
+            mx_project_forces(P, P->Fb, P->Gs, true);
+        } else {
+            launch_gemv(P->BinvT, P->Ftmp, P->Fb, P->nf, s);
+            project_forces(P, P->Fb, P->Gs, true, &fa);
+        }
     }
     std::swap(P->G, P->Gs);
     // ---- vort2flux!(state.ψ, state.q, state.Γ) (timestepping.jl:64); the

@@ -218,6 +218,11 @@ struct ScatterArgs {
     int ncell;
 };
 void launch_scatter(const ScatterArgs& a, hipStream_t s);
+struct ForceArgs;
+// the scatter and update_stress! (k_forces) in one launch (both read F̃b only)
+void launch_scatter_forces(const ScatterArgs& a, const ForceArgs& fa, hipStream_t s);
+// y = A x by the row GEMV of launch_gemv_forces, without the forces
+void launch_gemv(const double* AT, const double* x, double* y, int n, hipStream_t s);
 
 struct ForceArgs {
     const double* Fb;

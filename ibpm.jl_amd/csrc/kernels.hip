@@ -1611,6 +1611,37 @@ __global__ void __launch_bounds__(256) k_gemv_rows(const double* __restrict__ AT
 // CL = 2 sum(ds .* fb[:,2]) per body; one block
 __global__ void __launch_bounds__(256) k_forces(ForceArgs fa) { forces_block(fa); }
 
+// E' F̃b (k_scatter) and update_stress! (forces_block) in one launch: both only
+// read F̃b; the last work-group forms the forces
+__global__ void __launch_bounds__(256) k_scatter_forces(ScatterArgs a, ForceArgs fa, int nsb)
+{
+    if ((int)blockIdx.x == nsb) {
+        forces_block(fa);
+        return;
+    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ncell) return;
+    double acc = 0.0;
+    for (int e = a.off[t]; e < a.off[t + 1]; e++) acc = acc + a.ent_w[e] * a.fb[a.ent_f[e]];
+    a.q[a.cell[t]] = acc;
+}
+
+void launch_scatter_forces(const ScatterArgs& a, const ForceArgs& fa, hipStream_t s)
+{
+    const int nsb = (a.ncell + 255) / 256;
+    if (fa.nbodies == 0) {
+        launch_scatter(a, s);
+        return;
+    }
+    hipLaunchKernelGGL(k_scatter_forces, dim3(nsb + 1), dim3(256), 0, s, a, fa, nsb);
+}
+
+void launch_gemv(const double* AT, const double* x, double* y, int n, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_gemv_rows, dim3((n + 3) / 4), dim3(256), 0, s, AT, x, y, n);
+}
+
 void launch_gemv_forces(const double* AT, const double* x, double* y, int n, const ForceArgs& fa, hipStream_t s)
 {
     if (n == 0 || fa.nbodies == 0) return;
