@@ -35,7 +35,7 @@ import numpy as np  # noqa: E402
 METRIC = "timesteps/s, 5-level 1024² grid: FP64 vs mixed @CD err≤1e-6; % HBM peak"
 HBM_PEAK_GBS = 8000.0
 # launch-shape options of the autotuner (ibpm_set_option ids) by name
-TUNED_NAMES = {0: "ytile_batch", 1: "ytile_single", 4: "xwaves_batch", 7: "pipe"}
+TUNED_NAMES = {0: "ytile_batch", 1: "ytile_single", 4: "xwaves_batch", 7: "pipe", 8: "ysplit"}
 
 
 def algorithmic_bytes_per_step(nx, ny, mg, nb):
@@ -62,6 +62,13 @@ def measured_traffic(kernel, tile, nlev):
     return None
 
 
+def ysingle(tuned, bc):
+    """Name of the one-level y-pass launch shape the autotuner chose: tile,
+    boundary terms, and two wavefronts per column (split) or one."""
+    return (f"k_ypass<{tuned['ytile_single']}{', BC' if bc else ''}"
+            f"{', split' if tuned.get('ysplit') else ''}>")
+
+
 def time_kernels(prob, tuned, reps):
     """Average device ms per launch of each kernel kind (ibpm_time_kernel:
     HIP events on the problem's stream).  Returns ({name: ms}, name of the
@@ -71,8 +78,8 @@ def time_kernels(prob, tuned, reps):
     yname = f"k_ypass<{tuned['ytile_batch']}> op=A (all levels)"
     kb = {}
     for which, name in ((0, "k_xpass<X_PLAIN> (all levels)"), (1, yname), (2, "k_nonlinear (all levels)"),
-                        (4, "k_xpass<X_PLAIN> (level 1)"), (5, f"k_ypass<{tuned['ytile_single']}> op=Dinv (level 1)"),
-                        (8, f"k_ypass<{tuned['ytile_single']}, BC> op=Dinv + ψ boundary terms (level 1)"),
+                        (4, "k_xpass<X_PLAIN> (level 1)"), (5, f"{ysingle(tuned, False)} op=Dinv (level 1)"),
+                        (8, f"{ysingle(tuned, True)} op=Dinv + ψ boundary terms (level 1)"),
                         (6, "k_curl (level 1)"), (7, "k_copy (nG*mg doubles, counter calibration)")):
         v = ctypes.c_double(0.0)
         prob.ctx.check(C.lib().ibpm_time_kernel(prob.h, which, reps, ctypes.byref(v)))
@@ -345,13 +352,14 @@ def main():
     bc_bytes = 2.0 * 8.0 * N + 8.0 * (2 * (g.ny - 1) + 2 * (g.nx // 2 + 1))
     ypass_ms = y_ms
     roof = {"bound": "hbm",
-            "kernel": f"k_ypass<{tuned['ytile_single']}, BC> (one-level Δinv y-pass with the ψ boundary terms; "
+            "kernel": f"{ysingle(tuned, True)} (one-level Δinv y-pass with the ψ boundary terms; "
                       f"{2 * (L - 1)} per step, the vort2flux! chains' levels below the coarsest, the largest "
                       f"share of a step's kernel time; timed: the {L - 1} of the step's closing vort2flux!, which "
                       f"read and write the whole level (the trial chain's store only the rows its next pass reads))",
             "achieved": round(bc_bytes / (ypass_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(bc_bytes / (ypass_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic("k_ypass_bc", tuned["ytile_single"], 1),
+            "traffic": measured_traffic("k_ypass_bc_split" if tuned.get("ysplit") else "k_ypass_bc",
+                                        tuned["ytile_single"], 1),
             "bytes_per_launch": bc_bytes, "avg_launch_ms": round(ypass_ms, 5), "launches_timed": y_n}
     dst_bytes = 2.0 * 8.0 * N * L
     roof_all = {"bound": "hbm",

@@ -122,6 +122,7 @@ struct ibpm_prob {
     int ytile = 4;         // y-pass columns per workgroup, all-level launches (and batches of fields)
     int ytile_single = 4;  // same, one-level launches (vort2flux chain)
     int pipe = 1;          // IBPM_OPT_PIPE
+    int ysplit = 1;        // IBPM_OPT_YSPLIT
     int xwaves = 4;        // lines per work-group of the plain x-passes over all levels (or a batch)
 
     // device state (IBState) and work arrays
@@ -407,6 +408,7 @@ static YArgs yargs(ibpm_prob* P, int op, int lev0, int nlev, double* buf)
     a.pl = P->ty.plan;
     a.g = P->g;
     a.pipe = P->pipe;
+    a.split = P->ysplit;
     return a;
 }
 
@@ -2470,7 +2472,7 @@ struct ibpm_group {
     double* hist = nullptr;  // = hb.d
     size_t hist_steps = 0;
     int pending = 0;
-    int ytile = 4, ytile_single = 4, xwaves = 4, pipe = 1;
+    int ytile = 4, ytile_single = 4, xwaves = 4, pipe = 1, ysplit = 1;
     // HIP graphs of one group step, one per buffer role (as ibpm_prob's);
     // every launch argument is fixed by the roles (the force arguments and
     // history pointers are read from device memory when the kernels run)
@@ -2532,6 +2534,7 @@ YArgs gyargs(const ibpm_group* Gp, int op, double* buf)
     a.pl = Gp->ty.plan;
     a.g = Gp->g;
     a.pipe = Gp->pipe;
+    a.split = Gp->ysplit;
     return a;
 }
 
@@ -2797,7 +2800,7 @@ int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* probs, ibpm_group*
     for (int i = 0; i < n; i++) Gp->L = std::max(Gp->L, probs[i]->mg);
     Gp->g = P0->g;
     Gp->tx = P0->tx; Gp->ty = P0->ty; Gp->cx = P0->cx; Gp->cy = P0->cy; Gp->crange = P0->crange;
-    Gp->ytile = P0->ytile; Gp->ytile_single = P0->ytile_single; Gp->xwaves = P0->xwaves; Gp->pipe = P0->pipe;
+    Gp->ytile = P0->ytile; Gp->ytile_single = P0->ytile_single; Gp->xwaves = P0->xwaves; Gp->pipe = P0->pipe; Gp->ysplit = P0->ysplit;
     Gp->probs.assign(probs, probs + n);
     auto bail = [&](int code, const std::string& m) {
         for (void* p : Gp->allocs) (void)hipFree(p);
@@ -2982,6 +2985,7 @@ int ibpm_group_set_option(ibpm_group* Gp, int option, int value)
     else if (option == IBPM_OPT_XWAVES_BATCH) Gp->xwaves = value;
     else if (option == IBPM_OPT_GRAPHS) Gp->graphs = value == 1;
     else if (option == IBPM_OPT_PIPE) Gp->pipe = value;
+    else if (option == IBPM_OPT_YSPLIT) Gp->ysplit = value;
     return IBPM_OK;
 }
 
@@ -3355,6 +3359,11 @@ int ibpm_set_option(ibpm_prob* P, int option, int value)
         P->pipe = value;
         return IBPM_OK;
     }
+    if (option == IBPM_OPT_YSPLIT) {
+        if (value != 0 && value != 1) return fail(P->ctx, IBPM_E_ARG, "ysplit option is 0 or 1");
+        P->ysplit = value;
+        return IBPM_OK;
+    }
     if (option == IBPM_OPT_PROFILE) {
         if (value < 0 || value > 2) return fail(P->ctx, IBPM_E_ARG, "profile option is 0, 1 or 2");
         HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
@@ -3376,6 +3385,7 @@ int ibpm_get_option(ibpm_prob* P, int option, int* value)
     else if (option == IBPM_OPT_GRAPHS) *value = P->graphs ? 1 : 0;
     else if (option == IBPM_OPT_XWAVES_BATCH) *value = P->xwaves;
     else if (option == IBPM_OPT_PIPE) *value = P->pipe;
+    else if (option == IBPM_OPT_YSPLIT) *value = P->ysplit;
     else if (option == IBPM_OPT_PRECISION) *value = P->prec < 0 ? 0 : (P->prec == MX_FP32 ? 1 : 2);
     else if (option == IBPM_OPT_MX_SWEEPS) *value = P->mx_sweeps;
     else return fail(P->ctx, IBPM_E_ARG, "unknown option");

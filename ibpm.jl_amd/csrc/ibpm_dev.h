@@ -904,6 +904,271 @@ struct F1024Store {
 };
 
 // ---------------------------------------------------------------------------
+// The 1023-point DST-I split over two wavefronts that run at the same time,
+// for launches that hold one line per SIMD (a single level): each wavefront
+// transforms half of the outputs with a 256-point FFT, so a line's transform
+// is two independent chains of half the length.  Pairing x_j with x_{M-j}
+// (M = 1024, P = 512):
+//  * even outputs: Y_{2q} = 2 sum_{j<P} u_j sin(pi j q / P), u_j = x_j - x_{M-j},
+//    the 511-point DST-I of u, by the half-length algorithm of dst1024_line
+//    at P' = 256 (pre-processing, FFT, unpacking, prefix sum);
+//  * odd outputs: Y_{2q+1} = (-1)^q C_q with C the DCT-III (REDFT01) of
+//    X'_0 = 2 x_P, X'_t = x_{P-t} + x_{P+t} (t < P), by the half-length
+//    real-IDFT form of Makhoul's algorithm: H_t = (X'_t - i X'_{P-t}) e^{i pi t/M}
+//    (X'_P = x_0 + x_M = 0), Z_k = (H_k + H_{k+256}) + i e^{2 pi i k/P} (H_k - H_{k+256}),
+//    z = IDFT_256(Z) (unnormalised, as conj(FFT(conj Z))), w_{2m} = Re z_m,
+//    w_{2m+1} = Im z_m, C_{2r} = w_r, C_{2r+1} = w_{P-1-r}.  No prefix sum.
+// The 256-point FFTs are four radix-4 Stockham stages, one butterfly per
+// lane.  Lane L of either wavefront ends with the outputs n = 16 L + nl of its
+// parity, which it hands to out(L, nl, value).  The transform reads the
+// guarded line S (gsl layout, guards zero), works in its own padded complex
+// array z (fsplit::ZS doubles) and writes nothing else: the caller places the
+// outputs (out) where no wavefront of the line still reads.
+// ---------------------------------------------------------------------------
+namespace fsplit {
+constexpr int H = 256;  // FFT length
+// LDS tables, double2 units: W_512^k (k < 256) at WM + pw(k) (one pad per 4,
+// so that lanes reading 4 consecutive k each hit distinct banks); the radix-4
+// stage twiddles W_{4Ns}^{k r} (r = 1..3, k < Ns) at TW + off(Ns) + (r-1) Ns + k
+// for Ns = 4, 16, 64 (off 0, 12, 60); the even half's pre-processing sines
+// {sin(pi 2m/P), sin(pi (2m+1)/P)} at SA + m; then sin(pi j/M), j <= P, as doubles.
+__host__ __device__ constexpr int pw(int k) { return k + (k >> 2); }
+constexpr int WM = 0, TW = pw(H), SA = TW + 252, SN2 = SA + H;
+constexpr int TAB = 2 * SN2 + 514;  // doubles
+// doubles of one wavefront's complex work array: pz(H - 1) + 1 = 287 complex, + 2
+constexpr int ZS = 576;
+}  // namespace fsplit
+
+struct SplitTab {
+    const double2* wm;
+    const double2* tw;
+    const double2* sa;
+    const double* sn;
+};
+
+// The split tables from the P = 512 plan (tw = W_512^e, sn = sin(pi j/1024)),
+// in two phases as tab1024_issue / tab1024_store: NT threads take part.
+template <int NT>
+struct SplitTabRegs {
+    static constexpr int A = (fsplit::H + NT - 1) / NT;        // W_512^k, pre-processing sines
+    static constexpr int B = (252 + NT - 1) / NT;              // stage twiddles
+    static constexpr int C = (f1024::M / 2 + 1 + NT - 1) / NT;  // sines
+    double2 wm[A], sa[A], tw[B];
+    double sn[C];
+};
+
+template <int NT>
+__device__ __forceinline__ void splittab_issue(SplitTabRegs<NT>& R, const DstPlan& pl, int tid)
+{
+    // indices past the end are clamped (their values are never stored)
+#pragma unroll
+    for (int q = 0; q < SplitTabRegs<NT>::A; q++) {
+        const int m = min(tid + NT * q, fsplit::H - 1);
+        R.wm[q] = pl.tw[m];
+        const int e0 = 4 * m, e1 = 4 * m + 2;  // sin(pi i / 512) = sin(pi 2i / 1024), mirrored past 512
+        R.sa[q] = make_double2(pl.sn[min(e0, f1024::M - e0)], pl.sn[min(e1, f1024::M - e1)]);
+    }
+#pragma unroll
+    for (int q = 0; q < SplitTabRegs<NT>::B; q++) {
+        const int i = min(tid + NT * q, 251);
+        // i < 12: Ns = 4; i < 60: Ns = 16; else Ns = 64.  W_{4Ns}^{k r} = W_512^{k r 128 / Ns}
+        const int ns = i < 12 ? 4 : (i < 60 ? 16 : 64);
+        const int o = i < 12 ? i : (i < 60 ? i - 12 : i - 60);
+        const int r = o / ns + 1, k = o % ns;
+        R.tw[q] = pl.tw[k * r * (128 / ns)];
+    }
+#pragma unroll
+    for (int q = 0; q < SplitTabRegs<NT>::C; q++) R.sn[q] = pl.sn[min(tid + NT * q, f1024::M / 2)];
+}
+
+template <int NT>
+__device__ __forceinline__ SplitTab splittab_store(double* lds, const SplitTabRegs<NT>& R, int tid)
+{
+    using namespace fsplit;
+    double2* t2 = reinterpret_cast<double2*>(lds);
+    double* sn = lds + 2 * SN2;
+#pragma unroll
+    for (int q = 0; q < SplitTabRegs<NT>::A; q++) {
+        const int m = tid + NT * q;
+        if (m < H) {
+            t2[WM + pw(m)] = R.wm[q];
+            t2[SA + m] = R.sa[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < SplitTabRegs<NT>::B; q++) {
+        const int i = tid + NT * q;
+        if (i < 252) t2[TW + i] = R.tw[q];
+    }
+#pragma unroll
+    for (int q = 0; q < SplitTabRegs<NT>::C; q++) {
+        const int i = tid + NT * q;
+        if (i <= f1024::M / 2) sn[i] = R.sn[q];
+    }
+    SplitTab t;
+    t.wm = t2 + WM;
+    t.tw = t2 + TW;
+    t.sa = t2 + SA;
+    t.sn = sn;
+    return t;
+}
+
+// radix-4 Stockham stage Ns of the 256-point FFT, butterfly j = lane.  The
+// reads pz(j + 64 r) = pz(j) + 72 r; the writes pz(base + r Ns) are a lane
+// base plus a constant: Ns = 4: base + 4r + r/2, Ns = 16: base + 18 r,
+// Ns = 64: base + 72 r (base + r Ns stays inside one 8-group for r Ns < 8
+// and crosses whole groups otherwise).
+template <int NS>
+__device__ __forceinline__ void fft256_stage(double2* z, const double2* __restrict__ tw, int j, int pj)
+{
+    double2 v[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) v[r] = z[pj + 72 * r];
+    const int k = j & (NS - 1);
+#pragma unroll
+    for (int r = 1; r < 4; r++) v[r] = cmul(v[r], tw[(r - 1) * NS + k]);
+    wave_sync();
+    dft4(v);
+    const int base = (j - k) * 4 + k;  // (j / Ns) Ns 4 + k
+    if constexpr (NS == 4) {
+        const int pb = pz(base);  // base = 16 (j/4) + k, k < 4: pz(base + 4r) = pz(base) + 4r + r/2
+#pragma unroll
+        for (int r = 0; r < 4; r++) z[pb + 4 * r + (r >> 1)] = v[r];
+    } else if constexpr (NS == 16) {
+        const int pb = pz(base);  // base = 64 (j/16) + k, k < 16: pz(base + 16r) = pz(base) + 18r
+#pragma unroll
+        for (int r = 0; r < 4; r++) z[pb + 18 * r] = v[r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) z[pj + 72 * r] = v[r];  // base = j
+    }
+    wave_sync();
+}
+
+// half 0: even outputs, half 1: odd outputs (wave-uniform).  Line reads at
+// m = j + 64 r (or 2m) are a lane base plus a constant: gsl(e +- 64 s) =
+// gsl(e) +- 72 s for e, e +- 64 s >= 0.
+template <class OUT>
+__device__ __forceinline__ void dst1024_split(const double* S, double2* z, const SplitTab& tb, int lane, int half, OUT out)
+{
+    using f1024::gsl;
+    asm volatile("" : "+v"(lane));
+    half = __builtin_amdgcn_readfirstlane(half);  // uniform branches below
+    const int j = lane;
+    double2 v[4];
+    // --- stage 1 (Ns = 1) inputs m = j + 64 r, formed from the line
+    if (half == 0) {
+        // z_m = y_{2m} + i y_{2m+1}, y_i = sin(pi i/P) (u_i + u_{P-i}) + (u_i - u_{P-i}) / 2
+        const double* pa = S + gsl(2 * j);          // x_{2m}, x_{2m+1}: + 144 r
+        const double* pc = S + gsl(512 + 2 * j);    // x_{P+2m}, x_{P+2m+1}
+        const double* pb0 = S + gsl(1024 - 2 * j);  // x_{M-2m}: - 144 r
+        const double* pb1 = S + gsl(1023 - 2 * j);  // x_{M-2m-1}
+        const double* pd0 = S + gsl(512 - 2 * j);   // x_{P-2m}
+        const double* pd1 = S + gsl(511 - 2 * j);   // x_{P-2m-1}
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const double2 xa = *reinterpret_cast<const double2*>(pa + 144 * r);
+            const double2 xc = *reinterpret_cast<const double2*>(pc + 144 * r);
+            const double xb0 = pb0[-144 * r], xb1 = pb1[-144 * r];
+            const double xd0 = pd0[-144 * r], xd1 = pd1[-144 * r];
+            const double2 sv = tb.sa[j + 64 * r];
+            const double u0 = xa.x - xb0, w0 = xd0 - xc.x;  // u_{2m}, u_{P-2m}
+            const double u1 = xa.y - xb1, w1 = xd1 - xc.y;  // u_{2m+1}, u_{P-2m-1}
+            v[r] = make_double2(fma(sv.x, u0 + w0, 0.5 * (u0 - w0)), fma(sv.y, u1 + w1, 0.5 * (u1 - w1)));
+        }
+    } else {
+        // conj(Z_m)
+        const double* q0 = S + gsl(512 - j);   // x_{P-m}: - 72 r
+        const double* q1 = S + gsl(512 + j);   // x_{P+m}: + 72 r
+        const double* q2 = S + gsl(j);         // x_m
+        const double* q3 = S + gsl(1024 - j);  // x_{M-m}
+        const double* q4 = S + gsl(256 - j);   // x_{256-m}
+        const double* q5 = S + gsl(768 + j);   // x_{768+m}
+        const double* q6 = S + gsl(256 + j);   // x_{256+m}
+        const double* q7 = S + gsl(768 - j);   // x_{768-m}
+        const double* sn = tb.sn;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int d72 = 72 * r, d64 = 64 * r;
+            const double a0 = q0[-d72] + q1[d72];  // X'_m
+            const double b0 = q2[d72] + q3[-d72];  // X'_{P-m}
+            const double a1 = q4[-d72] + q5[d72];  // X'_{m+256}
+            const double b1 = q6[d72] + q7[-d72];  // X'_{256-m}
+            const double c0 = sn[512 - j - d64], s0 = sn[j + d64];        // cos, sin(pi m/M)
+            const double c1 = sn[256 - j - d64], s1 = sn[256 + j + d64];  // cos, sin(pi (m+256)/M)
+            const double2 h0 = make_double2(fma(a0, c0, b0 * s0), fma(a0, s0, -(b0 * c0)));
+            const double2 h1 = make_double2(fma(a1, c1, b1 * s1), fma(a1, s1, -(b1 * c1)));
+            const double2 d = csub(h0, h1);
+            const double2 e = tb.wm[fsplit::pw(j) + 80 * r];  // W_P^m; i conj(W_P^m) d:
+            const double2 t = make_double2(-fma(e.x, d.y, -(e.y * d.x)), fma(e.x, d.x, e.y * d.y));
+            v[r] = make_double2((h0.x + h1.x) + t.x, -((h0.y + h1.y) + t.y));
+        }
+    }
+    dft4(v);
+    {
+        const int p4 = 4 * j + (j >> 1);  // pz(4j + r) = p4 + r, r < 4
+#pragma unroll
+        for (int r = 0; r < 4; r++) z[p4 + r] = v[r];
+    }
+    wave_sync();
+    const int pj = pz(j);
+    fft256_stage<4>(z, tb.tw, j, pj);
+    fft256_stage<16>(z, tb.tw + 12, j, pj);
+    fft256_stage<64>(z, tb.tw + 60, j, pj);
+    if (half == 0) {
+        // unpacking, lane j owns k = 4j + i: 2 F_k = (a + conj b) + W_P^k (-i)(a - conj b),
+        // a = z_k, b = z_{256-k}; the outputs 4k (2 I_k) and 4k + 2 (prefix sum of R')
+        const int p4 = 4 * j + (j >> 1);  // pz(4j + i)
+        // pz(256 - 4j - i): e0 = 256 - 4j, pz(e0) - i, one less for i >= 1 when e0 = 0 mod 8
+        // (j even); lane 0 at i = 0 reads z_0
+        const int e0 = 256 - 4 * j;
+        const int pb0 = j == 0 ? 0 : e0 + (e0 >> 3);
+        const int pb1 = e0 + (e0 >> 3) - ((j & 1) ? 0 : 1);
+        double rr[4], ye[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const double2 a = z[p4 + i];
+            const double2 b = z[i == 0 ? pb0 : pb1 - i];
+            const double2 w = tb.wm[5 * j + i];  // pw(4j + i)
+            const double2 E2 = make_double2(a.x + b.x, a.y - b.y);
+            const double2 O2 = make_double2(a.y + b.y, -(a.x - b.x));
+            const double2 F2 = cadd(E2, cmul(w, O2));
+            rr[i] = F2.x;
+            ye[i] = -F2.y;
+        }
+        if (j == 0) {
+            rr[0] = 0.5 * rr[0];  // R'_0 = R_0
+            ye[0] = 0.0;          // n = 0: the guard cell x_0
+        }
+        double cs[4];
+        cs[0] = rr[0];
+#pragma unroll
+        for (int i = 1; i < 4; i++) cs[i] = cs[i - 1] + rr[i];
+        const double off = wave_excl_scan(cs[3]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            out(j, 4 * i, ye[i]);
+            out(j, 4 * i + 2, off + cs[i]);
+        }
+    } else {
+        // F = FFT(conj Z), z = conj F: C_q of q = 8j + t from F_{2j}, F_{2j+1},
+        // F_{254-2j}, F_{255-2j}; output n = 2q + 1 = 16 j + 2t + 1 is (-1)^t C_q
+        const int pa = 2 * j + (j >> 2);                    // pz(2j) (pz(2j + 1) = pa + 1)
+        const int pc = (254 - 2 * j) + ((255 - 2 * j) >> 3);  // pz(254 - 2j) (pz(255 - 2j) = pc + 1)
+        const double2 a = z[pa], b = z[pa + 1];
+        const double2 c = z[pc], d = z[pc + 1];
+        out(j, 1, a.x);
+        out(j, 3, d.y);
+        out(j, 5, -a.y);
+        out(j, 7, -d.x);
+        out(j, 9, b.x);
+        out(j, 11, c.y);
+        out(j, 13, -b.y);
+        out(j, 15, -c.x);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // get_bc! evaluated pointwise (src/fluid-operators/mg-utils.jl:7-32).
 // r: coarse-level (nx-1) x (ny-1) array.  k: 1-based index into the boundary
 // vector (offsets LEFT=0, RIGHT=ny+1, BOT=2(ny+1), TOP=2(ny+1)+nx+1).

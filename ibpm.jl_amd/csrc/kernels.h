@@ -103,12 +103,18 @@ struct YArgs {
     // pipe != 0: a launch of many tiles (no boundary terms, 4-column tiles)
     // runs as persistent work-groups that prefetch their next tile
     int pipe;
+    // split != 0 (1023-point columns, 4-column tiles, launches of at most one
+    // tile per CU: a single level): each column is transformed by two
+    // wavefronts (dst1024_split); 0: one wavefront per column
+    int split;
     DstPlan pl;
     GridDev g;
 };
 void launch_ypass(const YArgs& a, int tile, hipStream_t s);
 // LDS bytes one y-pass workgroup needs (tables, cos table, tile slots)
 size_t ypass_lds_bytes(const DstPlan& pl, int n2, int tile);
+// LDS bytes of the split y-pass work-group (ny = 1024), with or without boundary terms
+size_t ypass_split_lds_bytes(int ny, bool bc);
 
 // ---- field kernels ---------------------------------------------------------
 struct NLArgs {

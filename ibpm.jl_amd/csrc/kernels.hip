@@ -392,11 +392,35 @@ __host__ __device__ inline int cy_doubles_t(int n2)
 // that lanes reading pairs 8 apart (consecutive-row threads) hit distinct banks
 __host__ __device__ inline int ep_slot(int e) { return e + (e >> 3); }
 
+// dst1024_split output of the y-pass: as YOut1024, one value at a time
+// (output n = 16 L + nl of the wavefront's parity)
+struct YOutSplit {
+    double* S;
+    const double* cyp;
+    double cxk, qc, scale;
+    int op;
+    __device__ __forceinline__ void operator()(int L, int nl, double y) const
+    {
+        const int q = 18 * L + nl;  // gsl(16 L + nl)
+        if (op >= 0) y *= eig_factor(cxk, cyp[q], op, qc, scale);
+        S[q] = y;
+    }
+};
+
+// LDS doubles of one column of the split y-pass: the line, the slot the
+// first transform writes (the second writes the line again), and the complex
+// work arrays of the column's two wavefronts
+constexpr int YSPLIT_COL = 2 * f1024::SLOT + 2 * fsplit::ZS;
+
 // BC: the launch carries boundary terms (a.bcr != null); a separate instance
-// so that launches without them keep their register budget
-template <int TI, int PT, int YW, bool BC>
+// so that launches without them keep their register budget.  SPLIT (1023-point
+// lines, YW = 2 TI): each column is transformed by two wavefronts at once
+// (dst1024_split), which puts two wavefronts on every SIMD of a single-level
+// launch instead of one.
+template <int TI, int PT, int YW, bool BC, bool SPLIT = false>
 __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
 {
+    static_assert(!SPLIT || (PT == 512 && YW == 2 * TI), "the split y-pass takes two wavefronts per 1023-point column");
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
@@ -410,7 +434,7 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     const Entry en = batch_entry(b, a.lev0, a.levmod, a.fixlev, 1 << 30, a.ipar);
     if (!en.active) return;  // the whole work-group: a level its instance does not have
     const int i0 = (lin - b * ntiles) * TI;
-    const int ld = slot_doubles_t<PT>(a.pl);
+    const int ld = SPLIT ? YSPLIT_COL : slot_doubles_t<PT>(a.pl);
     double* base = a.buf + b * a.lstride;
     // cos(pi i/nx) of this wave's columns, loaded with the tile (not after it)
     constexpr int NCOL = (TI + YW - 1) / YW;
@@ -428,7 +452,7 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     // CH for the guarded 1023-point layout, whose padding keeps lanes CH rows
     // apart on distinct LDS banks, and 2 for the plain layouts.
     constexpr int RPI = YW * WAVE / TI;
-    constexpr int CH = 16;
+    constexpr int CH = SPLIT ? 8 : 16;
     constexpr int G = PT == 512 ? CH : 2;
     static_assert(CH % G == 0 && G % 2 == 0, "the column terms take rows in pairs");
     const int c0 = tid % TI, r0 = tid / TI;
@@ -440,16 +464,21 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
         else return rb + r0 + RPI * t;
     };
     // LDS: [DST tables][cos(pi j/ny) table][TI column slots]
-    const int ntab = tab_doubles_t<PT>(a.pl);
+    const int ntab = SPLIT ? fsplit::TAB : tab_doubles_t<PT>(a.pl);
     // the 1023-point tables and the cos table are loaded in two phases around
     // the boundary and tile loads below (all in flight together)
     constexpr int NT = YW * WAVE;
     constexpr int CYN = (f1024::M - 1 + NT - 1) / NT;
-    Tab1024Regs<NT> TR;
+    std::conditional_t<SPLIT, SplitTabRegs<NT>, Tab1024Regs<NT>> TR;
     double cyr[CYN];
     DstTab tb{};
+    SplitTab tbs{};
     double* cyl = lds + ntab;
-    if constexpr (PT == 512) {
+    if constexpr (SPLIT) {
+        splittab_issue(TR, a.pl, tid);
+#pragma unroll
+        for (int q = 0; q < CYN; q++) cyr[q] = a.cy[min(tid + NT * q, n2 - 1)];
+    } else if constexpr (PT == 512) {
         tab1024_issue(TR, a.pl, tid);
 #pragma unroll
         for (int q = 0; q < CYN; q++) cyr[q] = a.cy[min(tid + NT * q, n2 - 1)];
@@ -458,7 +487,10 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
         for (int k = tid; k < n2; k += NT) cyl[k] = a.cy[k];
     }
     double* tile = cyl + cy_doubles_t<PT>(n2);
-    if (tid < TI) set_guards<PT>(tile + tid * ld);
+    if (tid < TI) {
+        set_guards<PT>(tile + tid * ld);
+        if (SPLIT) set_guards<PT>(tile + tid * ld + f1024::SLOT);
+    }
     // apply_bc! (mg-utils.jl:43-60) with get_bc! (mg-utils.jl:7-32) of the next
     // coarser level, carried through the x-transform.  With S(t) = sin(pi t/nx),
     // C(t) = cos(pi t/nx), line k of the x-transformed field gets
@@ -474,6 +506,7 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     // ny/4 - 1 .. 3ny/4 - 1, pair e at ep_slot(e)
     double2* edl = reinterpret_cast<double2*>(tile + TI * ld);
     const int nep = a.g.ny / 2 + 1;
+    double* rowl = reinterpret_cast<double*>(edl + ep_slot(a.g.ny / 2) + 1);  // split: (addB, addT) per column
     constexpr int EPT = (MAXM / 2 + 1 + YW * WAVE - 1) / (YW * WAVE);  // pairs staged per thread
     constexpr int NCM = (MAXM / 2 + WAVE - 1) / WAVE;                  // coarse-row samples per lane
     double2 ep[EPT];
@@ -540,7 +573,8 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
         v[t] = (colok && r >= rlo && r <= rhi) ? base[ic + (long)n1 * r] : 0.0;
     }
     if constexpr (PT == 512) {
-        tb = tab1024_store(lds, TR, tid);
+        if constexpr (SPLIT) tbs = splittab_store(lds, TR, tid);
+        else tb = tab1024_store(lds, TR, tid);
 #pragma unroll
         for (int q = 0; q < CYN; q++) {
             const int k = tid + NT * q;
@@ -580,6 +614,12 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
             const int e = tid + YW * WAVE * q;
             if (e < nep) edl[ep_slot(e)] = ep[q];
         }
+        // split: the row terms of column w go with the tile into the column
+        // (its two wavefronts both read rows 0 and n2 - 1)
+        if (SPLIT && w < TI && lane == 0) {
+            rowl[2 * w] = addB[0];
+            rowl[2 * w + 1] = addT[0];
+        }
         __syncthreads();
     }
     // (4) the tile into LDS, with the column terms
@@ -617,6 +657,14 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
                 }
             }
         }
+        if (SPLIT && bc) {
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                const int r = trow(rb, t);
+                if (r == 0) v[t] += rowl[2 * c0];
+                if (r == n2 - 1) v[t] += rowl[2 * c0 + 1];
+            }
+        }
 #pragma unroll
         for (int t = 0; t < CH; t++) {
             const int r = trow(rb, t);
@@ -632,6 +680,20 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
         Re = a.ipar[en.inst].Re;
     }
     const double qc = dt / ((2.0 * Re) * (hcl * hcl));  // dt / (2 Re h^2), laplacian.jl:69,74
+    if constexpr (SPLIT) {
+        // column w / 2, outputs of parity w % 2: the first transform reads the
+        // line and writes the scaled outputs into the second slot, the second
+        // transform reads that slot and writes the line
+        const int c = w >> 1, half = w & 1;
+        const bool on = i0 + c < n1;
+        const double cxk = on ? a.cx[i0 + c] : 0.0;
+        double* s0 = tile + c * ld;
+        double* s1 = s0 + f1024::SLOT;
+        double2* zw = reinterpret_cast<double2*>(s0 + 2 * f1024::SLOT + half * fsplit::ZS);
+        if (on) dst1024_split(s0, zw, tbs, lane, half, YOutSplit{s1, cyl, cxk, qc, a.scale, a.op});
+        __syncthreads();
+        if (on) dst1024_split(s1, zw, tbs, lane, half, YOutSplit{s0, cyl, cxk, qc, a.scale, -1});
+    } else {
     // (column, transform) pairs: transform 0 is followed by the eigenvalue
     // scaling; one call site keeps a single DST body in the kernel
 #pragma unroll 1
@@ -667,6 +729,7 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
             }
         }
     }
+    }
     __syncthreads();
     if (colok) {
         const int wlo = a.wsel ? a.wlo : 0, whi = a.wsel ? a.whi : n2 - 1;
@@ -696,6 +759,24 @@ static size_t ypass_lds_bytes_t(const DstPlan& pl, int n2, int tile)
 size_t ypass_lds_bytes(const DstPlan& pl, int n2, int tile)
 {
     return (pl.kind == 0 && pl.P == 512) ? ypass_lds_bytes_t<512>(pl, n2, tile) : ypass_lds_bytes_t<0>(pl, n2, tile);
+}
+
+// the split y-pass: 4-column tiles of 1023-point columns, 8 wavefronts
+static void launch_y_split(const YArgs& a, hipStream_t s)
+{
+    static bool once = (allow_big_lds(k_ypass<4, 512, 8, false, true>), allow_big_lds(k_ypass<4, 512, 8, true, true>), true);
+    (void)once;
+    const int nblocks = ((a.g.n1 + 3) / 4) * a.nlev;
+    dim3 grid(NXCD * xcd_nper(nblocks));
+    const size_t lds = ypass_split_lds_bytes(a.g.ny, a.bcr != nullptr);
+    if (a.bcr) hipLaunchKernelGGL((k_ypass<4, 512, 8, true, true>), grid, dim3(8 * WAVE), lds, s, a);
+    else hipLaunchKernelGGL((k_ypass<4, 512, 8, false, true>), grid, dim3(8 * WAVE), lds, s, a);
+}
+
+size_t ypass_split_lds_bytes(int ny, bool bc)
+{
+    return sizeof(double) * ((size_t)fsplit::TAB + (size_t)cy_doubles_t<512>(ny - 1) + 4 * (size_t)YSPLIT_COL) +
+           (bc ? sizeof(double2) * (size_t)(ep_slot(ny / 2) + 1) + 8 * sizeof(double) : 0);
 }
 
 template <int TI, int PT, int YW>
@@ -896,6 +977,11 @@ void launch_ypass(const YArgs& a, int tile, hipStream_t s)
         case 100: launch_y_pipe_t<100>(a, nwg, s); return;
         default: launch_y_pipe_t<0>(a, nwg, s); return;
         }
+    }
+    if (tile == 4 && a.split && a.pl.kind == 0 && a.pl.P == 512 && ntiles4 <= 256 &&
+        ypass_split_lds_bytes(a.g.ny, a.bcr != nullptr) <= 160 * 1024) {
+        launch_y_split(a, s);
+        return;
     }
     if (tile == 4) launch_y_m<4, YWAVES>(a, s);
     else if (tile == 12) launch_y_m<12, 12>(a, s);

@@ -31,6 +31,7 @@ FIELD_FB_TILDE, FIELD_FB, FIELD_CD, FIELD_CL = 6, 7, 8, 9
 OPT_YTILE_BATCH, OPT_YTILE_SINGLE, OPT_PROFILE, OPT_GRAPHS, OPT_XWAVES_BATCH, OPT_PRECISION = 0, 1, 2, 3, 4, 5
 OPT_MX_SWEEPS = 6
 OPT_PIPE = 7
+OPT_YSPLIT = 8
 PREC_FP64, PREC_FP32_MFMA, PREC_BF16X3_MFMA = 0, 1, 2
 
 EXPORTED = [

@@ -21,6 +21,9 @@ SWEEPS = {
     # persistent prefetching work-groups for the passes over many tiles
     # (timed on the all-level y-pass, after its tile is chosen)
     C.OPT_PIPE: (1, (1, 0)),
+    # two wavefronts per column in the one-level y-passes of 1023-point
+    # columns (no effect on other grids: both values time the same kernel)
+    C.OPT_YSPLIT: (8, (1, 0)),
 }
 
 

@@ -305,6 +305,9 @@ int ibpm_op_update_bodies(ibpm_prob* prob, const double* kin);
 #define IBPM_OPT_PIPE 7          /* 1 (default): passes over more tiles than the chip holds at once run as
                                     persistent work-groups that load their next tile while transforming the
                                     current one; 0: one work-group per tile */
+#define IBPM_OPT_YSPLIT 8        /* 1 (default): the one-level y-passes over 1023-point columns (ny = 1024,
+                                    4-column tiles) transform each column with two wavefronts at once (its
+                                    even and its odd outputs); 0: one wavefront per column */
 int ibpm_set_option(ibpm_prob* prob, int option, int value);
 int ibpm_get_option(ibpm_prob* prob, int option, int* value);
 /* Average device ms and count of the y-pass launches timed inside the steps
