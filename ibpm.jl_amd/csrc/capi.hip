@@ -549,7 +549,15 @@ static void timed_ypass(ibpm_prob* P, const YArgs& y, int tile, int kind);
 // kept and whose rows ny/4 and 3ny/4 are stored (every line when tlo < 0).
 enum { CURL_ALL = 0, CURL_LEVEL1 = 1, CURL_NONE = 2, CURL_TRIAL = 3 };
 
-static void vort2flux_dev(ibpm_prob* P, double* Gb, double* psib, double* qout, int curl)
+// yhat (optional): a field whose x-transform is Gb on every line coarsify!
+// leaves alone: Gb = X(yhat) there, X the x-DST-I (Ainv's last pass wrote Gb
+// from yhat), so X(Gb) = X(X(yhat)) = 2 nx yhat on those lines.  Only the
+// lines coarsify! rewrites (the centre rows of levels 2..L) are then
+// x-transformed; the y-pass reads the others as 2 nx yhat.  ψ then differs
+// from the full transform's by rounding.  ysub0 (optional, with yhat): level
+// 1 of Gb is X(yhat) - X(ysub0) instead, read as 2 nx (yhat - ysub0).
+static void vort2flux_dev(ibpm_prob* P, double* Gb, double* psib, double* qout, int curl,
+                          const double* yhat = nullptr, const double* ysub0 = nullptr)
 {
     hipStream_t s = P->stream;
     const long nG = P->g.nG;
@@ -569,11 +577,26 @@ static void vort2flux_dev(ibpm_prob* P, double* Gb, double* psib, double* qout, 
     // the y-pass; the inverse x-pass of a level stores the columns the next
     // finer level's get_bc! reads.
     XArgs a = xargs(P);
-    a.src = Gb; a.dst = P->T1; a.nlev = L; a.lev0 = 0;
-    launch_xpass(X_PLAIN, a, s);
+    // 0-based lines of the centre rows coarsify! writes on levels 2..L
+    const int clo = P->crange[0].jc0 - 1, chi = P->crange[0].jc0 + P->crange[0].cj - 2;
+    if (!yhat) {
+        a.src = Gb; a.dst = P->T1; a.nlev = L; a.lev0 = 0;
+        launch_xpass(X_PLAIN, a, s);
+    } else if (L > 1 && chi >= clo) {
+        a.src = Gb + nG; a.dst = P->T1 + nG; a.nlev = L - 1; a.lev0 = 1;
+        a.lsel = 1; a.llo = clo; a.lhi = chi;
+        launch_xpass(X_PLAIN, a, s);
+    }
     const bool trial = curl == CURL_TRIAL && P->tlo >= 0;
     for (int l = L - 1; l >= 0; l--) {
         YArgs y = yargs(P, Y_DINV, l, 1, P->T1 + (long)l * nG);
+        if (yhat) {
+            y.ysrc = yhat + (long)l * nG;
+            y.afac = 2.0 * (double)P->g.nx;
+            y.clo = l > 0 ? clo : 1;  // level 1: every line from yhat
+            y.chi = l > 0 ? chi : 0;
+            if (l == 0) y.ysub = ysub0;
+        }
         if (l < L - 1) {
             y.bcr = psib + (long)(l + 1) * nG;
             y.edge = P->edge;
@@ -986,6 +1009,8 @@ static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, do
         mx_vort2flux(P, P->G, P->q, false);
         P->q_stale = false;
     } else {
+        // the full x-transform: the state's ψ stays vort2flux!(Γ) bit for bit
+        // (what a restart or a checkpoint recomputes from Γ alone)
         vort2flux_dev(P, P->G, P->psi, P->q, CURL_NONE);
     }
     (void)nG;
@@ -1022,7 +1047,9 @@ static void trial_dev(ibpm_prob* P)
     c.src = P->T2; c.dst = P->Gs; c.nlev = L; c.lev0 = 0;
     launch_xpass(X_PLAIN, c, s);
     P->nl1 = P->nl2 = out;  // state.nonlin[2] .= state.nonlin[1]
-    vort2flux_dev(P, P->Gs, P->psi, P->qs1, CURL_TRIAL);
+    // Γs = X(T2) on every line coarsify! leaves alone: the chain x-transforms
+    // only the centre rows, and reads the rest as 2 nx T2
+    vort2flux_dev(P, P->Gs, P->psi, P->qs1, CURL_TRIAL, P->T2);
 }
 
 // E of the trial flux qs[:,1] = curl!(ψ of the trial) read from ψ itself
@@ -2540,7 +2567,10 @@ YArgs gyargs(const ibpm_group* Gp, int op, double* buf)
 
 // vort2flux! (curl.jl:133-165) of every instance: the level chain runs over
 // all instances at once (one launch per pass and level)
-void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, bool level1_only)
+// yhat: as vort2flux_dev's (every instance's Ainv y-pass output, the trial
+// chain), so that a group's trial chain is an instance's bit for bit
+void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, bool level1_only,
+                     const double* yhat = nullptr)
 {
     hipStream_t s = Gp->stream;
     const long nG = Gp->g.nG, ist = (long)Gp->L * nG;
@@ -2558,10 +2588,22 @@ void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, boo
     // then the chain (boundary terms in the y-pass, as in vort2flux_dev)
     XArgs a = gxargs(Gp);
     a.src = Gb; a.dst = Gp->T1; a.nlev = n * L; a.lev0 = 0; a.levmod = L;
-    launch_xpass(X_PLAIN, a, s);
+    // with yhat only the centre rows coarsify! writes (level 1's are
+    // transformed too, and not read)
+    const int clo = Gp->crange[0].jc0 - 1, chi = Gp->crange[0].jc0 + Gp->crange[0].cj - 2;
+    if (yhat) {
+        a.lsel = 1; a.llo = clo; a.lhi = chi;
+    }
+    if (!yhat || (L > 1 && chi >= clo)) launch_xpass(X_PLAIN, a, s);
     for (int l = L - 1; l >= 0; l--) {
         YArgs y = gyargs(Gp, Y_DINV, Gp->T1 + (long)l * nG);
         y.lstride = ist; y.nlev = n; y.lev0 = l; y.fixlev = 1;
+        if (yhat) {
+            y.ysrc = yhat + (long)l * nG;
+            y.afac = 2.0 * (double)Gp->g.nx;
+            y.clo = l > 0 ? clo : 1;
+            y.chi = l > 0 ? chi : 0;
+        }
         if (l < L - 1) {  // (an instance whose coarsest level is l has none: the kernel checks)
             y.bcr = psib + (long)(l + 1) * nG;
             y.bcstride = ist;
@@ -2614,7 +2656,7 @@ void group_step(ibpm_group* Gp)
     c.src = Gp->T2; c.dst = Gp->Gs; c.nlev = NE; c.lev0 = 0; c.levmod = L;
     launch_xpass(X_PLAIN, c, s);
     Gp->nl1 = Gp->nl2 = out;
-    group_vort2flux(Gp, Gp->Gs, Gp->psi, Gp->qs1, true);
+    group_vort2flux(Gp, Gp->Gs, Gp->psi, Gp->qs1, true, Gp->T2);
     // ---- boundary_forces! and update_stress! of every instance
     GatherArgs ga{};
     ga.q = Gp->qs1; ga.q0 = Gp->q0; ga.w = Gp->w; ga.bidx = Gp->bidx; ga.out = Gp->Ftmp; ga.nbt = Gp->NBT; ga.g = g;

@@ -103,6 +103,14 @@ struct YArgs {
     // pipe != 0: a launch of many tiles (no boundary terms, 4-column tiles)
     // runs as persistent work-groups that prefetch their next tile
     int pipe;
+    // ysrc != null: the input rows outside clo..chi (0-based) are not read
+    // from buf but are afac * ysrc (ysub null) or afac * (ysrc - ysub), entry b
+    // at ysrc / ysub + b * lstride: rows whose x-transform is known as a
+    // multiple of another field's (the DST-I is its own inverse up to 2 nx)
+    const double* ysrc;
+    const double* ysub;
+    int clo, chi;
+    double afac;
     // split != 0 (1023-point columns, 4-column tiles, launches of at most one
     // tile per CU: a single level): each column is transformed by two
     // wavefronts (dst1024_split); 0: one wavefront per column

@@ -564,14 +564,31 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
             }
         }
     }
-    // (2) the first chunk of the tile (rows outside rlo..rhi are zero when rsel)
+    // (2) the first chunk of the tile (rows outside rlo..rhi are zero when rsel;
+    // rows outside clo..chi come from ysrc when it is given)
     const int rlo = a.rsel ? a.rlo : 0, rhi = a.rsel ? a.rhi : n2 - 1;
-    double v[CH];
+    const double* ys = a.ysrc ? a.ysrc + b * a.lstride : nullptr;
+    const double* yb = a.ysub ? a.ysub + b * a.lstride : nullptr;
+    auto load_chunk = [&](int rb, double* vv) {
+        bool alt[CH];
 #pragma unroll
-    for (int t = 0; t < CH; t++) {
-        const int r = trow(0, t);
-        v[t] = (colok && r >= rlo && r <= rhi) ? base[ic + (long)n1 * r] : 0.0;
-    }
+        for (int t = 0; t < CH; t++) {
+            const int r = trow(rb, t);
+            const bool ok = colok && r >= rlo && r <= rhi;
+            alt[t] = ok && ys && (r < a.clo || r > a.chi);
+            vv[t] = ok ? (alt[t] ? ys : base)[ic + (long)n1 * r] : 0.0;
+        }
+        if (ys) {
+            double sb[CH];
+#pragma unroll
+            for (int t = 0; t < CH; t++) sb[t] = (yb && alt[t]) ? yb[ic + (long)n1 * trow(rb, t)] : 0.0;
+#pragma unroll
+            for (int t = 0; t < CH; t++)
+                if (alt[t]) vv[t] = a.afac * (yb ? vv[t] - sb[t] : vv[t]);
+        }
+    };
+    double v[CH];
+    load_chunk(0, v);
     if constexpr (PT == 512) {
         if constexpr (SPLIT) tbs = splittab_store(lds, TR, tid);
         else tb = tab1024_store(lds, TR, tid);
@@ -624,13 +641,7 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     }
     // (4) the tile into LDS, with the column terms
     for (int rb = 0; rb < n2; rb += RPI * CH) {
-        if (rb > 0) {
-#pragma unroll
-            for (int t = 0; t < CH; t++) {
-                const int r = trow(rb, t);
-                v[t] = (colok && r >= rlo && r <= rhi) ? base[ic + (long)n1 * r] : 0.0;
-            }
-        }
+        if (rb > 0) load_chunk(rb, v);
         if (bc) {
             // get_bc! sample idx = j + 1 = r + 2 of the left and right columns:
             // the even row re = trow(rb, 2u) (idx = 2e, e = re/2 + 1) takes the
@@ -965,7 +976,7 @@ void launch_ypass(const YArgs& a, int tile, hipStream_t s)
     // pipelined persistent work-groups (tile option 4 with pipe): launches of
     // more 4-column tiles than two work-groups per CU hold at once
     const int ntiles4 = ((a.g.n1 + 3) / 4) * a.nlev;
-    if (tile == 4 && a.pipe && !a.bcr && ntiles4 > 2 * 256) {
+    if (tile == 4 && a.pipe && !a.bcr && !a.ysrc && ntiles4 > 2 * 256) {
         const int nwg = NXCD * 64;  // two per CU of the 256
         const int P = a.pl.kind == 0 ? a.pl.P : 0;
         switch (P) {
