@@ -348,8 +348,11 @@ def main():
     kb, yname = time_kernels(prob, tuned, args.kernel_reps)
     # algorithmic bytes per launch: read N + write N doubles per level for a DST
     # pass; the boundary-term pass also reads the two coarse columns (one
-    # pair per row) and the two coarse rows (nx/2 + 1 samples each)
-    bc_bytes = 2.0 * 8.0 * N + 8.0 * (2 * (g.ny - 1) + 2 * (g.nx // 2 + 1))
+    # pair per row) and the two coarse rows (nx/2 + 1 samples each).  Of the
+    # L - 1 timed launches, level 1's reads a second field (its input is
+    # 2 nx (Ainv's y-pass output - the projection's)): N more doubles, shown
+    # here as the average over the timed launches
+    bc_bytes = 2.0 * 8.0 * N + 8.0 * (2 * (g.ny - 1) + 2 * (g.nx // 2 + 1)) + 8.0 * N / (L - 1)
     ypass_ms = y_ms
     roof = {"bound": "hbm",
             "kernel": f"{ysingle(tuned, True)} (one-level Δinv y-pass with the ψ boundary terms; "

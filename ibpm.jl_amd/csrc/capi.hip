@@ -217,7 +217,7 @@ static void graphs_clear(ibpm_prob* P);
 
 // Device buffers of the tangent-linear and adjoint steps (MovingGrid bodies).
 // Per level: nG for circulation-like fields, nq for fluxes, nbc for boundary
-// vectors.  snap: the binomial checkpoints, (Γ, nonlin[2]) each.
+// vectors.  snap: the binomial checkpoints, (Γ, nonlin[2], ψ) each.
 struct AdjWork {
     int nsteps_cap = 0, nsnap = 0;
     double *dG = nullptr, *dGs = nullptr, *dN[2] = {nullptr, nullptr}, *tG = nullptr, *dpsi = nullptr;
@@ -1009,9 +1009,10 @@ static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, do
         mx_vort2flux(P, P->G, P->q, false);
         P->q_stale = false;
     } else {
-        // the full x-transform: the state's ψ stays vort2flux!(Γ) bit for bit
-        // (what a restart or a checkpoint recomputes from Γ alone)
-        vort2flux_dev(P, P->G, P->psi, P->q, CURL_NONE);
+        // Γ = Γs - X(Ŷ) on level 1 (Ŷ: the projection's y-pass output, in T1's
+        // level 1) and Γs = X(T2) off coarsify!'s rows, as in the trial chain.
+        // ψ differs from a full vort2flux!(Γ) by rounding: checkpoints keep ψ.
+        vort2flux_dev(P, P->G, P->psi, P->q, CURL_NONE, P->T2, P->nbt > 0 ? P->T1 : nullptr);
     }
     (void)nG;
 }
@@ -1394,7 +1395,7 @@ static int adj_setup(ibpm_prob* P, int nsteps, int nsnap)
     }
     while ((int)W->snap.size() < nsnap) {
         double* sp = nullptr;
-        HIPTRY(P->ctx, walloc(W, &sp, 2 * (size_t)(nG * L)));
+        HIPTRY(P->ctx, walloc(W, &sp, 3 * (size_t)(nG * L)));
         W->snap.push_back(sp);
     }
     return IBPM_OK;
@@ -1674,12 +1675,16 @@ static void aux_state(ibpm_prob* P, bool save)
     }
 }
 
+// A checkpoint keeps ψ beside Γ: the step's vort2flux! forms ψ from the
+// trial's transforms (equal to vort2flux!(Γ) up to rounding), so recomputing
+// it from Γ would not replay the forward run bit for bit.
 static void snap_store(ibpm_prob* P, int slot)
 {
     const long n = P->g.nG * P->mg;
     double* sp = P->adj->snap[slot];
     launch_copy(sp, P->G, n, P->stream);
     launch_copy(sp + n, P->nl[P->nl2], n, P->stream);
+    launch_copy(sp + 2 * n, P->psi, n, P->stream);
 }
 static void snap_restore(ibpm_prob* P, int slot)
 {
@@ -1688,7 +1693,8 @@ static void snap_restore(ibpm_prob* P, int slot)
     P->nl1 = P->nl2;
     launch_copy(P->G, sp, n, P->stream);
     launch_copy(P->nl[P->nl2], sp + n, n, P->stream);
-    vort2flux_dev(P, P->G, P->psi, P->q, CURL_ALL);
+    launch_copy(P->psi, sp + 2 * n, n, P->stream);
+    P->q_stale = true;  // formed from ψ where it is read
 }
 
 static long binom(int n, int k)
@@ -2567,10 +2573,11 @@ YArgs gyargs(const ibpm_group* Gp, int op, double* buf)
 
 // vort2flux! (curl.jl:133-165) of every instance: the level chain runs over
 // all instances at once (one launch per pass and level)
-// yhat: as vort2flux_dev's (every instance's Ainv y-pass output, the trial
-// chain), so that a group's trial chain is an instance's bit for bit
+// yhat, ysub0: as vort2flux_dev's (every instance's Ainv y-pass output, and
+// the projection's y-pass output), so that a group's chains are an
+// instance's bit for bit
 void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, bool level1_only,
-                     const double* yhat = nullptr)
+                     const double* yhat = nullptr, const double* ysub0 = nullptr)
 {
     hipStream_t s = Gp->stream;
     const long nG = Gp->g.nG, ist = (long)Gp->L * nG;
@@ -2588,11 +2595,11 @@ void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, boo
     // then the chain (boundary terms in the y-pass, as in vort2flux_dev)
     XArgs a = gxargs(Gp);
     a.src = Gb; a.dst = Gp->T1; a.nlev = n * L; a.lev0 = 0; a.levmod = L;
-    // with yhat only the centre rows coarsify! writes (level 1's are
-    // transformed too, and not read)
+    // with yhat only the centre rows coarsify! writes on levels 2..L (level
+    // 1 keeps the projection's output that ysub0 points at)
     const int clo = Gp->crange[0].jc0 - 1, chi = Gp->crange[0].jc0 + Gp->crange[0].cj - 2;
     if (yhat) {
-        a.lsel = 1; a.llo = clo; a.lhi = chi;
+        a.lsel = 1; a.llo = clo; a.lhi = chi; a.minlev = 1;
     }
     if (!yhat || (L > 1 && chi >= clo)) launch_xpass(X_PLAIN, a, s);
     for (int l = L - 1; l >= 0; l--) {
@@ -2603,6 +2610,7 @@ void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, boo
             y.afac = 2.0 * (double)Gp->g.nx;
             y.clo = l > 0 ? clo : 1;
             y.chi = l > 0 ? chi : 0;
+            if (l == 0) y.ysub = ysub0;
         }
         if (l < L - 1) {  // (an instance whose coarsest level is l has none: the kernel checks)
             y.bcr = psib + (long)(l + 1) * nG;
@@ -2682,7 +2690,7 @@ void group_step(ibpm_group* Gp)
     p2.src = Gp->T1; p2.dst = Gp->Gs; p2.lstride = ist; p2.nlev = n; p2.lev0 = 0; p2.fixlev = 1;
     launch_xpass(X_SUB, p2, s);
     std::swap(Gp->G, Gp->Gs);
-    group_vort2flux(Gp, Gp->G, Gp->psi, Gp->q, false);
+    group_vort2flux(Gp, Gp->G, Gp->psi, Gp->q, false, Gp->T2, Gp->T1);
     // the members follow the group's buffer roles
     for (int i = 0; i < n; i++) {
         ibpm_prob* P = Gp->probs[i];

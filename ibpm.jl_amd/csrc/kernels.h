@@ -59,6 +59,8 @@ struct XArgs {
     int levmod;
     int fixlev;
     const InstPar* ipar;
+    // entries of a level below minlev (0-based) are skipped
+    int minlev;
     DstPlan pl;
     GridDev g;
 };

@@ -104,7 +104,8 @@ __global__ void __launch_bounds__(XW * WAVE) k_xpass(XArgs a)
     const int b = blockIdx.y;
     const int j = line + 1;
     const Entry en = batch_entry(b, a.lev0, a.levmod, a.fixlev, a.mg, a.ipar);
-    if (!en.active) return;  // the whole work-group: a level its instance does not have
+    // the whole work-group: a level its instance does not have, or one the launch skips
+    if (!en.active || en.lev < a.minlev) return;
     const int ntab = tab_doubles_t<PT>(a.pl);
     const DstTab tb = load_tab_t<PT, XW * WAVE>(lds, a.pl, threadIdx.x);
     __syncthreads();
