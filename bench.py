@@ -260,6 +260,9 @@ def main():
                          "per configuration with the CPU oracle beside it")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only autotune and time each kernel kind (for counter profiling); prints kernel_ms")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes (counter passes over the steps then launch the same "
+                         "kernels in every pass)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -309,7 +312,10 @@ def main():
     state.Γ = gaussian_vortex_gamma(g)
     ibpm.vort2flux_(state, prob)
 
-    tuned = {TUNED_NAMES[k]: v[0] for k, v in autotune(prob).items()}
+    if args.no_autotune:
+        tuned = {name: prob.get_option(k) for k, name in TUNED_NAMES.items()}
+    else:
+        tuned = {TUNED_NAMES[k]: v[0] for k, v in autotune(prob).items()}
     print(f"# setup {setup_s:.2f}s (B assembly on GPU {prob.info.setup_seconds:.2f}s, "
           f"|B-B'|/|B| = {prob.info.b_asymmetry:.2e}); tiles {tuned}; warmup {args.warmup} steps",
           file=sys.stderr, flush=True)
