@@ -123,6 +123,7 @@ struct ibpm_prob {
     int ytile_single = 4;  // same, one-level launches (vort2flux chain)
     int pipe = 1;          // IBPM_OPT_PIPE
     int ysplit = 1;        // IBPM_OPT_YSPLIT
+    int xreuse = 0;        // IBPM_OPT_XREUSE (default by grid size, set at creation)
     int xwaves = 4;        // lines per work-group of the plain x-passes over all levels (or a batch)
 
     // device state (IBState) and work arrays
@@ -389,6 +390,12 @@ static void set_nl_q0(const ibpm_prob* P, NLArgs& na)
     na.QB = P->lin ? P->QB : nullptr;
     na.GB = P->lin ? P->GB : nullptr;
 }
+
+// IBPM_OPT_XREUSE by default: on for grids of 2^19 cells and more, where the
+// x-transform the step's vort2flux! chains skip is worth more than the second
+// source their y-passes then read (measured: +4 % at 1024 x 1024, -3 to -5 %
+// at 800 x 400 and below)
+static int xreuse_default(int nx, int ny) { return (long)nx * ny >= (1L << 19) ? 1 : 0; }
 
 static YArgs yargs(ibpm_prob* P, int op, int lev0, int nlev, double* buf)
 {
@@ -1012,7 +1019,8 @@ static void step_body(ibpm_prob* P, const ibpm_step_scalars& sc, double* cdh, do
         // Γ = Γs - X(Ŷ) on level 1 (Ŷ: the projection's y-pass output, in T1's
         // level 1) and Γs = X(T2) off coarsify!'s rows, as in the trial chain.
         // ψ differs from a full vort2flux!(Γ) by rounding: checkpoints keep ψ.
-        vort2flux_dev(P, P->G, P->psi, P->q, CURL_NONE, P->T2, P->nbt > 0 ? P->T1 : nullptr);
+        vort2flux_dev(P, P->G, P->psi, P->q, CURL_NONE, P->xreuse ? P->T2 : nullptr,
+                      P->xreuse && P->nbt > 0 ? P->T1 : nullptr);
     }
     (void)nG;
 }
@@ -1050,7 +1058,7 @@ static void trial_dev(ibpm_prob* P)
     P->nl1 = P->nl2 = out;  // state.nonlin[2] .= state.nonlin[1]
     // Γs = X(T2) on every line coarsify! leaves alone: the chain x-transforms
     // only the centre rows, and reads the rest as 2 nx T2
-    vort2flux_dev(P, P->Gs, P->psi, P->qs1, CURL_TRIAL, P->T2);
+    vort2flux_dev(P, P->Gs, P->psi, P->qs1, CURL_TRIAL, P->xreuse ? P->T2 : nullptr);
 }
 
 // E of the trial flux qs[:,1] = curl!(ψ of the trial) read from ψ itself
@@ -1919,6 +1927,7 @@ int ibpm_problem_create(ibpm_ctx* ctx, const ibpm_grid_desc* gd, const ibpm_body
     g.nx = nx; g.ny = ny; g.n1 = nx - 1; g.n2 = ny - 1;
     g.nG = (long)(nx - 1) * (ny - 1);
     g.nu = (long)ny * (nx + 1); g.nv = (long)nx * (ny + 1); g.nq = g.nu + g.nv;
+    P->xreuse = xreuse_default(nx, ny);
     g.LEFT = 0; g.RIGHT = ny + 1; g.BOT = 2 * (ny + 1); g.TOP = 2 * (ny + 1) + nx + 1;
     g.nbc = 2 * (nx + 1) + 2 * (ny + 1);
     g.h = gd->h;
@@ -2505,7 +2514,7 @@ struct ibpm_group {
     double* hist = nullptr;  // = hb.d
     size_t hist_steps = 0;
     int pending = 0;
-    int ytile = 4, ytile_single = 4, xwaves = 4, pipe = 1, ysplit = 1;
+    int ytile = 4, ytile_single = 4, xwaves = 4, pipe = 1, ysplit = 1, xreuse = 0;
     // HIP graphs of one group step, one per buffer role (as ibpm_prob's);
     // every launch argument is fixed by the roles (the force arguments and
     // history pointers are read from device memory when the kernels run)
@@ -2664,7 +2673,7 @@ void group_step(ibpm_group* Gp)
     c.src = Gp->T2; c.dst = Gp->Gs; c.nlev = NE; c.lev0 = 0; c.levmod = L;
     launch_xpass(X_PLAIN, c, s);
     Gp->nl1 = Gp->nl2 = out;
-    group_vort2flux(Gp, Gp->Gs, Gp->psi, Gp->qs1, true, Gp->T2);
+    group_vort2flux(Gp, Gp->Gs, Gp->psi, Gp->qs1, true, Gp->xreuse ? Gp->T2 : nullptr);
     // ---- boundary_forces! and update_stress! of every instance
     GatherArgs ga{};
     ga.q = Gp->qs1; ga.q0 = Gp->q0; ga.w = Gp->w; ga.bidx = Gp->bidx; ga.out = Gp->Ftmp; ga.nbt = Gp->NBT; ga.g = g;
@@ -2690,7 +2699,7 @@ void group_step(ibpm_group* Gp)
     p2.src = Gp->T1; p2.dst = Gp->Gs; p2.lstride = ist; p2.nlev = n; p2.lev0 = 0; p2.fixlev = 1;
     launch_xpass(X_SUB, p2, s);
     std::swap(Gp->G, Gp->Gs);
-    group_vort2flux(Gp, Gp->G, Gp->psi, Gp->q, false, Gp->T2, Gp->T1);
+    group_vort2flux(Gp, Gp->G, Gp->psi, Gp->q, false, Gp->xreuse ? Gp->T2 : nullptr, Gp->xreuse ? Gp->T1 : nullptr);
     // the members follow the group's buffer roles
     for (int i = 0; i < n; i++) {
         ibpm_prob* P = Gp->probs[i];
@@ -2851,6 +2860,7 @@ int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* probs, ibpm_group*
     Gp->g = P0->g;
     Gp->tx = P0->tx; Gp->ty = P0->ty; Gp->cx = P0->cx; Gp->cy = P0->cy; Gp->crange = P0->crange;
     Gp->ytile = P0->ytile; Gp->ytile_single = P0->ytile_single; Gp->xwaves = P0->xwaves; Gp->pipe = P0->pipe; Gp->ysplit = P0->ysplit;
+    Gp->xreuse = P0->xreuse;
     Gp->probs.assign(probs, probs + n);
     auto bail = [&](int code, const std::string& m) {
         for (void* p : Gp->allocs) (void)hipFree(p);
@@ -3036,6 +3046,7 @@ int ibpm_group_set_option(ibpm_group* Gp, int option, int value)
     else if (option == IBPM_OPT_GRAPHS) Gp->graphs = value == 1;
     else if (option == IBPM_OPT_PIPE) Gp->pipe = value;
     else if (option == IBPM_OPT_YSPLIT) Gp->ysplit = value;
+    else if (option == IBPM_OPT_XREUSE) Gp->xreuse = value;
     return IBPM_OK;
 }
 
@@ -3414,6 +3425,11 @@ int ibpm_set_option(ibpm_prob* P, int option, int value)
         P->ysplit = value;
         return IBPM_OK;
     }
+    if (option == IBPM_OPT_XREUSE) {
+        if (value != 0 && value != 1) return fail(P->ctx, IBPM_E_ARG, "xreuse option is 0 or 1");
+        P->xreuse = value;
+        return IBPM_OK;
+    }
     if (option == IBPM_OPT_PROFILE) {
         if (value < 0 || value > 2) return fail(P->ctx, IBPM_E_ARG, "profile option is 0, 1 or 2");
         HIPTRY(P->ctx, hipStreamSynchronize(P->stream));
@@ -3436,6 +3452,7 @@ int ibpm_get_option(ibpm_prob* P, int option, int* value)
     else if (option == IBPM_OPT_XWAVES_BATCH) *value = P->xwaves;
     else if (option == IBPM_OPT_PIPE) *value = P->pipe;
     else if (option == IBPM_OPT_YSPLIT) *value = P->ysplit;
+    else if (option == IBPM_OPT_XREUSE) *value = P->xreuse;
     else if (option == IBPM_OPT_PRECISION) *value = P->prec < 0 ? 0 : (P->prec == MX_FP32 ? 1 : 2);
     else if (option == IBPM_OPT_MX_SWEEPS) *value = P->mx_sweeps;
     else return fail(P->ctx, IBPM_E_ARG, "unknown option");

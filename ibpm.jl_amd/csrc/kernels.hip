@@ -418,7 +418,9 @@ constexpr int YSPLIT_COL = 2 * f1024::SLOT + 2 * fsplit::ZS;
 // lines, YW = 2 TI): each column is transformed by two wavefronts at once
 // (dst1024_split), which puts two wavefronts on every SIMD of a single-level
 // launch instead of one.
-template <int TI, int PT, int YW, bool BC, bool SPLIT = false>
+// YSRC: the launch reads some rows from a.ysrc (a separate instance, so that
+// launches without keep their register budget and occupancy).
+template <int TI, int PT, int YW, bool BC, bool SPLIT = false, bool YSRC = false>
 __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
 {
     static_assert(!SPLIT || (PT == 512 && YW == 2 * TI), "the split y-pass takes two wavefronts per 1023-point column");
@@ -568,18 +570,25 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     // (2) the first chunk of the tile (rows outside rlo..rhi are zero when rsel;
     // rows outside clo..chi come from ysrc when it is given)
     const int rlo = a.rsel ? a.rlo : 0, rhi = a.rsel ? a.rhi : n2 - 1;
-    const double* ys = a.ysrc ? a.ysrc + b * a.lstride : nullptr;
+    const double* ys = a.ysrc + b * a.lstride;  // read only in the YSRC instance
     const double* yb = a.ysub ? a.ysub + b * a.lstride : nullptr;
     auto load_chunk = [&](int rb, double* vv) {
-        bool alt[CH];
+        if constexpr (!YSRC) {
 #pragma unroll
-        for (int t = 0; t < CH; t++) {
-            const int r = trow(rb, t);
-            const bool ok = colok && r >= rlo && r <= rhi;
-            alt[t] = ok && ys && (r < a.clo || r > a.chi);
-            vv[t] = ok ? (alt[t] ? ys : base)[ic + (long)n1 * r] : 0.0;
-        }
-        if (ys) {
+            for (int t = 0; t < CH; t++) {
+                const int r = trow(rb, t);
+                vv[t] = (colok && r >= rlo && r <= rhi) ? base[ic + (long)n1 * r] : 0.0;
+            }
+        } else {
+            // every load of the chunk is issued before the first is used
+            bool alt[CH];
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                const int r = trow(rb, t);
+                const bool ok = colok && r >= rlo && r <= rhi;
+                alt[t] = ok && (r < a.clo || r > a.chi);
+                vv[t] = ok ? (alt[t] ? ys : base)[ic + (long)n1 * r] : 0.0;
+            }
             double sb[CH];
 #pragma unroll
             for (int t = 0; t < CH; t++) sb[t] = (yb && alt[t]) ? yb[ic + (long)n1 * trow(rb, t)] : 0.0;
@@ -776,13 +785,20 @@ size_t ypass_lds_bytes(const DstPlan& pl, int n2, int tile)
 // the split y-pass: 4-column tiles of 1023-point columns, 8 wavefronts
 static void launch_y_split(const YArgs& a, hipStream_t s)
 {
-    static bool once = (allow_big_lds(k_ypass<4, 512, 8, false, true>), allow_big_lds(k_ypass<4, 512, 8, true, true>), true);
+    static bool once = (allow_big_lds(k_ypass<4, 512, 8, false, true>), allow_big_lds(k_ypass<4, 512, 8, true, true>),
+                        allow_big_lds(k_ypass<4, 512, 8, false, true, true>),
+                        allow_big_lds(k_ypass<4, 512, 8, true, true, true>), true);
     (void)once;
     const int nblocks = ((a.g.n1 + 3) / 4) * a.nlev;
     dim3 grid(NXCD * xcd_nper(nblocks));
     const size_t lds = ypass_split_lds_bytes(a.g.ny, a.bcr != nullptr);
-    if (a.bcr) hipLaunchKernelGGL((k_ypass<4, 512, 8, true, true>), grid, dim3(8 * WAVE), lds, s, a);
-    else hipLaunchKernelGGL((k_ypass<4, 512, 8, false, true>), grid, dim3(8 * WAVE), lds, s, a);
+    if (a.ysrc) {
+        if (a.bcr) hipLaunchKernelGGL((k_ypass<4, 512, 8, true, true, true>), grid, dim3(8 * WAVE), lds, s, a);
+        else hipLaunchKernelGGL((k_ypass<4, 512, 8, false, true, true>), grid, dim3(8 * WAVE), lds, s, a);
+    } else {
+        if (a.bcr) hipLaunchKernelGGL((k_ypass<4, 512, 8, true, true>), grid, dim3(8 * WAVE), lds, s, a);
+        else hipLaunchKernelGGL((k_ypass<4, 512, 8, false, true>), grid, dim3(8 * WAVE), lds, s, a);
+    }
 }
 
 size_t ypass_split_lds_bytes(int ny, bool bc)
@@ -794,14 +810,21 @@ size_t ypass_split_lds_bytes(int ny, bool bc)
 template <int TI, int PT, int YW>
 static void launch_y_t(const YArgs& a, hipStream_t s)
 {
-    static bool once = (allow_big_lds(k_ypass<TI, PT, YW, false>), allow_big_lds(k_ypass<TI, PT, YW, true>), true);
+    static bool once = (allow_big_lds(k_ypass<TI, PT, YW, false>), allow_big_lds(k_ypass<TI, PT, YW, true>),
+                        allow_big_lds(k_ypass<TI, PT, YW, false, false, true>),
+                        allow_big_lds(k_ypass<TI, PT, YW, true, false, true>), true);
     (void)once;
     const int nblocks = ((a.g.n1 + TI - 1) / TI) * a.nlev;
     dim3 grid(NXCD * xcd_nper(nblocks));
     const size_t lds = ypass_lds_bytes_t<PT>(a.pl, a.g.n2, TI) +
                        (a.bcr ? sizeof(double2) * (size_t)(ep_slot(a.g.ny / 2) + 1) : 0);
-    if (a.bcr) hipLaunchKernelGGL((k_ypass<TI, PT, YW, true>), grid, dim3(YW * WAVE), lds, s, a);
-    else hipLaunchKernelGGL((k_ypass<TI, PT, YW, false>), grid, dim3(YW * WAVE), lds, s, a);
+    if (a.ysrc) {
+        if (a.bcr) hipLaunchKernelGGL((k_ypass<TI, PT, YW, true, false, true>), grid, dim3(YW * WAVE), lds, s, a);
+        else hipLaunchKernelGGL((k_ypass<TI, PT, YW, false, false, true>), grid, dim3(YW * WAVE), lds, s, a);
+    } else {
+        if (a.bcr) hipLaunchKernelGGL((k_ypass<TI, PT, YW, true>), grid, dim3(YW * WAVE), lds, s, a);
+        else hipLaunchKernelGGL((k_ypass<TI, PT, YW, false>), grid, dim3(YW * WAVE), lds, s, a);
+    }
 }
 
 template <int TI, int YW>

@@ -32,6 +32,7 @@ OPT_YTILE_BATCH, OPT_YTILE_SINGLE, OPT_PROFILE, OPT_GRAPHS, OPT_XWAVES_BATCH, OP
 OPT_MX_SWEEPS = 6
 OPT_PIPE = 7
 OPT_YSPLIT = 8
+OPT_XREUSE = 9
 PREC_FP64, PREC_FP32_MFMA, PREC_BF16X3_MFMA = 0, 1, 2
 
 EXPORTED = [

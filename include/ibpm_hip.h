@@ -308,6 +308,10 @@ int ibpm_op_update_bodies(ibpm_prob* prob, const double* kin);
 #define IBPM_OPT_YSPLIT 8        /* 1 (default): the one-level y-passes over 1023-point columns (ny = 1024,
                                     4-column tiles) transform each column with two wavefronts at once (its
                                     even and its odd outputs); 0: one wavefront per column */
+#define IBPM_OPT_XREUSE 9        /* 1: the step's vort2flux! chains x-transform only the lines coarsify!
+                                    rewrites and read the others as 2 nx times Ainv's y-pass output (psi
+                                    then equals vort2flux!(Gamma) up to rounding); 0: the full x-transform.
+                                    Default 1 for grids of nx ny >= 2^19 cells, else 0 */
 int ibpm_set_option(ibpm_prob* prob, int option, int value);
 int ibpm_get_option(ibpm_prob* prob, int option, int* value);
 /* Average device ms and count of the y-pass launches timed inside the steps

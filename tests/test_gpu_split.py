@@ -1,9 +1,13 @@
-"""The one-level y-pass of 1023-point columns in both forms (IBPM_OPT_YSPLIT):
+"""The faster forms of the step against the oracle.
+
+The one-level y-pass of 1023-point columns in both forms (IBPM_OPT_YSPLIT):
 1 (the default), each column transformed by two wavefronts at once, the even
 outputs as a 511-point DST-I of the differences x_j - x_{M-j} and the odd
 outputs as a DCT-III of the sums; 0, one wavefront per column.  Against the
 oracle with the tolerances of test_gpu_parity's full-size cases: operators
-1e-11/1e-12, steps 1e-10."""
+1e-11/1e-12, steps 1e-10.  And the vort2flux! chains that read Γ's
+x-transform back from Ainv's output (IBPM_OPT_XREUSE) on a small grid, where
+it is off by default, alone and in a group."""
 import os
 
 import numpy as np
@@ -103,3 +107,43 @@ def test_split_steps_match_oracle(ysplit):
     errs["F̃b smoothed"] = relerr(_smooth_along_body(st.F̃b), _smooth_along_body(S.Fb))
     print("ysplit", ysplit, {k: f"{v:.1e}" for k, v in errs.items()})
     assert errs["F̃b smoothed"] < STEP_TOL
+
+
+@pytest.mark.parametrize("name,xreuse", [("small3", 1), ("moving", 1), ("small3", 0)])
+def test_xreuse_steps_match_oracle(name, xreuse):
+    """IBPM_OPT_XREUSE on a small grid (where it is off by default): the
+    vort2flux! chains of the step x-transform only the lines coarsify!
+    rewrites and read the rest from Ainv's y-pass output; 30 steps against
+    the oracle, static and moving grid."""
+    from ibpm import _capi as C
+    from test_gpu_parity import build
+    c, op, gp = build(name, share_binv=True)
+    gp.set_option(C.OPT_XREUSE, xreuse)
+    S, st = initial_states(c, op, gp)
+    ts = np.arange(30) * c["dt"]
+    cd_o, cl_o = op.run(S, np.stack([sc7(c, t) for t in ts]))
+    cd, cl, done = ibpm.advance_n_(st, gp, ts)
+    assert done == 30
+    assert relerr(cd, cd_o) < STEP_TOL
+    compare_state(S, st, STEP_TOL)
+
+
+def test_group_equals_single_with_xreuse():
+    """Groups follow their problems' IBPM_OPT_XREUSE and stay bit-identical to
+    the members stepped alone with it on."""
+    from ibpm import _capi as C
+    from test_gpu_group import fields, members
+    pa, sa = members()
+    pg, sg = members([p.Binv for p in pa])
+    for p in pa + pg:
+        p.set_option(C.OPT_XREUSE, 1)
+    n1 = 8
+    alone = [ibpm.advance_n_(st, p, np.arange(n1) * p.scheme.dt) for p, st in zip(pa, sa)]
+    grp = ibpm.Group(pg, sg)
+    res = grp.advance_n_(n1)
+    for (cd, cl, d), (cdg, clg, dg), st, stg in zip(alone, res, sa, sg):
+        assert d == dg == n1
+        assert np.array_equal(cd, cdg) and np.array_equal(cl, clg)
+        for a, b in zip(fields(st), fields(stg)):
+            assert np.array_equal(a, b)
+    grp.close()
