@@ -2519,9 +2519,13 @@ struct ibpm_group {
     // every launch argument is fixed by the roles (the force arguments and
     // history pointers are read from device memory when the kernels run)
     bool graphs = true;
+    // q_stale: the group's q is not formed (the steps leave it to ψ, as the
+    // members' own FP64 steps do); its first step after creation reads q
+    bool q_stale = false;
     struct StepGraph {
         double* G;
         int out, nl2;
+        bool qs;
         hipGraphExec_t exec;
     };
     std::vector<StepGraph> gcache;
@@ -2585,6 +2589,7 @@ YArgs gyargs(const ibpm_group* Gp, int op, double* buf)
 // yhat, ysub0: as vort2flux_dev's (every instance's Ainv y-pass output, and
 // the projection's y-pass output), so that a group's chains are an
 // instance's bit for bit
+// qout null: no curl! (the step's readers form the fluxes from ψ)
 void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, bool level1_only,
                      const double* yhat = nullptr, const double* ysub0 = nullptr)
 {
@@ -2637,6 +2642,7 @@ void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, boo
         }
         launch_xpass(X_PLAIN, c, s);
     }
+    if (!qout) return;
     CurlArgs ca{};
     ca.psi = psib; ca.q = qout; ca.qstride = Gp->g.nq; ca.mg = L; ca.g = Gp->g; ca.lev0 = 0; ca.ipar = Gp->ipar;
     if (level1_only) {
@@ -2645,6 +2651,22 @@ void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, boo
         ca.nlev = n * L; ca.levmod = L;
     }
     launch_curl(ca, s);
+}
+
+// the host side of a group step: the members follow the group's buffer roles,
+// and q is left to ψ (formed where something reads it, ensure_q)
+static void group_roles(ibpm_group* Gp)
+{
+    const long ist = (long)Gp->L * Gp->g.nG;
+    Gp->q_stale = true;
+    for (int i = 0; i < Gp->n; i++) {
+        ibpm_prob* P = Gp->probs[i];
+        P->G = Gp->G + (long)i * ist;
+        P->Gs = Gp->Gs + (long)i * ist;
+        P->nl1 = Gp->nl1;
+        P->nl2 = Gp->nl2;
+        P->q_stale = true;
+    }
 }
 
 // advance!(state, prob, t) of every instance (timestepping.jl:35-67), static bodies
@@ -2658,6 +2680,7 @@ void group_step(ibpm_group* Gp)
     const int out = (Gp->nl1 != Gp->nl2) ? Gp->nl1 : 1 - Gp->nl2;
     NLArgs na{};
     na.q = Gp->q; na.q0 = Gp->q0; na.G = Gp->G; na.out = Gp->nl[out];
+    na.psi = Gp->q_stale ? Gp->psi : nullptr;  // the fluxes from ψ once the steps leave q unformed
     na.nlev = NE; na.lev0 = 0; na.mg = L; na.g = g; na.levmod = L; na.ipar = Gp->ipar;
     na.q0_uniform = Gp->q0_uniform ? 1 : 0;
     launch_nonlinear(na, s);
@@ -2673,10 +2696,12 @@ void group_step(ibpm_group* Gp)
     c.src = Gp->T2; c.dst = Gp->Gs; c.nlev = NE; c.lev0 = 0; c.levmod = L;
     launch_xpass(X_PLAIN, c, s);
     Gp->nl1 = Gp->nl2 = out;
-    group_vort2flux(Gp, Gp->Gs, Gp->psi, Gp->qs1, true, Gp->xreuse ? Gp->T2 : nullptr);
-    // ---- boundary_forces! and update_stress! of every instance
+    group_vort2flux(Gp, Gp->Gs, Gp->psi, nullptr, true, Gp->xreuse ? Gp->T2 : nullptr);
+    // ---- boundary_forces! and update_stress! of every instance; E forms the
+    // trial flux qs[:,1] from the trial ψ (curl!'s values)
     GatherArgs ga{};
-    ga.q = Gp->qs1; ga.q0 = Gp->q0; ga.w = Gp->w; ga.bidx = Gp->bidx; ga.out = Gp->Ftmp; ga.nbt = Gp->NBT; ga.g = g;
+    ga.q = nullptr; ga.q0 = Gp->q0; ga.w = Gp->w; ga.bidx = Gp->bidx; ga.out = Gp->Ftmp; ga.nbt = Gp->NBT; ga.g = g;
+    ga.psi = Gp->psi; ga.psic = L > 1 ? Gp->psi + nG : nullptr; ga.psistride = ist; ga.ipar = Gp->ipar;
     ga.qstride = g.nq; ga.q0stride = (long)L * g.nq;
     ga.pinst = Gp->pinst; ga.plocal = Gp->plocal; ga.inbt = Gp->inbt; ga.ifoff = Gp->ifoff;
     launch_gather(ga, s);
@@ -2699,15 +2724,8 @@ void group_step(ibpm_group* Gp)
     p2.src = Gp->T1; p2.dst = Gp->Gs; p2.lstride = ist; p2.nlev = n; p2.lev0 = 0; p2.fixlev = 1;
     launch_xpass(X_SUB, p2, s);
     std::swap(Gp->G, Gp->Gs);
-    group_vort2flux(Gp, Gp->G, Gp->psi, Gp->q, false, Gp->xreuse ? Gp->T2 : nullptr, Gp->xreuse ? Gp->T1 : nullptr);
-    // the members follow the group's buffer roles
-    for (int i = 0; i < n; i++) {
-        ibpm_prob* P = Gp->probs[i];
-        P->G = Gp->G + (long)i * ist;
-        P->Gs = Gp->Gs + (long)i * ist;
-        P->nl1 = Gp->nl1;
-        P->nl2 = Gp->nl2;
-    }
+    group_vort2flux(Gp, Gp->G, Gp->psi, nullptr, false, Gp->xreuse ? Gp->T2 : nullptr, Gp->xreuse ? Gp->T1 : nullptr);
+    group_roles(Gp);
 }
 
 // history blocks of every instance for `steps` steps, and their pointers in
@@ -2753,25 +2771,19 @@ void group_step_dev(ibpm_group* Gp)
     }
     const int out = (Gp->nl1 != Gp->nl2) ? Gp->nl1 : 1 - Gp->nl2;
     for (auto& e : Gp->gcache) {
-        if (e.G == Gp->G && e.out == out && e.nl2 == Gp->nl2) {
+        if (e.G == Gp->G && e.out == out && e.nl2 == Gp->nl2 && e.qs == Gp->q_stale) {
             (void)hipGraphLaunch(e.exec, Gp->stream);
             // the host side of group_step: roles of the buffers
             Gp->nl1 = Gp->nl2 = out;
             std::swap(Gp->G, Gp->Gs);
-            for (int i = 0; i < Gp->n; i++) {
-                ibpm_prob* P = Gp->probs[i];
-                const long ist = (long)Gp->L * Gp->g.nG;
-                P->G = Gp->G + (long)i * ist;
-                P->Gs = Gp->Gs + (long)i * ist;
-                P->nl1 = Gp->nl1;
-                P->nl2 = Gp->nl2;
-            }
+            group_roles(Gp);
             return;
         }
     }
-    ibpm_group::StepGraph e{Gp->G, out, Gp->nl2, nullptr};
+    ibpm_group::StepGraph e{Gp->G, out, Gp->nl2, Gp->q_stale, nullptr};
     double *G0 = Gp->G, *Gs0 = Gp->Gs;
     const int nl1 = Gp->nl1, nl2 = Gp->nl2;
+    const bool qs0 = Gp->q_stale;
     hipGraph_t graph = nullptr;
     bool ok = hipStreamBeginCapture(Gp->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
@@ -2789,7 +2801,8 @@ void group_step_dev(ibpm_group* Gp)
     if (e.exec) (void)hipGraphExecDestroy(e.exec);
     (void)hipGetLastError();
     Gp->graphs = false;
-    Gp->G = G0; Gp->Gs = Gs0; Gp->nl1 = nl1; Gp->nl2 = nl2;
+    Gp->G = G0; Gp->Gs = Gs0; Gp->nl1 = nl1; Gp->nl2 = nl2; Gp->q_stale = qs0;
+    for (int i = 0; i < Gp->n; i++) Gp->probs[i]->q_stale = qs0;
     group_step(Gp);
 }
 
@@ -2801,7 +2814,7 @@ void group_step_dev(ibpm_group* Gp)
 struct MultiGraph {
     std::vector<ibpm_group*> groups;
     std::vector<double*> G;
-    std::vector<int> out, nl2;
+    std::vector<int> out, nl2, qs;
     hipGraphExec_t exec;
 };
 std::vector<MultiGraph> g_multi;
@@ -3019,7 +3032,7 @@ int ibpm_group_destroy(ibpm_group* Gp)
         if (!ok) rc = IBPM_E_HIP;
         P->G = o.G; P->Gs = o.Gs; P->psi = o.psi; P->nl[0] = o.nl0; P->nl[1] = o.nl1b; P->q = o.q; P->q0 = o.q0;
         P->Fb = o.Fb;
-        P->q_stale = false;  // a group step stores q
+        P->q_stale = Gp->q_stale;  // formed from ψ where it is read, as after the member's own steps
         P->group = nullptr;
     }
     (void)hipStreamSynchronize(Gp->stream);
@@ -3110,7 +3123,8 @@ int ibpm_groups_advance_async(int ngroups, ibpm_group* const* groups, int nsteps
             for (int a = 0; a < ngroups && same; a++) {
                 ibpm_group* Gp = groups[a];
                 const int out = (Gp->nl1 != Gp->nl2) ? Gp->nl1 : 1 - Gp->nl2;
-                same = m.groups[a] == Gp && m.G[a] == Gp->G && m.out[a] == out && m.nl2[a] == Gp->nl2;
+                same = m.groups[a] == Gp && m.G[a] == Gp->G && m.out[a] == out && m.nl2[a] == Gp->nl2 &&
+                       m.qs[a] == (int)Gp->q_stale;
             }
             if (same) { mg = &m; break; }
         }
@@ -3121,14 +3135,7 @@ int ibpm_groups_advance_async(int ngroups, ibpm_group* const* groups, int nsteps
                 const int out = (Gp->nl1 != Gp->nl2) ? Gp->nl1 : 1 - Gp->nl2;
                 Gp->nl1 = Gp->nl2 = out;
                 std::swap(Gp->G, Gp->Gs);
-                const long ist = (long)Gp->L * Gp->g.nG;
-                for (int i = 0; i < Gp->n; i++) {
-                    ibpm_prob* P = Gp->probs[i];
-                    P->G = Gp->G + (long)i * ist;
-                    P->Gs = Gp->Gs + (long)i * ist;
-                    P->nl1 = Gp->nl1;
-                    P->nl2 = Gp->nl2;
-                }
+                group_roles(Gp);
             }
             continue;
         }
@@ -3139,6 +3146,7 @@ int ibpm_groups_advance_async(int ngroups, ibpm_group* const* groups, int nsteps
             m.G.push_back(Gp->G);
             m.out.push_back((Gp->nl1 != Gp->nl2) ? Gp->nl1 : 1 - Gp->nl2);
             m.nl2.push_back(Gp->nl2);
+            m.qs.push_back((int)Gp->q_stale);
         }
         hipGraph_t graph = nullptr;
         HIPTRY(C, hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));

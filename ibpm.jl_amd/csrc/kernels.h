@@ -190,11 +190,15 @@ void launch_rot(const double* q, double* G, int ib0, int ib1, int jb0, int jb1, 
 
 struct GatherArgs {
     const double* q;
-    // when psi is non-null (one problem, level 1), the fluxes are formed from
-    // psi by curl! with the boundary values of psic (the level-2 psi; null:
-    // zero) instead of being read from q
+    // when psi is non-null (level 1), the fluxes are formed from psi by curl!
+    // with the boundary values of psic (the level-2 psi; null: zero) instead
+    // of being read from q.  Over a group, instance inst's psi is at
+    // psi + inst * psistride, its psic nG further on (when psic != null and
+    // the instance has two levels or more, ipar[inst].L)
     const double* psi;
     const double* psic;
+    long psistride;
+    const InstPar* ipar;
     const double* q0;  // may be null
     const double* w;   // [(2k + c) * 169 + l + 13 m]
     const int* bidx;   // [k] = bx, [nbt + k] = by (1-based)

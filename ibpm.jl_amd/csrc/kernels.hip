@@ -1580,12 +1580,20 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs a)
     const int c = wid / a.nbt, k = wid % a.nbt;
     const double* __restrict__ qf = a.q + (long)bq * a.qstride;
     const double* __restrict__ q0f = a.q0;
+    const double* __restrict__ ps = a.psi;
+    const double* __restrict__ pc = a.psic;
+    bool hasbc = a.psic != nullptr;
     long o = (long)bq * a.ostride + c * a.nbt + k;
     if (a.pinst) {  // a group: point k is point plocal[k] of instance pinst[k]
         const int inst = a.pinst[k];
         qf = a.q + (long)inst * a.qstride;
         if (q0f) q0f += (long)inst * a.q0stride;
         o = a.ifoff[inst] + (long)c * a.inbt[inst] + a.plocal[k];
+        if (ps) {
+            ps = a.psi + (long)inst * a.psistride;
+            pc = ps + a.g.nG;
+            hasbc = hasbc && (!a.ipar || a.ipar[inst].L > 1);
+        }
     }
     const int bx = a.bidx[k], by = a.bidx[a.nbt + k];
     const double* w = a.w + (long)(2 * k + c) * 169;
@@ -1596,8 +1604,7 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs a)
         const int i = bx + l - 6, j = by + m - 6;
         long p = (c == 0) ? (i - 1) + (long)(nx + 1) * (j - 1) : a.g.nu + (i - 1) + (long)nx * (j - 1);
         double f;
-        if (a.psi) f = (c == 0) ? curl_qx(a.psi, a.psic, a.psic != nullptr, a.g, i, j)
-                                : curl_qy(a.psi, a.psic, a.psic != nullptr, a.g, i, j);
+        if (ps) f = (c == 0) ? curl_qx(ps, pc, hasbc, a.g, i, j) : curl_qy(ps, pc, hasbc, a.g, i, j);
         else f = qf[p];
         double v = q0f ? (f + q0f[p]) : f;
         acc += w[e] * v;
