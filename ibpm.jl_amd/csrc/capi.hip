@@ -2522,6 +2522,8 @@ struct ibpm_group {
     // q_stale: the group's q is not formed (the steps leave it to ψ, as the
     // members' own FP64 steps do); its first step after creation reads q
     bool q_stale = false;
+    // level-1 lines E reads of every member (their union; -1: not all interior)
+    int tlo = -1, thi = -1;
     struct StepGraph {
         double* G;
         int out, nl2;
@@ -2589,9 +2591,11 @@ YArgs gyargs(const ibpm_group* Gp, int op, double* buf)
 // yhat, ysub0: as vort2flux_dev's (every instance's Ainv y-pass output, and
 // the projection's y-pass output), so that a group's chains are an
 // instance's bit for bit
-// qout null: no curl! (the step's readers form the fluxes from ψ)
+// qout null: no curl! (the step's readers form the fluxes from ψ).  trial:
+// as vort2flux_dev's CURL_TRIAL, only the ψ lines E and the chain read are
+// transformed and stored (the members' level-1 lines tlo..thi, their union)
 void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, bool level1_only,
-                     const double* yhat = nullptr, const double* ysub0 = nullptr)
+                     const double* yhat = nullptr, const double* ysub0 = nullptr, bool trial = false)
 {
     hipStream_t s = Gp->stream;
     const long nG = Gp->g.nG, ist = (long)Gp->L * nG;
@@ -2616,9 +2620,15 @@ void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, boo
         a.lsel = 1; a.llo = clo; a.lhi = chi; a.minlev = 1;
     }
     if (!yhat || (L > 1 && chi >= clo)) launch_xpass(X_PLAIN, a, s);
+    const bool rtrial = trial && !qout && Gp->tlo >= 0;
     for (int l = L - 1; l >= 0; l--) {
         YArgs y = gyargs(Gp, Y_DINV, Gp->T1 + (long)l * nG);
         y.lstride = ist; y.nlev = n; y.lev0 = l; y.fixlev = 1;
+        if (rtrial) {  // the rows the trial's x-pass of this level reads (below)
+            y.wsel = 1;
+            y.wlo = l > 0 ? Gp->g.ny / 4 - 1 : Gp->tlo;
+            y.whi = l > 0 ? 3 * Gp->g.ny / 4 - 1 : Gp->thi;
+        }
         if (yhat) {
             y.ysrc = yhat + (long)l * nG;
             y.afac = 2.0 * (double)Gp->g.nx;
@@ -2639,6 +2649,17 @@ void group_vort2flux(ibpm_group* Gp, double* Gb, double* psib, double* qout, boo
         if (l > 0) {
             c.edge = Gp->edge;
             c.edstride = 2L * Gp->g.n2;
+        }
+        if (rtrial) {
+            c.lsel = 1;
+            if (l > 0) {
+                c.llo = c.keep0 = Gp->g.ny / 4 - 1;
+                c.lhi = c.keep1 = 3 * Gp->g.ny / 4 - 1;
+                c.keep = 1;
+            } else {
+                c.llo = Gp->tlo;
+                c.lhi = Gp->thi;
+            }
         }
         launch_xpass(X_PLAIN, c, s);
     }
@@ -2696,7 +2717,7 @@ void group_step(ibpm_group* Gp)
     c.src = Gp->T2; c.dst = Gp->Gs; c.nlev = NE; c.lev0 = 0; c.levmod = L;
     launch_xpass(X_PLAIN, c, s);
     Gp->nl1 = Gp->nl2 = out;
-    group_vort2flux(Gp, Gp->Gs, Gp->psi, nullptr, true, Gp->xreuse ? Gp->T2 : nullptr);
+    group_vort2flux(Gp, Gp->Gs, Gp->psi, nullptr, true, Gp->xreuse ? Gp->T2 : nullptr, nullptr, true);
     // ---- boundary_forces! and update_stress! of every instance; E forms the
     // trial flux qs[:,1] from the trial ψ (curl!'s values)
     GatherArgs ga{};
@@ -2874,6 +2895,15 @@ int ibpm_group_create(ibpm_ctx* ctx, int n, ibpm_prob* const* probs, ibpm_group*
     Gp->tx = P0->tx; Gp->ty = P0->ty; Gp->cx = P0->cx; Gp->cy = P0->cy; Gp->crange = P0->crange;
     Gp->ytile = P0->ytile; Gp->ytile_single = P0->ytile_single; Gp->xwaves = P0->xwaves; Gp->pipe = P0->pipe; Gp->ysplit = P0->ysplit;
     Gp->xreuse = P0->xreuse;
+    Gp->tlo = P0->tlo;
+    Gp->thi = P0->thi;
+    for (int i = 1; i < n && Gp->tlo >= 0; i++) {
+        if (probs[i]->tlo < 0) Gp->tlo = Gp->thi = -1;
+        else {
+            Gp->tlo = std::min(Gp->tlo, probs[i]->tlo);
+            Gp->thi = std::max(Gp->thi, probs[i]->thi);
+        }
+    }
     Gp->probs.assign(probs, probs + n);
     auto bail = [&](int code, const std::string& m) {
         for (void* p : Gp->allocs) (void)hipFree(p);
