@@ -570,8 +570,8 @@ __global__ void __launch_bounds__(YW * WAVE) k_ypass(YArgs a)
     // (2) the first chunk of the tile (rows outside rlo..rhi are zero when rsel;
     // rows outside clo..chi come from ysrc when it is given)
     const int rlo = a.rsel ? a.rlo : 0, rhi = a.rsel ? a.rhi : n2 - 1;
-    const double* ys = a.ysrc + b * a.lstride;  // read only in the YSRC instance
-    const double* yb = a.ysub ? a.ysub + b * a.lstride : nullptr;
+    const double* ys = YSRC ? a.ysrc + b * a.lstride : nullptr;
+    const double* yb = (YSRC && a.ysub) ? a.ysub + b * a.lstride : nullptr;
     auto load_chunk = [&](int rb, double* vv) {
         if constexpr (!YSRC) {
 #pragma unroll
