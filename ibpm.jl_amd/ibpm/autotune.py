@@ -50,3 +50,50 @@ def autotune(prob, reps: int = 20) -> dict:
         prob.ctx.check(L.ibpm_set_option(prob.h, opt, best), "ibpm_set_option")
         out[opt] = (best, times)
     return out
+
+
+# options of a group's all-level passes, swept by timing whole group steps:
+# the per-problem sweep times one instance's launch, while a group launches
+# every instance at once and may prefer other shapes
+GROUP_SWEEPS = (
+    ((C.OPT_YTILE_BATCH, (4, 8, 12)), (C.OPT_PIPE, (1, 0))),
+    ((C.OPT_XWAVES_BATCH, (4, 8, 12, 14)),),
+)
+
+
+def autotune_group(group, steps: int = 6) -> dict:
+    """Sweep the all-level launch shapes of `group` (the y tile with the
+    persistent work-groups, then the x-pass work-group size) by the wall time
+    of `steps` group steps per candidate, best of two; apply the fastest and
+    return {option: best value}.  The timed steps advance the group's state,
+    as warm-up steps would; every shape gives the same results."""
+    import itertools
+    import time
+
+    def step_us():
+        best = float("inf")
+        for _ in range(2):
+            t = time.perf_counter()
+            group.advance_async(steps)
+            group.wait()
+            best = min(best, (time.perf_counter() - t) / steps * 1e6)
+        return best
+
+    chosen = {}
+    for sweep in GROUP_SWEEPS:
+        opts = [o for o, _ in sweep]
+        times = {}
+        for combo in itertools.product(*[vals for _, vals in sweep]):
+            try:
+                for o, v in zip(opts, combo):
+                    group.set_option(o, v)
+            except C.IBPMError:
+                continue  # does not fit this grid
+            group.advance_async(1)
+            group.wait()
+            times[combo] = step_us()
+        best = min(times, key=times.get)
+        for o, v in zip(opts, best):
+            group.set_option(o, v)
+            chosen[o] = v
+    return chosen

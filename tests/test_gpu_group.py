@@ -187,3 +187,32 @@ def test_joint_graph_of_three_naca_shapes_matches_oracle():
         g = c["grid"]
         print(f"{g.nx}x{g.ny}x{g.mg}: final CD {cd[-1, 0]:.8f} CL {cl[-1, 0]:.3e};", errs)
         assert all(v <= 1e-10 for v in errs.values()), errs
+
+
+def test_autotune_group_keeps_results():
+    """autotune_group picks the all-level launch shapes of a group by timing
+    group steps; the shapes change no result: the tuned group's state equals
+    an untuned group's after the same number of steps, bit for bit."""
+    from ibpm.autotune import autotune_group
+    pa, sa = members()
+    pg, sg = members([p.Binv for p in pa])
+    ga, gb = ibpm.Group(pa, sa), ibpm.Group(pg, sg)
+    taken = []
+    orig = gb.advance_async
+    gb.advance_async = lambda n: (taken.append(n), orig(n))[1]
+    chosen = autotune_group(gb, steps=2)
+    assert set(chosen) == {C.OPT_YTILE_BATCH, C.OPT_PIPE, C.OPT_XWAVES_BATCH}
+    for o, v in chosen.items():
+        assert pg[0].get_option(o) == v
+    ga.advance_async(sum(taken))
+    ga.wait()
+    ga.advance_async(3)
+    gb.advance_async(3)
+    for (cda, cla, da), (cdb, clb, db) in zip(ga.wait(), gb.wait()):
+        assert da == db == 3
+        assert np.array_equal(cda, cdb) and np.array_equal(cla, clb)
+    ga.close()
+    gb.close()
+    for st, stg in zip(sa, sg):
+        for a, b in zip(fields(st), fields(stg)):
+            assert np.array_equal(a, b)
