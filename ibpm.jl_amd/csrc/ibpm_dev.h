@@ -674,7 +674,7 @@ __device__ __forceinline__ void dst1_line_t(double* s, const DstTab& tb, int lan
 // other length runs dst1_line.
 __host__ __device__ inline bool dst_specialised(int P)
 {
-    return P == 512 || P == 400 || P == 256 || P == 200 || P == 128 || P == 100;
+    return P == 512 || P == 400 || P == 300 || P == 256 || P == 200 || P == 150 || P == 128 || P == 100;
 }
 
 // ---------------------------------------------------------------------------

@@ -305,8 +305,10 @@ static void launch_x_m(const XArgs& a, hipStream_t s)
     switch (P) {
     case 512: launch_x_w<MODE, 512>(a, s); break;
     case 400: launch_x_w<MODE, 400>(a, s); break;
+    case 300: launch_x_w<MODE, 300>(a, s); break;
     case 256: launch_x_w<MODE, 256>(a, s); break;
     case 200: launch_x_w<MODE, 200>(a, s); break;
+    case 150: launch_x_w<MODE, 150>(a, s); break;
     case 128: launch_x_w<MODE, 128>(a, s); break;
     case 100: launch_x_w<MODE, 100>(a, s); break;
     default: launch_x_w<MODE, 0>(a, s); break;
@@ -834,8 +836,10 @@ static void launch_y_m(const YArgs& a, hipStream_t s)
     switch (P) {
     case 512: launch_y_t<TI, 512, YW>(a, s); break;
     case 400: launch_y_t<TI, 400, YW>(a, s); break;
+    case 300: launch_y_t<TI, 300, YW>(a, s); break;
     case 256: launch_y_t<TI, 256, YW>(a, s); break;
     case 200: launch_y_t<TI, 200, YW>(a, s); break;
+    case 150: launch_y_t<TI, 150, YW>(a, s); break;
     case 128: launch_y_t<TI, 128, YW>(a, s); break;
     case 100: launch_y_t<TI, 100, YW>(a, s); break;
     default: launch_y_t<TI, 0, YW>(a, s); break;
@@ -1006,8 +1010,10 @@ void launch_ypass(const YArgs& a, int tile, hipStream_t s)
         switch (P) {
         case 512: launch_y_pipe_t<512>(a, nwg, s); return;
         case 400: launch_y_pipe_t<400>(a, nwg, s); return;
+        case 300: launch_y_pipe_t<300>(a, nwg, s); return;
         case 256: launch_y_pipe_t<256>(a, nwg, s); return;
         case 200: launch_y_pipe_t<200>(a, nwg, s); return;
+        case 150: launch_y_pipe_t<150>(a, nwg, s); return;
         case 128: launch_y_pipe_t<128>(a, nwg, s); return;
         case 100: launch_y_pipe_t<100>(a, nwg, s); return;
         default: launch_y_pipe_t<0>(a, nwg, s); return;

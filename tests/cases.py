@@ -56,6 +56,10 @@ def case(name: str):
         grid = ibpm.make_grid(1.0 / 32, (-4.0, 4.0, -3.125, 3.125), mg=2)
         bodies = [ibpm.make_cylinder(0.5, grid.h, 0.3, -0.2, n=100)]
         return dict(grid=grid, bodies=bodies, Re=150.0, dt=0.005, freestream=U1)
+    if name == "spec2":  # 600 x 300, 2 levels: compile-time transform lengths P = 300 (x, radices 4 5 5 3) and 150 (y, 2 5 5 3)
+        grid = ibpm.make_grid(0.02, (-4.0, 8.0, -3.0, 3.0), mg=2)
+        bodies = [ibpm.make_cylinder(0.25, grid.h, 0.1, -0.05, n=80)]
+        return dict(grid=grid, bodies=bodies, Re=120.0, dt=0.005, freestream=U1)
     if name == "direct":  # 56 x 44, 2 levels: P = 28 and 22 have factors 7 and 11, so both directions take the direct O(n^2) DST
         grid = ibpm.make_grid(0.125, (-2.0, 5.0, -2.75, 2.75), mg=2)
         bodies = [ibpm.make_cylinder(0.5, grid.h, 0.0, 0.0, n=26)]

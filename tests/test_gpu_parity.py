@@ -32,7 +32,7 @@ def build(name, share_binv=False):
     return c, op, gp
 
 
-@pytest.fixture(scope="module", params=["tiny", "small3", "mixed", "moving", "spec", "direct", "single"])
+@pytest.fixture(scope="module", params=["tiny", "small3", "mixed", "moving", "spec", "spec2", "direct", "single"])
 def pair(request):
     return build(request.param)
 
@@ -187,7 +187,7 @@ def test_one_step_shared_binv(name):
 
 
 @pytest.mark.parametrize("name,nsteps", [("tiny", 40), ("small3", 30), ("mixed", 30), ("moving", 40), ("spec", 30),
-                                          ("direct", 30), ("single", 30)])
+                                          ("spec2", 20), ("direct", 30), ("single", 30)])
 def test_many_steps(name, nsteps):
     c, op, gp = build(name)
     S, st = initial_states(c, op, gp)
