@@ -156,11 +156,10 @@ __device__ __forceinline__ void dft8(double2* v)
 // complex) then fall in distinct LDS banks instead of one.
 __device__ __forceinline__ int pz(int e) { return e + (e >> 3); }
 
-// 10- and 12-point DFTs by the prime-factor (Good-Thomas) index maps, so
-// that no twiddles enter between the two sets of small DFTs.  10 = 2 x 5:
-// input n = (5 n1 + 2 n2) mod 10, output k = (5 k1 + 6 k2) mod 10, and
-// W10^{nk} = W2^{n1 k1} W5^{n2 k2}.  12 = 4 x 3: n = (3 n1 + 4 n2) mod 12,
-// k = (9 k1 + 4 k2) mod 12, W12^{nk} = W4^{n1 k1} W3^{n2 k2}.
+// 10-point DFT as 2 x 5 by the prime-factor (Good-Thomas) index map, so
+// that no twiddles enter between the two sets of small DFTs: input
+// n = (5 n1 + 2 n2) mod 10, output k = (5 k1 + 6 k2) mod 10, and
+// W10^{nk} = W2^{n1 k1} W5^{n2 k2}.
 __device__ __forceinline__ void dft10(double2* v)
 {
     double2 a[2][5];
@@ -178,24 +177,6 @@ __device__ __forceinline__ void dft10(double2* v)
         for (int k2 = 0; k2 < 5; k2++) v[(5 * k1 + 6 * k2) % 10] = a[k1][k2];
 }
 
-__device__ __forceinline__ void dft12(double2* v)
-{
-    double2 a[3][4];
-#pragma unroll
-    for (int n2 = 0; n2 < 3; n2++) {
-#pragma unroll
-        for (int n1 = 0; n1 < 4; n1++) a[n2][n1] = v[(3 * n1 + 4 * n2) % 12];
-        dft4(a[n2]);
-    }
-#pragma unroll
-    for (int k1 = 0; k1 < 4; k1++) {
-        double2 b[3] = {a[0][k1], a[1][k1], a[2][k1]};
-        dft3(b);
-#pragma unroll
-        for (int k2 = 0; k2 < 3; k2++) v[(9 * k1 + 4 * k2) % 12] = b[k2];
-    }
-}
-
 template <int R>
 __device__ __forceinline__ void dft_r(double2* v)
 {
@@ -205,7 +186,6 @@ __device__ __forceinline__ void dft_r(double2* v)
     else if constexpr (R == 5) dft5(v);
     else if constexpr (R == 8) dft8(v);
     else if constexpr (R == 10) dft10(v);
-    else if constexpr (R == 12) dft12(v);
 }
 
 // One in-place Stockham stage of radix R over P complex points in LDS `z`
@@ -541,11 +521,11 @@ struct RadPlan {
 };
 
 // radices of P: 8s first, then the remaining 4 or 2, then 5s and 3s, as the
-// host plan, except that a remaining 4 takes a 3 along as one radix-12 stage
-// and a remaining 2 a 5 as one radix-10 stage where that stage still has half
-// a wavefront of butterflies (one LDS round trip fewer; with fewer lanes
-// busy the merged stage costs more than it saves).  The host plan, which only
-// the runtime-length path follows, keeps them apart.
+// host plan, except that a remaining 2 takes a 5 along as one radix-10 stage
+// where that stage still has half a wavefront of butterflies (one LDS round
+// trip fewer; with fewer lanes busy, as for P = 150, the merged stage cost
+// more than it saved).  The host plan, which only the runtime-length path
+// follows, keeps them apart.
 constexpr RadPlan make_radplan(int P)
 {
     RadPlan rp{0, {0, 0, 0, 0, 0, 0, 0, 0}};
@@ -554,8 +534,7 @@ constexpr RadPlan make_radplan(int P)
     while (m % 3 == 0) { m /= 3; e3++; }
     while (m % 5 == 0) { m /= 5; e5++; }
     while (e2 >= 3) { rp.r[rp.n++] = 8; e2 -= 3; }
-    if (e2 == 2 && e3 > 0 && P / 12 >= 32) { rp.r[rp.n++] = 12; e3--; }
-    else if (e2 == 2) rp.r[rp.n++] = 4;
+    if (e2 == 2) rp.r[rp.n++] = 4;
     if (e2 == 1 && e5 > 0 && P / 10 >= 32) { rp.r[rp.n++] = 10; e5--; }
     else if (e2 == 1) rp.r[rp.n++] = 2;
     for (int k = 0; k < e5; k++) rp.r[rp.n++] = 5;
