@@ -9,7 +9,8 @@ configuration run in FP64).  A step is one advance!(state, prob, t).
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed] [--graphs 0|1]
     python bench.py --kernels-only [--kernel-reps R]   (per-kernel timing only)
     python bench.py --workload naca64 [--steps K]       (batched 64-instance NACA workload)
-    python bench.py --workload configs                  (every configuration, CPU oracle beside it)
+    python bench.py --workload configs [--config NAME] [--no-cpu] [--graphs 0|1]
+                                                        (every configuration, CPU oracle beside it)
 
 Prints one JSON line.  Timed region: K consecutive steps enqueued by
 ibpm_advance_n with the state resident in HBM, bracketed by device
@@ -204,9 +205,12 @@ def bench_configs(args):
     """One line per configuration: GPU FP64 timesteps/s (state resident, one
     ibpm_advance_n call), the same rate as a fraction of 8 TB/s by the bytes
     formula, and the C oracle's timesteps/s on 1 and on 16 host threads
-    (a few seconds of steps each, after one untimed step)."""
+    (a few seconds of steps each, after one untimed step).  --config keeps
+    one configuration, --no-cpu skips the oracle legs, --graphs 0 launches
+    every kernel from the host (for a profiler)."""
     import ibpm
     import oracle as O
+    from ibpm import _capi as C
     from ibpm.autotune import autotune
     from cases import boundary_of, case, gaussian_vortex_gamma, oracle_bodies, oracle_steps
 
@@ -215,6 +219,10 @@ def bench_configs(args):
             ("cyl_re200_1024x1024x5", "cylinder Re=200, 1024x1024, 5 levels, Gaussian vortex", 300, True),
             ("plate_pitch_512x256x4", "MovingGrid pitching plate Re=200, 512x256, 4 levels", 300, False),
             ("eldredge_lab", "MotionFunction pitching plate Re=100, 400x200, 5 levels (lab frame)", 200, False)]
+    if args.config:
+        runs = [r for r in runs if r[0] == args.config]
+        if not runs:
+            raise SystemExit(f"unknown configuration {args.config}")
     for name, desc, steps, vortex in runs:
         c = case(name)
         g = c["grid"]
@@ -222,6 +230,7 @@ def bench_configs(args):
         motion = 2 if isinstance(c["bodies"][0].motion, ibpm.MotionFunction) else (1 if "motion" in c else 0)
         prob = ibpm.IBProblem(g, c["bodies"], c["dt"], c["Re"], freestream=c["freestream"])
         autotune(prob, reps=5)
+        prob.set_option(C.OPT_GRAPHS, args.graphs)
         st = ibpm.IBState(prob)
         if vortex:
             st.Γ = gaussian_vortex_gamma(g)
@@ -234,8 +243,8 @@ def bench_configs(args):
         el = time.perf_counter() - t0
         sps = done / el
         byts = algorithmic_bytes_per_step(g.nx, g.ny, g.mg, nb) + (8.0 * g.mg * g.nq if motion == 1 else 0.0)
-        cpu = {}
-        for threads in (1, 16):
+        cpu = {1: None, 16: None}
+        for threads in (() if args.no_cpu else (1, 16)):
             O.set_threads(threads)
             op = O.OracleProblem(g.h, boundary_of(g), g.mg, c["Re"], c["dt"], oracle_bodies(c), motion=motion,
                                  binv=None if motion == 2 else prob.Binv)
@@ -252,7 +261,8 @@ def bench_configs(args):
         print(json.dumps({"workload": name, "desc": desc, "grid": [g.nx, g.ny], "levels": g.mg, "body_points": nb,
                           "gpu_timesteps_per_s": round(sps, 1), "steps": done,
                           "hbm_frac_by_bytes_formula": round(byts * sps / 1e9 / HBM_PEAK_GBS, 4),
-                          "cpu_oracle_1_thread": round(cpu[1], 3), "cpu_oracle_16_threads": round(cpu[16], 3)}),
+                          "cpu_oracle_1_thread": cpu[1] and round(cpu[1], 3),
+                          "cpu_oracle_16_threads": cpu[16] and round(cpu[16], 3)}),
               flush=True)
 
 
@@ -271,6 +281,8 @@ def main():
     ap.add_argument("--workload", choices=("cylinder1024", "naca64", "configs"), default="cylinder1024",
                     help="naca64: the batched 64-instance NACA workload (secondary line); configs: one line "
                          "per configuration with the CPU oracle beside it")
+    ap.add_argument("--config", default=None,
+                    help="with --workload configs: only the named configuration (e.g. cyl_re100_800x400x5)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only autotune and time each kernel kind (for counter profiling); prints kernel_ms")
     ap.add_argument("--no-autotune", action="store_true",
